@@ -1,0 +1,264 @@
+// Shared definitions of the MI355X DFSPH step: the parameter blocks the
+// kernels take by value, and the device math of the step.
+//
+// Byte layouts follow the reference's SimParams (crates/fluid-core/src/sim.rs:325-350,
+// sim_solve.wgsl:11-23) and Step (sim.rs:200-252, sim_solve.wgsl:29-41).
+// Every expression below keeps the reference shader's operand order, and the
+// library is built with -ffp-contract=off and IEEE division and square root,
+// so each float result is that of single precision evaluated in that order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fluid {
+
+// ---- byte blocks of the reference's parameter API -----------------------
+
+struct SimParamsBlock {  // 64 B, sim.rs:330-350
+    float box_min[3];
+    float cell;
+    uint32_t dims[3];
+    uint32_t count;
+    float h, mass, rho0, inv_h, sigma, sigma_h, support_sq, pad;
+};
+static_assert(sizeof(SimParamsBlock) == 64, "SimParams is 64 bytes");
+
+struct StepBlock {  // 144 B, sim.rs:215-252
+    float force[3];
+    float dt;
+    float omega[3];
+    float v_clamp;
+    float domega[3];
+    uint32_t seed;
+    float touch_r;
+    uint32_t touch_n;
+    uint32_t pad[2];
+    float touches[5][4];
+};
+static_assert(sizeof(StepBlock) == 144, "Step is 144 bytes");
+
+// ---- constants of the method, sim_solve.wgsl:84-159, sim_tracers.wgsl:39-52
+
+constexpr float XSPH_RATE = 6.0f;
+constexpr float TOUCH_RATE = 40.0f;
+constexpr float LENS_RATE = 20.0f;
+constexpr float K_NEAR = 3000.0f;
+constexpr float DYNAMIC_VISCOSITY = 1.002e-3f;
+constexpr float HEAT_CAPACITY = 4184.0f;
+constexpr float CONDUCTIVITY = 0.598f;
+constexpr float EXPANSION = 2.07e-4f;
+constexpr float SIGMA_WATER = 0.0728f;
+constexpr float LATTICE_SIGMA = 0.8665f;
+constexpr float COS_CONTACT = -0.342f;
+constexpr float FIXED = 65536.0f;
+constexpr float TAU = 3.0f;
+constexpr float TAU_STRAY = 0.25f;
+constexpr float T_CHARGE = 4.0f;
+constexpr float HUGE_F = 1e30f;
+
+// ---- what the kernels take by value --------------------------------------
+
+// Scene constants: the SimParams block plus the pair-loop constants the
+// shader hoists (sim_solve.wgsl:161-172, 534-537), evaluated once on the host
+// in single precision in the shader's operand order.
+struct SceneK {
+    float box_min[3];
+    float cell;
+    uint32_t dims[3];
+    uint32_t count;
+    float h, mass, rho0, inv_h, sigma, sigma_h, support_sq;
+    float wall_lo[3], wall_hi[3];
+    float inv_h_wall;  // 1.0 / h as wall_grad_sum and wall_adh_sum evaluate it
+    float coh_c, coh_k, coh_near, gamma, beta;
+    float visc2, heat_cond, heat_visc, eps_h2;
+};
+
+// Substep constants: the Step block plus the exponentials the shader
+// evaluates per invocation, which are uniform over a substep.
+struct StepK {
+    float force[3];
+    float dt;
+    float omega[3];
+    float v_clamp;
+    float domega[3];
+    uint32_t seed;
+    float touch_r;
+    uint32_t touch_n;
+    float touches[5][4];
+    float c_xsph;    // 1 - exp(-XSPH_RATE dt), sim_solve.wgsl:657
+    float chase;     // 1 - exp(-LENS_RATE dt), sim_solve.wgsl:741
+    float decay;     // exp(-dt / T_CHARGE), sim_tracers.wgsl:182
+    float dt_guard;  // max(dt, 1e-9), sim_solve.wgsl:516
+};
+
+SceneK make_scene(const SimParamsBlock& p);
+StepK make_step(const StepBlock& s);
+
+// ---- device math (the reference's WGSL helpers) --------------------------
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return {x, y, z}; }
+__device__ __forceinline__ f3 add(f3 a, f3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 scl(float s, f3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float wmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float wmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ f3 xyz(float4 v) { return {v.x, v.y, v.z}; }
+
+// float -> u32 as WGSL and Rust convert: truncation, negatives and NaN to 0,
+// saturation at the top.
+__device__ __forceinline__ uint32_t sat_u32(float f) {
+    if (!(f > 0.0f)) return 0u;
+    if (f >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)f;
+}
+__device__ __forceinline__ int32_t sat_i32(float f) {
+    if (f != f) return 0;
+    if (f >= 2147483648.0f) return 2147483647;
+    if (f <= -2147483648.0f) return (int32_t)0x80000000u;
+    return (int32_t)f;
+}
+
+// sim_grid.wgsl:36-42: the clamped, truncated cell coordinate.
+__device__ __forceinline__ uint32_t cell_coord(float p, float lo, float cell, uint32_t dim) {
+    uint32_t c = sat_u32((p - lo) / cell);
+    return c < dim - 1u ? c : dim - 1u;
+}
+
+// sim_solve.wgsl:176-182
+__device__ __forceinline__ float kernel_q(const SceneK& k, float q) {
+    if (q < 1.0f) return k.sigma * (1.0f - 1.5f * q * q * (1.0f - 0.5f * q));
+    float t = 2.0f - q;
+    return k.sigma * 0.25f * t * t * t;
+}
+// sim_solve.wgsl:184-196: the scalar dW/r that multiplies x.
+__device__ __forceinline__ float grad_scale(const SceneK& k, float r, float q) {
+    float dw;
+    if (q < 1.0f) {
+        dw = k.sigma_h * (-3.0f * q + 2.25f * q * q);
+    } else {
+        float t = 2.0f - q;
+        dw = k.sigma_h * (-0.75f * t * t);
+    }
+    return dw / r;
+}
+__device__ __forceinline__ f3 grad_q(const SceneK& k, f3 x, float r, float q) {
+    if (r < 1e-9f) return mk(0.0f, 0.0f, 0.0f);
+    return scl(grad_scale(k, r, q), x);
+}
+// sim_solve.wgsl:201-210
+__device__ __forceinline__ float cohesion(float r, float c, float kk, float nearc) {
+    if (r >= c || r < 1e-9f) return 0.0f;
+    float a = (c - r) * (c - r) * (c - r) * r * r * r;
+    if (2.0f * r > c) return kk * a;
+    return kk * (2.0f * a - nearc);
+}
+// sim_solve.wgsl:213-222
+__device__ __forceinline__ float wall_density(float t) {
+    if (t >= 2.0f) return 0.0f;
+    float t3 = t * t * t;
+    if (t < 1.0f) return 0.5f - 0.7f * t + t3 * ((float)(1.0 / 3.0) + t * t * (-0.15f + 0.05f * t));
+    return (float)(8.0 / 15.0) - 0.8f * t + t3 * ((float)(2.0 / 3.0) + t * (-0.5f + t * (0.15f - t / 60.0f)));
+}
+// sim_solve.wgsl:226-235
+__device__ __forceinline__ float wall_gradient(float t) {
+    if (t >= 2.0f) return 0.0f;
+    float t2 = t * t;
+    if (t < 1.0f) return 0.7f + t2 * (-1.0f + t2 * (0.75f - t * 0.3f));
+    return 0.8f + t2 * (-2.0f + t * (2.0f + t * (-0.75f + t * 0.1f)));
+}
+// sim_solve.wgsl:249-270 — the reference's fitted coefficient table.
+__device__ __forceinline__ float wedge(float t1, float t2) {
+    if (t1 * t1 + t2 * t2 >= 4.0f) return 0.0f;
+    float c0 = 2.5005807e-01f + t2 * (-3.4681153e-01f + t2 * (-4.0986882e-02f + t2 * (3.0301620e-01f + t2 * (-1.9202736e-01f + t2 * (4.6947582e-02f + t2 * (-3.8543515e-03f))))));
+    float c1 = -3.4681153e-01f + t2 * (4.8587248e-01f + t2 * (1.0092180e-02f + t2 * (-2.9978363e-01f + t2 * (1.5320335e-01f + t2 * (-2.2544282e-02f)))));
+    float c2 = -4.0986882e-02f + t2 * (1.0092180e-02f + t2 * (-5.4145610e-02f + t2 * (7.9597369e-02f + t2 * (-2.3350373e-02f))));
+    float c3 = 3.0301620e-01f + t2 * (-2.9978363e-01f + t2 * (7.9597369e-02f + t2 * (-8.2431946e-03f)));
+    float c4 = -1.9202736e-01f + t2 * (1.5320335e-01f + t2 * (-2.3350373e-02f));
+    float c5 = 4.6947582e-02f + t2 * (-2.2544282e-02f);
+    return c0 + t1 * (c1 + t1 * (c2 + t1 * (c3 + t1 * (c4 + t1 * (c5 + t1 * (-3.8543515e-03f))))));
+}
+// sim_solve.wgsl:272-293
+__device__ __forceinline__ float wedge_d(float t1, float t2) {
+    if (t1 * t1 + t2 * t2 >= 4.0f) return 0.0f;
+    float c0 = -3.4611994e-01f + t2 * (4.4757673e-01f + t2 * (1.1932939e-01f + t2 * (-4.6460101e-01f + t2 * (2.7637253e-01f + t2 * (-6.2846025e-02f + t2 * (4.3871652e-03f))))));
+    float c1 = -6.3622823e-02f + t2 * (2.0521006e-01f + t2 * (-3.7311339e-01f + t2 * (3.7508155e-01f + t2 * (-1.6629713e-01f + t2 * (2.5030129e-02f)))));
+    float c2 = 7.8352497e-01f + t2 * (-1.2658344e+00f + t2 * (5.8898945e-01f + t2 * (-1.0772903e-01f + t2 * (1.3535777e-02f))));
+    float c3 = -4.9470084e-01f + t2 * (9.1302071e-01f + t2 * (-3.4330355e-01f + t2 * (2.4034609e-02f)));
+    float c4 = -2.5605481e-02f + t2 * (-2.0067981e-01f + t2 * (5.9854468e-02f));
+    float c5 = 8.8898623e-02f + t2 * (3.9914838e-03f);
+    return c0 + t1 * (c1 + t1 * (c2 + t1 * (c3 + t1 * (c4 + t1 * (c5 + t1 * (-1.7788321e-02f))))));
+}
+// sim_solve.wgsl:334-340
+__device__ __forceinline__ float wall_adhesion(float t) {
+    float u = wmin(wmax(2.0f - t, 0.0f), 2.0f);
+    return u * u * (1.7847015e-3f + u * (3.3392196e-3f + u * (-4.3394677e-3f + u * (1.6711868e-3f + u * (-2.1788750e-4f)))));
+}
+__device__ __forceinline__ float wedge_dsum(float t, float a, float b, float c, float d) {
+    return wedge_d(t, a) + wedge_d(t, b) + wedge_d(t, c) + wedge_d(t, d);
+}
+// sim_solve.wgsl:299-304
+__device__ __forceinline__ float wedge_sum(f3 tl, f3 th) {
+    float w = wedge(tl.x, tl.y) + wedge(tl.x, th.y) + wedge(th.x, tl.y) + wedge(th.x, th.y);
+    w += wedge(tl.x, tl.z) + wedge(tl.x, th.z) + wedge(th.x, tl.z) + wedge(th.x, th.z);
+    w += wedge(tl.y, tl.z) + wedge(tl.y, th.z) + wedge(th.y, tl.z) + wedge(th.y, th.z);
+    return w;
+}
+// sim_solve.wgsl:310-328
+__device__ __forceinline__ f3 wall_grad_sum(const SceneK& k, f3 pos) {
+    float ih = k.inv_h_wall;
+    f3 tl = mk((pos.x - k.wall_lo[0]) * ih, (pos.y - k.wall_lo[1]) * ih, (pos.z - k.wall_lo[2]) * ih);
+    f3 th = mk((k.wall_hi[0] - pos.x) * ih, (k.wall_hi[1] - pos.y) * ih, (k.wall_hi[2] - pos.z) * ih);
+    float scale = k.rho0 * ih;
+    f3 g = mk(0.0f, 0.0f, 0.0f);
+    g.x -= scale * (wall_gradient(tl.x) + wedge_dsum(tl.x, tl.y, th.y, tl.z, th.z));
+    g.x += scale * (wall_gradient(th.x) + wedge_dsum(th.x, tl.y, th.y, tl.z, th.z));
+    g.y -= scale * (wall_gradient(tl.y) + wedge_dsum(tl.y, tl.x, th.x, tl.z, th.z));
+    g.y += scale * (wall_gradient(th.y) + wedge_dsum(th.y, tl.x, th.x, tl.z, th.z));
+    g.z -= scale * (wall_gradient(tl.z) + wedge_dsum(tl.z, tl.x, th.x, tl.y, th.y));
+    g.z += scale * (wall_gradient(th.z) + wedge_dsum(th.z, tl.x, th.x, tl.y, th.y));
+    return g;
+}
+// sim_solve.wgsl:349-361
+__device__ __forceinline__ f3 wall_adh_sum(const SceneK& k, f3 pos) {
+    float ih = k.inv_h_wall;
+    f3 a = mk(0.0f, 0.0f, 0.0f);
+    a.x -= wall_adhesion((pos.x - k.wall_lo[0]) * ih);
+    a.x += wall_adhesion((k.wall_hi[0] - pos.x) * ih);
+    a.y -= wall_adhesion((pos.y - k.wall_lo[1]) * ih);
+    a.y += wall_adhesion((k.wall_hi[1] - pos.y) * ih);
+    a.z -= wall_adhesion((pos.z - k.wall_lo[2]) * ih);
+    a.z += wall_adhesion((k.wall_hi[2] - pos.z) * ih);
+    return a;
+}
+
+// ---- launch geometry -----------------------------------------------------
+
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;
+
+// Maps the dispatch-order workgroup id onto a tile index so that each of the
+// 8 XCDs (which take workgroups round-robin) owns one contiguous run of
+// tiles: a run of cell-sorted particles shares its neighbours, and this keeps
+// their gathers in the XCD's own L2. A bijection on [0, nblocks).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
+    if (nblocks < 2u * kXcds) return b;
+    uint32_t per = (nblocks + kXcds - 1) / kXcds;
+    uint32_t tall = nblocks % kXcds;
+    if (tall == 0) tall = kXcds;
+    uint32_t xcd = b % kXcds, local = b / kXcds;
+    if (xcd < tall) return xcd * per + local;
+    return tall * per + (xcd - tall) * (per - 1) + local;
+}
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+}  // namespace fluid

@@ -1,0 +1,115 @@
+// Internal interface between the host-side sim (sim.cpp) and the kernel
+// translation units (grid.hip, solve.hip, stats_tracers.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fluid_common.hpp"
+
+namespace fluid {
+
+// Cell arithmetic only (the SimParams head every grid kernel reads).
+struct GridGeom {
+    float box_min[3];
+    float cell;
+    uint32_t dims[3];
+};
+
+// The five persistent records of one particle set (resting or working),
+// plus the optional identity payload carried through the sort.
+struct Records {
+    float4* pos;            // xyz; w = 0 at rest
+    float4* vel;            // xyz; w = acceleration EMA
+    float4* prev_vel;       // xyz; w = pressure EMA
+    float* prev_pressure;   // last substep's pressure
+    float* temperature;     // kelvin
+    uint32_t* ids;          // null unless ids are tracked
+};
+
+struct GridBufs {
+    uint32_t* cellid;    // [N] cell of each resting particle
+    uint32_t* rank;      // [N] provisional rank from the histogram atomics
+    uint32_t* counts;    // [C] histogram (cleared by the scan)
+    uint32_t* starts;    // [C + 1]
+    uint32_t* perm_tmp;  // [N] provisional slot -> resting index
+    uint32_t* perm;      // [N] stable slot -> resting index
+    uint64_t* status;    // [tiles] look-back words
+    uint32_t* ticket;    // [1] tile ticket
+};
+
+// Per-substep scratch of the solver, all in slot (cell) order.
+struct SolveBufs {
+    float* density;
+    float* alpha;
+    float* kd;        // kappa / rho
+    float* pressure;
+    float* rho_near;
+    float4* accel;    // xyz; w = dT/dt
+    float4* xsph;
+    float4* wall_grad;
+    uint32_t* nbr;    // [cap][stride] neighbour slots, neighbour-major
+    uint32_t* nbr_n;  // [N]
+    uint32_t cap;
+    uint32_t stride;
+    uint32_t* counters;  // [0] CFL clamps, [1] neighbour overflow (cumulative)
+};
+
+enum KernelId {
+    KERNEL_GRID_COUNT = 0,
+    KERNEL_GRID_SCAN,
+    KERNEL_GRID_SCATTER,
+    KERNEL_DENSITY_DIV,
+    KERNEL_DIV_APPLY,
+    KERNEL_FORCES_EVAL,
+    KERNEL_FORCES_DEN_APPLY,
+    KERNEL_DEN_KAPPA,
+    KERNEL_DEN_APPLY,
+    KERNEL_INTEGRATE,
+    KERNEL_REDUCE_STATS,
+    KERNEL_TRACERS,
+    KERNEL_COUNT_
+};
+
+// HIP-event timing of kernel classes on the launch stream (profiling mode).
+struct KernelTimer {
+    virtual ~KernelTimer() = default;
+    virtual void begin(hipStream_t st, int kernel) = 0;
+    virtual void end(hipStream_t st, int kernel) = 0;
+};
+
+uint32_t scan_tiles(uint32_t cells);
+void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Records& rest, const Records& work,
+                       const GridBufs& b, KernelTimer* timer);
+void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,
+                      uint32_t* ticket);
+
+// One substep's solve on the working set (after the grid build), ending with
+// integrate writing the resting set: density_div, div_apply, forces_eval,
+// forces_den_apply, refine x (den_kappa, den_apply), integrate.
+void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
+                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t refine,
+                  KernelTimer* timer);
+// The density sweep alone (the zero-substep path of the reference's tests).
+void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
+                         const Records& work, const SolveBufs& b, KernelTimer* timer);
+
+struct StatsBufs {
+    float* partials;   // [blocks][16]
+    uint32_t blocks;
+    float* stats;      // [14]
+};
+uint32_t stats_blocks(uint32_t n);
+void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
+                         const uint32_t* counters, const StatsBufs& b, KernelTimer* timer);
+
+struct TracerBufs {
+    int32_t* vel_grid;   // [C][4] 16.16 sums (cleared by resolve)
+    int32_t* vel_flat;   // [C][4]
+    uint32_t* tracers;   // [T][2] packed
+    uint32_t count;
+};
+void launch_tracers(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const Records& rest,
+                    const TracerBufs& b, KernelTimer* timer);
+
+}  // namespace fluid

@@ -1,0 +1,239 @@
+// Neighbour-grid build for gfx950: cell hashing with an atomic per-cell
+// histogram, a single-pass decoupled-look-back exclusive scan built on wave64
+// DPP and ballot, and a stable scatter of the resting set into cell-sorted
+// SoA arrays.
+//
+// Replaces the reference's count (sim_grid.wgsl:44-50), scan_single
+// (sim_scan.wgsl:97-120) and scatter (sim_grid.wgsl:52-72). The reference's
+// scatter takes an atomic cursor, so its in-cell order is whatever the
+// atomics gave; this build fixes the stable order (within a cell, particles
+// keep the order of the resting set), one of the orders the reference admits.
+//
+// Launches (all on the caller's stream, no host synchronisation):
+//   grid_count   cell id per particle; histogram via one atomic per run of
+//                equal cells in a wave; a provisional rank; clears the scan's
+//                tile state for this build.
+//   grid_scan    starts[c] = sum of counts below c, starts[C] = N; clears
+//                the counts for the next build.
+//   grid_place   provisional slot starts[c] + rank <- particle index.
+//   grid_gather  stable rank within the cell; permutation; copies the five
+//                persistent records (plus ids when tracked) into slot order.
+#include "fluid_common.hpp"
+#include "fluid_kernels.hpp"
+
+namespace fluid {
+
+namespace {
+
+constexpr int kScanItems = 8;
+constexpr uint32_t kScanTile = kBlock * kScanItems;
+constexpr uint64_t kFlagAggregate = 1ull << 32;
+constexpr uint64_t kFlagInclusive = 2ull << 32;
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Inclusive prefix sum over the 64 lanes of a wave: four row shifts build
+// the 16-lane row scans, then two row broadcasts (lane 15 into rows 1 and 3,
+// lane 31 into rows 2 and 3) carry the row totals across the wave.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)v;
+}
+
+__device__ __forceinline__ uint32_t wave_total(uint32_t x) {
+    return (uint32_t)__shfl((int)wave_inclusive_sum(x), 63);
+}
+
+__device__ __forceinline__ uint64_t load_status(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_count(GridGeom g, const float4* __restrict__ pos, uint32_t n,
+                                                       uint32_t* __restrict__ cellid, uint32_t* __restrict__ rank,
+                                                       uint32_t* __restrict__ counts, uint64_t* __restrict__ status,
+                                                       uint32_t ntiles, uint32_t* __restrict__ ticket) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < ntiles) status[i] = 0;
+    if (i == 0) *ticket = 0;
+    bool live = i < n;
+    uint32_t c = 0xffffffffu;
+    if (live) {
+        float4 p = pos[i];
+        uint32_t cx = cell_coord(p.x, g.box_min[0], g.cell, g.dims[0]);
+        uint32_t cy = cell_coord(p.y, g.box_min[1], g.cell, g.dims[1]);
+        uint32_t cz = cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]);
+        c = (cz * g.dims[1] + cy) * g.dims[0] + cx;
+        cellid[i] = c;
+    }
+    // Lanes that share a cell (neighbours in a cell-sorted resting set mostly
+    // do) take one atomic between them; each gets the old count plus its rank
+    // among the equal lanes below it.
+    uint64_t active = __ballot(live);
+    uint32_t lane = lane_id();
+    uint32_t mine = 0;
+    while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        uint32_t lc = (uint32_t)__shfl((int)c, leader);
+        uint64_t m = __ballot(c == lc) & active;
+        uint32_t base = 0;
+        if ((int)lane == leader) base = atomicAdd(&counts[lc], (uint32_t)__popcll(m));
+        base = (uint32_t)__shfl((int)base, leader);
+        if ((m >> lane) & 1ull) mine = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        active &= ~m;
+    }
+    if (live) rank[i] = mine;
+}
+
+// One tile of kScanTile counts per workgroup, tiles taken in ticket order so
+// every predecessor of a tile is already resident: its look-back waits only
+// on workgroups that publish without waiting.
+__global__ __launch_bounds__(kBlock) void k_grid_scan(uint32_t* __restrict__ counts, uint32_t* __restrict__ starts,
+                                                      uint32_t cells, uint64_t* __restrict__ status,
+                                                      uint32_t* __restrict__ ticket) {
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave[kBlock / 64];
+    __shared__ uint32_t s_prefix;
+    if (threadIdx.x == 0) s_tile = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint32_t base = tile * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint32_t c = base + k;
+        v[k] = c < cells ? counts[c] : 0u;
+        tsum += v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (base + k < cells) counts[base + k] = 0u;  // ready for the next build
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t incl = wave_inclusive_sum(tsum);
+    if (lane == 63u) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t wave_prefix = 0, agg = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) wave_prefix += s_wave[w];
+        agg += s_wave[w];
+    }
+    if (wave == 0) {
+        uint32_t prefix = 0;
+        if (tile == 0) {
+            if (lane == 0) store_status(&status[0], kFlagInclusive | agg);
+        } else {
+            if (lane == 0) store_status(&status[tile], kFlagAggregate | agg);
+            int64_t j = (int64_t)tile - 1;
+            while (true) {
+                int64_t t = j - (int64_t)lane;
+                uint64_t w = kFlagInclusive;  // before tile 0: an inclusive zero
+                if (t >= 0) {
+                    do {
+                        w = load_status(&status[t]);
+                    } while ((w >> 32) == 0);
+                }
+                uint64_t incl_mask = __ballot((w >> 32) == 2u);
+                uint32_t val = (uint32_t)w;
+                if (incl_mask) {
+                    uint32_t first = (uint32_t)(__ffsll((unsigned long long)incl_mask) - 1);
+                    prefix += wave_total(lane <= first ? val : 0u);
+                    break;
+                }
+                prefix += wave_total(val);
+                j -= 64;
+            }
+            if (lane == 0) store_status(&status[tile], kFlagInclusive | (uint64_t)(prefix + agg));
+        }
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    uint32_t run = s_prefix + wave_prefix + (incl - tsum);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        uint32_t c = base + k;
+        if (c < cells) starts[c] = run;
+        run += v[k];
+        if (c + 1 == cells) starts[cells] = run;  // the sentinel: N
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_place(uint32_t n, const uint32_t* __restrict__ cellid,
+                                                       const uint32_t* __restrict__ rank,
+                                                       const uint32_t* __restrict__ starts,
+                                                       uint32_t* __restrict__ perm_tmp) {
+    uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    perm_tmp[starts[cellid[i]] + rank[i]] = i;
+}
+
+// A slot's particle takes the final slot start + (particles of its cell with
+// a smaller resting index): the stable counting-sort order, whatever order
+// the atomics of grid_count handed out.
+__global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32_t* __restrict__ perm_tmp,
+                                                        const uint32_t* __restrict__ cellid,
+                                                        const uint32_t* __restrict__ starts, Records rest,
+                                                        Records work, uint32_t* __restrict__ perm) {
+    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= n) return;
+    uint32_t i = perm_tmp[s];
+    uint32_t c = cellid[i];
+    uint32_t st = starts[c], en = starts[c + 1];
+    uint32_t r = 0;
+    for (uint32_t t = st; t < en; ++t) r += perm_tmp[t] < i ? 1u : 0u;
+    uint32_t f = st + r;
+    perm[f] = i;
+    work.pos[f] = rest.pos[i];
+    work.vel[f] = rest.vel[i];
+    work.prev_vel[f] = rest.prev_vel[i];
+    work.prev_pressure[f] = rest.prev_pressure[i];
+    work.temperature[f] = rest.temperature[i];
+    if (rest.ids) work.ids[f] = rest.ids[i];
+}
+
+}  // namespace
+
+uint32_t scan_tiles(uint32_t cells) { return (cells + kScanTile - 1) / kScanTile; }
+
+void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Records& rest, const Records& work,
+                       const GridBufs& b, KernelTimer* timer) {
+    const uint32_t cells = g.dims[0] * g.dims[1] * g.dims[2];
+    const uint32_t ntiles = scan_tiles(cells);
+    uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
+    if (timer) timer->begin(st, KERNEL_GRID_COUNT);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, g, rest.pos, n,
+                       b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
+    if (timer) timer->end(st, KERNEL_GRID_COUNT);
+    if (timer) timer->begin(st, KERNEL_GRID_SCAN);
+    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, b.counts, b.starts, cells, b.status,
+                       b.ticket);
+    if (timer) timer->end(st, KERNEL_GRID_SCAN);
+    if (n == 0) return;
+    if (timer) timer->begin(st, KERNEL_GRID_SCATTER);
+    hipLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.cellid, b.rank, b.starts,
+                       b.perm_tmp);
+    hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp, b.cellid,
+                       b.starts, rest, work, b.perm);
+    if (timer) timer->end(st, KERNEL_GRID_SCATTER);
+}
+
+// A stand-alone exclusive scan of `cells` counts (the grid scan, exposed for
+// its own tests): counts are cleared, starts gets cells + 1 entries.
+void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,
+                      uint32_t* ticket) {
+    const uint32_t ntiles = scan_tiles(cells);
+    hipMemsetAsync(status, 0, (size_t)ntiles * sizeof(uint64_t), st);
+    hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, counts, starts, cells, status, ticket);
+}
+
+}  // namespace fluid

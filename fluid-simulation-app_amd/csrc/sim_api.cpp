@@ -1,0 +1,625 @@
+// Host side of the MI355X DFSPH step: device buffers, the substep schedule
+// and the C ABI of include/fluid_sim.h.
+//
+// Takes the role of the reference's Sim (crates/fluid-core/src/render.rs:1382-1612
+// for the state it allocates, 3164-3228 for the dispatch order). Everything
+// runs on one HIP stream per handle; nothing synchronises unless a call
+// returns host data.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fluid_common.hpp"
+#include "fluid_kernels.hpp"
+#include "../../include/fluid_sim.h"
+
+namespace fluid {
+
+SceneK make_scene(const SimParamsBlock& p) {
+    SceneK k;
+    for (int a = 0; a < 3; ++a) {
+        k.box_min[a] = p.box_min[a];
+        k.dims[a] = p.dims[a];
+        // wall_lo = box_min + cell, walls at -wall_lo (sim_solve.wgsl:237-239)
+        k.wall_lo[a] = p.box_min[a] + p.cell;
+        k.wall_hi[a] = -k.wall_lo[a];
+    }
+    k.cell = p.cell;
+    k.count = p.count;
+    k.h = p.h;
+    k.mass = p.mass;
+    k.rho0 = p.rho0;
+    k.inv_h = p.inv_h;
+    k.sigma = p.sigma;
+    k.sigma_h = p.sigma_h;
+    k.support_sq = p.support_sq;
+    k.inv_h_wall = 1.0f / p.h;
+    // forces_eval's hoisted cohesion constants, sim_solve.wgsl:534-537
+    k.coh_c = 2.0f * p.h;
+    const float c3 = k.coh_c * k.coh_c * k.coh_c;
+    k.coh_k = 32.0f / (3.14159265f * c3 * c3 * c3);
+    k.coh_near = c3 * c3 / 64.0f;
+    // tension_gamma and adhesion_beta, sim_solve.wgsl:161-172
+    const float c = 2.0f * p.h;
+    k.gamma = (SIGMA_WATER * 7040.0f) / (21.0f * LATTICE_SIGMA * p.rho0 * p.rho0 * c * c);
+    k.beta = (SIGMA_WATER * (1.0f + COS_CONTACT) * 1558.4f) / (LATTICE_SIGMA * p.rho0 * p.rho0 * c * c);
+    // the constant factors of forces_eval's pair terms, sim_solve.wgsl:554-565
+    k.visc2 = 2.0f * DYNAMIC_VISCOSITY;
+    k.heat_cond = 2.0f * CONDUCTIVITY / HEAT_CAPACITY;
+    k.heat_visc = 0.5f * DYNAMIC_VISCOSITY / HEAT_CAPACITY;
+    k.eps_h2 = 0.01f * p.h * p.h;
+    return k;
+}
+
+StepK make_step(const StepBlock& s) {
+    StepK k;
+    for (int a = 0; a < 3; ++a) {
+        k.force[a] = s.force[a];
+        k.omega[a] = s.omega[a];
+        k.domega[a] = s.domega[a];
+    }
+    k.dt = s.dt;
+    k.v_clamp = s.v_clamp;
+    k.seed = s.seed;
+    k.touch_r = s.touch_r;
+    // The shader's touches array has five slots (sim_solve.wgsl:40).
+    k.touch_n = s.touch_n < 5u ? s.touch_n : 5u;
+    std::memcpy(k.touches, s.touches, sizeof k.touches);
+    k.c_xsph = 1.0f - std::exp(-XSPH_RATE * s.dt);
+    k.chase = 1.0f - std::exp(-LENS_RATE * s.dt);
+    k.decay = std::exp(-s.dt / T_CHARGE);
+    k.dt_guard = std::max(s.dt, 1e-9f);
+    return k;
+}
+
+namespace {
+
+thread_local std::string g_error;
+
+void set_error(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    fprintf(stderr, "fluid_sim: %s\n", buf);
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return FLUID_ERR_HIP;                                                       \
+        }                                                                               \
+    } while (0)
+
+const char* kKernelNames[KERNEL_COUNT_] = {
+    "grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
+    "forces_den_apply", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"};
+
+class EventTimer final : public KernelTimer {
+  public:
+    uint32_t mask = 0;
+    struct Pair {
+        hipEvent_t a, b;
+        int kernel;
+    };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open[KERNEL_COUNT_] = {};
+    double ms[KERNEL_COUNT_] = {};
+    uint64_t launches[KERNEL_COUNT_] = {};
+
+    ~EventTimer() override {
+        for (auto& p : pending) {
+            hipEventDestroy(p.a);
+            hipEventDestroy(p.b);
+        }
+        for (auto e : pool) hipEventDestroy(e);
+    }
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    bool wants(int kernel) const { return (mask >> kernel) & 1u; }
+    void begin(hipStream_t st, int kernel) override {
+        if (!wants(kernel)) return;
+        open[kernel] = take();
+        hipEventRecord(open[kernel], st);
+    }
+    void end(hipStream_t st, int kernel) override {
+        if (!wants(kernel) || !open[kernel]) return;
+        hipEvent_t e = take();
+        hipEventRecord(e, st);
+        pending.push_back({open[kernel], e, kernel});
+        open[kernel] = nullptr;
+    }
+    void collect() {
+        for (auto& p : pending) {
+            hipEventSynchronize(p.b);
+            float t = 0.0f;
+            hipEventElapsedTime(&t, p.a, p.b);
+            ms[p.kernel] += t;
+            launches[p.kernel] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        collect();
+        std::fill(ms, ms + KERNEL_COUNT_, 0.0);
+        std::fill(launches, launches + KERNEL_COUNT_, 0);
+    }
+};
+
+}  // namespace
+}  // namespace fluid
+
+using namespace fluid;
+
+struct FluidSim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    SimParamsBlock params{};
+    SceneK scene{};
+    GridGeom geom{};
+    uint32_t n = 0, cells = 0, cap = 0, ntr = 0;
+    Records rest{}, work{};
+    GridBufs grid{};
+    SolveBufs solve{};
+    StatsBufs stats{};
+    TracerBufs tracers{};
+    bool track_ids = false;
+    EventTimer timer;
+    std::vector<void*> allocs;
+
+    KernelTimer* t() { return timer.mask ? &timer : nullptr; }
+
+    template <typename P>
+    int alloc(P*& out, size_t count, bool zero = true) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(P);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+            return FLUID_ERR_NOMEM;
+        }
+        allocs.push_back(p);
+        if (zero) {
+            e = hipMemset(p, 0, bytes);
+            if (e != hipSuccess) {
+                set_error("hipMemset failed: %s", hipGetErrorString(e));
+                return FLUID_ERR_HIP;
+            }
+        }
+        out = static_cast<P*>(p);
+        return FLUID_OK;
+    }
+
+    ~FluidSim() {
+        if (stream) hipStreamSynchronize(stream);
+        for (void* p : allocs) hipFree(p);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int check_launch(const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
+        return FLUID_ERR_HIP;
+    }
+    return FLUID_OK;
+}
+
+int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_cap) {
+    std::memcpy(&s->params, params64, sizeof s->params);
+    const SimParamsBlock& p = s->params;
+    if (p.dims[0] < 2 || p.dims[1] < 2 || p.dims[2] < 2) {
+        set_error("grid dims %u x %u x %u: each must be at least 2", p.dims[0], p.dims[1], p.dims[2]);
+        return FLUID_ERR_ARG;
+    }
+    const uint64_t cells64 = (uint64_t)p.dims[0] * p.dims[1] * p.dims[2];
+    if (cells64 >= 0xffffffffull) {
+        set_error("grid of %llu cells exceeds 32-bit cell ids", (unsigned long long)cells64);
+        return FLUID_ERR_ARG;
+    }
+    if (!(p.cell > 0.0f) || !(p.h > 0.0f)) {
+        set_error("cell %g and h %g must be positive", p.cell, p.h);
+        return FLUID_ERR_ARG;
+    }
+    s->scene = make_scene(p);
+    for (int a = 0; a < 3; ++a) {
+        s->geom.box_min[a] = p.box_min[a];
+        s->geom.dims[a] = p.dims[a];
+    }
+    s->geom.cell = p.cell;
+    s->n = p.count;
+    s->cells = (uint32_t)cells64;
+    s->cap = nbr_cap ? nbr_cap : FLUID_SIM_NBR_CAP;
+    s->ntr = ntr;
+    HIP_TRY(hipGetDevice(&s->device));
+    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const size_t n = s->n, c = s->cells;
+    int rc = FLUID_OK;
+#define A(ptr, cnt)                            \
+    if ((rc = s->alloc(ptr, cnt)) != FLUID_OK) \
+        return rc;
+    for (Records* r : {&s->rest, &s->work}) {
+        A(r->pos, n);
+        A(r->vel, n);
+        A(r->prev_vel, n);
+        A(r->prev_pressure, n);
+        A(r->temperature, n);
+        r->ids = nullptr;
+    }
+    A(s->grid.cellid, n);
+    A(s->grid.rank, n);
+    A(s->grid.counts, c);
+    A(s->grid.starts, c + 1);
+    A(s->grid.perm_tmp, n);
+    A(s->grid.perm, n);
+    A(s->grid.status, scan_tiles(s->cells) + 1);
+    A(s->grid.ticket, 1);
+    SolveBufs& b = s->solve;
+    A(b.density, n);
+    A(b.alpha, n);
+    A(b.kd, n);
+    A(b.pressure, n);
+    A(b.rho_near, n);
+    A(b.accel, n);
+    A(b.xsph, n);
+    A(b.wall_grad, n);
+    b.cap = s->cap;
+    b.stride = (uint32_t)((n + 63) / 64 * 64);
+    if (b.stride == 0) b.stride = 64;
+    if ((rc = s->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
+    A(b.nbr_n, n);
+    A(b.counters, 4);
+    s->stats.blocks = stats_blocks(s->n);
+    A(s->stats.partials, (size_t)s->stats.blocks * 16);
+    A(s->stats.stats, 16);
+    A(s->tracers.vel_grid, c * 4);
+    A(s->tracers.vel_flat, c * 4);
+    A(s->tracers.tracers, (size_t)std::max<uint32_t>(ntr, 1) * 2);
+#undef A
+    s->tracers.count = ntr;
+    // The box starts at 293.15 K (render.rs:1540-1555); all else at zero.
+    std::vector<float> warm(std::max<size_t>(n, 1), 293.15f);
+    HIP_TRY(hipMemcpy(s->rest.temperature, warm.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    return FLUID_OK;
+}
+
+bool valid(const FluidSim* s) {
+    if (!s) {
+        set_error("null sim handle");
+        return false;
+    }
+    return true;
+}
+
+int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t refine) {
+    StepBlock sb;
+    std::memcpy(&sb, step144, sizeof sb);
+    // dt = 0 encodes no solve: kappa divides by dt^2 (render.rs:3165-3167).
+    if (!(sb.dt > 0.0f) || nsub == 0) return FLUID_OK;
+    const StepK k = make_step(sb);
+    for (uint32_t i = 0; i < nsub; ++i) {
+        launch_grid_build(s->stream, s->geom, s->n, s->rest, s->work, s->grid, s->t());
+        launch_solve(s->stream, s->scene, k, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
+    }
+    return check_launch("substep");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fluid_sim_last_error(void) { return g_error.c_str(); }
+
+int fluid_sim_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+FluidSim* fluid_sim_create(const uint8_t* params64, uint32_t tracers, uint32_t nbr_cap) {
+    if (!params64) {
+        set_error("null SimParams block");
+        return nullptr;
+    }
+    if (fluid_sim_device_count() < 1) {
+        set_error("no HIP device");
+        return nullptr;
+    }
+    FluidSim* s = new FluidSim();
+    if (init_sim(s, params64, tracers, nbr_cap) != FLUID_OK) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+void fluid_sim_destroy(FluidSim* sim) { delete sim; }
+
+uint32_t fluid_sim_count(const FluidSim* sim) { return sim ? sim->n : 0; }
+uint32_t fluid_sim_cells(const FluidSim* sim) { return sim ? sim->cells : 0; }
+uint32_t fluid_sim_tracer_count(const FluidSim* sim) { return sim ? sim->ntr : 0; }
+
+int fluid_sim_upload_state(FluidSim* s, const float* pos4, const float* vel4, const float* prev_vel4,
+                           const float* prev_pressure, const float* temperature) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    const size_t n = s->n;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (pos4) HIP_TRY(hipMemcpy(s->rest.pos, pos4, n * 16, hipMemcpyHostToDevice));
+    if (vel4) HIP_TRY(hipMemcpy(s->rest.vel, vel4, n * 16, hipMemcpyHostToDevice));
+    if (prev_vel4) HIP_TRY(hipMemcpy(s->rest.prev_vel, prev_vel4, n * 16, hipMemcpyHostToDevice));
+    if (prev_pressure) HIP_TRY(hipMemcpy(s->rest.prev_pressure, prev_pressure, n * 4, hipMemcpyHostToDevice));
+    if (temperature) HIP_TRY(hipMemcpy(s->rest.temperature, temperature, n * 4, hipMemcpyHostToDevice));
+    return FLUID_OK;
+}
+
+int fluid_sim_download_state(FluidSim* s, float* pos4, float* vel4, float* prev_vel4, float* prev_pressure,
+                             float* temperature) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    const size_t n = s->n;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (pos4) HIP_TRY(hipMemcpy(pos4, s->rest.pos, n * 16, hipMemcpyDeviceToHost));
+    if (vel4) HIP_TRY(hipMemcpy(vel4, s->rest.vel, n * 16, hipMemcpyDeviceToHost));
+    if (prev_vel4) HIP_TRY(hipMemcpy(prev_vel4, s->rest.prev_vel, n * 16, hipMemcpyDeviceToHost));
+    if (prev_pressure) HIP_TRY(hipMemcpy(prev_pressure, s->rest.prev_pressure, n * 4, hipMemcpyDeviceToHost));
+    if (temperature) HIP_TRY(hipMemcpy(temperature, s->rest.temperature, n * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_download_field(FluidSim* s, uint32_t field, float* out) {
+    if (!valid(s) || !out) return FLUID_ERR_ARG;
+    const void* src = nullptr;
+    size_t per = 1;
+    switch (field) {
+        case FLUID_FIELD_DENSITY: src = s->solve.density; break;
+        case FLUID_FIELD_ALPHA: src = s->solve.alpha; break;
+        case FLUID_FIELD_KD: src = s->solve.kd; break;
+        case FLUID_FIELD_PRESSURE: src = s->solve.pressure; break;
+        case FLUID_FIELD_RHO_NEAR: src = s->solve.rho_near; break;
+        case FLUID_FIELD_ACCEL: src = s->solve.accel; per = 4; break;
+        case FLUID_FIELD_XSPH: src = s->solve.xsph; per = 4; break;
+        case FLUID_FIELD_WALL_GRAD: src = s->solve.wall_grad; per = 4; break;
+        case FLUID_FIELD_WORK_POS: src = s->work.pos; per = 4; break;
+        case FLUID_FIELD_WORK_VEL: src = s->work.vel; per = 4; break;
+        default:
+            set_error("unknown field %u", field);
+            return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, src, (size_t)s->n * per * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_download_grid(FluidSim* s, uint32_t* cellid, uint32_t* counts, uint32_t* starts, uint32_t* perm,
+                            uint32_t* nbr_n) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const size_t n = s->n, c = s->cells;
+    if (cellid) HIP_TRY(hipMemcpy(cellid, s->grid.cellid, n * 4, hipMemcpyDeviceToHost));
+    if (starts) HIP_TRY(hipMemcpy(starts, s->grid.starts, (c + 1) * 4, hipMemcpyDeviceToHost));
+    if (counts) {
+        // The scan clears the histogram for the next build; the counts of the
+        // last build are the differences of the starts.
+        std::vector<uint32_t> st(c + 1);
+        HIP_TRY(hipMemcpy(st.data(), s->grid.starts, (c + 1) * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < c; ++i) counts[i] = st[i + 1] - st[i];
+    }
+    if (perm) HIP_TRY(hipMemcpy(perm, s->grid.perm, n * 4, hipMemcpyDeviceToHost));
+    if (nbr_n) HIP_TRY(hipMemcpy(nbr_n, s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
+    if (!valid(s) || !nbr) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    const size_t n = s->n, cap = s->cap, stride = s->solve.stride;
+    std::vector<uint32_t> raw(cap * stride);
+    std::vector<uint32_t> cnt(std::max<size_t>(n, 1));
+    HIP_TRY(hipMemcpy(raw.data(), s->solve.nbr, raw.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i)
+        for (size_t m = 0; m < cap; ++m) nbr[i * cap + m] = m < cnt[i] ? raw[m * stride + i] : 0u;
+    return FLUID_OK;
+}
+
+int fluid_sim_track_ids(FluidSim* s, bool enable) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (!enable) {
+        s->rest.ids = nullptr;
+        s->work.ids = nullptr;
+        s->track_ids = false;
+        return FLUID_OK;
+    }
+    if (!s->track_ids) {
+        uint32_t *a = nullptr, *b = nullptr;
+        int rc = s->alloc(a, s->n);
+        if (rc) return rc;
+        if ((rc = s->alloc(b, s->n))) return rc;
+        std::vector<uint32_t> iota(std::max<size_t>(s->n, 1));
+        for (uint32_t i = 0; i < s->n; ++i) iota[i] = i;
+        HIP_TRY(hipMemcpy(a, iota.data(), (size_t)s->n * 4, hipMemcpyHostToDevice));
+        s->rest.ids = a;
+        s->work.ids = b;
+        s->track_ids = true;
+    }
+    return FLUID_OK;
+}
+
+int fluid_sim_download_ids(FluidSim* s, uint32_t* ids) {
+    if (!valid(s) || !ids) return FLUID_ERR_ARG;
+    if (!s->track_ids) {
+        set_error("ids are not tracked");
+        return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(ids, s->rest.ids, (size_t)s->n * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_substeps(FluidSim* s, const uint8_t* step144, uint32_t n, uint32_t refine) {
+    if (!valid(s) || !step144) return FLUID_ERR_ARG;
+    return run_substeps(s, step144, n, refine);
+}
+
+int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
+    if (!valid(s) || !step144) return FLUID_ERR_ARG;
+    StepBlock sb;
+    std::memcpy(&sb, step144, sizeof sb);
+    const StepK k = make_step(sb);
+    launch_grid_build(s->stream, s->geom, s->n, s->rest, s->work, s->grid, s->t());
+    launch_density_only(s->stream, s->scene, k, s->n, s->grid.starts, s->work, s->solve, s->t());
+    return check_launch("density_only");
+}
+
+int fluid_sim_reduce_stats(FluidSim* s) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    if (s->n == 0) return FLUID_OK;
+    launch_reduce_stats(s->stream, s->params.rho0, s->n, s->solve.density, s->rest, s->solve.counters, s->stats,
+                        s->t());
+    return check_launch("reduce_stats");
+}
+
+int fluid_sim_read_stats(FluidSim* s, float* out14) {
+    if (!valid(s) || !out14) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out14, s->stats.stats, 14 * sizeof(float), hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_tracers(FluidSim* s, const uint8_t* step144) {
+    if (!valid(s) || !step144) return FLUID_ERR_ARG;
+    StepBlock sb;
+    std::memcpy(&sb, step144, sizeof sb);
+    launch_tracers(s->stream, s->scene, make_step(sb), s->n, s->rest, s->tracers, s->t());
+    return check_launch("tracers");
+}
+
+int fluid_sim_upload_tracers(FluidSim* s, const uint32_t* packed2) {
+    if (!valid(s) || !packed2) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(s->tracers.tracers, packed2, (size_t)s->ntr * 8, hipMemcpyHostToDevice));
+    return FLUID_OK;
+}
+
+int fluid_sim_download_tracers(FluidSim* s, uint32_t* packed2) {
+    if (!valid(s) || !packed2) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(packed2, s->tracers.tracers, (size_t)s->ntr * 8, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_frame(FluidSim* s, const uint8_t* step144, uint32_t n, uint32_t refine, const uint8_t* tracer_step144) {
+    if (!valid(s) || !step144) return FLUID_ERR_ARG;
+    StepBlock sb;
+    std::memcpy(&sb, step144, sizeof sb);
+    if (!(sb.dt > 0.0f)) return FLUID_OK;  // render.rs:3165-3167: no solve, no stats, no tracers
+    int rc = run_substeps(s, step144, n, refine);
+    if (rc) return rc;
+    if ((rc = fluid_sim_reduce_stats(s))) return rc;
+    if (s->ntr > 0 && tracer_step144) return fluid_sim_tracers(s, tracer_step144);
+    return FLUID_OK;
+}
+
+int fluid_sim_counters(FluidSim* s, uint64_t* clamps, uint64_t* overflow) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    uint32_t c[4];
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(c, s->solve.counters, sizeof c, hipMemcpyDeviceToHost));
+    if (clamps) *clamps = c[0];
+    if (overflow) *overflow = c[1];
+    return FLUID_OK;
+}
+
+int fluid_sim_synchronize(FluidSim* s) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return FLUID_OK;
+}
+
+int fluid_sim_profile(FluidSim* s, uint32_t kernel_mask) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->timer.reset();
+    s->timer.mask = kernel_mask;
+    return FLUID_OK;
+}
+
+int fluid_sim_profile_read(FluidSim* s, double* ms, uint64_t* launches) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->timer.collect();
+    for (int k = 0; k < KERNEL_COUNT_; ++k) {
+        if (ms) ms[k] = s->timer.ms[k];
+        if (launches) launches[k] = s->timer.launches[k];
+    }
+    return FLUID_OK;
+}
+
+const char* fluid_sim_kernel_name(uint32_t kernel) {
+    return kernel < (uint32_t)KERNEL_COUNT_ ? kKernelNames[kernel] : "";
+}
+
+int fluid_scan_u32(const uint32_t* counts, uint32_t cells, uint32_t* starts) {
+    if (!counts || !starts || cells == 0) {
+        set_error("scan needs counts, starts and cells > 0");
+        return FLUID_ERR_ARG;
+    }
+    if (fluid_sim_device_count() < 1) {
+        set_error("no HIP device");
+        return FLUID_ERR_DEVICE;
+    }
+    uint32_t *dc = nullptr, *ds = nullptr, *ticket = nullptr;
+    uint64_t* status = nullptr;
+    const uint32_t tiles = scan_tiles(cells);
+    hipStream_t st = nullptr;
+    int rc = FLUID_OK;
+    hipError_t e = hipSuccess;
+    do {
+        if ((e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) break;
+        if ((e = hipMalloc(&dc, (size_t)cells * 4)) != hipSuccess) break;
+        if ((e = hipMalloc(&ds, ((size_t)cells + 1) * 4)) != hipSuccess) break;
+        if ((e = hipMalloc(&status, ((size_t)tiles + 1) * 8)) != hipSuccess) break;
+        if ((e = hipMalloc(&ticket, 4)) != hipSuccess) break;
+        if ((e = hipMemcpy(dc, counts, (size_t)cells * 4, hipMemcpyHostToDevice)) != hipSuccess) break;
+        launch_scan_only(st, dc, ds, cells, status, ticket);
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
+        if ((e = hipMemcpy(starts, ds, ((size_t)cells + 1) * 4, hipMemcpyDeviceToHost)) != hipSuccess) break;
+    } while (false);
+    if (e != hipSuccess) {
+        set_error("scan failed: %s", hipGetErrorString(e));
+        rc = FLUID_ERR_HIP;
+    }
+    hipFree(dc);
+    hipFree(ds);
+    hipFree(status);
+    hipFree(ticket);
+    if (st) hipStreamDestroy(st);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,303 @@
+// Frame statistics and the passive tracers for gfx950.
+//
+// reduce_stats replaces the reference's single-workgroup reduction
+// (crates/fluid-core/src/sim_solve.wgsl:800-874) by a device-wide one: a
+// grid of workgroups folds strided slices into per-workgroup partials, and
+// one workgroup folds the partials in workgroup order. Min and max are exact;
+// the compression sum is a float sum in a different association than the
+// reference's 256-way strided sum.
+//
+// The tracers follow sim_tracers.wgsl: splat (98-114) sums particle
+// velocities per cell in 16.16 fixed point with integer atomics (exact in
+// any order), resolve (122-138) copies and clears that grid, advect (140-183)
+// samples it trilinearly and moves, recycles and repacks each tracer.
+#include "fluid_common.hpp"
+#include "fluid_kernels.hpp"
+
+namespace fluid {
+
+namespace {
+
+constexpr int kStatsMaxBlocks = 1024;
+constexpr int kStatsFields = 13;  // sum, max c, rho lo/hi, v lo/hi, a lo/hi, p lo/hi, T lo/hi, pad
+
+__device__ __forceinline__ float lane_min(float a, float b) { return wmin(a, b); }
+__device__ __forceinline__ float lane_max(float a, float b) { return wmax(a, b); }
+
+__global__ __launch_bounds__(kBlock) void k_stats_partial(float rho0, uint32_t n, const float* __restrict__ density,
+                                                          const float4* __restrict__ vel,
+                                                          const float4* __restrict__ prev_vel,
+                                                          const float* __restrict__ temperature,
+                                                          float* __restrict__ partials) {
+    float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const float d = density[i];
+        const float c = d / rho0 - 1.0f;
+        acc[0] += wmax(c, 0.0f);
+        acc[1] = wmax(acc[1], c);
+        acc[2] = wmin(acc[2], d);
+        acc[3] = wmax(acc[3], d);
+        const float4 q = vel[i];
+        const float sp = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z);
+        acc[4] = wmin(acc[4], sp);
+        acc[5] = wmax(acc[5], sp);
+        acc[6] = wmin(acc[6], q.w);
+        acc[7] = wmax(acc[7], q.w);
+        const float pw = prev_vel[i].w;
+        acc[8] = wmin(acc[8], pw);
+        acc[9] = wmax(acc[9], pw);
+        const float t = temperature[i];
+        acc[10] = wmin(acc[10], t);
+        acc[11] = wmax(acc[11], t);
+    }
+    __shared__ float red[12][kBlock];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) red[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            red[0][threadIdx.x] += red[0][o];
+            red[1][threadIdx.x] = wmax(red[1][threadIdx.x], red[1][o]);
+#pragma unroll
+            for (int f = 2; f < 12; f += 2) {
+                red[f][threadIdx.x] = wmin(red[f][threadIdx.x], red[f][o]);
+                red[f + 1][threadIdx.x] = wmax(red[f + 1][threadIdx.x], red[f + 1][o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) partials[(size_t)blockIdx.x * 16 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(64) void k_stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
+                                                    const uint32_t* __restrict__ counters, float* __restrict__ stats) {
+    if (threadIdx.x != 0) return;
+    float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const float* p = partials + (size_t)b * 16;
+        acc[0] += p[0];
+        acc[1] = wmax(acc[1], p[1]);
+        for (int f = 2; f < 12; f += 2) {
+            acc[f] = wmin(acc[f], p[f]);
+            acc[f + 1] = wmax(acc[f + 1], p[f + 1]);
+        }
+    }
+    stats[0] = acc[0] / (float)n;
+    stats[1] = wmax(acc[1], 0.0f);
+    stats[2] = acc[2];
+    stats[3] = acc[3];
+    stats[4] = acc[8];
+    stats[5] = acc[9];
+    stats[6] = acc[5];
+    stats[7] = acc[10];
+    stats[8] = acc[11];
+    stats[9] = (float)counters[0];
+    stats[10] = (float)counters[1];
+    stats[11] = acc[4];
+    stats[12] = acc[6];
+    stats[13] = acc[7];
+}
+
+// ---- tracers -------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t pcg(uint32_t x) {
+    uint32_t h = x * 747796405u + 2891336453u;
+    h = ((h >> ((h >> 28u) + 4u)) ^ h) * 277803737u;
+    return (h >> 22u) ^ h;
+}
+
+__device__ __forceinline__ uint32_t unorm16(float v) {
+    const float c = wmin(wmax(v, 0.0f), 1.0f);
+    return (uint32_t)(0.5f + 65535.0f * c);
+}
+
+// IEEE binary16 bits, round to nearest even (pack2x16float).
+__device__ __forceinline__ uint32_t half_bits(float f) {
+    const uint32_t x = __float_as_uint(f);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) return sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u : 0u);
+    if (ax >= 0x477ff000u) return sign | 0x7c00u;
+    if (ax < 0x38800000u) {
+        if (ax < 0x33000000u) return sign;
+        const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+        const uint32_t shift = 126u - (ax >> 23);
+        uint32_t q = mant >> shift;
+        const uint32_t rem = mant & ((1u << shift) - 1u);
+        const uint32_t half = 1u << (shift - 1u);
+        if (rem > half || (rem == half && (q & 1u))) q++;
+        return sign | q;
+    }
+    uint32_t q = (ax - 0x38000000u) >> 13;
+    const uint32_t rem = ax & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+    return sign | q;
+}
+
+__device__ __forceinline__ float half_value(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    if (e == 0) {
+        const float v = (float)m * (1.0f / 16777216.0f);
+        return __uint_as_float(__float_as_uint(v) | sign);
+    }
+    if (e == 31) return __uint_as_float(sign | 0x7f800000u | (m << 13));
+    return __uint_as_float(sign | ((e + 112u) << 23) | (m << 13));
+}
+
+__device__ __forceinline__ float box_extent(const SceneK& k, int a) { return -(k.box_min[a] + k.cell); }
+
+__device__ __forceinline__ void store_tracer(const SceneK& k, uint32_t* tr, uint32_t i, float x, float y, float z,
+                                             float sp) {
+    const uint32_t ux = unorm16(x / box_extent(k, 0) * 0.5f + 0.5f);
+    const uint32_t uy = unorm16(y / box_extent(k, 1) * 0.5f + 0.5f);
+    uint2 out;
+    out.x = ux | (uy << 16);
+    out.y = half_bits(z) | (half_bits(sp) << 16);
+    reinterpret_cast<uint2*>(tr)[i] = out;
+}
+
+__device__ __forceinline__ void respawn(const SceneK& k, uint32_t n, const float4* __restrict__ pos,
+                                        const float4* __restrict__ vel, uint32_t* tr, uint32_t slot, uint32_t r) {
+    const uint32_t r1 = pcg(r), r2 = pcg(r1), r3 = pcg(r2), r4 = pcg(r3);
+    const float hc = k.cell * 0.5f;
+    const float jx = ((float)(r2 >> 20) / 4096.0f - 0.5f) * hc;
+    const float jy = ((float)(r3 >> 20) / 4096.0f - 0.5f) * hc;
+    const float jz = ((float)(r4 >> 20) / 4096.0f - 0.5f) * hc;
+    const uint32_t j = r1 % n;
+    const float4 pj = pos[j];
+    const float4 vj = vel[j];
+    float q[3] = {pj.x + jx, pj.y + jy, pj.z + jz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float e = box_extent(k, a);
+        q[a] = wmin(wmax(q[a], -e), e);
+    }
+    store_tracer(k, tr, slot, q[0], q[1], q[2], sqrtf(vj.x * vj.x + vj.y * vj.y + vj.z * vj.z));
+}
+
+__global__ __launch_bounds__(kBlock) void k_splat(GridGeom g, uint32_t n, const float4* __restrict__ pos,
+                                                  const float4* __restrict__ vel, int32_t* __restrict__ grid) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = pos[i];
+    const uint32_t c = ((cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]) * g.dims[1] +
+                         cell_coord(p.y, g.box_min[1], g.cell, g.dims[1])) *
+                            g.dims[0] +
+                        cell_coord(p.x, g.box_min[0], g.cell, g.dims[0])) *
+                       4u;
+    const float4 v = vel[i];
+    atomicAdd(&grid[c], sat_i32(v.x * FIXED));
+    atomicAdd(&grid[c + 1], sat_i32(v.y * FIXED));
+    atomicAdd(&grid[c + 2], sat_i32(v.z * FIXED));
+    atomicAdd(&grid[c + 3], 1);
+}
+
+__global__ __launch_bounds__(kBlock) void k_resolve(uint32_t cells, int4* __restrict__ grid, int4* __restrict__ flat) {
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= cells) return;
+    flat[c] = grid[c];
+    grid[c] = make_int4(0, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(kBlock) void k_advect(SceneK k, StepK s, uint32_t n, const float4* __restrict__ pos,
+                                                   const float4* __restrict__ vel, const int4* __restrict__ flat,
+                                                   uint32_t* __restrict__ tr, uint32_t ntr) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= ntr) return;
+    const uint32_t r0 = pcg(i ^ pcg(s.seed));
+    if ((float)r0 / 4294967295.0f < s.dt / TAU) {
+        respawn(k, n, pos, vel, tr, i, r0);
+        return;
+    }
+    const uint2 t = reinterpret_cast<const uint2*>(tr)[i];
+    const float ex = box_extent(k, 0), ey = box_extent(k, 1), ez = box_extent(k, 2);
+    const float px = ((float)(t.x & 0xffffu) / 65535.0f * 2.0f - 1.0f) * ex;
+    const float py = ((float)(t.x >> 16) / 65535.0f * 2.0f - 1.0f) * ey;
+    const float pz = half_value(t.y & 0xffffu);
+    const float ow = half_value(t.y >> 16);
+    const float pp[3] = {px, py, pz};
+    int32_t base[3];
+    float f[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float gp = (pp[a] - k.box_min[a]) / k.cell - 0.5f;
+        int32_t b = sat_i32(floorf(gp));
+        const int32_t hi = (int32_t)k.dims[a] - 2;
+        b = b < 0 ? 0 : (b > hi ? hi : b);
+        base[a] = b;
+        f[a] = wmin(wmax(gp - (float)(uint32_t)b, 0.0f), 1.0f);
+    }
+    float mx = 0.0f, my = 0.0f, mz = 0.0f, wc = 0.0f;
+#pragma unroll
+    for (uint32_t kk = 0; kk < 8; ++kk) {
+        const uint32_t o0 = kk & 1u, o1 = (kk >> 1) & 1u, o2 = (kk >> 2) & 1u;
+        const float w0 = wmix(1.0f - f[0], f[0], (float)o0);
+        const float w1 = wmix(1.0f - f[1], f[1], (float)o1);
+        const float w2 = wmix(1.0f - f[2], f[2], (float)o2);
+        const float weight = w0 * w1 * w2;
+        const uint32_t c = (((uint32_t)base[2] + o2) * k.dims[1] + ((uint32_t)base[1] + o1)) * k.dims[0] +
+                           ((uint32_t)base[0] + o0);
+        const int4 cv = flat[c];
+        mx += weight * (float)cv.x;
+        my += weight * (float)cv.y;
+        mz += weight * (float)cv.z;
+        wc += weight * (float)cv.w;
+    }
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    if (wc > 0.0f) {
+        const float den = wc * FIXED;
+        vx = mx / den;
+        vy = my / den;
+        vz = mz / den;
+    } else if ((float)pcg(r0 ^ 0x85ebca6bu) / 4294967295.0f < s.dt / TAU_STRAY) {
+        respawn(k, n, pos, vel, tr, i, pcg(r0 ^ 0x632be59bu));
+        return;
+    }
+    const float q0 = wmin(wmax(px + vx * s.dt, -ex), ex);
+    const float q1 = wmin(wmax(py + vy * s.dt, -ey), ey);
+    const float q2 = wmin(wmax(pz + vz * s.dt, -ez), ez);
+    const float sp = wmax(sqrtf(vx * vx + vy * vy + vz * vz), ow * s.decay);
+    store_tracer(k, tr, i, q0, q1, q2, sp);
+}
+
+}  // namespace
+
+uint32_t stats_blocks(uint32_t n) {
+    uint32_t b = blocks_for(n);
+    if (b < 1) b = 1;
+    return b < (uint32_t)kStatsMaxBlocks ? b : (uint32_t)kStatsMaxBlocks;
+}
+
+void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
+                         const uint32_t* counters, const StatsBufs& b, KernelTimer* timer) {
+    if (timer) timer->begin(st, KERNEL_REDUCE_STATS);
+    hipLaunchKernelGGL(k_stats_partial, dim3(b.blocks), dim3(kBlock), 0, st, rho0, n, density, rest.vel,
+                       rest.prev_vel, rest.temperature, b.partials);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, st, n, b.blocks, b.partials, counters, b.stats);
+    if (timer) timer->end(st, KERNEL_REDUCE_STATS);
+}
+
+void launch_tracers(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const Records& rest,
+                    const TracerBufs& b, KernelTimer* timer) {
+    const uint32_t cells = k.dims[0] * k.dims[1] * k.dims[2];
+    GridGeom g;
+    for (int a = 0; a < 3; ++a) {
+        g.box_min[a] = k.box_min[a];
+        g.dims[a] = k.dims[a];
+    }
+    g.cell = k.cell;
+    if (timer) timer->begin(st, KERNEL_TRACERS);
+    if (n > 0)
+        hipLaunchKernelGGL(k_splat, dim3(blocks_for(n)), dim3(kBlock), 0, st, g, n, rest.pos, rest.vel, b.vel_grid);
+    hipLaunchKernelGGL(k_resolve, dim3(blocks_for(cells)), dim3(kBlock), 0, st, cells,
+                       reinterpret_cast<int4*>(b.vel_grid), reinterpret_cast<int4*>(b.vel_flat));
+    if (n > 0 && b.count > 0)
+        hipLaunchKernelGGL(k_advect, dim3(blocks_for(b.count)), dim3(kBlock), 0, st, k, s, n, rest.pos, rest.vel,
+                           reinterpret_cast<const int4*>(b.vel_flat), b.tracers, b.count);
+    if (timer) timer->end(st, KERNEL_TRACERS);
+}
+
+}  // namespace fluid

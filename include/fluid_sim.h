@@ -1,0 +1,222 @@
+#ifndef FLUID_SIM_H
+#define FLUID_SIM_H
+
+/*
+ * Step-level C ABI of the MI355X DFSPH step.
+ *
+ * The reference keeps its step crate-private behind `Sim` (render.rs:1382-2226)
+ * and drives it with two byte blocks: the 64-byte SimParams uniform
+ * (pack_sim_params, crates/fluid-core/src/sim.rs:325-350) and the 144-byte
+ * Step immediates (pack_step, sim.rs:200-252), dispatching a substep as
+ * count, scan, scatter, density_div, div_apply, forces_eval,
+ * forces_den_apply, refine x (den_kappa, den_apply), integrate, then per
+ * frame reduce_stats and the tracer kernels (render.rs:3168-3228). This
+ * header exposes exactly that surface, taking the two blocks verbatim, so a
+ * Rust (or any C) caller of the reference's step binds it unchanged.
+ *
+ * Conventions (the reference's fluid_ffi.h ones): an opaque handle; NULL or
+ * a negative status on failure with the reason on stderr and in
+ * fluid_sim_last_error(); one thread per handle. Kernels run on the handle's
+ * HIP stream; calls that return host data synchronise it.
+ *
+ * Every function has no counterpart in fluid_ffi.h unless its comment names
+ * one; each cites the reference code whose role it takes.
+ */
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct FluidSim FluidSim;
+
+#define FLUID_SIM_PARAMS_BYTES 64u  /* sim.rs:330 */
+#define FLUID_SIM_STEP_BYTES 144u   /* sim.rs:202 STEP_BYTES */
+#define FLUID_SIM_STATS 14u         /* sim_solve.wgsl:61 stats[14] */
+#define FLUID_SIM_NBR_CAP 96u       /* sim_solve.wgsl:388 NBR_CAP */
+
+#define FLUID_OK 0
+#define FLUID_ERR_ARG (-1)
+#define FLUID_ERR_HIP (-2)
+#define FLUID_ERR_NOMEM (-3)
+#define FLUID_ERR_DEVICE (-4)
+
+/* Fields for fluid_sim_download_field, all in slot (cell) order. */
+#define FLUID_FIELD_DENSITY 0u    /* f32 [N]    sim_solve.wgsl:501 */
+#define FLUID_FIELD_ALPHA 1u      /* f32 [N]    :509 */
+#define FLUID_FIELD_KD 2u         /* f32 [N]    kappa/rho, :516, 589, 695 */
+#define FLUID_FIELD_PRESSURE 3u   /* f32 [N]    :590, 696 */
+#define FLUID_FIELD_RHO_NEAR 4u   /* f32 [N]    positions.w during a substep, :507 */
+#define FLUID_FIELD_ACCEL 5u      /* f32 [N][4] :577 */
+#define FLUID_FIELD_XSPH 6u       /* f32 [N][4] :581 */
+#define FLUID_FIELD_WALL_GRAD 7u  /* f32 [N][4] :499 */
+#define FLUID_FIELD_WORK_POS 8u   /* f32 [N][4] working positions */
+#define FLUID_FIELD_WORK_VEL 9u   /* f32 [N][4] working velocities */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *fluid_sim_last_error(void);
+
+/* Number of HIP devices visible (0 without a GPU). */
+int fluid_sim_device_count(void);
+
+/*
+ * Builds a sim on the current HIP device from a 64-byte SimParams block
+ * (count at byte 28 is the particle count). Takes the role of Sim::new
+ * (render.rs:1468-1612): velocities and previous velocities start at zero,
+ * temperature at 293.15 K, previous pressure at zero; positions start at
+ * zero until fluid_sim_upload_state. `tracers` packed tracers are allocated
+ * (zero = none). `nbr_cap` is the neighbour-list cap (0 = NBR_CAP, 96).
+ * Returns NULL on failure.
+ */
+FluidSim *fluid_sim_create(const uint8_t *params64, uint32_t tracers, uint32_t nbr_cap);
+
+/* Frees every device buffer; the handle is dead afterwards. */
+void fluid_sim_destroy(FluidSim *sim);
+
+uint32_t fluid_sim_count(const FluidSim *sim);
+uint32_t fluid_sim_cells(const FluidSim *sim);
+uint32_t fluid_sim_tracer_count(const FluidSim *sim);
+
+/*
+ * Resting-set upload (Appendix of the record layouts: positions vec4 with
+ * w = 0, velocities vec4 with w = acceleration EMA, prev_vel vec4 with
+ * w = pressure EMA, prev_pressure f32, temperature f32). Any pointer may be
+ * NULL to leave that record unchanged. Host pointers; synchronous.
+ */
+int fluid_sim_upload_state(FluidSim *sim, const float *pos4, const float *vel4, const float *prev_vel4,
+                           const float *prev_pressure, const float *temperature);
+int fluid_sim_download_state(FluidSim *sim, float *pos4, float *vel4, float *prev_vel4, float *prev_pressure,
+                             float *temperature);
+
+/* Scratch field of the last substep (FLUID_FIELD_*), 1 or 4 floats a particle. */
+int fluid_sim_download_field(FluidSim *sim, uint32_t field, float *out);
+
+/*
+ * Grid of the last rebuild: cell id per resting particle (N), per-cell
+ * counts (C), starts (C + 1, starts[C] = N), stable slot -> resting-index
+ * permutation (N), neighbour counts (N). Any pointer may be NULL.
+ */
+int fluid_sim_download_grid(FluidSim *sim, uint32_t *cellid, uint32_t *counts, uint32_t *starts, uint32_t *perm,
+                            uint32_t *nbr_n);
+
+/* Neighbour list of the last density sweep: N x cap slot indices. */
+int fluid_sim_download_neighbours(FluidSim *sim, uint32_t *nbr);
+
+/*
+ * Carries a particle identity (initially the upload index) through every
+ * sort, so callers can follow particles across substeps. Off by default: it
+ * adds 8 bytes a particle to the scatter.
+ */
+int fluid_sim_track_ids(FluidSim *sim, bool enable);
+int fluid_sim_download_ids(FluidSim *sim, uint32_t *ids);
+
+/*
+ * `n` substeps of the reference's dispatch order (render.rs:3175-3203) with
+ * one Step block (dt in it is the substep length), `refine` density
+ * iterations each (render.rs:77-79: 5 or 2). dt = 0 encodes no solve, as
+ * render.rs:3165-3167 does. Asynchronous.
+ */
+int fluid_sim_substeps(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32_t refine);
+
+/*
+ * Grid rebuild plus the density sweep alone: the zero-substep path of the
+ * reference's test driver (render.rs:3902-3937). Asynchronous.
+ */
+int fluid_sim_density_only(FluidSim *sim, const uint8_t *step144);
+
+/* reduce_stats (sim_solve.wgsl:800-874) into the device stats block. */
+int fluid_sim_reduce_stats(FluidSim *sim);
+/* Copies the 14-float stats block (synchronises). Layout: sim_solve.wgsl:819-873. */
+int fluid_sim_read_stats(FluidSim *sim, float *out14);
+
+/* splat, resolve, advect (sim_tracers.wgsl:98-183) with a frame Step block. */
+int fluid_sim_tracers(FluidSim *sim, const uint8_t *step144);
+int fluid_sim_upload_tracers(FluidSim *sim, const uint32_t *packed2);
+int fluid_sim_download_tracers(FluidSim *sim, uint32_t *packed2);
+
+/*
+ * One frame as render.rs:3164-3228 encodes it: `n` substeps of `step144`,
+ * reduce_stats, then the tracers with `tracer_step144` (when the sim has
+ * tracers and tracer_step144 is not NULL).
+ */
+int fluid_sim_frame(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32_t refine,
+                    const uint8_t *tracer_step144);
+
+/* Cumulative CFL clamps and neighbour-list overflow (stats[9], stats[10]). */
+int fluid_sim_counters(FluidSim *sim, uint64_t *clamps, uint64_t *overflow);
+
+/* Waits for the handle's stream. */
+int fluid_sim_synchronize(FluidSim *sim);
+
+/*
+ * Per-kernel HIP-event timing of the launches that follow, on the handle's
+ * stream (the timing adds an event pair around each timed launch).
+ * `kernel_mask` selects kernel classes by bit (FLUID_KERNEL_*); 0 stops.
+ */
+#define FLUID_KERNEL_GRID_COUNT 0u
+#define FLUID_KERNEL_GRID_SCAN 1u
+#define FLUID_KERNEL_GRID_SCATTER 2u
+#define FLUID_KERNEL_DENSITY_DIV 3u
+#define FLUID_KERNEL_DIV_APPLY 4u
+#define FLUID_KERNEL_FORCES_EVAL 5u
+#define FLUID_KERNEL_FORCES_DEN_APPLY 6u
+#define FLUID_KERNEL_DEN_KAPPA 7u
+#define FLUID_KERNEL_DEN_APPLY 8u
+#define FLUID_KERNEL_INTEGRATE 9u
+#define FLUID_KERNEL_REDUCE_STATS 10u
+#define FLUID_KERNEL_TRACERS 11u
+#define FLUID_KERNEL_CLASSES 12u
+int fluid_sim_profile(FluidSim *sim, uint32_t kernel_mask);
+/* Total milliseconds and launch count per kernel class since profiling began (synchronises). */
+int fluid_sim_profile_read(FluidSim *sim, double *ms, uint64_t *launches);
+const char *fluid_sim_kernel_name(uint32_t kernel);
+
+/*
+ * The grid's exclusive scan on its own (the decoupled-look-back scan that
+ * replaces scan_single, sim_scan.wgsl:97-120): starts[c] = sum of
+ * counts[0..c), starts[cells] = total. Host arrays; synchronous.
+ */
+int fluid_scan_u32(const uint32_t *counts, uint32_t cells, uint32_t *starts);
+
+/* ---- Host helpers: the reference's CPU substrate and schedule ------------
+ * Pure host functions (no device needed), C++ restatements of
+ * crates/fluid-core/src/sim.rs:1-350, lib.rs:34-36 and render.rs:57-106,
+ * 1362-1380. The grid takes three half-extents: the reference's Grid::new
+ * takes two and fixes z at SLAB_DEPTH / 2 (sim.rs:24-26).
+ */
+typedef struct FluidGrid {
+    float min[3];
+    float cell;
+    uint32_t dims[3];
+} FluidGrid;
+
+uint32_t fluid_hash(uint32_t x);                                                  /* sim.rs:318-323 */
+void fluid_grid_new(const float half[3], float support, FluidGrid *out);          /* sim.rs:24-32 */
+uint32_t fluid_grid_cell_of(const FluidGrid *grid, const float pos[3]);           /* sim.rs:38-43 */
+float fluid_kernel(float r, float h);                                             /* sim.rs:49-60 */
+void fluid_pack_sim_params(const FluidGrid *grid, uint32_t count, float h, float mass, float rho0,
+                           uint8_t *out64);                                       /* sim.rs:330-350 */
+/* touches20: 5 fingers x (x, y, vx, vy), may be NULL. */
+void fluid_pack_step(const float force[3], const float omega[3], const float domega[3], float dt, float v_clamp,
+                     uint32_t seed, const float *touches20, uint32_t touch_n, float touch_r,
+                     uint8_t *out144);                                            /* sim.rs:215-252 */
+void fluid_body_force(const float gravity[3], const float user_acceleration[3], float out[3]); /* lib.rs:34-36 */
+void fluid_lattice_dims(float spacing, const float half[3], float fill, uint32_t out[3]);     /* sim.rs:65-72 */
+/* Writes 4 floats a particle when out4 is not NULL; returns the count. */
+uint32_t fluid_seed_slab(float spacing, const float half[3], float fill, float *out4);       /* sim.rs:77-100 */
+void fluid_seed_tracers(uint32_t count, const float half[3], float fill, uint32_t *out2);    /* sim.rs:105-122 */
+void fluid_pack_tracer(const float pos[3], float speed, const float extent[2], uint32_t out[2]); /* sim.rs:127-133 */
+uint16_t fluid_half_bits(float x);                                                /* sim.rs:139-151 */
+uint32_t fluid_refine_passes(float dt_sub);                                       /* render.rs:77-79 */
+uint32_t fluid_substeps_for(float dt, float v_max, float spacing, uint32_t cap);  /* render.rs:97-106 */
+float fluid_spacing_for(float scale, const float extent[2]);                      /* render.rs:1362-1380 */
+float fluid_slab_depth(void);                                                     /* sim.rs:10 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLUID_SIM_H */

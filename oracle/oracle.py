@@ -1,0 +1,206 @@
+"""ctypes wrapper over the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker. The product (the HIP library under
+fluid-simulation-app_amd/) never imports it.
+
+The oracle restates the reference's step (crates/fluid-core/src/sim_grid.wgsl,
+sim_scan.wgsl:97-120, sim_solve.wgsl, sim_tracers.wgsl) in float (parity)
+and double (the FP64 oracle); see dfsph_oracle.cpp for the ordering choices.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+FIELDS = {
+    "density": (0, 1),
+    "alpha": (1, 1),
+    "kd": (2, 1),
+    "pressure": (3, 1),
+    "rho_near": (4, 1),
+    "accel": (5, 4),
+    "xsph": (6, 4),
+    "wall_grad": (7, 4),
+    "work_pos": (8, 4),
+    "work_vel": (9, 4),
+}
+
+
+def build(force: bool = False) -> str:
+    """Compiles liboracle.so with the recipe in oracle/Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-B" if force else "-s", "-C", _HERE], check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp, u32, f32, f64, i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_double, C.c_int
+        P = C.POINTER
+        L.oracle_create.argtypes = [vp, u32, i32]
+        L.oracle_create.restype = vp
+        L.oracle_destroy.argtypes = [vp]
+        L.oracle_set_threads.argtypes = [i32]
+        L.oracle_max_threads.restype = i32
+        L.oracle_set_state.argtypes = [vp] + [P(f32)] * 5
+        L.oracle_get_state.argtypes = [vp] + [P(f32)] * 5 + [P(u32)]
+        L.oracle_get_state_f64.argtypes = [vp] + [P(f64)] * 5
+        L.oracle_get_field.argtypes = [vp, i32, P(f64)]
+        L.oracle_get_field.restype = C.c_size_t
+        L.oracle_get_grid.argtypes = [vp] + [P(u32)] * 5
+        L.oracle_get_nbr.argtypes = [vp, P(u32)]
+        L.oracle_substep.argtypes = [vp, vp, u32]
+        L.oracle_density_only.argtypes = [vp, vp]
+        L.oracle_grid_build.argtypes = [vp]
+        L.oracle_reduce_stats.argtypes = [vp, P(f32)]
+        L.oracle_counters.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
+        L.oracle_tracers.argtypes = [vp, vp, P(u32), u32]
+        L.oracle_tracers.restype = i32
+        L.oracle_cell_of.argtypes = [vp, f32, f32, f32]
+        L.oracle_cell_of.restype = u32
+        for name in ("oracle_wall_density", "oracle_wall_gradient", "oracle_wall_adhesion"):
+            getattr(L, name).argtypes = [f64]
+            getattr(L, name).restype = f64
+        for name in ("oracle_wedge", "oracle_wedge_d"):
+            getattr(L, name).argtypes = [f64, f64]
+            getattr(L, name).restype = f64
+        L.oracle_wall_density_f32.argtypes = [f32]
+        L.oracle_wall_density_f32.restype = f32
+        L.oracle_pcg.argtypes = [u32]
+        L.oracle_pcg.restype = u32
+        L.oracle_f16.argtypes = [f32]
+        L.oracle_f16.restype = C.c_uint16
+        L.oracle_f16_to_f32.argtypes = [C.c_uint16]
+        L.oracle_f16_to_f32.restype = f32
+        _lib = L
+    return _lib
+
+
+def _fp(a, ctype=C.c_float):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _bytes_ptr(b: bytes):
+    buf = C.create_string_buffer(bytes(b), len(b))
+    return buf
+
+
+class OracleSim:
+    """One scene on the CPU oracle: the resting set plus the scratch fields."""
+
+    def __init__(self, params64: bytes, nbr_cap: int = 96, precision: int = 32):
+        assert len(params64) == 64
+        self.L = lib()
+        self._params = _bytes_ptr(params64)
+        self.h = self.L.oracle_create(self._params, nbr_cap, precision)
+        self.n = int(np.frombuffer(params64, dtype=np.uint32, count=1, offset=28)[0])
+        d = np.frombuffer(params64, dtype=np.uint32, count=3, offset=16)
+        self.cells = int(d[0]) * int(d[1]) * int(d[2])
+        self.cap = nbr_cap
+        self.precision = precision
+
+    def close(self):
+        if self.h:
+            self.L.oracle_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_state(self, pos4=None, vel4=None, prev_vel4=None, prev_pressure=None, temperature=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+                for a in (pos4, vel4, prev_vel4, prev_pressure, temperature)]
+        self.L.oracle_set_state(self.h, *[_fp(a) for a in arrs])
+
+    def state(self):
+        n = self.n
+        pos = np.empty((n, 4), np.float32)
+        vel = np.empty((n, 4), np.float32)
+        pvel = np.empty((n, 4), np.float32)
+        pp = np.empty(n, np.float32)
+        temp = np.empty(n, np.float32)
+        ids = np.empty(n, np.uint32)
+        self.L.oracle_get_state(self.h, _fp(pos), _fp(vel), _fp(pvel), _fp(pp), _fp(temp),
+                                _fp(ids, C.c_uint32))
+        return dict(pos=pos, vel=vel, prev_vel=pvel, prev_pressure=pp, temperature=temp, ids=ids)
+
+    def state_f64(self):
+        n = self.n
+        out = dict(pos=np.empty((n, 4)), vel=np.empty((n, 4)), prev_vel=np.empty((n, 4)),
+                   prev_pressure=np.empty(n), temperature=np.empty(n))
+        self.L.oracle_get_state_f64(self.h, *[_fp(out[k], C.c_double) for k in
+                                              ("pos", "vel", "prev_vel", "prev_pressure", "temperature")])
+        return out
+
+    def field(self, name):
+        which, per = FIELDS[name]
+        out = np.empty(self.n * per, np.float64)
+        self.L.oracle_get_field(self.h, which, _fp(out, C.c_double))
+        return out.reshape(self.n, per) if per > 1 else out
+
+    def grid(self):
+        n, c = self.n, self.cells
+        cellid = np.empty(n, np.uint32)
+        counts = np.empty(c, np.uint32)
+        starts = np.empty(c + 1, np.uint32)
+        perm = np.empty(n, np.uint32)
+        nbr_n = np.empty(n, np.uint32)
+        u = C.c_uint32
+        self.L.oracle_get_grid(self.h, _fp(cellid, u), _fp(counts, u), _fp(starts, u), _fp(perm, u),
+                               _fp(nbr_n, u))
+        return dict(cellid=cellid, counts=counts, starts=starts, perm=perm, nbr_n=nbr_n)
+
+    def nbr(self):
+        out = np.empty((self.n, self.cap), np.uint32)
+        self.L.oracle_get_nbr(self.h, _fp(out, C.c_uint32))
+        return out
+
+    def substep(self, step144: bytes, refine: int):
+        b = _bytes_ptr(step144)
+        self.L.oracle_substep(self.h, b, refine)
+
+    def density_only(self, step144: bytes):
+        self.L.oracle_density_only(self.h, _bytes_ptr(step144))
+
+    def grid_build(self):
+        self.L.oracle_grid_build(self.h)
+
+    def reduce_stats(self):
+        out = np.empty(14, np.float32)
+        self.L.oracle_reduce_stats(self.h, _fp(out))
+        return out
+
+    def counters(self):
+        a, b = C.c_uint64(), C.c_uint64()
+        self.L.oracle_counters(self.h, C.byref(a), C.byref(b))
+        return int(a.value), int(b.value)
+
+    def tracers(self, step144: bytes, tracers: np.ndarray):
+        tr = np.ascontiguousarray(tracers, dtype=np.uint32).copy()
+        rc = self.L.oracle_tracers(self.h, _bytes_ptr(step144), _fp(tr, C.c_uint32), tr.size // 2)
+        assert rc == 0
+        return tr
+
+
+def set_threads(n: int):
+    lib().oracle_set_threads(int(n))
+
+
+def max_threads() -> int:
+    return int(lib().oracle_max_threads())

@@ -1,0 +1,182 @@
+"""Benchmark of the MI355X DFSPH step: particle-substeps/s at 1,048,576
+particles (a block filling 40% of the unit box, sinusoidal tilt gravity,
+4,096 tracers), the reference's fixed iteration schedule.
+
+A bench step is one full DFSPH substep (grid build, density and alpha,
+divergence solve, forces, density solve with R refine iterations,
+integrate) over the whole scene, inputs resident in HBM. Per-frame work
+(stats, tracers) runs once per frame outside the timed substeps and is
+reported on its own.
+
+Prints one JSON line. `roofline` is priced on the canonical algorithmic
+bytes per particle of each kernel over the kernel's average
+launch time measured with HIP events on the sim's own stream during the
+timed region. `cpu_baseline` times the CPU oracle (a C++ restatement of the
+reference's step, OpenMP over the host cores) on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "fluid-simulation-app_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Canonical algorithmic bytes per particle of each kernel class (per launch).
+BYTES_PER_PARTICLE = {
+    "grid_count": 24, "grid_scatter": 124, "density_div": 64, "div_apply": 68, "forces_eval": 84,
+    "forces_den_apply": 108, "den_kappa": 68, "den_apply": 68, "integrate": 120,
+}
+BYTES_PER_CELL = {"grid_scan": 16}
+
+
+def canonical_bytes(n, cells, refine):
+    return n * (592 + 136 * refine) + 16 * cells
+
+
+def kernel_bytes(name, n, cells):
+    if name in BYTES_PER_PARTICLE:
+        return BYTES_PER_PARTICLE[name] * n
+    if name in BYTES_PER_CELL:
+        return BYTES_PER_CELL[name] * cells
+    return None
+
+
+def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=8):
+    """Times the oracle's FP32 step on this host's cores on a bounded sample."""
+    import oracle
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "1024")))
+    oracle.set_threads(cores)
+    o = oracle.OracleSim(scene.params, precision=32)
+    o.set_state(scene.pos4)
+    o.substep(step, refine)  # warm-up substep (untimed)
+    done, t0 = 0, time.perf_counter()
+    while done < max_substeps:
+        o.substep(step, refine)
+        done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": scene.n * done / dt, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+            "sample": f"{done} substeps (R={refine}) of the same {scene.n}-particle scene after 1 warm-up "
+                      f"substep, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
+    ap.add_argument("--workload", default="block_1m", choices=["block_1m", "block_8m", "dam_break_16k"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
+              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import fluid_amd as fa
+    if fa.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    maker = {"block_1m": fa.scenes.block_1m, "block_8m": fa.scenes.block_8m,
+             "dam_break_16k": fa.scenes.dam_break}[args.workload]
+    sc = maker(seed=args.seed)
+    n_sub, dt_sub, refine, v_clamp = sc.schedule()
+    ntr = len(sc.tracers)
+    sim = fa.Sim(sc.params, tracers=ntr)
+    sim.upload_state(sc.pos4)
+    if ntr:
+        sim.upload_tracers(sc.tracers)
+
+    total = args.warmup + args.steps
+    steps = [sc.step_block(k * dt_sub, dt_sub, v_clamp) for k in range(total)]
+    frame_blocks = [fa.pack_step(force=sc.gravity(k * dt_sub), dt=dt_sub * n_sub, seed=k) for k in range(total)]
+
+    # warm-up substeps, then a short all-kernel profile to name the dominant kernel
+    for k in range(args.warmup):
+        sim.substeps(steps[k], 1, refine)
+    sim.synchronize()
+    sim.profile(fa.KERNELS)
+    for k in range(3):
+        sim.substeps(steps[args.warmup + k], 1, refine)
+    prof = sim.profile_read()
+    sim.profile([])
+    per_kernel_ms = {k: v[0] / 3.0 for k, v in prof.items()}
+    dominant = max((k for k in per_kernel_ms if kernel_bytes(k, sc.n, sim.cells)), key=lambda k: per_kernel_ms[k])
+
+    # the timed region: exactly `steps` substeps, events on the dominant kernel only
+    sim.profile([dominant])
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.substeps(steps[args.warmup + k], 1, refine)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    dom = sim.profile_read()[dominant]
+    sim.profile([])
+    dom_ms = dom[0] / max(dom[1], 1)
+
+    # per-frame work, timed on its own: reduce_stats + tracers
+    sim.synchronize()
+    t1 = time.perf_counter()
+    frames = 5
+    for k in range(frames):
+        sim.reduce_stats()
+        if ntr:
+            sim.tracers(frame_blocks[k])
+    sim.synchronize()
+    frame_extra_ms = (time.perf_counter() - t1) * 1e3 / frames
+    stats = sim.read_stats()
+
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = sc.n * args.steps / elapsed
+    b_dom = kernel_bytes(dominant, sc.n, sim.cells)
+    achieved = b_dom / (dom_ms * 1e-3) / 1e9
+    step_bytes = canonical_bytes(sc.n, sim.cells, refine)
+    out = {
+        "metric": "DFSPH particle-steps/s @1M (plain)",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded jittered lattice block, tilt gravity",
+        "config": {"workload": sc.name, "particles": sc.n, "cells": sim.cells, "refine": refine,
+                   "substep_dt_s": dt_sub, "substeps_per_frame": n_sub, "h_over_spacing": sc.h / sc.spacing,
+                   "tracers": ntr, "schedule": "reference fixed (1 divergence + warm start + R density)"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "avg_launch_us": dom_ms * 1e3, "algorithmic_bytes": b_dom},
+        "step_roofline": {"canonical_bytes": step_bytes, "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "per_kernel_ms": per_kernel_ms,
+        "frame_extra_ms": frame_extra_ms,
+        "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
+                  "compression_avg": float(stats[0])},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, steps[0], refine)
+        out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    sim.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1052,6 +1053,30 @@ void oracle_density_only(void* hv, const void* step144) {
     std::memcpy(&sb, step144, sizeof sb);
     dispatch((Handle*)hv, [&](auto& s) {
         s.density_only(sb);
+        return 0;
+    });
+}
+
+// One kernel of the substep by name (grid, density_div, div_apply,
+// forces_eval, forces_den_apply, den_kappa, den_apply, integrate), so tests
+// can check invariants of the pieces. Returns -1 for an unknown name.
+int oracle_kernel(void* hv, const char* name, const void* step144) {
+    StepBlock sb;
+    std::memcpy(&sb, step144, sizeof sb);
+    std::string k(name);
+    return dispatch((Handle*)hv, [&](auto& s) -> int {
+        using S = std::remove_reference_t<decltype(s)>;
+        using T = std::remove_reference_t<decltype(s.density[0])>;
+        (void)sizeof(S);
+        StepT<T> st(sb);
+        if (k == "grid") s.grid_build();
+        else if (k == "density_div") s.density_div(st);
+        else if (k == "div_apply" || k == "den_apply") s.kappa_apply(st);
+        else if (k == "forces_eval") s.forces_eval();
+        else if (k == "forces_den_apply") s.forces_den_apply(st);
+        else if (k == "den_kappa") s.den_kappa(st);
+        else if (k == "integrate") s.integrate(st);
+        else return -1;
         return 0;
     });
 }

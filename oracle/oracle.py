@@ -65,6 +65,8 @@ def lib():
         L.oracle_substep.argtypes = [vp, vp, u32]
         L.oracle_density_only.argtypes = [vp, vp]
         L.oracle_grid_build.argtypes = [vp]
+        L.oracle_kernel.argtypes = [vp, C.c_char_p, vp]
+        L.oracle_kernel.restype = i32
         L.oracle_reduce_stats.argtypes = [vp, P(f32)]
         L.oracle_counters.argtypes = [vp, P(C.c_uint64), P(C.c_uint64)]
         L.oracle_tracers.argtypes = [vp, vp, P(u32), u32]
@@ -177,6 +179,10 @@ class OracleSim:
 
     def density_only(self, step144: bytes):
         self.L.oracle_density_only(self.h, _bytes_ptr(step144))
+
+    def kernel(self, name: str, step144: bytes):
+        rc = self.L.oracle_kernel(self.h, name.encode(), _bytes_ptr(step144))
+        assert rc == 0, name
 
     def grid_build(self):
         self.L.oracle_grid_build(self.h)

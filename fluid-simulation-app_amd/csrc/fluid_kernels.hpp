@@ -48,7 +48,9 @@ struct SolveBufs {
     float4* accel;    // xyz; w = dT/dt
     float4* xsph;
     float4* wall_grad;
-    uint32_t* nbr;    // [cap][stride] neighbour slots, neighbour-major
+    // [cap][stride] per pair: (neighbour slot, bits of dW/r), neighbour-major,
+    // so a wave's reads of entry m are one contiguous 512-byte line.
+    uint2* nbr;
     uint32_t* nbr_n;  // [N]
     uint32_t cap;
     uint32_t stride;

@@ -231,8 +231,8 @@ void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Reco
 void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,
                       uint32_t* ticket) {
     const uint32_t ntiles = scan_tiles(cells);
-    hipMemsetAsync(status, 0, (size_t)ntiles * sizeof(uint64_t), st);
-    hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
+    (void)hipMemsetAsync(status, 0, (size_t)ntiles * sizeof(uint64_t), st);
+    (void)hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
     hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, counts, starts, cells, status, ticket);
 }
 

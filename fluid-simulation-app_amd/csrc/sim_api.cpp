@@ -120,10 +120,10 @@ class EventTimer final : public KernelTimer {
 
     ~EventTimer() override {
         for (auto& p : pending) {
-            hipEventDestroy(p.a);
-            hipEventDestroy(p.b);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
-        for (auto e : pool) hipEventDestroy(e);
+        for (auto e : pool) (void)hipEventDestroy(e);
     }
     hipEvent_t take() {
         if (!pool.empty()) {
@@ -132,27 +132,27 @@ class EventTimer final : public KernelTimer {
             return e;
         }
         hipEvent_t e;
-        hipEventCreate(&e);
+        (void)hipEventCreate(&e);
         return e;
     }
     bool wants(int kernel) const { return (mask >> kernel) & 1u; }
     void begin(hipStream_t st, int kernel) override {
         if (!wants(kernel)) return;
         open[kernel] = take();
-        hipEventRecord(open[kernel], st);
+        (void)hipEventRecord(open[kernel], st);
     }
     void end(hipStream_t st, int kernel) override {
         if (!wants(kernel) || !open[kernel]) return;
         hipEvent_t e = take();
-        hipEventRecord(e, st);
+        (void)hipEventRecord(e, st);
         pending.push_back({open[kernel], e, kernel});
         open[kernel] = nullptr;
     }
     void collect() {
         for (auto& p : pending) {
-            hipEventSynchronize(p.b);
+            (void)hipEventSynchronize(p.b);
             float t = 0.0f;
-            hipEventElapsedTime(&t, p.a, p.b);
+            (void)hipEventElapsedTime(&t, p.a, p.b);
             ms[p.kernel] += t;
             launches[p.kernel] += 1;
             pool.push_back(p.a);
@@ -212,9 +212,9 @@ struct FluidSim {
     }
 
     ~FluidSim() {
-        if (stream) hipStreamSynchronize(stream);
-        for (void* p : allocs) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -436,12 +436,12 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     if (!valid(s) || !nbr) return FLUID_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(s->stream));
     const size_t n = s->n, cap = s->cap, stride = s->solve.stride;
-    std::vector<uint32_t> raw(cap * stride);
+    std::vector<uint2> raw(cap * stride);
     std::vector<uint32_t> cnt(std::max<size_t>(n, 1));
-    HIP_TRY(hipMemcpy(raw.data(), s->solve.nbr, raw.size() * 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(raw.data(), s->solve.nbr, raw.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i)
-        for (size_t m = 0; m < cap; ++m) nbr[i * cap + m] = m < cnt[i] ? raw[m * stride + i] : 0u;
+        for (size_t m = 0; m < cap; ++m) nbr[i * cap + m] = m < cnt[i] ? raw[m * stride + i].x : 0u;
     return FLUID_OK;
 }
 
@@ -614,11 +614,11 @@ int fluid_scan_u32(const uint32_t* counts, uint32_t cells, uint32_t* starts) {
         set_error("scan failed: %s", hipGetErrorString(e));
         rc = FLUID_ERR_HIP;
     }
-    hipFree(dc);
-    hipFree(ds);
-    hipFree(status);
-    hipFree(ticket);
-    if (st) hipStreamDestroy(st);
+    (void)hipFree(dc);
+    (void)hipFree(ds);
+    (void)hipFree(status);
+    (void)hipFree(ticket);
+    if (st) (void)hipStreamDestroy(st);
     return rc;
 }
 

@@ -17,10 +17,15 @@
 // which is the reference's LANES = 1 reading. Each x-row of the 3x3x3
 // stencil is one contiguous slot range, so the walk reads 9 ranges.
 //
-// The neighbour list keeps only slot indices (neighbour-major, so a wave's
-// reads of entry m are one coalesced line); the sweeps recompute each pair's
-// kernel gradient from the positions, bit-identical to density_div's, which
-// reads 4 bytes per pair from HBM instead of the reference's 16.
+// density_div walks the stencil in two phases: a distance test per candidate
+// that appends accepted slots to the neighbour list, then the per-pair math
+// over that compact list (so the wave does not run the heavy path on every
+// candidate some lane accepts). The list holds, per pair, the slot and the
+// scalar dW/r (neighbour-major, so a wave's reads of entry m are one
+// coalesced line); the later sweeps rebuild each gradient as dW/r times
+// x_i - x_j, the product grad_q itself forms, so it is bit-identical, with
+// 8 bytes a pair from HBM instead of the reference's 16 and no square root
+// or division.
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 
@@ -32,10 +37,58 @@ __device__ __forceinline__ uint32_t particle_index() {
     return xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 }
 
-__device__ __forceinline__ f3 pair_grad(const SceneK& k, f3 pi, float4 pj) {
-    f3 x = sub(pi, xyz(pj));
-    float r = sqrtf(dot3(x, x));
-    return grad_q(k, x, r, r * k.inv_h);
+// The gradient of a listed pair: the stored dW/r times x_i - x_j, which is
+// grad_q's own product (sim_solve.wgsl:195), so it is bit-identical to the
+// gradient density_div computed.
+__device__ __forceinline__ f3 listed_grad(f3 pi, float4 pj, float g) {
+    return scl(g, sub(pi, xyz(pj)));
+}
+
+// The heavy per-pair terms of density_div for one accepted neighbour.
+struct DensitySums {
+    float w_sum, rn, gsq, drho;
+    f3 gsum;
+};
+
+__device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, float4 pj, float4 vj, DensitySums& a) {
+    const f3 x = sub(pi, xyz(pj));
+    const float r2 = dot3(x, x);
+    const float r = sqrtf(r2);
+    const float q = r * k.inv_h;
+    a.w_sum += kernel_q(k, q);
+    const float wn = wmax(1.0f - q, 0.0f);
+    a.rn += wn * wn * wn;
+    const float g = r < 1e-9f ? 0.0f : grad_scale(k, r, q);
+    const f3 gw = r < 1e-9f ? mk(0.0f, 0.0f, 0.0f) : scl(g, x);
+    a.gsum = add(a.gsum, gw);
+    a.gsq += dot3(gw, gw);
+    a.drho += dot3(sub(vi, xyz(vj)), gw);
+    return g;
+}
+
+// The candidate rows of the 3x3x3 stencil around cell (bx, by, bz): row
+// (dy, dz) spans x cells [x0, x1]. A row or an end cell whose box lies
+// farther from the particle than the support is left out: no particle in it
+// can pass the r^2 < support^2 test, so the accepted neighbours, and their
+// order, are those of the full 27-cell walk (sim_solve.wgsl:436-468).
+struct Stencil {
+    float dl[3], dh[3];  // distance to the low / high face of the own cell, per axis
+    float reach;         // support^2 with a margin over the rounding of the faces
+};
+
+__device__ __forceinline__ Stencil make_stencil(const SceneK& k, f3 p, uint32_t bx, uint32_t by, uint32_t bz) {
+    Stencil st;
+    const float pp[3] = {p.x, p.y, p.z};
+    const uint32_t b[3] = {bx, by, bz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float lo = k.box_min[a] + (float)b[a] * k.cell;
+        const float hi = k.box_min[a] + (float)(b[a] + 1u) * k.cell;
+        st.dl[a] = wmax(pp[a] - lo, 0.0f);
+        st.dh[a] = wmax(hi - pp[a], 0.0f);
+    }
+    st.reach = k.support_sq * 1.001f;
+    return st;
 }
 
 __global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint32_t n, const uint32_t* __restrict__ starts,
@@ -48,45 +101,76 @@ __global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint3
     const uint32_t bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
     const uint32_t by = cell_coord(pi.y, k.box_min[1], k.cell, k.dims[1]);
     const uint32_t bz = cell_coord(pi.z, k.box_min[2], k.cell, k.dims[2]);
-    const uint32_t x0 = bx > 0 ? bx - 1 : 0;
-    const uint32_t x1 = bx + 1 < k.dims[0] ? bx + 1 : k.dims[0] - 1;
-    float w_sum = k.sigma, rn = 0.0f, gsq = 0.0f, drho = 0.0f;
-    f3 gsum = mk(0.0f, 0.0f, 0.0f);
+    const Stencil st = make_stencil(k, pi, bx, by, bz);
+    uint2* __restrict__ list = b.nbr + i;
+    const uint32_t stride = b.stride, cap = b.cap;
+    // Phase 1: the candidate walk, in the reference's cell order, appending
+    // accepted slots to the list (a distance test per candidate and nothing
+    // else, so the wave stays converged).
     uint32_t found = 0;
     for (int dz = -1; dz <= 1; ++dz) {
-        int cz = (int)bz + dz;
+        const int cz = (int)bz + dz;
         if (cz < 0 || cz >= (int)k.dims[2]) continue;
+        const float ez = dz < 0 ? st.dl[2] : (dz > 0 ? st.dh[2] : 0.0f);
         for (int dy = -1; dy <= 1; ++dy) {
-            int cy = (int)by + dy;
+            const int cy = (int)by + dy;
             if (cy < 0 || cy >= (int)k.dims[1]) continue;
+            const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
+            const float e2 = ey * ey + ez * ez;
+            if (e2 > st.reach) continue;
+            const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.reach) ? bx - 1 : bx;
+            const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.reach) ? bx + 1 : bx;
             const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
             const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
             for (uint32_t kk = k0; kk < k1; ++kk) {
-                if (kk == i) continue;
-                const float4 pj = pos[kk];
-                const f3 x = sub(pi, xyz(pj));
+                const f3 x = sub(pi, xyz(pos[kk]));
                 const float r2 = dot3(x, x);
-                if (r2 >= k.support_sq) continue;
-                const float r = sqrtf(r2);
-                const float q = r * k.inv_h;
-                w_sum += kernel_q(k, q);
-                const float wn = wmax(1.0f - q, 0.0f);
-                rn += wn * wn * wn;
-                const f3 gw = grad_q(k, x, r, q);
-                gsum = add(gsum, gw);
-                gsq += dot3(gw, gw);
-                drho += dot3(sub(vi, xyz(vel[kk])), gw);
-                if (found < b.cap) b.nbr[(size_t)found * b.stride + i] = kk;
-                found++;
+                if (r2 < k.support_sq && kk != i) {
+                    if (found < cap) list[(size_t)found * stride].x = kk;
+                    found++;
+                }
             }
         }
     }
-    b.nbr_n[i] = found < b.cap ? found : b.cap;
-    if (found > b.cap) atomicAdd(&b.counters[1], found - b.cap);
-    float rho = k.mass * w_sum;
-    gsq = k.mass * k.mass * gsq;
-    drho = k.mass * drho;
-    gsum = scl(k.mass, gsum);
+    // Phase 2: the sums over the compact list, in list order.
+    DensitySums a{k.sigma, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
+    const uint32_t nn = found < cap ? found : cap;
+    for (uint32_t m = 0; m < nn; ++m) {
+        const uint32_t j = list[(size_t)m * stride].x;
+        const float g = density_pair(k, pi, vi, pos[j], vel[j], a);
+        list[(size_t)m * stride].y = __float_as_uint(g);
+    }
+    if (found > cap) {
+        // Past the cap the list keeps its first `cap` neighbours (the later
+        // sweeps see only those, as in the reference) but the density sums
+        // take every neighbour: walk again for the ones the list dropped.
+        atomicAdd(&b.counters[1], found - cap);
+        uint32_t seen = 0;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int cz = (int)bz + dz;
+            if (cz < 0 || cz >= (int)k.dims[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int cy = (int)by + dy;
+                if (cy < 0 || cy >= (int)k.dims[1]) continue;
+                const uint32_t x0 = bx > 0 ? bx - 1 : bx;
+                const uint32_t x1 = bx + 1 < k.dims[0] ? bx + 1 : bx;
+                const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
+                const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
+                for (uint32_t kk = k0; kk < k1; ++kk) {
+                    const f3 x = sub(pi, xyz(pos[kk]));
+                    if (dot3(x, x) < k.support_sq && kk != i) {
+                        if (seen >= cap) density_pair(k, pi, vi, pos[kk], vel[kk], a);
+                        seen++;
+                    }
+                }
+            }
+        }
+    }
+    b.nbr_n[i] = nn;
+    float rho = k.mass * a.w_sum;
+    const float gsq = k.mass * k.mass * a.gsq;
+    float drho = k.mass * a.drho;
+    f3 gsum = scl(k.mass, a.gsum);
     const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h, (pi.z - k.wall_lo[2]) * k.inv_h);
     const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h, (k.wall_hi[2] - pi.z) * k.inv_h);
     float fill = 0.0f;
@@ -99,11 +183,11 @@ __global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint3
     b.wall_grad[i] = make_float4(wg.x, wg.y, wg.z, 0.0f);
     gsum = add(gsum, wg);
     b.density[i] = rho;
-    b.rho_near[i] = rn;
-    const float a = rho / wmax(dot3(gsum, gsum) + gsq, 1e-4f);
-    b.alpha[i] = a;
+    b.rho_near[i] = a.rn;
+    const float al = rho / wmax(dot3(gsum, gsum) + gsq, 1e-4f);
+    b.alpha[i] = al;
     drho += dot3(vi, wg);
-    b.kd[i] = wmax(drho, 0.0f) * a / s.dt_guard / rho;
+    b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
 }
 
 // div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
@@ -116,9 +200,11 @@ __global__ __launch_bounds__(kBlock) void k_kappa_apply(SceneK k, StepK s, uint3
     const float kdi = b.kd[i];
     const uint32_t nn = b.nbr_n[i];
     f3 dv = mk(0.0f, 0.0f, 0.0f);
+    const uint2* __restrict__ list = b.nbr + i;
+#pragma unroll 4
     for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = b.nbr[(size_t)m * b.stride + i];
-        dv = add(dv, scl(kdi + b.kd[j], pair_grad(k, pi, pos[j])));
+        const uint2 e = list[(size_t)m * b.stride];
+        dv = add(dv, scl(kdi + b.kd[e.x], listed_grad(pi, pos[e.x], __uint_as_float(e.y))));
     }
     const float4 wg = b.wall_grad[i];
     const f3 full = add(scl(k.mass, dv), scl(kdi, xyz(wg)));
@@ -138,12 +224,14 @@ __global__ __launch_bounds__(kBlock) void k_forces_eval(SceneK k, uint32_t n, co
     f3 visc = mk(0.0f, 0.0f, 0.0f), blend = visc, nearf = visc, tension = visc;
     float heat = 0.0f;
     const uint32_t nn = b.nbr_n[i];
+    const uint2* __restrict__ list = b.nbr + i;
     for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = b.nbr[(size_t)m * b.stride + i];
+        const uint2 e = list[(size_t)m * b.stride];
+        const uint32_t j = e.x;
         const f3 x = sub(pi, xyz(pos[j]));
         const float r2 = dot3(x, x);
         const float r = sqrtf(r2);
-        const f3 g = grad_q(k, x, r, r * k.inv_h);
+        const f3 g = scl(__uint_as_float(e.y), x);
         const float rho_j = b.density[j];
         const float f = dot3(x, g) / (r2 + k.eps_h2);
         const float pair = k.mass / (rho_i * rho_j) * f;
@@ -197,9 +285,11 @@ __global__ __launch_bounds__(kBlock) void k_forces_den_apply(SceneK k, StepK s, 
     const float kdi = b.kd[i];
     const uint32_t nn = b.nbr_n[i];
     f3 dv = mk(0.0f, 0.0f, 0.0f);
+    const uint2* __restrict__ list = b.nbr + i;
+#pragma unroll 4
     for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = b.nbr[(size_t)m * b.stride + i];
-        dv = add(dv, scl(kdi + b.kd[j], pair_grad(k, pi, pos[j])));
+        const uint2 e = list[(size_t)m * b.stride];
+        dv = add(dv, scl(kdi + b.kd[e.x], listed_grad(pi, pos[e.x], __uint_as_float(e.y))));
     }
     const float4 a = b.accel[i];
     const float4 was = vel[i];
@@ -227,9 +317,11 @@ __global__ __launch_bounds__(kBlock) void k_den_kappa(SceneK k, StepK s, uint32_
     const f3 vi = xyz(vel[i]);
     const uint32_t nn = b.nbr_n[i];
     float drho = 0.0f;
+    const uint2* __restrict__ list = b.nbr + i;
+#pragma unroll 4
     for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = b.nbr[(size_t)m * b.stride + i];
-        drho += dot3(sub(vi, xyz(vel[j])), pair_grad(k, pi, pos[j]));
+        const uint2 e = list[(size_t)m * b.stride];
+        drho += dot3(sub(vi, xyz(vel[e.x])), listed_grad(pi, pos[e.x], __uint_as_float(e.y)));
     }
     const float rho = b.density[i];
     const float4 wg = b.wall_grad[i];

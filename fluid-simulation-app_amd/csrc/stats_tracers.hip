@@ -19,7 +19,6 @@ namespace fluid {
 namespace {
 
 constexpr int kStatsMaxBlocks = 1024;
-constexpr int kStatsFields = 13;  // sum, max c, rho lo/hi, v lo/hi, a lo/hi, p lo/hi, T lo/hi, pad
 
 __device__ __forceinline__ float lane_min(float a, float b) { return wmin(a, b); }
 __device__ __forceinline__ float lane_max(float a, float b) { return wmax(a, b); }

@@ -133,6 +133,22 @@ __device__ __forceinline__ uint32_t cell_coord(float p, float lo, float cell, ui
     return c < dim - 1u ? c : dim - 1u;
 }
 
+// e^x for the fingers' per-particle blend (sim_solve.wgsl:124), evaluated
+// with a fixed sequence of IEEE operations so that host code following the
+// same sequence gets the same bits: Cody-Waite reduction x = k ln2 + r with
+// ln2 split into a 16-bit head (k * head is exact) and a tail, a degree-7
+// Taylor polynomial for e^r on |r| <= ln2/2 (truncation below 2^-27), and an
+// exact scaling by 2^k. Valid for x in [-87, 88]; WGSL allows exp() 3 + 2|x|
+// ULP, which this meets.
+__device__ __forceinline__ float exp_fixed(float x) {
+    x = wmin(wmax(x, -87.0f), 88.0f);
+    const float k = rintf(x * 1.44269504f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+    const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f + r * (0.00833333377f +
+                    r * (0.00138888892f + r * 0.000198412701f))))));
+    return ldexpf(p, (int)k);
+}
+
 // sim_solve.wgsl:176-182
 __device__ __forceinline__ float kernel_q(const SceneK& k, float q) {
     if (q < 1.0f) return k.sigma * (1.0f - 1.5f * q * q * (1.0f - 0.5f * q));

@@ -272,7 +272,7 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
         }
     }
     if (weight <= 0.0f) return v;
-    const float blend = 1.0f - expf(-TOUCH_RATE * weight * s.dt);
+    const float blend = 1.0f - exp_fixed(-TOUCH_RATE * weight * s.dt);
     return mk(wmix(v.x, pullx / weight, blend), wmix(v.y, pully / weight, blend), v.z);
 }
 

@@ -70,33 +70,55 @@ __global__ __launch_bounds__(kBlock) void k_stats_partial(float rho0, uint32_t n
     if (threadIdx.x < 12) partials[(size_t)blockIdx.x * 16 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ __launch_bounds__(64) void k_stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
-                                                    const uint32_t* __restrict__ counters, float* __restrict__ stats) {
-    if (threadIdx.x != 0) return;
+// Folds the per-workgroup partials in a fixed order (thread t takes
+// partials t, t + 256, ..., then a tree over the threads), so the result is
+// the same on every run for a given particle count.
+__global__ __launch_bounds__(kBlock) void k_stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
+                                                        const uint32_t* __restrict__ counters,
+                                                        float* __restrict__ stats) {
     float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
-    for (uint32_t b = 0; b < blocks; ++b) {
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
         const float* p = partials + (size_t)b * 16;
         acc[0] += p[0];
         acc[1] = wmax(acc[1], p[1]);
+#pragma unroll
         for (int f = 2; f < 12; f += 2) {
             acc[f] = wmin(acc[f], p[f]);
             acc[f + 1] = wmax(acc[f + 1], p[f + 1]);
         }
     }
-    stats[0] = acc[0] / (float)n;
-    stats[1] = wmax(acc[1], 0.0f);
-    stats[2] = acc[2];
-    stats[3] = acc[3];
-    stats[4] = acc[8];
-    stats[5] = acc[9];
-    stats[6] = acc[5];
-    stats[7] = acc[10];
-    stats[8] = acc[11];
+    __shared__ float red[12][kBlock];
+#pragma unroll
+    for (int f = 0; f < 12; ++f) red[f][threadIdx.x] = acc[f];
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            const int o = threadIdx.x + off;
+            red[0][threadIdx.x] += red[0][o];
+            red[1][threadIdx.x] = wmax(red[1][threadIdx.x], red[1][o]);
+#pragma unroll
+            for (int f = 2; f < 12; f += 2) {
+                red[f][threadIdx.x] = wmin(red[f][threadIdx.x], red[f][o]);
+                red[f + 1][threadIdx.x] = wmax(red[f + 1][threadIdx.x], red[f + 1][o]);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    stats[0] = red[0][0] / (float)n;
+    stats[1] = wmax(red[1][0], 0.0f);
+    stats[2] = red[2][0];
+    stats[3] = red[3][0];
+    stats[4] = red[8][0];
+    stats[5] = red[9][0];
+    stats[6] = red[5][0];
+    stats[7] = red[10][0];
+    stats[8] = red[11][0];
     stats[9] = (float)counters[0];
     stats[10] = (float)counters[1];
-    stats[11] = acc[4];
-    stats[12] = acc[6];
-    stats[13] = acc[7];
+    stats[11] = red[4][0];
+    stats[12] = red[6][0];
+    stats[13] = red[7][0];
 }
 
 // ---- tracers -------------------------------------------------------------
@@ -177,21 +199,52 @@ __device__ __forceinline__ void respawn(const SceneK& k, uint32_t n, const float
     store_tracer(k, tr, slot, q[0], q[1], q[2], sqrtf(vj.x * vj.x + vj.y * vj.y + vj.z * vj.z));
 }
 
+// Particles of one cell sit next to each other in the resting set (it is in
+// the last substep's cell order), so each wave first sums its runs of equal
+// cells (a segmented scan over run ids) and the last lane of a run adds the
+// run's sums: one set of atomics per run instead of per particle. Integer
+// sums, so the grid holds the same values in any order.
 __global__ __launch_bounds__(kBlock) void k_splat(GridGeom g, uint32_t n, const float4* __restrict__ pos,
                                                   const float4* __restrict__ vel, int32_t* __restrict__ grid) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = pos[i];
-    const uint32_t c = ((cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]) * g.dims[1] +
-                         cell_coord(p.y, g.box_min[1], g.cell, g.dims[1])) *
-                            g.dims[0] +
-                        cell_coord(p.x, g.box_min[0], g.cell, g.dims[0])) *
-                       4u;
-    const float4 v = vel[i];
-    atomicAdd(&grid[c], sat_i32(v.x * FIXED));
-    atomicAdd(&grid[c + 1], sat_i32(v.y * FIXED));
-    atomicAdd(&grid[c + 2], sat_i32(v.z * FIXED));
-    atomicAdd(&grid[c + 3], 1);
+    const bool live = i < n;
+    uint32_t c = 0xffffffffu;
+    int32_t v[4] = {0, 0, 0, 0};
+    if (live) {
+        const float4 p = pos[i];
+        c = (cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]) * g.dims[1] +
+             cell_coord(p.y, g.box_min[1], g.cell, g.dims[1])) *
+                g.dims[0] +
+            cell_coord(p.x, g.box_min[0], g.cell, g.dims[0]);
+        const float4 q = vel[i];
+        v[0] = sat_i32(q.x * FIXED);
+        v[1] = sat_i32(q.y * FIXED);
+        v[2] = sat_i32(q.z * FIXED);
+        v[3] = 1;
+    }
+    const uint32_t lane = __lane_id();
+    const uint32_t prev = (uint32_t)__shfl_up((int)c, 1);
+    const bool head = lane == 0 || prev != c;
+    const uint64_t heads = __ballot(head);
+    const uint32_t run = (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));  // run id, 1-based
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t other = (uint32_t)__shfl_up((int)run, d);
+        int32_t t[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) t[a] = __shfl_up(v[a], d);
+        if (lane >= d && other == run) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a) v[a] = (int32_t)((uint32_t)v[a] + (uint32_t)t[a]);
+        }
+    }
+    const uint32_t next_run = (uint32_t)__shfl_down((int)run, 1);
+    const bool tail = lane == 63 || next_run != run;
+    if (live && tail) {
+        const uint32_t base = c * 4u;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) atomicAdd(&grid[base + a], v[a]);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_resolve(uint32_t cells, int4* __restrict__ grid, int4* __restrict__ flat) {
@@ -275,7 +328,7 @@ void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* de
     if (timer) timer->begin(st, KERNEL_REDUCE_STATS);
     hipLaunchKernelGGL(k_stats_partial, dim3(b.blocks), dim3(kBlock), 0, st, rho0, n, density, rest.vel,
                        rest.prev_vel, rest.temperature, b.partials);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, st, n, b.blocks, b.partials, counters, b.stats);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, st, n, b.blocks, b.partials, counters, b.stats);
     if (timer) timer->end(st, KERNEL_REDUCE_STATS);
 }
 

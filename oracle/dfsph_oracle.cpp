@@ -372,6 +372,20 @@ struct StepT {
     }
 };
 
+// e^x as a fixed sequence of float operations (the same sequence the HIP
+// kernels run, so both sides round identically): Cody-Waite reduction with
+// ln2 = 0.693145751953125 + 1.42860677e-06, a degree-7 Taylor polynomial,
+// exact scaling by 2^k. The double oracle uses std::exp.
+inline float exp_fixed(float x) {
+    x = std::min(std::max(x, -87.0f), 88.0f);
+    const float k = std::nearbyint(x * 1.44269504f);
+    const float r = (x - k * 0.693145751953125f) - k * 1.42860677e-06f;
+    const float p = 1.0f + r * (1.0f + r * (0.5f + r * (0.166666672f + r * (0.0416666679f + r * (0.00833333377f +
+                    r * (0.00138888892f + r * 0.000198412701f))))));
+    return std::ldexp(p, (int)k);
+}
+inline double exp_fixed(double x) { return std::exp(x); }
+
 // sim_solve.wgsl:103-127
 template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v) {
     T pullx = 0, pully = 0, weight = 0;
@@ -387,7 +401,7 @@ template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v
         }
     }
     if (weight <= T(0)) return v;
-    T blend = T(1) - std::exp(-T(TOUCH_RATE) * weight * s.dt);
+    T blend = T(1) - exp_fixed(-T(TOUCH_RATE) * weight * s.dt);
     return v3(mix(v.x, pullx / weight, blend), mix(v.y, pully / weight, blend), v.z);
 }
 
@@ -1125,6 +1139,7 @@ double oracle_wall_gradient(double t) { return wall_gradient<double>(t); }
 double oracle_wedge(double a, double b) { return wedge<double>(a, b); }
 double oracle_wedge_d(double a, double b) { return wedge_d<double>(a, b); }
 double oracle_wall_adhesion(double t) { return wall_adhesion<double>(t); }
+float oracle_exp_fixed(float x) { return exp_fixed(x); }
 float oracle_wall_density_f32(float t) { return wall_density<float>(t); }
 uint32_t oracle_pcg(uint32_t x) { return pcg(x); }
 uint16_t oracle_f16(float x) { return f32_to_f16_rne(x); }

@@ -81,6 +81,8 @@ def lib():
             getattr(L, name).restype = f64
         L.oracle_wall_density_f32.argtypes = [f32]
         L.oracle_wall_density_f32.restype = f32
+        L.oracle_exp_fixed.argtypes = [f32]
+        L.oracle_exp_fixed.restype = f32
         L.oracle_pcg.argtypes = [u32]
         L.oracle_pcg.restype = u32
         L.oracle_f16.argtypes = [f32]

@@ -408,3 +408,13 @@ def test_tracers_ride_a_uniform_flow():
     respawned = ~moved
     assert respawned.mean() < 0.02   # recycling rate dt / 3 s plus the cells the block does not cover
     assert moved.sum() > 450
+
+
+def test_fixed_sequence_exp_is_within_two_ulp():
+    L = olib()
+    xs = np.concatenate([np.random.default_rng(4).uniform(-80, 80, 20000), [0.0, -1e-8, 1e-8, -87.0, 88.0]])
+    xs = xs.astype(np.float32)
+    got = np.array([L.oracle_exp_fixed(float(x)) for x in xs], np.float64)
+    ref = np.exp(xs.astype(np.float64))
+    ulp = np.abs(got - ref) / np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert ulp.max() <= 2.0, ulp.max()

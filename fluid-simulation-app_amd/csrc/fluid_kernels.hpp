@@ -52,10 +52,21 @@ struct SolveBufs {
     // so a wave's reads of entry m are one contiguous 512-byte line.
     uint2* nbr;
     uint32_t* nbr_n;  // [N]
+    // [N] per-row counts of each particle's list, 7 bits for each of the 9
+    // stencil rows (dz, dy) in walk order: which LDS range a listed slot is in.
+    uint64_t* nbr_rows;
     uint32_t cap;
     uint32_t stride;
     uint32_t* counters;  // [0] CFL clamps, [1] neighbour overflow (cumulative)
+    // LDS capacities (neighbour-region entries) of the staged sweeps; a tile
+    // whose region is larger gathers from global memory instead.
+    uint32_t cap_density, cap_kappa, cap_kappa_vel, cap_forces;
 };
+
+// Sets the LDS capacities for this device's per-workgroup LDS limit.
+void configure_solve(SolveBufs& b, size_t lds_bytes);
+// Raises the sweeps' dynamic-LDS limit to `bytes` where the device allows it.
+void enable_large_lds(size_t bytes);
 
 enum KernelId {
     KERNEL_GRID_COUNT = 0,

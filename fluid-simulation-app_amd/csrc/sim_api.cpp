@@ -185,6 +185,7 @@ struct FluidSim {
     StatsBufs stats{};
     TracerBufs tracers{};
     bool track_ids = false;
+    size_t lds_bytes = 0;
     EventTimer timer;
     std::vector<void*> allocs;
 
@@ -254,6 +255,11 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     s->n = p.count;
     s->cells = (uint32_t)cells64;
     s->cap = nbr_cap ? nbr_cap : FLUID_SIM_NBR_CAP;
+    if (s->cap > 127) {
+        // Each stencil row's share of the list is counted in 7 bits.
+        set_error("neighbour cap %u exceeds 127", s->cap);
+        return FLUID_ERR_ARG;
+    }
     s->ntr = ntr;
     HIP_TRY(hipGetDevice(&s->device));
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -292,7 +298,13 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     if (b.stride == 0) b.stride = 64;
     if ((rc = s->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
+    A(b.nbr_rows, n);
     A(b.counters, 4);
+    int lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
+    s->lds_bytes = (size_t)lds;
+    enable_large_lds(s->lds_bytes);
+    configure_solve(b, s->lds_bytes);
     s->stats.blocks = stats_blocks(s->n);
     A(s->stats.partials, (size_t)s->stats.blocks * 16);
     A(s->stats.stats, 16);
@@ -576,6 +588,18 @@ int fluid_sim_profile_read(FluidSim* s, double* ms, uint64_t* launches) {
         if (ms) ms[k] = s->timer.ms[k];
         if (launches) launches[k] = s->timer.launches[k];
     }
+    return FLUID_OK;
+}
+
+int fluid_sim_describe(const FluidSim* s, char* buf, size_t size) {
+    if (!valid(s) || !buf || size == 0) return FLUID_ERR_ARG;
+    hipDeviceProp_t prop;
+    const char* name = hipGetDeviceProperties(&prop, s->device) == hipSuccess ? prop.gcnArchName : "?";
+    snprintf(buf, size,
+             "{\"arch\": \"%s\", \"lds_bytes\": %zu, \"nbr_cap\": %u, \"lds_capacity\": {\"density\": %u, "
+             "\"kappa\": %u, \"den_kappa\": %u, \"forces\": %u}}",
+             name, s->lds_bytes, s->cap, s->solve.cap_density, s->solve.cap_kappa, s->solve.cap_kappa_vel,
+             s->solve.cap_forces);
     return FLUID_OK;
 }
 

@@ -11,21 +11,28 @@
 //   den_apply          700-703
 //   integrate          709-756
 //
-// Layout: every array is in slot (cell) order after the grid build; one
-// thread owns one particle and walks its neighbours serially in the
-// reference's cell numbering (x fastest, then y, then z; in-cell slot order),
-// which is the reference's LANES = 1 reading. Each x-row of the 3x3x3
-// stencil is one contiguous slot range, so the walk reads 9 ranges.
+// Layout: every array is in slot (cell) order after the grid build. A
+// workgroup owns a tile of kTile consecutive slots. All neighbours of a
+// tile lie in nine slot ranges, one per stencil row (dy, dz) — in cell-id
+// space, row (dy, dz) of every particle of the tile falls in
+// [first cell + off(dy, dz) - 1, last cell + off(dy, dz) + 1] — so each sweep
+// first stages the fields it gathers for those ranges into LDS (coalesced,
+// each line fetched once per tile) and then reads neighbours from LDS. A
+// tile whose ranges exceed the LDS capacity gathers from global memory; the
+// values read, and so the results, are the same either way.
 //
-// density_div walks the stencil in two phases: a distance test per candidate
-// that appends accepted slots to the neighbour list, then the per-pair math
-// over that compact list (so the wave does not run the heavy path on every
-// candidate some lane accepts). The list holds, per pair, the slot and the
-// scalar dW/r (neighbour-major, so a wave's reads of entry m are one
-// coalesced line); the later sweeps rebuild each gradient as dW/r times
-// x_i - x_j, the product grad_q itself forms, so it is bit-identical, with
-// 8 bytes a pair from HBM instead of the reference's 16 and no square root
-// or division.
+// density_div: one thread a particle. It walks the stencil rows in the
+// reference's cell order (x fastest, then y, then z; in-cell slot order) —
+// the reference's LANES = 1 reading — testing distances only and appending
+// accepted slots to the neighbour list with per-row counts; then it runs the
+// per-pair math over that compact list. The list holds, per pair, the slot
+// and the scalar dW/r (neighbour-major, so entry m of consecutive particles is
+// contiguous); the later sweeps rebuild each gradient as dW/r times x_i - x_j,
+// the product grad_q itself forms, so it is bit-identical.
+//
+// The list sweeps run kLanes lanes a particle, as the reference runs 8: lane l
+// takes list entries l, l + kLanes, ..., and the partials fold with the
+// reference's xor butterfly (lane_fold, sim_solve.wgsl:398-402).
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 
@@ -33,25 +40,85 @@ namespace fluid {
 
 namespace {
 
-__device__ __forceinline__ uint32_t particle_index() {
-    return xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+constexpr uint32_t kTile = 256;                      // particles a workgroup
+constexpr uint32_t kLanes = 4;                       // lanes a particle in the list sweeps
+constexpr uint32_t kSweepThreads = kTile * kLanes;  // 1024
+constexpr uint32_t kRows = 9;
+
+// The nine stencil-row slot ranges of a tile and where each sits in LDS.
+struct Region {
+    uint32_t start[kRows], end[kRows];
+    int32_t delta[kRows];  // LDS index of slot j in row r: j + delta[r]
+    uint32_t total;
+    uint32_t staged;
+};
+
+__device__ __forceinline__ uint32_t tile_index() { return xcd_tile(blockIdx.x, gridDim.x); }
+
+__device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
+    const uint32_t cx = cell_coord(p.x, k.box_min[0], k.cell, k.dims[0]);
+    const uint32_t cy = cell_coord(p.y, k.box_min[1], k.cell, k.dims[1]);
+    const uint32_t cz = cell_coord(p.z, k.box_min[2], k.cell, k.dims[2]);
+    return (cz * k.dims[1] + cy) * k.dims[0] + cx;
 }
 
-// The gradient of a listed pair: the stored dW/r times x_i - x_j, which is
-// grad_q's own product (sim_solve.wgsl:195), so it is bit-identical to the
-// gradient density_div computed.
-__device__ __forceinline__ f3 listed_grad(f3 pi, float4 pj, float g) {
-    return scl(g, sub(pi, xyz(pj)));
+// One thread builds the region of slots [s0, s1).
+__device__ void build_region(const SceneK& k, const uint32_t* __restrict__ starts, const float4* __restrict__ pos,
+                             uint32_t s0, uint32_t s1, uint32_t capacity, Region& R) {
+    const int64_t c_first = cell_of_pos(k, pos[s0]);
+    const int64_t c_last = cell_of_pos(k, pos[s1 - 1]);
+    const int64_t cells = (int64_t)k.dims[0] * k.dims[1] * k.dims[2];
+    const int64_t plane = (int64_t)k.dims[0] * k.dims[1];
+    uint32_t total = 0;
+    for (uint32_t r = 0; r < kRows; ++r) {
+        const int64_t dz = (int64_t)(r / 3) - 1, dy = (int64_t)(r % 3) - 1;
+        const int64_t off = dz * plane + dy * (int64_t)k.dims[0];
+        int64_t lo = c_first + off - 1, hi = c_last + off + 1;
+        if (lo < 0) lo = 0;
+        if (hi > cells - 1) hi = cells - 1;
+        uint32_t a = 0, e = 0;
+        if (lo <= hi) {
+            a = starts[lo];
+            e = starts[hi + 1];
+        }
+        R.start[r] = a;
+        R.end[r] = e;
+        R.delta[r] = (int32_t)total - (int32_t)a;
+        total += e - a;
+    }
+    R.total = total;
+    R.staged = total <= capacity ? 1u : 0u;
 }
 
-// The heavy per-pair terms of density_div for one accepted neighbour.
+__device__ __forceinline__ uint32_t row_count(uint64_t rows, uint32_t r) { return (uint32_t)(rows >> (7u * r)) & 127u; }
+
+// Advances (r, row_end) until entry m lies in row r.
+__device__ __forceinline__ void seek_row(uint64_t rows, uint32_t m, uint32_t& r, uint32_t& row_end) {
+    while (m >= row_end && r + 1 < kRows) {
+        ++r;
+        row_end += row_count(rows, r);
+    }
+}
+
+// lane_fold over kLanes = 4 consecutive lanes: xor 2, then xor 1 (lane 0
+// holds (p0 + p2) + (p1 + p3)).
+__device__ __forceinline__ float fold4(float v) {
+    const float t = v + __shfl_xor(v, 2);
+    return t + __shfl_xor(t, 1);
+}
+__device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
+
+__device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
+
+// ---- density_div -------------------------------------------------------------
+
 struct DensitySums {
     float w_sum, rn, gsq, drho;
     f3 gsum;
 };
 
-__device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, float4 pj, float4 vj, DensitySums& a) {
-    const f3 x = sub(pi, xyz(pj));
+__device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, f3 pj, f3 vj, DensitySums& a) {
+    const f3 x = sub(pi, pj);
     const float r2 = dot3(x, x);
     const float r = sqrtf(r2);
     const float q = r * k.inv_h;
@@ -62,22 +129,20 @@ __device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, flo
     const f3 gw = r < 1e-9f ? mk(0.0f, 0.0f, 0.0f) : scl(g, x);
     a.gsum = add(a.gsum, gw);
     a.gsq += dot3(gw, gw);
-    a.drho += dot3(sub(vi, xyz(vj)), gw);
+    a.drho += dot3(sub(vi, vj), gw);
     return g;
 }
 
-// The candidate rows of the 3x3x3 stencil around cell (bx, by, bz): row
-// (dy, dz) spans x cells [x0, x1]. A row or an end cell whose box lies
-// farther from the particle than the support is left out: no particle in it
-// can pass the r^2 < support^2 test, so the accepted neighbours, and their
-// order, are those of the full 27-cell walk (sim_solve.wgsl:436-468).
-struct Stencil {
-    float dl[3], dh[3];  // distance to the low / high face of the own cell, per axis
-    float reach;         // support^2 with a margin over the rounding of the faces
+// Distances from the particle to the faces of its own cell: a stencil row or
+// end cell farther than the support holds no neighbour, so the walk skips it
+// (the accepted neighbours and their order are those of the full walk).
+struct Reach {
+    float dl[3], dh[3];
+    float lim;
 };
 
-__device__ __forceinline__ Stencil make_stencil(const SceneK& k, f3 p, uint32_t bx, uint32_t by, uint32_t bz) {
-    Stencil st;
+__device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, uint32_t by, uint32_t bz) {
+    Reach st;
     const float pp[3] = {p.x, p.y, p.z};
     const uint32_t b[3] = {bx, by, bz};
 #pragma unroll
@@ -87,57 +152,90 @@ __device__ __forceinline__ Stencil make_stencil(const SceneK& k, f3 p, uint32_t 
         st.dl[a] = wmax(pp[a] - lo, 0.0f);
         st.dh[a] = wmax(hi - pp[a], 0.0f);
     }
-    st.reach = k.support_sq * 1.001f;
+    st.lim = k.support_sq * 1.001f;  // a margin over the rounding of the faces
     return st;
 }
 
-__global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint32_t n, const uint32_t* __restrict__ starts,
-                                                        const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                        SolveBufs b) {
-    uint32_t i = particle_index();
-    if (i >= n) return;
+// LDS layout of density_div: per entry the position and velocity (24 bytes).
+struct DensityLds {
+    float4* pv;  // x, y, z, vx
+    float2* vw;  // vy, vz
+};
+
+template <bool STAGED>
+__device__ __forceinline__ void load_pv(const DensityLds& L, const float4* __restrict__ pos,
+                                        const float4* __restrict__ vel, uint32_t j, int32_t delta, f3& p, f3& v) {
+    if (STAGED) {
+        const float4 a = L.pv[j + delta];
+        const float2 b = L.vw[j + delta];
+        p = mk(a.x, a.y, a.z);
+        v = mk(a.w, b.x, b.y);
+    } else {
+        p = xyz(pos[j]);
+        v = xyz(vel[j]);
+    }
+}
+
+template <bool STAGED>
+__device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const uint32_t* __restrict__ starts,
+                             const float4* __restrict__ pos, const float4* __restrict__ vel, const SolveBufs& b,
+                             const Region& R, const DensityLds& L) {
     const f3 pi = xyz(pos[i]);
     const f3 vi = xyz(vel[i]);
     const uint32_t bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
     const uint32_t by = cell_coord(pi.y, k.box_min[1], k.cell, k.dims[1]);
     const uint32_t bz = cell_coord(pi.z, k.box_min[2], k.cell, k.dims[2]);
-    const Stencil st = make_stencil(k, pi, bx, by, bz);
+    const Reach st = make_reach(k, pi, bx, by, bz);
     uint2* __restrict__ list = b.nbr + i;
     const uint32_t stride = b.stride, cap = b.cap;
-    // Phase 1: the candidate walk, in the reference's cell order, appending
-    // accepted slots to the list (a distance test per candidate and nothing
-    // else, so the wave stays converged).
+    // Phase 1: distance tests only, appending accepted slots and row counts.
     uint32_t found = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int cz = (int)bz + dz;
-        if (cz < 0 || cz >= (int)k.dims[2]) continue;
+    uint64_t rows = 0;
+#pragma unroll 1
+    for (uint32_t r = 0; r < kRows; ++r) {
+        const int dz = (int)(r / 3) - 1, dy = (int)(r % 3) - 1;
+        const int cz = (int)bz + dz, cy = (int)by + dy;
+        if (cz < 0 || cz >= (int)k.dims[2] || cy < 0 || cy >= (int)k.dims[1]) continue;
         const float ez = dz < 0 ? st.dl[2] : (dz > 0 ? st.dh[2] : 0.0f);
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int cy = (int)by + dy;
-            if (cy < 0 || cy >= (int)k.dims[1]) continue;
-            const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
-            const float e2 = ey * ey + ez * ez;
-            if (e2 > st.reach) continue;
-            const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.reach) ? bx - 1 : bx;
-            const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.reach) ? bx + 1 : bx;
-            const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
-            const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
-            for (uint32_t kk = k0; kk < k1; ++kk) {
-                const f3 x = sub(pi, xyz(pos[kk]));
-                const float r2 = dot3(x, x);
-                if (r2 < k.support_sq && kk != i) {
-                    if (found < cap) list[(size_t)found * stride].x = kk;
-                    found++;
+        const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
+        const float e2 = ey * ey + ez * ez;
+        if (e2 > st.lim) continue;
+        const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.lim) ? bx - 1 : bx;
+        const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
+        const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
+        const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
+        const int32_t delta = R.delta[r];
+        uint32_t in_row = 0;
+        for (uint32_t kk = k0; kk < k1; ++kk) {
+            f3 pj;
+            if (STAGED) {
+                const float4 a = L.pv[kk + delta];
+                pj = mk(a.x, a.y, a.z);
+            } else {
+                pj = xyz(pos[kk]);
+            }
+            const f3 x = sub(pi, pj);
+            const float r2 = dot3(x, x);
+            if (r2 < k.support_sq && kk != i) {
+                if (found < cap) {
+                    list[(size_t)found * stride].x = kk;
+                    in_row++;
                 }
+                found++;
             }
         }
+        rows |= (uint64_t)in_row << (7u * r);
     }
     // Phase 2: the sums over the compact list, in list order.
     DensitySums a{k.sigma, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
     const uint32_t nn = found < cap ? found : cap;
+    uint32_t r = 0, row_end = row_count(rows, 0);
     for (uint32_t m = 0; m < nn; ++m) {
+        seek_row(rows, m, r, row_end);
         const uint32_t j = list[(size_t)m * stride].x;
-        const float g = density_pair(k, pi, vi, pos[j], vel[j], a);
+        f3 pj, vj;
+        load_pv<STAGED>(L, pos, vel, j, R.delta[r], pj, vj);
+        const float g = density_pair(k, pi, vi, pj, vj, a);
         list[(size_t)m * stride].y = __float_as_uint(g);
     }
     if (found > cap) {
@@ -146,27 +244,26 @@ __global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint3
         // take every neighbour: walk again for the ones the list dropped.
         atomicAdd(&b.counters[1], found - cap);
         uint32_t seen = 0;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int cz = (int)bz + dz;
-            if (cz < 0 || cz >= (int)k.dims[2]) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int cy = (int)by + dy;
-                if (cy < 0 || cy >= (int)k.dims[1]) continue;
-                const uint32_t x0 = bx > 0 ? bx - 1 : bx;
-                const uint32_t x1 = bx + 1 < k.dims[0] ? bx + 1 : bx;
-                const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
-                const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
-                for (uint32_t kk = k0; kk < k1; ++kk) {
-                    const f3 x = sub(pi, xyz(pos[kk]));
-                    if (dot3(x, x) < k.support_sq && kk != i) {
-                        if (seen >= cap) density_pair(k, pi, vi, pos[kk], vel[kk], a);
-                        seen++;
-                    }
+        for (uint32_t rr = 0; rr < kRows; ++rr) {
+            const int dz = (int)(rr / 3) - 1, dy = (int)(rr % 3) - 1;
+            const int cz = (int)bz + dz, cy = (int)by + dy;
+            if (cz < 0 || cz >= (int)k.dims[2] || cy < 0 || cy >= (int)k.dims[1]) continue;
+            const uint32_t x0 = bx > 0 ? bx - 1 : bx;
+            const uint32_t x1 = bx + 1 < k.dims[0] ? bx + 1 : bx;
+            const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
+            const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
+            for (uint32_t kk = k0; kk < k1; ++kk) {
+                const f3 pj = xyz(pos[kk]);
+                const f3 x = sub(pi, pj);
+                if (dot3(x, x) < k.support_sq && kk != i) {
+                    if (seen >= cap) density_pair(k, pi, vi, pj, xyz(vel[kk]), a);
+                    seen++;
                 }
             }
         }
     }
     b.nbr_n[i] = nn;
+    b.nbr_rows[i] = rows;
     float rho = k.mass * a.w_sum;
     const float gsq = k.mass * k.mass * a.gsq;
     float drho = k.mass * a.drho;
@@ -190,72 +287,226 @@ __global__ __launch_bounds__(kBlock) void k_density_div(SceneK k, StepK s, uint3
     b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
 }
 
-// div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
-// Reads only kd of the neighbours, so the velocity update is in place.
-__global__ __launch_bounds__(kBlock) void k_kappa_apply(SceneK k, StepK s, uint32_t n, const float4* __restrict__ pos,
-                                                        float4* __restrict__ vel, SolveBufs b) {
-    uint32_t i = particle_index();
-    if (i >= n) return;
-    const f3 pi = xyz(pos[i]);
-    const float kdi = b.kd[i];
-    const uint32_t nn = b.nbr_n[i];
-    f3 dv = mk(0.0f, 0.0f, 0.0f);
-    const uint2* __restrict__ list = b.nbr + i;
-#pragma unroll 4
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint2 e = list[(size_t)m * b.stride];
-        dv = add(dv, scl(kdi + b.kd[e.x], listed_grad(pi, pos[e.x], __uint_as_float(e.y))));
+__global__ __launch_bounds__(kTile) void k_density_div(SceneK k, StepK s, uint32_t n, const uint32_t* __restrict__ starts,
+                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                                       SolveBufs b) {
+    extern __shared__ float4 smem[];
+    __shared__ Region R;
+    const uint32_t s0 = tile_index() * kTile;
+    const uint32_t s1 = s0 + kTile < n ? s0 + kTile : n;
+    if (threadIdx.x == 0) build_region(k, starts, pos, s0, s1, b.cap_density, R);
+    __syncthreads();
+    DensityLds L{smem, reinterpret_cast<float2*>(smem + b.cap_density)};
+    if (R.staged) {
+        for (uint32_t r = 0; r < kRows; ++r) {
+            const uint32_t a = R.start[r], len = R.end[r] - a;
+            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
+            for (uint32_t t = threadIdx.x; t < len; t += kTile) {
+                const float4 p = pos[a + t];
+                const float4 v = vel[a + t];
+                L.pv[base + t] = make_float4(p.x, p.y, p.z, v.x);
+                L.vw[base + t] = make_float2(v.y, v.z);
+            }
+        }
     }
-    const float4 wg = b.wall_grad[i];
-    const f3 full = add(scl(k.mass, dv), scl(kdi, xyz(wg)));
-    const float4 w = vel[i];
-    vel[i] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
+    __syncthreads();
+    const uint32_t i = s0 + threadIdx.x;
+    if (i >= n) return;
+    if (R.staged)
+        density_body<true>(k, s, i, starts, pos, vel, b, R, L);
+    else
+        density_body<false>(k, s, i, starts, pos, vel, b, R, L);
 }
 
-__global__ __launch_bounds__(kBlock) void k_forces_eval(SceneK k, uint32_t n, const float4* __restrict__ pos,
-                                                        const float4* __restrict__ vel,
-                                                        const float* __restrict__ temperature,
-                                                        const float* __restrict__ prev_pressure, SolveBufs b) {
-    uint32_t i = particle_index();
-    if (i >= n) return;
-    const f3 pi = xyz(pos[i]);
-    const f3 vi = xyz(vel[i]);
-    const float w_i = b.rho_near[i], temp_i = temperature[i], rho_i = b.density[i];
-    f3 visc = mk(0.0f, 0.0f, 0.0f), blend = visc, nearf = visc, tension = visc;
-    float heat = 0.0f;
-    const uint32_t nn = b.nbr_n[i];
-    const uint2* __restrict__ list = b.nbr + i;
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint2 e = list[(size_t)m * b.stride];
-        const uint32_t j = e.x;
-        const f3 x = sub(pi, xyz(pos[j]));
-        const float r2 = dot3(x, x);
-        const float r = sqrtf(r2);
-        const f3 g = scl(__uint_as_float(e.y), x);
-        const float rho_j = b.density[j];
-        const float f = dot3(x, g) / (r2 + k.eps_h2);
-        const float pair = k.mass / (rho_i * rho_j) * f;
-        const f3 dv = sub(vi, xyz(vel[j]));
-        const float wn = wmax(1.0f - r * k.inv_h, 0.0f);
-        const float rm = wmax(r, 1e-5f);
-        const float sn = (w_i + b.rho_near[j]) * wn * wn;
-        nearf = add(nearf, mk(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm)));
-        visc = add(visc, scl(k.visc2 * pair, dv));
-        blend = sub(blend, scl(k.mass / rho_j * kernel_q(k, r * k.inv_h), dv));
-        heat += k.heat_cond * pair * (temp_i - temperature[j]);
-        heat -= k.heat_visc * pair * dot3(dv, dv);
-        const float corr = 2.0f * k.rho0 / (rho_i + rho_j);
-        const float cm = corr * k.mass * cohesion(r, k.coh_c, k.coh_k, k.coh_near);
-        const float rg = wmax(r, 1e-9f);
-        tension = sub(tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
+// ---- list sweeps -------------------------------------------------------------
+
+// Stages position and kd (the gathers of div_apply, den_apply and
+// forces_den_apply) as one float4 per slot.
+__device__ __forceinline__ void stage_pos_kd(const Region& R, const float4* __restrict__ pos,
+                                             const float* __restrict__ kd, float4* lds) {
+    for (uint32_t r = 0; r < kRows; ++r) {
+        const uint32_t a = R.start[r], len = R.end[r] - a;
+        const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
+        for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
+            const float4 p = pos[a + t];
+            lds[base + t] = make_float4(p.x, p.y, p.z, kd[a + t]);
+        }
     }
-    const f3 adh = scl(k.beta * k.rho0, wall_adh_sum(k, pi));
-    const f3 acc = add(add(add(visc, scl(K_NEAR, nearf)), scl(k.gamma, tension)), adh);
-    b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
-    b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
-    const float half = 0.5f * prev_pressure[i];
-    b.kd[i] = half / (rho_i * rho_i);
-    b.pressure[i] = half;
+}
+
+// Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
+// its share of the list (kappa_partial, sim_solve.wgsl:596-606).
+template <bool STAGED>
+__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, uint32_t p, uint32_t lane, f3 pi, float kdi,
+                                            const float4* __restrict__ pos, const Region& R, const float4* lds) {
+    f3 dv = mk(0.0f, 0.0f, 0.0f);
+    const uint32_t nn = b.nbr_n[p];
+    const uint64_t rows = b.nbr_rows[p];
+    uint32_t r = 0, row_end = row_count(rows, 0);
+    const uint2* __restrict__ list = b.nbr + p;
+    for (uint32_t m = lane; m < nn; m += kLanes) {
+        seek_row(rows, m, r, row_end);
+        const uint2 e = list[(size_t)m * b.stride];
+        float4 q;
+        if (STAGED) {
+            q = lds[e.x + R.delta[r]];
+        } else {
+            const float4 pj = pos[e.x];
+            q = make_float4(pj.x, pj.y, pj.z, b.kd[e.x]);
+        }
+        dv = add(dv, scl(kdi + q.w, listed_grad(pi, xyz(q), __uint_as_float(e.y))));
+    }
+    return dv;
+}
+
+struct LaneId {
+    uint32_t s0, s1, p, lane;
+    bool live;
+};
+
+__device__ __forceinline__ LaneId lane_id(uint32_t n) {
+    LaneId id;
+    id.s0 = tile_index() * kTile;
+    id.s1 = id.s0 + kTile < n ? id.s0 + kTile : n;
+    id.p = id.s0 + threadIdx.x / kLanes;
+    id.lane = threadIdx.x % kLanes;
+    id.live = id.p < n;
+    return id;
+}
+
+// div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
+// Reads only kd of the neighbours, so the velocity update is in place.
+__global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(SceneK k, StepK s, uint32_t n,
+                                                               const uint32_t* __restrict__ starts,
+                                                               const float4* __restrict__ pos, float4* __restrict__ vel,
+                                                               SolveBufs b) {
+    extern __shared__ float4 smem[];
+    __shared__ Region R;
+    const LaneId id = lane_id(n);
+    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa, R);
+    __syncthreads();
+    if (R.staged) stage_pos_kd(R, pos, b.kd, smem);
+    __syncthreads();
+    f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
+    float kdi = 0.0f;
+    if (id.live) {
+        pi = xyz(pos[id.p]);
+        kdi = b.kd[id.p];
+        dv = R.staged ? kappa_partial<true>(b, id.p, id.lane, pi, kdi, pos, R, smem)
+                      : kappa_partial<false>(b, id.p, id.lane, pi, kdi, pos, R, smem);
+    }
+    const f3 total = fold4(dv);
+    if (id.live && id.lane == 0) {
+        const float4 wg = b.wall_grad[id.p];
+        const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
+        const float4 w = vel[id.p];
+        vel[id.p] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
+    }
+}
+
+// forces_eval's gathers: position, velocity, density, near-density and
+// temperature, 36 bytes a slot as float4 (x, y, z, vx), float4 (vy, vz,
+// rho, rho_near), float (temperature).
+struct ForcesLds {
+    float4* a;
+    float4* c;
+    float* t;
+};
+
+__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const uint32_t* __restrict__ starts,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ vel,
+                                                               const float* __restrict__ temperature,
+                                                               const float* __restrict__ prev_pressure, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    __shared__ Region R;
+    const LaneId id = lane_id(n);
+    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_forces, R);
+    __syncthreads();
+    ForcesLds L{smem, smem + b.cap_forces, reinterpret_cast<float*>(smem + 2 * b.cap_forces)};
+    if (R.staged) {
+        for (uint32_t r = 0; r < kRows; ++r) {
+            const uint32_t a = R.start[r], len = R.end[r] - a;
+            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
+            for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
+                const float4 p = pos[a + t];
+                const float4 v = vel[a + t];
+                L.a[base + t] = make_float4(p.x, p.y, p.z, v.x);
+                L.c[base + t] = make_float4(v.y, v.z, b.density[a + t], b.rho_near[a + t]);
+                L.t[base + t] = temperature[a + t];
+            }
+        }
+    }
+    __syncthreads();
+    f3 visc = mk(0.0f, 0.0f, 0.0f), blend = visc, nearf = visc, tension = visc, pi = visc;
+    float heat = 0.0f, rho_i = 0.0f;
+    if (id.live) {
+        const uint32_t i = id.p;
+        pi = xyz(pos[i]);
+        const f3 vi = xyz(vel[i]);
+        const float w_i = b.rho_near[i], temp_i = temperature[i];
+        rho_i = b.density[i];
+        const uint32_t nn = b.nbr_n[i];
+        const uint64_t rows = b.nbr_rows[i];
+        uint32_t r = 0, row_end = row_count(rows, 0);
+        const uint2* __restrict__ list = b.nbr + i;
+        for (uint32_t m = id.lane; m < nn; m += kLanes) {
+            seek_row(rows, m, r, row_end);
+            const uint2 e = list[(size_t)m * b.stride];
+            const uint32_t j = e.x;
+            f3 pj, vj;
+            float rho_j, near_j, temp_j;
+            if (R.staged) {
+                const uint32_t q = j + R.delta[r];
+                const float4 a = L.a[q], c = L.c[q];
+                pj = mk(a.x, a.y, a.z);
+                vj = mk(a.w, c.x, c.y);
+                rho_j = c.z;
+                near_j = c.w;
+                temp_j = L.t[q];
+            } else {
+                pj = xyz(pos[j]);
+                vj = xyz(vel[j]);
+                rho_j = b.density[j];
+                near_j = b.rho_near[j];
+                temp_j = temperature[j];
+            }
+            const f3 x = sub(pi, pj);
+            const float r2 = dot3(x, x);
+            const float rr = sqrtf(r2);
+            const f3 g = scl(__uint_as_float(e.y), x);
+            const float f = dot3(x, g) / (r2 + k.eps_h2);
+            const float pair = k.mass / (rho_i * rho_j) * f;
+            const f3 dv = sub(vi, vj);
+            const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
+            const float rm = wmax(rr, 1e-5f);
+            const float sn = (w_i + near_j) * wn * wn;
+            nearf = add(nearf, mk(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm)));
+            visc = add(visc, scl(k.visc2 * pair, dv));
+            blend = sub(blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
+            heat += k.heat_cond * pair * (temp_i - temp_j);
+            heat -= k.heat_visc * pair * dot3(dv, dv);
+            const float corr = 2.0f * k.rho0 / (rho_i + rho_j);
+            const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
+            const float rg = wmax(rr, 1e-9f);
+            tension = sub(tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
+        }
+    }
+    visc = fold4(visc);
+    blend = fold4(blend);
+    nearf = fold4(nearf);
+    tension = fold4(tension);
+    heat = fold4(heat);
+    if (id.live && id.lane == 0) {
+        const uint32_t i = id.p;
+        const f3 adh = scl(k.beta * k.rho0, wall_adh_sum(k, pi));
+        const f3 acc = add(add(add(visc, scl(K_NEAR, nearf)), scl(k.gamma, tension)), adh);
+        b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
+        b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
+        const float half = 0.5f * prev_pressure[i];
+        b.kd[i] = half / (rho_i * rho_i);
+        b.pressure[i] = half;
+    }
 }
 
 // sim_solve.wgsl:103-127
@@ -276,66 +527,110 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
     return mk(wmix(v.x, pullx / weight, blend), wmix(v.y, pully / weight, blend), v.z);
 }
 
-__global__ __launch_bounds__(kBlock) void k_forces_den_apply(SceneK k, StepK s, uint32_t n,
-                                                             const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                             float* __restrict__ temperature, SolveBufs b) {
-    uint32_t i = particle_index();
-    if (i >= n) return;
-    const f3 pi = xyz(pos[i]);
-    const float kdi = b.kd[i];
-    const uint32_t nn = b.nbr_n[i];
-    f3 dv = mk(0.0f, 0.0f, 0.0f);
-    const uint2* __restrict__ list = b.nbr + i;
-#pragma unroll 4
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint2 e = list[(size_t)m * b.stride];
-        dv = add(dv, scl(kdi + b.kd[e.x], listed_grad(pi, pos[e.x], __uint_as_float(e.y))));
+__global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, StepK s, uint32_t n,
+                                                                    const uint32_t* __restrict__ starts,
+                                                                    const float4* __restrict__ pos,
+                                                                    float4* __restrict__ vel,
+                                                                    float* __restrict__ temperature, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    __shared__ Region R;
+    const LaneId id = lane_id(n);
+    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa, R);
+    __syncthreads();
+    if (R.staged) stage_pos_kd(R, pos, b.kd, smem);
+    __syncthreads();
+    f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
+    float kdi = 0.0f;
+    if (id.live) {
+        pi = xyz(pos[id.p]);
+        kdi = b.kd[id.p];
+        dv = R.staged ? kappa_partial<true>(b, id.p, id.lane, pi, kdi, pos, R, smem)
+                      : kappa_partial<false>(b, id.p, id.lane, pi, kdi, pos, R, smem);
     }
-    const float4 a = b.accel[i];
-    const float4 was = vel[i];
-    const f3 v = xyz(was);
-    const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
-    const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
-    const f3 c1 = cross3(dom, pi);
-    const f3 c2 = cross3(om, cross3(om, pi));
-    const f3 c3 = scl(2.0f, cross3(om, v));
-    const f3 fict = sub(sub(mk(-c1.x, -c1.y, -c1.z), c2), c3);
-    const f3 fsum = add(add(mk(s.force[0], s.force[1], s.force[2]), fict), xyz(a));
-    const float4 xs = b.xsph[i];
-    const f3 forced = fingers(s, pi, add(add(v, scl(s.dt, fsum)), scl(s.c_xsph, xyz(xs))));
-    temperature[i] = temperature[i] + s.dt * a.w;
-    const float4 wg = b.wall_grad[i];
-    const f3 full = add(scl(k.mass, dv), scl(kdi, xyz(wg)));
-    vel[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z, was.w);
+    const f3 total = fold4(dv);
+    if (id.live && id.lane == 0) {
+        const uint32_t i = id.p;
+        const float4 a = b.accel[i];
+        const float4 was = vel[i];
+        const f3 v = xyz(was);
+        const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
+        const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
+        const f3 c1 = cross3(dom, pi);
+        const f3 c2 = cross3(om, cross3(om, pi));
+        const f3 c3 = scl(2.0f, cross3(om, v));
+        const f3 fict = sub(sub(mk(-c1.x, -c1.y, -c1.z), c2), c3);
+        const f3 fsum = add(add(mk(s.force[0], s.force[1], s.force[2]), fict), xyz(a));
+        const float4 xs = b.xsph[i];
+        const f3 forced = fingers(s, pi, add(add(v, scl(s.dt, fsum)), scl(s.c_xsph, xyz(xs))));
+        temperature[i] = temperature[i] + s.dt * a.w;
+        const float4 wg = b.wall_grad[i];
+        const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
+        vel[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z, was.w);
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_den_kappa(SceneK k, StepK s, uint32_t n, const float4* __restrict__ pos,
-                                                      const float4* __restrict__ vel, SolveBufs b) {
-    uint32_t i = particle_index();
-    if (i >= n) return;
-    const f3 pi = xyz(pos[i]);
-    const f3 vi = xyz(vel[i]);
-    const uint32_t nn = b.nbr_n[i];
-    float drho = 0.0f;
-    const uint2* __restrict__ list = b.nbr + i;
-#pragma unroll 4
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint2 e = list[(size_t)m * b.stride];
-        drho += dot3(sub(vi, xyz(vel[e.x])), listed_grad(pi, pos[e.x], __uint_as_float(e.y)));
+// den_kappa's gathers: position and velocity, 24 bytes a slot.
+__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(SceneK k, StepK s, uint32_t n,
+                                                             const uint32_t* __restrict__ starts,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    __shared__ Region R;
+    const LaneId id = lane_id(n);
+    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa_vel, R);
+    __syncthreads();
+    DensityLds L{smem, reinterpret_cast<float2*>(smem + b.cap_kappa_vel)};
+    if (R.staged) {
+        for (uint32_t r = 0; r < kRows; ++r) {
+            const uint32_t a = R.start[r], len = R.end[r] - a;
+            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
+            for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
+                const float4 p = pos[a + t];
+                const float4 v = vel[a + t];
+                L.pv[base + t] = make_float4(p.x, p.y, p.z, v.x);
+                L.vw[base + t] = make_float2(v.y, v.z);
+            }
+        }
     }
-    const float rho = b.density[i];
-    const float4 wg = b.wall_grad[i];
-    drho = k.mass * drho + dot3(vi, xyz(wg));
-    const float rho_star = rho + s.dt * drho;
-    const float kap = wmax(rho_star - k.rho0, 0.0f) * b.alpha[i] / (s.dt * s.dt);
-    b.kd[i] = kap / rho;
-    b.pressure[i] = b.pressure[i] + kap * rho;
+    __syncthreads();
+    float drho = 0.0f;
+    f3 vi = mk(0.0f, 0.0f, 0.0f);
+    if (id.live) {
+        const uint32_t i = id.p;
+        const f3 pi = xyz(pos[i]);
+        vi = xyz(vel[i]);
+        const uint32_t nn = b.nbr_n[i];
+        const uint64_t rows = b.nbr_rows[i];
+        uint32_t r = 0, row_end = row_count(rows, 0);
+        const uint2* __restrict__ list = b.nbr + i;
+        for (uint32_t m = id.lane; m < nn; m += kLanes) {
+            seek_row(rows, m, r, row_end);
+            const uint2 e = list[(size_t)m * b.stride];
+            f3 pj, vj;
+            if (R.staged)
+                load_pv<true>(L, pos, vel, e.x, R.delta[r], pj, vj);
+            else
+                load_pv<false>(L, pos, vel, e.x, 0, pj, vj);
+            drho += dot3(sub(vi, vj), listed_grad(pi, pj, __uint_as_float(e.y)));
+        }
+    }
+    const float total = fold4(drho);
+    if (id.live && id.lane == 0) {
+        const uint32_t i = id.p;
+        const float rho = b.density[i];
+        const float4 wg = b.wall_grad[i];
+        const float d = k.mass * total + dot3(vi, xyz(wg));
+        const float rho_star = rho + s.dt * d;
+        const float kap = wmax(rho_star - k.rho0, 0.0f) * b.alpha[i] / (s.dt * s.dt);
+        b.kd[i] = kap / rho;
+        b.pressure[i] = b.pressure[i] + kap * rho;
+    }
 }
 
 // Reads the working set, writes the resting set at the same slot.
 __global__ __launch_bounds__(kBlock) void k_integrate(SceneK k, StepK s, uint32_t n, Records work, Records rest,
                                                       SolveBufs b) {
-    uint32_t i = particle_index();
+    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
     bool clamped_speed = false;
     if (i < n) {
         const float4 was = work.vel[i];
@@ -376,14 +671,34 @@ __global__ __launch_bounds__(kBlock) void k_integrate(SceneK k, StepK s, uint32_
         atomicAdd(&b.counters[0], (uint32_t)__popcll(m));
 }
 
+constexpr size_t kDensityEntry = sizeof(float4) + sizeof(float2);      // 24
+constexpr size_t kKappaEntry = sizeof(float4);                         // 16
+constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
+
+uint32_t tiles_for(uint32_t n) { return (n + kTile - 1) / kTile; }
+
 }  // namespace
+
+void configure_solve(SolveBufs& b, size_t lds_bytes) {
+    // Leave room for the static Region block; keep capacities multiples of 64.
+    const size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
+    auto cap = [&](size_t entry, size_t want) {
+        size_t c = avail / entry;
+        if (c > want) c = want;
+        return (uint32_t)(c / 64 * 64);
+    };
+    b.cap_density = cap(kDensityEntry, 3072);
+    b.cap_kappa = cap(kKappaEntry, 4096);
+    b.cap_kappa_vel = cap(kDensityEntry, 3072);
+    b.cap_forces = cap(kForcesEntry, 2560);
+}
 
 void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
                          const Records& work, const SolveBufs& b, KernelTimer* timer) {
     if (n == 0) return;
-    const dim3 grid(blocks_for(n)), block(kBlock);
     if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
-    hipLaunchKernelGGL(k_density_div, grid, block, 0, st, k, s, n, starts, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_density_div, dim3(tiles_for(n)), dim3(kTile), b.cap_density * kDensityEntry, st, k, s, n,
+                       starts, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DENSITY_DIV);
 }
 
@@ -391,29 +706,46 @@ void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, c
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t refine,
                   KernelTimer* timer) {
     if (n == 0) return;
-    const dim3 grid(blocks_for(n)), block(kBlock);
+    const dim3 tiles(tiles_for(n)), sweep(kSweepThreads);
+    const size_t lds_kappa = b.cap_kappa * kKappaEntry;
     launch_density_only(st, k, s, n, starts, work, b, timer);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
-    hipLaunchKernelGGL(k_kappa_apply, grid, block, 0, st, k, s, n, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval, grid, block, 0, st, k, n, work.pos, work.vel, work.temperature,
-                       work.prev_pressure, b);
+    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.cap_forces * kForcesEntry, st, k, n, starts, work.pos,
+                       work.vel, work.temperature, work.prev_pressure, b);
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
-    hipLaunchKernelGGL(k_forces_den_apply, grid, block, 0, st, k, s, n, work.pos, work.vel, work.temperature, b);
+    hipLaunchKernelGGL(k_forces_den_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel,
+                       work.temperature, b);
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
     for (uint32_t r = 0; r < refine; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa, grid, block, 0, st, k, s, n, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.cap_kappa_vel * kDensityEntry, st, k, s, n, starts,
+                           work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
-        hipLaunchKernelGGL(k_kappa_apply, grid, block, 0, st, k, s, n, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_APPLY);
     }
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
-    hipLaunchKernelGGL(k_integrate, grid, block, 0, st, k, s, n, work, rest, b);
+    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n)), dim3(kBlock), 0, st, k, s, n, work, rest, b);
     if (timer) timer->end(st, KERNEL_INTEGRATE);
+}
+
+// Lets the sweeps take LDS beyond the default per-launch limit.
+void enable_large_lds(size_t bytes) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_div),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kappa_apply),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forces_eval),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forces_den_apply),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_den_kappa),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace fluid

@@ -280,6 +280,12 @@ class Sim:
     def synchronize(self):
         check(self.L.fluid_sim_synchronize(self.h), "synchronize")
 
+    def describe(self) -> dict:
+        import json
+        buf = C.create_string_buffer(512)
+        check(self.L.fluid_sim_describe(self.h, buf, 512), "describe")
+        return json.loads(buf.value.decode())
+
     def profile(self, kernels=None):
         mask = 0
         for k in (kernels or []):

@@ -68,7 +68,7 @@ int fluid_sim_device_count(void);
  * (render.rs:1468-1612): velocities and previous velocities start at zero,
  * temperature at 293.15 K, previous pressure at zero; positions start at
  * zero until fluid_sim_upload_state. `tracers` packed tracers are allocated
- * (zero = none). `nbr_cap` is the neighbour-list cap (0 = NBR_CAP, 96).
+ * (zero = none). `nbr_cap` is the neighbour-list cap (0 = NBR_CAP, 96; at most 127).
  * Returns NULL on failure.
  */
 FluidSim *fluid_sim_create(const uint8_t *params64, uint32_t tracers, uint32_t nbr_cap);
@@ -173,6 +173,9 @@ int fluid_sim_profile(FluidSim *sim, uint32_t kernel_mask);
 /* Total milliseconds and launch count per kernel class since profiling began (synchronises). */
 int fluid_sim_profile_read(FluidSim *sim, double *ms, uint64_t *launches);
 const char *fluid_sim_kernel_name(uint32_t kernel);
+
+/* A JSON description of the handle's device and launch configuration. */
+int fluid_sim_describe(const FluidSim *sim, char *buf, size_t size);
 
 /*
  * The grid's exclusive scan on its own (the decoupled-look-back scan that

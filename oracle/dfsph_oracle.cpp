@@ -20,11 +20,14 @@
 //     particles keep the order of the resting set they came from.
 //   * The reference's sweeps split each particle over 8 lanes and fold
 //     (sim_solve.wgsl:377, 398-402) and append the neighbour list through a
-//     workgroup atomic (sim_solve.wgsl:463). The oracle is the LANES = 1
-//     reading of the same code: one serial walk of the 27 cells in the
+//     workgroup atomic (sim_solve.wgsl:463). The oracle's density sweep is the
+//     LANES = 1 reading of that code: one serial walk of the 27 cells in the
 //     reference's cell numbering (x fastest, then y, then z,
 //     sim_solve.wgsl:436-438), particles in slot order within a cell, and
-//     the list in that same order.
+//     the list in that same order. The list sweeps (div_apply, forces_eval,
+//     forces_den_apply, den_kappa, den_apply) split each list over 4 lanes
+//     with the reference's strided assignment and xor-butterfly fold
+//     (kListLanes, lane_fold below).
 //   * Per-substep constants the reference evaluates per invocation with
 //     exp() (XSPH blend, lens chase, tracer charge decay) are evaluated once
 //     per substep with the host expf/exp.
@@ -239,6 +242,18 @@ struct Consts {
         eps_h2 = K(0.01) * h * h;
     }
 };
+
+// The list sweeps split each particle's neighbour list over kListLanes
+// lanes, as the reference splits it over LANES = 8 (sim_solve.wgsl:377,
+// 546, 601, 684): lane l takes entries l, l + kListLanes, ... in order, and
+// the lane partials fold the way lane_fold does (sim_solve.wgsl:398-402),
+// xor 2 then xor 1, so lane 0 holds (p0 + p2) + (p1 + p3).
+constexpr uint32_t kListLanes = 4;
+template <typename A> inline A lane_fold(const A p[kListLanes]) {
+    const A t0 = p[0] + p[2];
+    const A t1 = p[1] + p[3];
+    return t0 + t1;
+}
 
 // sim_solve.wgsl:176-182
 template <typename T> inline T kernel_q(const Consts<T>& k, T q) {
@@ -577,11 +592,13 @@ struct Sim {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i);
             T kd_i = kd[i];
-            V3<T> dv = v3(T(0), T(0), T(0));
+            V3<T> part[kListLanes];
+            for (auto& q : part) q = v3(T(0), T(0), T(0));
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                dv = dv + (kd_i + kd[j]) * pair_grad(pos_i, j);
+                part[m % kListLanes] = part[m % kListLanes] + (kd_i + kd[j]) * pair_grad(pos_i, j);
             }
+            const V3<T> dv = lane_fold(part);
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             V3<T> full = k.mass * dv + kd_i * wg;
             V3<T> v = wv(i) - s.dt * full;
@@ -598,12 +615,20 @@ struct Sim {
 #pragma omp parallel for schedule(static)
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t i = (uint32_t)pi;
-            V3<T> visc = v3(T(0), T(0), T(0)), blend = visc, nearf = visc, tension = visc;
-            T heat = 0;
+            V3<T> visc_l[kListLanes], blend_l[kListLanes], near_l[kListLanes], tension_l[kListLanes];
+            T heat_l[kListLanes] = {};
+            for (uint32_t l = 0; l < kListLanes; ++l)
+                visc_l[l] = blend_l[l] = near_l[l] = tension_l[l] = v3(T(0), T(0), T(0));
             V3<T> pos_i = wp(i), vel_i = wv(i);
             T w_i = rho_near[i], temp_i = wtemp[i], rho_i = density[i];
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
+                const uint32_t l = m % kListLanes;
+                V3<T>& visc = visc_l[l];
+                V3<T>& blend = blend_l[l];
+                V3<T>& nearf = near_l[l];
+                V3<T>& tension = tension_l[l];
+                T& heat = heat_l[l];
                 V3<T> x = pos_i - wp(j);
                 T r2 = dot(x, x);
                 T r = std::sqrt(r2);
@@ -625,6 +650,9 @@ struct Sim {
                 T rg = tmax(r, K(1e-9));
                 tension = tension - v3(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg);
             }
+            const V3<T> visc = lane_fold(visc_l), blend = lane_fold(blend_l), nearf = lane_fold(near_l),
+                        tension = lane_fold(tension_l);
+            const T heat = lane_fold(heat_l);
             V3<T> adh = (k.beta * k.rho0) * wall_adh_sum(k, pos_i);
             V3<T> acc = visc + T(K_NEAR) * nearf + k.gamma * tension + adh;
             accel[4 * i] = acc.x;
@@ -650,11 +678,13 @@ struct Sim {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i);
             T kd_i = kd[i];
-            V3<T> dv = v3(T(0), T(0), T(0));
+            V3<T> part[kListLanes];
+            for (auto& q : part) q = v3(T(0), T(0), T(0));
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                dv = dv + (kd_i + kd[j]) * pair_grad(pos_i, j);
+                part[m % kListLanes] = part[m % kListLanes] + (kd_i + kd[j]) * pair_grad(pos_i, j);
             }
+            const V3<T> dv = lane_fold(part);
             V3<T> a = v3(accel[4 * i], accel[4 * i + 1], accel[4 * i + 2]);
             V3<T> r = pos_i, v = wv(i);
             V3<T> fict = -cross(s.domega, r) - cross(s.omega, cross(s.omega, r)) - T(2) * cross(s.omega, v);
@@ -678,11 +708,12 @@ struct Sim {
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i), vel_i = wv(i);
-            T drho = 0;
+            T part[kListLanes] = {};
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                drho += dot(vel_i - wv(j), pair_grad(pos_i, j));
+                part[m % kListLanes] += dot(vel_i - wv(j), pair_grad(pos_i, j));
             }
+            T drho = lane_fold(part);
             T rho = density[i];
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             drho = k.mass * drho + dot(vel_i, wg);

@@ -167,6 +167,7 @@ def main():
         "step_roofline": {"canonical_bytes": step_bytes, "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "per_kernel_ms": per_kernel_ms,
+        "launch": sim.describe(),
         "frame_extra_ms": frame_extra_ms,
         "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
                   "compression_avg": float(stats[0])},

@@ -38,6 +38,17 @@ struct GridBufs {
     uint32_t* ticket;    // [1] tile ticket
 };
 
+// The nine stencil-row slot ranges holding every neighbour of one tile of
+// consecutive slots, and their offsets in the tile's LDS staging area: a
+// neighbour slot j found in stencil row r sits at LDS index j + delta[r]. Built
+// once per substep after the grid build.
+struct TileRegion {
+    int32_t delta[9];
+    uint32_t base[9];  // LDS offset of row r's first slot (prefix of the lengths)
+    uint32_t total;    // staged slots
+    uint32_t pad;
+};
+
 // Per-substep scratch of the solver, all in slot (cell) order.
 struct SolveBufs {
     float* density;
@@ -61,7 +72,11 @@ struct SolveBufs {
     // LDS capacities (neighbour-region entries) of the staged sweeps; a tile
     // whose region is larger gathers from global memory instead.
     uint32_t cap_density, cap_kappa, cap_kappa_vel, cap_forces;
+    TileRegion* regions;  // [tiles]
 };
+
+// Tiles of the solve (consecutive slots a workgroup owns).
+uint32_t solve_tiles(uint32_t n);
 
 // Sets the LDS capacities for this device's per-workgroup LDS limit.
 void configure_solve(SolveBufs& b, size_t lds_bytes);

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -299,10 +300,16 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     if ((rc = s->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
     A(b.nbr_rows, n);
+    A(b.regions, solve_tiles(s->n));
     A(b.counters, 4);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_bytes = (size_t)lds;
+    // FLUID_LDS_BYTES caps the LDS a sweep workgroup may take (a tuning knob).
+    if (const char* env = getenv("FLUID_LDS_BYTES")) {
+        const long v = strtol(env, nullptr, 10);
+        if (v > 0 && (size_t)v < s->lds_bytes) s->lds_bytes = (size_t)v;
+    }
     enable_large_lds(s->lds_bytes);
     configure_solve(b, s->lds_bytes);
     s->stats.blocks = stats_blocks(s->n);

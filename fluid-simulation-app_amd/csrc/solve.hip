@@ -13,13 +13,13 @@
 //
 // Layout: every array is in slot (cell) order after the grid build. A
 // workgroup owns a tile of kTile consecutive slots. All neighbours of a
-// tile lie in nine slot ranges, one per stencil row (dy, dz) — in cell-id
+// tile lie in nine slot ranges, one per stencil row (dy, dz): in cell-id
 // space, row (dy, dz) of every particle of the tile falls in
-// [first cell + off(dy, dz) - 1, last cell + off(dy, dz) + 1] — so each sweep
-// first stages the fields it gathers for those ranges into LDS (coalesced,
-// each line fetched once per tile) and then reads neighbours from LDS. A
-// tile whose ranges exceed the LDS capacity gathers from global memory; the
-// values read, and so the results, are the same either way.
+// [first cell + off(dy, dz) - 1, last cell + off(dy, dz) + 1]. k_tile_regions
+// records those ranges once per substep; each sweep then stages the fields it
+// gathers for them into LDS with independent coalesced loads, and reads
+// neighbours from LDS. A tile whose ranges exceed the LDS capacity gathers
+// from global memory; the values read, and so the results, are the same.
 //
 // density_div: one thread a particle. It walks the stencil rows in the
 // reference's cell order (x fastest, then y, then z; in-cell slot order) —
@@ -32,7 +32,8 @@
 //
 // The list sweeps run kLanes lanes a particle, as the reference runs 8: lane l
 // takes list entries l, l + kLanes, ..., and the partials fold with the
-// reference's xor butterfly (lane_fold, sim_solve.wgsl:398-402).
+// reference's xor butterfly (lane_fold, sim_solve.wgsl:398-402). Each lane
+// loads its first kPrefetch entries up front, so their latencies overlap.
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 
@@ -44,14 +45,7 @@ constexpr uint32_t kTile = 256;                      // particles a workgroup
 constexpr uint32_t kLanes = 4;                       // lanes a particle in the list sweeps
 constexpr uint32_t kSweepThreads = kTile * kLanes;  // 1024
 constexpr uint32_t kRows = 9;
-
-// The nine stencil-row slot ranges of a tile and where each sits in LDS.
-struct Region {
-    uint32_t start[kRows], end[kRows];
-    int32_t delta[kRows];  // LDS index of slot j in row r: j + delta[r]
-    uint32_t total;
-    uint32_t staged;
-};
+constexpr uint32_t kPrefetch = 8;                    // list entries a lane loads ahead
 
 __device__ __forceinline__ uint32_t tile_index() { return xcd_tile(blockIdx.x, gridDim.x); }
 
@@ -62,14 +56,21 @@ __device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
     return (cz * k.dims[1] + cy) * k.dims[0] + cx;
 }
 
-// One thread builds the region of slots [s0, s1).
-__device__ void build_region(const SceneK& k, const uint32_t* __restrict__ starts, const float4* __restrict__ pos,
-                             uint32_t s0, uint32_t s1, uint32_t capacity, Region& R) {
+// One thread a tile: the tile's nine row ranges.
+__global__ __launch_bounds__(kBlock) void k_tile_regions(SceneK k, uint32_t n, const uint32_t* __restrict__ starts,
+                                                         const float4* __restrict__ pos, TileRegion* __restrict__ out,
+                                                         uint32_t tiles) {
+    const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+    if (t >= tiles) return;
+    const uint32_t s0 = t * kTile;
+    const uint32_t s1 = s0 + kTile < n ? s0 + kTile : n;
     const int64_t c_first = cell_of_pos(k, pos[s0]);
     const int64_t c_last = cell_of_pos(k, pos[s1 - 1]);
     const int64_t cells = (int64_t)k.dims[0] * k.dims[1] * k.dims[2];
     const int64_t plane = (int64_t)k.dims[0] * k.dims[1];
+    TileRegion R;
     uint32_t total = 0;
+#pragma unroll
     for (uint32_t r = 0; r < kRows; ++r) {
         const int64_t dz = (int64_t)(r / 3) - 1, dy = (int64_t)(r % 3) - 1;
         const int64_t off = dz * plane + dy * (int64_t)k.dims[0];
@@ -81,13 +82,70 @@ __device__ void build_region(const SceneK& k, const uint32_t* __restrict__ start
             a = starts[lo];
             e = starts[hi + 1];
         }
-        R.start[r] = a;
-        R.end[r] = e;
         R.delta[r] = (int32_t)total - (int32_t)a;
+        R.base[r] = total;
         total += e - a;
     }
     R.total = total;
-    R.staged = total <= capacity ? 1u : 0u;
+    R.pad = 0;
+    out[t] = R;
+}
+
+// A tile's region as uniform scalars (SGPRs): the staging loop selects the
+// row of each flat LDS index among them.
+struct RegionS {
+    int32_t d0, d1, d2, d3, d4, d5, d6, d7, d8;
+    uint32_t b1, b2, b3, b4, b5, b6, b7, b8;
+    uint32_t total;
+};
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+
+__device__ __forceinline__ RegionS load_region(const TileRegion* __restrict__ regions, uint32_t tile) {
+    const TileRegion* R = regions + tile;
+    RegionS s;
+    s.d0 = (int32_t)uni((uint32_t)R->delta[0]);
+    s.d1 = (int32_t)uni((uint32_t)R->delta[1]);
+    s.d2 = (int32_t)uni((uint32_t)R->delta[2]);
+    s.d3 = (int32_t)uni((uint32_t)R->delta[3]);
+    s.d4 = (int32_t)uni((uint32_t)R->delta[4]);
+    s.d5 = (int32_t)uni((uint32_t)R->delta[5]);
+    s.d6 = (int32_t)uni((uint32_t)R->delta[6]);
+    s.d7 = (int32_t)uni((uint32_t)R->delta[7]);
+    s.d8 = (int32_t)uni((uint32_t)R->delta[8]);
+    s.b1 = uni(R->base[1]);
+    s.b2 = uni(R->base[2]);
+    s.b3 = uni(R->base[3]);
+    s.b4 = uni(R->base[4]);
+    s.b5 = uni(R->base[5]);
+    s.b6 = uni(R->base[6]);
+    s.b7 = uni(R->base[7]);
+    s.b8 = uni(R->base[8]);
+    s.total = uni(R->total);
+    return s;
+}
+
+// The delta of the region row holding flat LDS index idx: that of the last
+// row whose base <= idx (empty rows share the next row's base, so the last one
+// is the non-empty one).
+// Accumulated as d0 plus the step at each row boundary passed (a chain of
+// adds, which keeps every value in registers).
+__device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) {
+    int32_t d = R.d0;
+    d += (int32_t)(idx >= R.b1) * (R.d1 - R.d0);
+    d += (int32_t)(idx >= R.b2) * (R.d2 - R.d1);
+    d += (int32_t)(idx >= R.b3) * (R.d3 - R.d2);
+    d += (int32_t)(idx >= R.b4) * (R.d4 - R.d3);
+    d += (int32_t)(idx >= R.b5) * (R.d5 - R.d4);
+    d += (int32_t)(idx >= R.b6) * (R.d6 - R.d5);
+    d += (int32_t)(idx >= R.b7) * (R.d7 - R.d6);
+    d += (int32_t)(idx >= R.b8) * (R.d8 - R.d7);
+    return d;
+}
+
+// Every sweep keeps the nine row deltas in LDS for its lanes' row lookups.
+__device__ __forceinline__ void load_deltas(const TileRegion* __restrict__ regions, uint32_t tile, int32_t* s_delta) {
+    if (threadIdx.x < kRows) s_delta[threadIdx.x] = regions[tile].delta[threadIdx.x];
 }
 
 __device__ __forceinline__ uint32_t row_count(uint64_t rows, uint32_t r) { return (uint32_t)(rows >> (7u * r)) & 127u; }
@@ -109,6 +167,25 @@ __device__ __forceinline__ float fold4(float v) {
 __device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
 
 __device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
+
+// Copies `total` staged slots into LDS: NT threads, ITERS slots each, all
+// loads issued before the stores. `load(slot, it)` fetches into registers,
+// `store(idx, it)` writes them to LDS.
+template <uint32_t NT, uint32_t ITERS, class Load, class Store>
+__device__ __forceinline__ void stage_flat(const RegionS& R, Load&& load, Store&& store) {
+    for (uint32_t chunk = 0; chunk < R.total; chunk += NT * ITERS) {
+#pragma unroll
+        for (uint32_t it = 0; it < ITERS; ++it) {
+            const uint32_t idx = chunk + it * NT + threadIdx.x;
+            if (idx < R.total) load((uint32_t)((int32_t)idx - region_delta(R, idx)), it);
+        }
+#pragma unroll
+        for (uint32_t it = 0; it < ITERS; ++it) {
+            const uint32_t idx = chunk + it * NT + threadIdx.x;
+            if (idx < R.total) store(idx, it);
+        }
+    }
+}
 
 // ---- density_div -------------------------------------------------------------
 
@@ -156,14 +233,15 @@ __device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, 
     return st;
 }
 
-// LDS layout of density_div: per entry the position and velocity (24 bytes).
-struct DensityLds {
+// LDS layout of density_div and den_kappa: per slot the position and the
+// velocity, 24 bytes.
+struct PvLds {
     float4* pv;  // x, y, z, vx
     float2* vw;  // vy, vz
 };
 
 template <bool STAGED>
-__device__ __forceinline__ void load_pv(const DensityLds& L, const float4* __restrict__ pos,
+__device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict__ pos,
                                         const float4* __restrict__ vel, uint32_t j, int32_t delta, f3& p, f3& v) {
     if (STAGED) {
         const float4 a = L.pv[j + delta];
@@ -176,10 +254,27 @@ __device__ __forceinline__ void load_pv(const DensityLds& L, const float4* __res
     }
 }
 
+template <uint32_t NT>
+__device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restrict__ pos,
+                                         const float4* __restrict__ vel, const PvLds& L) {
+    constexpr uint32_t IT = 4;
+    float4 pp[IT], vv[IT];
+    stage_flat<NT, IT>(
+        R,
+        [&](uint32_t slot, uint32_t it) {
+            pp[it] = pos[slot];
+            vv[it] = vel[slot];
+        },
+        [&](uint32_t idx, uint32_t it) {
+            L.pv[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
+            L.vw[idx] = make_float2(vv[it].y, vv[it].z);
+        });
+}
+
 template <bool STAGED>
 __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const uint32_t* __restrict__ starts,
                              const float4* __restrict__ pos, const float4* __restrict__ vel, const SolveBufs& b,
-                             const Region& R, const DensityLds& L) {
+                             const int32_t* deltas, const PvLds& L) {
     const f3 pi = xyz(pos[i]);
     const f3 vi = xyz(vel[i]);
     const uint32_t bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
@@ -204,8 +299,9 @@ __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const 
         const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
         const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
         const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
-        const int32_t delta = R.delta[r];
+        const int32_t delta = deltas[r];
         uint32_t in_row = 0;
+#pragma unroll 4
         for (uint32_t kk = k0; kk < k1; ++kk) {
             f3 pj;
             if (STAGED) {
@@ -226,17 +322,27 @@ __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const 
         }
         rows |= (uint64_t)in_row << (7u * r);
     }
-    // Phase 2: the sums over the compact list, in list order.
+    // Phase 2: the sums over the compact list, in list order, in chunks of
+    // kPrefetch entries whose slots are read back together.
     DensitySums a{k.sigma, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
     const uint32_t nn = found < cap ? found : cap;
     uint32_t r = 0, row_end = row_count(rows, 0);
-    for (uint32_t m = 0; m < nn; ++m) {
-        seek_row(rows, m, r, row_end);
-        const uint32_t j = list[(size_t)m * stride].x;
-        f3 pj, vj;
-        load_pv<STAGED>(L, pos, vel, j, R.delta[r], pj, vj);
-        const float g = density_pair(k, pi, vi, pj, vj, a);
-        list[(size_t)m * stride].y = __float_as_uint(g);
+    for (uint32_t m0 = 0; m0 < nn; m0 += kPrefetch) {
+        uint32_t js[kPrefetch];
+#pragma unroll
+        for (uint32_t t = 0; t < kPrefetch; ++t)
+            js[t] = m0 + t < nn ? list[(size_t)(m0 + t) * stride].x : 0u;
+#pragma unroll
+        for (uint32_t t = 0; t < kPrefetch; ++t) {
+            const uint32_t m = m0 + t;
+            if (m < nn) {
+                seek_row(rows, m, r, row_end);
+                f3 pj, vj;
+                load_pv<STAGED>(L, pos, vel, js[t], deltas[r], pj, vj);
+                const float g = density_pair(k, pi, vi, pj, vj, a);
+                list[(size_t)m * stride].y = __float_as_uint(g);
+            }
+        }
     }
     if (found > cap) {
         // Past the cap the list keeps its first `cap` neighbours (the later
@@ -291,109 +397,122 @@ __global__ __launch_bounds__(kTile) void k_density_div(SceneK k, StepK s, uint32
                                                        const float4* __restrict__ pos, const float4* __restrict__ vel,
                                                        SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ Region R;
-    const uint32_t s0 = tile_index() * kTile;
-    const uint32_t s1 = s0 + kTile < n ? s0 + kTile : n;
-    if (threadIdx.x == 0) build_region(k, starts, pos, s0, s1, b.cap_density, R);
+    __shared__ int32_t s_delta[kRows];
+    const uint32_t tile = tile_index();
+    const RegionS R = load_region(b.regions, tile);
+    const bool staged = R.total <= b.cap_density;
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.cap_density)};
+    load_deltas(b.regions, tile, s_delta);
+    if (staged) stage_pv<kTile>(R, pos, vel, L);
     __syncthreads();
-    DensityLds L{smem, reinterpret_cast<float2*>(smem + b.cap_density)};
-    if (R.staged) {
-        for (uint32_t r = 0; r < kRows; ++r) {
-            const uint32_t a = R.start[r], len = R.end[r] - a;
-            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
-            for (uint32_t t = threadIdx.x; t < len; t += kTile) {
-                const float4 p = pos[a + t];
-                const float4 v = vel[a + t];
-                L.pv[base + t] = make_float4(p.x, p.y, p.z, v.x);
-                L.vw[base + t] = make_float2(v.y, v.z);
-            }
-        }
-    }
-    __syncthreads();
-    const uint32_t i = s0 + threadIdx.x;
+    const uint32_t i = tile * kTile + threadIdx.x;
     if (i >= n) return;
-    if (R.staged)
-        density_body<true>(k, s, i, starts, pos, vel, b, R, L);
+    if (staged)
+        density_body<true>(k, s, i, starts, pos, vel, b, s_delta, L);
     else
-        density_body<false>(k, s, i, starts, pos, vel, b, R, L);
+        density_body<false>(k, s, i, starts, pos, vel, b, s_delta, L);
 }
 
 // ---- list sweeps -------------------------------------------------------------
 
-// Stages position and kd (the gathers of div_apply, den_apply and
-// forces_den_apply) as one float4 per slot.
-__device__ __forceinline__ void stage_pos_kd(const Region& R, const float4* __restrict__ pos,
-                                             const float* __restrict__ kd, float4* lds) {
-    for (uint32_t r = 0; r < kRows; ++r) {
-        const uint32_t a = R.start[r], len = R.end[r] - a;
-        const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
-        for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
-            const float4 p = pos[a + t];
-            lds[base + t] = make_float4(p.x, p.y, p.z, kd[a + t]);
-        }
-    }
-}
-
-// Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
-// its share of the list (kappa_partial, sim_solve.wgsl:596-606).
-template <bool STAGED>
-__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, uint32_t p, uint32_t lane, f3 pi, float kdi,
-                                            const float4* __restrict__ pos, const Region& R, const float4* lds) {
-    f3 dv = mk(0.0f, 0.0f, 0.0f);
-    const uint32_t nn = b.nbr_n[p];
-    const uint64_t rows = b.nbr_rows[p];
-    uint32_t r = 0, row_end = row_count(rows, 0);
-    const uint2* __restrict__ list = b.nbr + p;
-    for (uint32_t m = lane; m < nn; m += kLanes) {
-        seek_row(rows, m, r, row_end);
-        const uint2 e = list[(size_t)m * b.stride];
-        float4 q;
-        if (STAGED) {
-            q = lds[e.x + R.delta[r]];
-        } else {
-            const float4 pj = pos[e.x];
-            q = make_float4(pj.x, pj.y, pj.z, b.kd[e.x]);
-        }
-        dv = add(dv, scl(kdi + q.w, listed_grad(pi, xyz(q), __uint_as_float(e.y))));
-    }
-    return dv;
-}
-
 struct LaneId {
-    uint32_t s0, s1, p, lane;
+    uint32_t tile, p, lane;
     bool live;
 };
 
 __device__ __forceinline__ LaneId lane_id(uint32_t n) {
     LaneId id;
-    id.s0 = tile_index() * kTile;
-    id.s1 = id.s0 + kTile < n ? id.s0 + kTile : n;
-    id.p = id.s0 + threadIdx.x / kLanes;
+    id.tile = tile_index();
+    id.p = id.tile * kTile + threadIdx.x / kLanes;
     id.lane = threadIdx.x % kLanes;
     id.live = id.p < n;
     return id;
 }
 
+// Walks lane `lane`'s share of particle p's list in order, calling
+// f(slot, dW/r, delta of the slot's row) for each entry.
+template <class F>
+__device__ __forceinline__ void for_list(const SolveBufs& b, const int32_t* deltas, uint32_t p, uint32_t lane, F&& f) {
+    const uint32_t nn = b.nbr_n[p];
+    const uint64_t rows = b.nbr_rows[p];
+    const uint2* __restrict__ list = b.nbr + p;
+    uint32_t r = 0, row_end = row_count(rows, 0);
+    uint2 e[kPrefetch];
+#pragma unroll
+    for (uint32_t t = 0; t < kPrefetch; ++t) {
+        const uint32_t m = lane + t * kLanes;
+        e[t] = m < nn ? list[(size_t)m * b.stride] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (uint32_t t = 0; t < kPrefetch; ++t) {
+        const uint32_t m = lane + t * kLanes;
+        if (m < nn) {
+            seek_row(rows, m, r, row_end);
+            f(e[t].x, __uint_as_float(e[t].y), deltas[r]);
+        }
+    }
+    for (uint32_t m = lane + kPrefetch * kLanes; m < nn; m += kLanes) {
+        seek_row(rows, m, r, row_end);
+        const uint2 q = list[(size_t)m * b.stride];
+        f(q.x, __uint_as_float(q.y), deltas[r]);
+    }
+}
+
+// Stages position and kd (the gathers of div_apply, den_apply and
+// forces_den_apply) as one float4 per slot.
+__device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __restrict__ pos,
+                                             const float* __restrict__ kd, float4* lds) {
+    constexpr uint32_t IT = 4;
+    float4 pp[IT];
+    float kk[IT];
+    stage_flat<kSweepThreads, IT>(
+        R,
+        [&](uint32_t slot, uint32_t it) {
+            pp[it] = pos[slot];
+            kk[it] = kd[slot];
+        },
+        [&](uint32_t idx, uint32_t it) { lds[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, kk[it]); });
+}
+
+// Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
+// its share of the list (kappa_partial, sim_solve.wgsl:596-606).
+template <bool STAGED>
+__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const int32_t* deltas, uint32_t p, uint32_t lane,
+                                            f3 pi, float kdi, const float4* __restrict__ pos, const float4* lds) {
+    f3 dv = mk(0.0f, 0.0f, 0.0f);
+    for_list(b, deltas, p, lane, [&](uint32_t j, float g, int32_t delta) {
+        float4 q;
+        if (STAGED) {
+            q = lds[j + delta];
+        } else {
+            const float4 pj = pos[j];
+            q = make_float4(pj.x, pj.y, pj.z, b.kd[j]);
+        }
+        dv = add(dv, scl(kdi + q.w, listed_grad(pi, xyz(q), g)));
+    });
+    return dv;
+}
+
 // div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
 // Reads only kd of the neighbours, so the velocity update is in place.
 __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(SceneK k, StepK s, uint32_t n,
-                                                               const uint32_t* __restrict__ starts,
                                                                const float4* __restrict__ pos, float4* __restrict__ vel,
                                                                SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ Region R;
+    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
-    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa, R);
-    __syncthreads();
-    if (R.staged) stage_pos_kd(R, pos, b.kd, smem);
+    const RegionS R = load_region(b.regions, id.tile);
+    load_deltas(b.regions, id.tile, s_delta);
+    const bool staged = R.total <= b.cap_kappa;
+    if (staged) stage_pos_kd(R, pos, b.kd, smem);
     __syncthreads();
     f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
     float kdi = 0.0f;
     if (id.live) {
         pi = xyz(pos[id.p]);
         kdi = b.kd[id.p];
-        dv = R.staged ? kappa_partial<true>(b, id.p, id.lane, pi, kdi, pos, R, smem)
-                      : kappa_partial<false>(b, id.p, id.lane, pi, kdi, pos, R, smem);
+        dv = staged ? kappa_partial<true>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem)
+                    : kappa_partial<false>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem);
     }
     const f3 total = fold4(dv);
     if (id.live && id.lane == 0) {
@@ -413,29 +532,35 @@ struct ForcesLds {
     float* t;
 };
 
-__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const uint32_t* __restrict__ starts,
-                                                               const float4* __restrict__ pos,
+__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel,
                                                                const float* __restrict__ temperature,
                                                                const float* __restrict__ prev_pressure, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ Region R;
+    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
-    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_forces, R);
-    __syncthreads();
-    ForcesLds L{smem, smem + b.cap_forces, reinterpret_cast<float*>(smem + 2 * b.cap_forces)};
-    if (R.staged) {
-        for (uint32_t r = 0; r < kRows; ++r) {
-            const uint32_t a = R.start[r], len = R.end[r] - a;
-            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
-            for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
-                const float4 p = pos[a + t];
-                const float4 v = vel[a + t];
-                L.a[base + t] = make_float4(p.x, p.y, p.z, v.x);
-                L.c[base + t] = make_float4(v.y, v.z, b.density[a + t], b.rho_near[a + t]);
-                L.t[base + t] = temperature[a + t];
-            }
-        }
+    const RegionS R = load_region(b.regions, id.tile);
+    load_deltas(b.regions, id.tile, s_delta);
+    const bool staged = R.total <= b.cap_forces;
+    const ForcesLds L{smem, smem + b.cap_forces, reinterpret_cast<float*>(smem + 2 * b.cap_forces)};
+    if (staged) {
+        constexpr uint32_t IT = 2;
+        float4 pp[IT], vv[IT];
+        float dd[IT], nr[IT], tt[IT];
+        stage_flat<kSweepThreads, IT>(
+            R,
+            [&](uint32_t slot, uint32_t it) {
+                pp[it] = pos[slot];
+                vv[it] = vel[slot];
+                dd[it] = b.density[slot];
+                nr[it] = b.rho_near[slot];
+                tt[it] = temperature[slot];
+            },
+            [&](uint32_t idx, uint32_t it) {
+                L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
+                L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
+                L.t[idx] = tt[it];
+            });
     }
     __syncthreads();
     f3 visc = mk(0.0f, 0.0f, 0.0f), blend = visc, nearf = visc, tension = visc, pi = visc;
@@ -446,18 +571,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_
         const f3 vi = xyz(vel[i]);
         const float w_i = b.rho_near[i], temp_i = temperature[i];
         rho_i = b.density[i];
-        const uint32_t nn = b.nbr_n[i];
-        const uint64_t rows = b.nbr_rows[i];
-        uint32_t r = 0, row_end = row_count(rows, 0);
-        const uint2* __restrict__ list = b.nbr + i;
-        for (uint32_t m = id.lane; m < nn; m += kLanes) {
-            seek_row(rows, m, r, row_end);
-            const uint2 e = list[(size_t)m * b.stride];
-            const uint32_t j = e.x;
+        const float rhoi = rho_i;
+        for_list(b, s_delta, i, id.lane, [&](uint32_t j, float gs, int32_t delta) {
             f3 pj, vj;
             float rho_j, near_j, temp_j;
-            if (R.staged) {
-                const uint32_t q = j + R.delta[r];
+            if (staged) {
+                const uint32_t q = j + delta;
                 const float4 a = L.a[q], c = L.c[q];
                 pj = mk(a.x, a.y, a.z);
                 vj = mk(a.w, c.x, c.y);
@@ -474,9 +593,9 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_
             const f3 x = sub(pi, pj);
             const float r2 = dot3(x, x);
             const float rr = sqrtf(r2);
-            const f3 g = scl(__uint_as_float(e.y), x);
+            const f3 g = scl(gs, x);
             const float f = dot3(x, g) / (r2 + k.eps_h2);
-            const float pair = k.mass / (rho_i * rho_j) * f;
+            const float pair = k.mass / (rhoi * rho_j) * f;
             const f3 dv = sub(vi, vj);
             const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
             const float rm = wmax(rr, 1e-5f);
@@ -486,11 +605,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_
             blend = sub(blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
             heat += k.heat_cond * pair * (temp_i - temp_j);
             heat -= k.heat_visc * pair * dot3(dv, dv);
-            const float corr = 2.0f * k.rho0 / (rho_i + rho_j);
+            const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
             const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
             const float rg = wmax(rr, 1e-9f);
             tension = sub(tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
-        }
+        });
     }
     visc = fold4(visc);
     blend = fold4(blend);
@@ -528,24 +647,24 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
 }
 
 __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, StepK s, uint32_t n,
-                                                                    const uint32_t* __restrict__ starts,
                                                                     const float4* __restrict__ pos,
                                                                     float4* __restrict__ vel,
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ Region R;
+    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
-    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa, R);
-    __syncthreads();
-    if (R.staged) stage_pos_kd(R, pos, b.kd, smem);
+    const RegionS R = load_region(b.regions, id.tile);
+    load_deltas(b.regions, id.tile, s_delta);
+    const bool staged = R.total <= b.cap_kappa;
+    if (staged) stage_pos_kd(R, pos, b.kd, smem);
     __syncthreads();
     f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
     float kdi = 0.0f;
     if (id.live) {
         pi = xyz(pos[id.p]);
         kdi = b.kd[id.p];
-        dv = R.staged ? kappa_partial<true>(b, id.p, id.lane, pi, kdi, pos, R, smem)
-                      : kappa_partial<false>(b, id.p, id.lane, pi, kdi, pos, R, smem);
+        dv = staged ? kappa_partial<true>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem)
+                    : kappa_partial<false>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem);
     }
     const f3 total = fold4(dv);
     if (id.live && id.lane == 0) {
@@ -569,29 +688,17 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, St
     }
 }
 
-// den_kappa's gathers: position and velocity, 24 bytes a slot.
 __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(SceneK k, StepK s, uint32_t n,
-                                                             const uint32_t* __restrict__ starts,
                                                              const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ Region R;
+    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
-    if (threadIdx.x == 0) build_region(k, starts, pos, id.s0, id.s1, b.cap_kappa_vel, R);
-    __syncthreads();
-    DensityLds L{smem, reinterpret_cast<float2*>(smem + b.cap_kappa_vel)};
-    if (R.staged) {
-        for (uint32_t r = 0; r < kRows; ++r) {
-            const uint32_t a = R.start[r], len = R.end[r] - a;
-            const uint32_t base = (uint32_t)((int32_t)a + R.delta[r]);
-            for (uint32_t t = threadIdx.x; t < len; t += kSweepThreads) {
-                const float4 p = pos[a + t];
-                const float4 v = vel[a + t];
-                L.pv[base + t] = make_float4(p.x, p.y, p.z, v.x);
-                L.vw[base + t] = make_float2(v.y, v.z);
-            }
-        }
-    }
+    const RegionS R = load_region(b.regions, id.tile);
+    load_deltas(b.regions, id.tile, s_delta);
+    const bool staged = R.total <= b.cap_kappa_vel;
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.cap_kappa_vel)};
+    if (staged) stage_pv<kSweepThreads>(R, pos, vel, L);
     __syncthreads();
     float drho = 0.0f;
     f3 vi = mk(0.0f, 0.0f, 0.0f);
@@ -599,20 +706,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(SceneK k, StepK s, 
         const uint32_t i = id.p;
         const f3 pi = xyz(pos[i]);
         vi = xyz(vel[i]);
-        const uint32_t nn = b.nbr_n[i];
-        const uint64_t rows = b.nbr_rows[i];
-        uint32_t r = 0, row_end = row_count(rows, 0);
-        const uint2* __restrict__ list = b.nbr + i;
-        for (uint32_t m = id.lane; m < nn; m += kLanes) {
-            seek_row(rows, m, r, row_end);
-            const uint2 e = list[(size_t)m * b.stride];
+        const f3 vii = vi;
+        for_list(b, s_delta, i, id.lane, [&](uint32_t j, float g, int32_t delta) {
             f3 pj, vj;
-            if (R.staged)
-                load_pv<true>(L, pos, vel, e.x, R.delta[r], pj, vj);
+            if (staged)
+                load_pv<true>(L, pos, vel, j, delta, pj, vj);
             else
-                load_pv<false>(L, pos, vel, e.x, 0, pj, vj);
-            drho += dot3(sub(vi, vj), listed_grad(pi, pj, __uint_as_float(e.y)));
-        }
+                load_pv<false>(L, pos, vel, j, 0, pj, vj);
+            drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
+        });
     }
     const float total = fold4(drho);
     if (id.live && id.lane == 0) {
@@ -671,33 +773,49 @@ __global__ __launch_bounds__(kBlock) void k_integrate(SceneK k, StepK s, uint32_
         atomicAdd(&b.counters[0], (uint32_t)__popcll(m));
 }
 
-constexpr size_t kDensityEntry = sizeof(float4) + sizeof(float2);      // 24
+constexpr size_t kPvEntry = sizeof(float4) + sizeof(float2);          // 24
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
 constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
 
-uint32_t tiles_for(uint32_t n) { return (n + kTile - 1) / kTile; }
-
 }  // namespace
 
+uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
+
 void configure_solve(SolveBufs& b, size_t lds_bytes) {
-    // Leave room for the static Region block; keep capacities multiples of 64.
-    const size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
+    // Keep capacities multiples of 64 slots; a typical tile region holds about
+    // 9 x 34 cells x 8 slots = 2,448 slots at the configurations' spacing.
     auto cap = [&](size_t entry, size_t want) {
-        size_t c = avail / entry;
+        size_t c = lds_bytes / entry;
         if (c > want) c = want;
         return (uint32_t)(c / 64 * 64);
     };
-    b.cap_density = cap(kDensityEntry, 3072);
+    b.cap_density = cap(kPvEntry, 3072);
     b.cap_kappa = cap(kKappaEntry, 4096);
-    b.cap_kappa_vel = cap(kDensityEntry, 3072);
-    b.cap_forces = cap(kForcesEntry, 2560);
+    b.cap_kappa_vel = cap(kPvEntry, 3072);
+    b.cap_forces = cap(kForcesEntry, 3072);
+}
+
+void enable_large_lds(size_t bytes) {
+    const void* fns[] = {reinterpret_cast<const void*>(&k_density_div), reinterpret_cast<const void*>(&k_kappa_apply),
+                         reinterpret_cast<const void*>(&k_forces_eval),
+                         reinterpret_cast<const void*>(&k_forces_den_apply),
+                         reinterpret_cast<const void*>(&k_den_kappa)};
+    for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+static void launch_regions(hipStream_t st, const SceneK& k, uint32_t n, const uint32_t* starts, const Records& work,
+                           const SolveBufs& b) {
+    const uint32_t tiles = solve_tiles(n);
+    hipLaunchKernelGGL(k_tile_regions, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, k, n, starts, work.pos,
+                       b.regions, tiles);
 }
 
 void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
                          const Records& work, const SolveBufs& b, KernelTimer* timer) {
     if (n == 0) return;
     if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
-    hipLaunchKernelGGL(k_density_div, dim3(tiles_for(n)), dim3(kTile), b.cap_density * kDensityEntry, st, k, s, n,
+    launch_regions(st, k, n, starts, work, b);
+    hipLaunchKernelGGL(k_density_div, dim3(solve_tiles(n)), dim3(kTile), b.cap_density * kPvEntry, st, k, s, n,
                        starts, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DENSITY_DIV);
 }
@@ -706,46 +824,32 @@ void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, c
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t refine,
                   KernelTimer* timer) {
     if (n == 0) return;
-    const dim3 tiles(tiles_for(n)), sweep(kSweepThreads);
+    const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
     const size_t lds_kappa = b.cap_kappa * kKappaEntry;
     launch_density_only(st, k, s, n, starts, work, b, timer);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
-    hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.cap_forces * kForcesEntry, st, k, n, starts, work.pos,
-                       work.vel, work.temperature, work.prev_pressure, b);
+    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.cap_forces * kForcesEntry, st, k, n, work.pos, work.vel,
+                       work.temperature, work.prev_pressure, b);
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
-    hipLaunchKernelGGL(k_forces_den_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel,
+    hipLaunchKernelGGL(k_forces_den_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel,
                        work.temperature, b);
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
     for (uint32_t r = 0; r < refine; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.cap_kappa_vel * kDensityEntry, st, k, s, n, starts,
-                           work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.cap_kappa_vel * kPvEntry, st, k, s, n, work.pos, work.vel,
+                           b);
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
-        hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, starts, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_APPLY);
     }
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
     hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n)), dim3(kBlock), 0, st, k, s, n, work, rest, b);
     if (timer) timer->end(st, KERNEL_INTEGRATE);
-}
-
-// Lets the sweeps take LDS beyond the default per-launch limit.
-void enable_large_lds(size_t bytes) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_density_div),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_kappa_apply),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forces_eval),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_forces_den_apply),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_den_kappa),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
 }  // namespace fluid

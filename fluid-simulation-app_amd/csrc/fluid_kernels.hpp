@@ -78,10 +78,9 @@ struct SolveBufs {
 // Tiles of the solve (consecutive slots a workgroup owns).
 uint32_t solve_tiles(uint32_t n);
 
-// Sets the LDS capacities for this device's per-workgroup LDS limit.
-void configure_solve(SolveBufs& b, size_t lds_bytes);
-// Raises the sweeps' dynamic-LDS limit to `bytes` where the device allows it.
-void enable_large_lds(size_t bytes);
+// Sets the sweeps' LDS capacities for the device's per-workgroup LDS (and
+// raises their dynamic-LDS limit); returns the dynamic LDS budget used.
+size_t configure_solve(SolveBufs& b, size_t lds_bytes);
 
 enum KernelId {
     KERNEL_GRID_COUNT = 0,

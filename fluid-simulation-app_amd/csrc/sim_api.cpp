@@ -310,8 +310,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
         const long v = strtol(env, nullptr, 10);
         if (v > 0 && (size_t)v < s->lds_bytes) s->lds_bytes = (size_t)v;
     }
-    enable_large_lds(s->lds_bytes);
-    configure_solve(b, s->lds_bytes);
+    s->lds_bytes = configure_solve(b, s->lds_bytes);
     s->stats.blocks = stats_blocks(s->n);
     A(s->stats.partials, (size_t)s->stats.blocks * 16);
     A(s->stats.stats, 16);

@@ -781,11 +781,24 @@ constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
 
 uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 
-void configure_solve(SolveBufs& b, size_t lds_bytes) {
+// Sizes the sweeps' LDS staging areas from the device's per-workgroup LDS
+// (less 1 KB for the kernels' static LDS) and raises the kernels' dynamic-LDS
+// limit to that size. If the device refuses the larger limit, the areas fit
+// the default 64 KB.
+size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
+    size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
+    const void* fns[] = {reinterpret_cast<const void*>(&k_density_div), reinterpret_cast<const void*>(&k_kappa_apply),
+                         reinterpret_cast<const void*>(&k_forces_eval),
+                         reinterpret_cast<const void*>(&k_forces_den_apply),
+                         reinterpret_cast<const void*>(&k_den_kappa)};
+    for (const void* f : fns)
+        if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
+            avail = 65536 - 1024;
+    (void)hipGetLastError();
     // Keep capacities multiples of 64 slots; a typical tile region holds about
     // 9 x 34 cells x 8 slots = 2,448 slots at the configurations' spacing.
     auto cap = [&](size_t entry, size_t want) {
-        size_t c = lds_bytes / entry;
+        size_t c = avail / entry;
         if (c > want) c = want;
         return (uint32_t)(c / 64 * 64);
     };
@@ -793,14 +806,7 @@ void configure_solve(SolveBufs& b, size_t lds_bytes) {
     b.cap_kappa = cap(kKappaEntry, 4096);
     b.cap_kappa_vel = cap(kPvEntry, 3072);
     b.cap_forces = cap(kForcesEntry, 3072);
-}
-
-void enable_large_lds(size_t bytes) {
-    const void* fns[] = {reinterpret_cast<const void*>(&k_density_div), reinterpret_cast<const void*>(&k_kappa_apply),
-                         reinterpret_cast<const void*>(&k_forces_eval),
-                         reinterpret_cast<const void*>(&k_forces_den_apply),
-                         reinterpret_cast<const void*>(&k_den_kappa)};
-    for (const void* f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    return avail;
 }
 
 static void launch_regions(hipStream_t st, const SceneK& k, uint32_t n, const uint32_t* starts, const Records& work,

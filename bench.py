@@ -72,6 +72,26 @@ def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=8):
                       f"substep, {dt:.1f} s"}
 
 
+def copy_calibration(nbytes: int, reps: int = 4) -> None:
+    """Runs `reps` device-to-device copies of `nbytes` (HIP's own copy kernel)
+    so that a counter run can read what FETCH_SIZE / WRITE_SIZE report for a
+    plain copy of known size; the ratio scales the solver kernels' counts."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(nbytes))
+    for _ in range(reps):
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,11 +100,16 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--workload", default="block_1m", choices=["block_1m", "block_8m", "dam_break_16k"])
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device copies of BYTES (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
               "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import fluid_amd as fa
     if fa.device_count() < 1:
@@ -100,7 +125,7 @@ def main():
     if ntr:
         sim.upload_tracers(sc.tracers)
 
-    total = args.warmup + args.steps
+    total = args.warmup + max(args.steps, 3)  # the kernel profile below takes 3 substeps
     steps = [sc.step_block(k * dt_sub, dt_sub, v_clamp) for k in range(total)]
     frame_blocks = [fa.pack_step(force=sc.gravity(k * dt_sub), dt=dt_sub * n_sub, seed=k) for k in range(total)]
 

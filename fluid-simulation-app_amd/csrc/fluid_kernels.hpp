@@ -41,12 +41,14 @@ struct GridBufs {
 // The nine stencil-row slot ranges holding every neighbour of one tile of
 // consecutive slots, and their offsets in the tile's LDS staging area: a
 // neighbour slot j found in stencil row r sits at LDS index j + delta[r]. Built
-// once per substep after the grid build.
+// once per substep after the grid build. A tile whose ranges fit the staging
+// capacity is staged in every sweep, and its neighbour list then holds LDS
+// indices; otherwise the list holds slots and the sweeps gather globally.
 struct TileRegion {
     int32_t delta[9];
     uint32_t base[9];  // LDS offset of row r's first slot (prefix of the lengths)
-    uint32_t total;    // staged slots
-    uint32_t pad;
+    uint32_t total;    // slots in the nine ranges
+    uint32_t staged;   // total <= the staging capacity
 };
 
 // Per-substep scratch of the solver, all in slot (cell) order.
@@ -59,26 +61,29 @@ struct SolveBufs {
     float4* accel;    // xyz; w = dT/dt
     float4* xsph;
     float4* wall_grad;
-    // [cap][stride] per pair: (neighbour slot, bits of dW/r), neighbour-major,
-    // so a wave's reads of entry m are one contiguous 512-byte line.
+    // Per pair (neighbour, bits of dW/r), quad-major: entry m of particle p at
+    // ((m / 4) * stride + p) * 4 + m % 4, so the four list lanes of a particle
+    // read one 32-byte quad and a wave's 64 lanes one 512-byte line. The
+    // neighbour is its LDS index when the particle's tile is staged, else its
+    // slot.
     uint2* nbr;
     uint32_t* nbr_n;  // [N]
-    // [N] per-row counts of each particle's list, 7 bits for each of the 9
-    // stencil rows (dz, dy) in walk order: which LDS range a listed slot is in.
-    uint64_t* nbr_rows;
-    uint32_t cap;
-    uint32_t stride;
+    uint32_t cap;     // a multiple of 4
+    uint32_t stride;  // >= the tiles' slot count
     uint32_t* counters;  // [0] CFL clamps, [1] neighbour overflow (cumulative)
-    // LDS capacities (neighbour-region entries) of the staged sweeps; a tile
-    // whose region is larger gathers from global memory instead.
-    uint32_t cap_density, cap_kappa, cap_kappa_vel, cap_forces;
+    // Staging capacity (slots) shared by every sweep.
+    uint32_t stage_cap;
     TileRegion* regions;  // [tiles]
 };
 
 // Tiles of the solve (consecutive slots a workgroup owns).
 uint32_t solve_tiles(uint32_t n);
+// List lanes a particle in the sweeps (entry m belongs to lane m % 4).
+constexpr uint32_t kListLanes = 4;
+// Slots a solve tile holds.
+constexpr uint32_t kSolveTile = 256;
 
-// Sets the sweeps' LDS capacities for the device's per-workgroup LDS (and
+// Sets the sweeps' staging capacity for the device's per-workgroup LDS (and
 // raises their dynamic-LDS limit); returns the dynamic LDS budget used.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes);
 

@@ -256,9 +256,10 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     s->n = p.count;
     s->cells = (uint32_t)cells64;
     s->cap = nbr_cap ? nbr_cap : FLUID_SIM_NBR_CAP;
-    if (s->cap > 127) {
-        // Each stencil row's share of the list is counted in 7 bits.
-        set_error("neighbour cap %u exceeds 127", s->cap);
+    if (s->cap % kListLanes != 0 || s->cap > 256) {
+        // The list is laid out in quads, one entry for each of a particle's
+        // four lanes.
+        set_error("neighbour cap %u is not a multiple of 4 in [4, 256]", s->cap);
         return FLUID_ERR_ARG;
     }
     s->ntr = ntr;
@@ -295,12 +296,10 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.xsph, n);
     A(b.wall_grad, n);
     b.cap = s->cap;
-    b.stride = (uint32_t)((n + 63) / 64 * 64);
-    if (b.stride == 0) b.stride = 64;
+    b.stride = std::max<uint32_t>(solve_tiles(s->n), 1) * kSolveTile;
     if ((rc = s->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
-    A(b.nbr_rows, n);
-    A(b.regions, solve_tiles(s->n));
+    A(b.regions, std::max<uint32_t>(solve_tiles(s->n), 1));
     A(b.counters, 4);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
@@ -454,12 +453,29 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     if (!valid(s) || !nbr) return FLUID_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(s->stream));
     const size_t n = s->n, cap = s->cap, stride = s->solve.stride;
+    if (n == 0) return FLUID_OK;
     std::vector<uint2> raw(cap * stride);
-    std::vector<uint32_t> cnt(std::max<size_t>(n, 1));
+    std::vector<uint32_t> cnt(n);
+    std::vector<TileRegion> reg(solve_tiles(s->n));
     HIP_TRY(hipMemcpy(raw.data(), s->solve.nbr, raw.size() * sizeof(uint2), hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < n; ++i)
-        for (size_t m = 0; m < cap; ++m) nbr[i * cap + m] = m < cnt[i] ? raw[m * stride + i].x : 0u;
+    HIP_TRY(hipMemcpy(reg.data(), s->solve.regions, reg.size() * sizeof(TileRegion), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) {
+        const TileRegion& R = reg[i / kSolveTile];
+        for (size_t m = 0; m < cap; ++m) {
+            uint32_t j = 0;
+            if (m < cnt[i]) {
+                j = raw[((m / kListLanes) * stride + i) * kListLanes + m % kListLanes].x;
+                if (R.staged) {
+                    // An LDS index: back to the slot through its stencil row.
+                    int r = 8;
+                    while (r > 0 && (j < R.base[r] || R.base[r] == (r < 8 ? R.base[r + 1] : R.total))) --r;
+                    j = (uint32_t)((int32_t)j - R.delta[r]);
+                }
+            }
+            nbr[i * cap + m] = j;
+        }
+    }
     return FLUID_OK;
 }
 
@@ -602,10 +618,9 @@ int fluid_sim_describe(const FluidSim* s, char* buf, size_t size) {
     hipDeviceProp_t prop;
     const char* name = hipGetDeviceProperties(&prop, s->device) == hipSuccess ? prop.gcnArchName : "?";
     snprintf(buf, size,
-             "{\"arch\": \"%s\", \"lds_bytes\": %zu, \"nbr_cap\": %u, \"lds_capacity\": {\"density\": %u, "
-             "\"kappa\": %u, \"den_kappa\": %u, \"forces\": %u}}",
-             name, s->lds_bytes, s->cap, s->solve.cap_density, s->solve.cap_kappa, s->solve.cap_kappa_vel,
-             s->solve.cap_forces);
+             "{\"arch\": \"%s\", \"lds_bytes\": %zu, \"nbr_cap\": %u, \"stage_cap\": %u, "
+             "\"list_lanes\": %u, \"tile\": %u}",
+             name, s->lds_bytes, s->cap, s->solve.stage_cap, kListLanes, kSolveTile);
     return FLUID_OK;
 }
 

@@ -12,28 +12,27 @@
 //   integrate          709-756
 //
 // Layout: every array is in slot (cell) order after the grid build. A
-// workgroup owns a tile of kTile consecutive slots. All neighbours of a
-// tile lie in nine slot ranges, one per stencil row (dy, dz): in cell-id
-// space, row (dy, dz) of every particle of the tile falls in
-// [first cell + off(dy, dz) - 1, last cell + off(dy, dz) + 1]. k_tile_regions
-// records those ranges once per substep; each sweep then stages the fields it
-// gathers for them into LDS with independent coalesced loads, and reads
-// neighbours from LDS. A tile whose ranges exceed the LDS capacity gathers
-// from global memory; the values read, and so the results, are the same.
+// workgroup owns a tile of kTile consecutive slots, kLanes threads a particle
+// (kTile * kLanes = 1024 threads). All neighbours of a tile lie in nine slot
+// ranges, one per stencil row (dy, dz): in cell-id space, row (dy, dz) of
+// every particle of the tile falls in [first cell + off(dy, dz) - 1, last
+// cell + off(dy, dz) + 1]. k_tile_regions records those ranges once per
+// substep; when they fit the staging capacity, every sweep copies the fields
+// it gathers for them into LDS with independent coalesced loads, and the
+// neighbour list of the tile's particles holds LDS indices. Otherwise the
+// list holds slots and the sweeps gather from global memory; the values read,
+// and so the results, are the same.
 //
-// density_div: one thread a particle. It walks the stencil rows in the
-// reference's cell order (x fastest, then y, then z; in-cell slot order) —
-// the reference's LANES = 1 reading — testing distances only and appending
-// accepted slots to the neighbour list with per-row counts; then it runs the
-// per-pair math over that compact list. The list holds, per pair, the slot
-// and the scalar dW/r (neighbour-major, so entry m of consecutive particles is
-// contiguous); the later sweeps rebuild each gradient as dW/r times x_i - x_j,
-// the product grad_q itself forms, so it is bit-identical.
-//
-// The list sweeps run kLanes lanes a particle, as the reference runs 8: lane l
-// takes list entries l, l + kLanes, ..., and the partials fold with the
-// reference's xor butterfly (lane_fold, sim_solve.wgsl:398-402). Each lane
-// loads its first kPrefetch entries up front, so their latencies overlap.
+// density_div walks the stencil rows in the reference's cell order (x
+// fastest, then y, then z; in-cell slot order) with the particle's four lanes
+// testing consecutive candidates; a ballot over the four orders the accepted
+// ones, so the list is in walk order. The list sweeps, density_div's pair sums
+// included, then split the list over the lanes — lane l takes entries l, l +
+// 4, ... — and fold the partials with the reference's xor butterfly
+// (lane_fold, sim_solve.wgsl:398-402) over four lanes. The list holds, per
+// pair, the neighbour and the scalar dW/r; the later sweeps rebuild each
+// gradient as dW/r times x_i - x_j, the product grad_q itself forms, so it is
+// bit-identical.
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 
@@ -41,11 +40,11 @@ namespace fluid {
 
 namespace {
 
-constexpr uint32_t kTile = 256;                      // particles a workgroup
-constexpr uint32_t kLanes = 4;                       // lanes a particle in the list sweeps
+constexpr uint32_t kTile = kSolveTile;              // particles a workgroup
+constexpr uint32_t kLanes = kListLanes;             // lanes a particle
 constexpr uint32_t kSweepThreads = kTile * kLanes;  // 1024
 constexpr uint32_t kRows = 9;
-constexpr uint32_t kPrefetch = 8;                    // list entries a lane loads ahead
+constexpr uint32_t kPrefetch = 4;                   // list quads a lane loads ahead
 
 __device__ __forceinline__ uint32_t tile_index() { return xcd_tile(blockIdx.x, gridDim.x); }
 
@@ -59,7 +58,7 @@ __device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
 // One thread a tile: the tile's nine row ranges.
 __global__ __launch_bounds__(kBlock) void k_tile_regions(SceneK k, uint32_t n, const uint32_t* __restrict__ starts,
                                                          const float4* __restrict__ pos, TileRegion* __restrict__ out,
-                                                         uint32_t tiles) {
+                                                         uint32_t tiles, uint32_t stage_cap) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= tiles) return;
     const uint32_t s0 = t * kTile;
@@ -87,16 +86,16 @@ __global__ __launch_bounds__(kBlock) void k_tile_regions(SceneK k, uint32_t n, c
         total += e - a;
     }
     R.total = total;
-    R.pad = 0;
+    R.staged = total <= stage_cap ? 1u : 0u;
     out[t] = R;
 }
 
-// A tile's region as uniform scalars (SGPRs): the staging loop selects the
-// row of each flat LDS index among them.
+// A tile's region as uniform scalars (SGPRs).
 struct RegionS {
     int32_t d0, d1, d2, d3, d4, d5, d6, d7, d8;
     uint32_t b1, b2, b3, b4, b5, b6, b7, b8;
     uint32_t total;
+    bool staged;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
@@ -122,14 +121,14 @@ __device__ __forceinline__ RegionS load_region(const TileRegion* __restrict__ re
     s.b7 = uni(R->base[7]);
     s.b8 = uni(R->base[8]);
     s.total = uni(R->total);
+    s.staged = uni(R->staged) != 0u;
     return s;
 }
 
 // The delta of the region row holding flat LDS index idx: that of the last
 // row whose base <= idx (empty rows share the next row's base, so the last one
-// is the non-empty one).
-// Accumulated as d0 plus the step at each row boundary passed (a chain of
-// adds, which keeps every value in registers).
+// is the non-empty one). Accumulated as d0 plus the step at each row boundary
+// passed, a chain of adds that stays in registers.
 __device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) {
     int32_t d = R.d0;
     d += (int32_t)(idx >= R.b1) * (R.d1 - R.d0);
@@ -142,31 +141,6 @@ __device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) 
     d += (int32_t)(idx >= R.b8) * (R.d8 - R.d7);
     return d;
 }
-
-// Every sweep keeps the nine row deltas in LDS for its lanes' row lookups.
-__device__ __forceinline__ void load_deltas(const TileRegion* __restrict__ regions, uint32_t tile, int32_t* s_delta) {
-    if (threadIdx.x < kRows) s_delta[threadIdx.x] = regions[tile].delta[threadIdx.x];
-}
-
-__device__ __forceinline__ uint32_t row_count(uint64_t rows, uint32_t r) { return (uint32_t)(rows >> (7u * r)) & 127u; }
-
-// Advances (r, row_end) until entry m lies in row r.
-__device__ __forceinline__ void seek_row(uint64_t rows, uint32_t m, uint32_t& r, uint32_t& row_end) {
-    while (m >= row_end && r + 1 < kRows) {
-        ++r;
-        row_end += row_count(rows, r);
-    }
-}
-
-// lane_fold over kLanes = 4 consecutive lanes: xor 2, then xor 1 (lane 0
-// holds (p0 + p2) + (p1 + p3)).
-__device__ __forceinline__ float fold4(float v) {
-    const float t = v + __shfl_xor(v, 2);
-    return t + __shfl_xor(t, 1);
-}
-__device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
-
-__device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
 
 // Copies `total` staged slots into LDS: NT threads, ITERS slots each, all
 // loads issued before the stores. `load(slot, it)` fetches into registers,
@@ -184,6 +158,68 @@ __device__ __forceinline__ void stage_flat(const RegionS& R, Load&& load, Store&
             const uint32_t idx = chunk + it * NT + threadIdx.x;
             if (idx < R.total) store(idx, it);
         }
+    }
+}
+
+// lane_fold over the kLanes = 4 lanes of a quad: xor 2, then xor 1 (every
+// lane ends with (p0 + p2) + (p1 + p3)). DPP quad permutations, no LDS.
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float fold4(float v) {
+    const float t = v + quad_perm<0x4E>(v);  // [2, 3, 0, 1]: lane ^ 2
+    return t + quad_perm<0xB1>(t);           // [1, 0, 3, 2]: lane ^ 1
+}
+__device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
+
+__device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
+
+struct LaneId {
+    uint32_t tile, p, lane, gid;
+    bool live;
+};
+
+__device__ __forceinline__ LaneId lane_id(uint32_t n) {
+    LaneId id;
+    id.tile = tile_index();
+    id.gid = id.tile * kSweepThreads + threadIdx.x;  // = p * kLanes + lane
+    id.p = id.tile * kTile + threadIdx.x / kLanes;
+    id.lane = threadIdx.x % kLanes;
+    id.live = id.p < n;
+    return id;
+}
+
+// Entry m of particle p.
+__device__ __forceinline__ size_t list_entry(uint32_t stride, uint32_t p, uint32_t m) {
+    return ((size_t)(m / kLanes) * stride + p) * kLanes + m % kLanes;
+}
+
+// Lane `lane` of particle p walks its share of the list in order (entries
+// lane, lane + 4, ...: quad t of the list holds entry 4t + lane at quad
+// offset gid), calling f(neighbour, dW/r). The quad's address is uniform
+// across the wave; kPrefetch quads are loaded before they are used.
+template <class F>
+__device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, F&& f) {
+    const uint32_t nn = b.nbr_n[id.p];
+    const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
+    const size_t step = (size_t)b.stride * kLanes;
+    uint32_t t = 0;
+    for (; t + kPrefetch <= cnt; t += kPrefetch) {
+        uint2 e[kPrefetch];
+#pragma unroll
+        for (uint32_t u = 0; u < kPrefetch; ++u) e[u] = (b.nbr + (size_t)(t + u) * step)[id.gid];
+#pragma unroll
+        for (uint32_t u = 0; u < kPrefetch; ++u) f(e[u].x, __uint_as_float(e[u].y));
+    }
+    if (t < cnt) {
+        uint2 e[kPrefetch - 1];
+#pragma unroll
+        for (uint32_t u = 0; u < kPrefetch - 1; ++u)
+            e[u] = t + u < cnt ? (b.nbr + (size_t)(t + u) * step)[id.gid] : make_uint2(0u, 0u);
+#pragma unroll
+        for (uint32_t u = 0; u < kPrefetch - 1; ++u)
+            if (t + u < cnt) f(e[u].x, __uint_as_float(e[u].y));
     }
 }
 
@@ -242,10 +278,10 @@ struct PvLds {
 
 template <bool STAGED>
 __device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict__ pos,
-                                        const float4* __restrict__ vel, uint32_t j, int32_t delta, f3& p, f3& v) {
+                                        const float4* __restrict__ vel, uint32_t j, f3& p, f3& v) {
     if (STAGED) {
-        const float4 a = L.pv[j + delta];
-        const float2 b = L.vw[j + delta];
+        const float4 a = L.pv[j];
+        const float2 b = L.vw[j];
         p = mk(a.x, a.y, a.z);
         v = mk(a.w, b.x, b.y);
     } else {
@@ -257,7 +293,7 @@ __device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict
 template <uint32_t NT>
 __device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restrict__ pos,
                                          const float4* __restrict__ vel, const PvLds& L) {
-    constexpr uint32_t IT = 4;
+    constexpr uint32_t IT = 3;
     float4 pp[IT], vv[IT];
     stage_flat<NT, IT>(
         R,
@@ -271,22 +307,20 @@ __device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restr
         });
 }
 
+// Phase 1 of density_div: the four lanes of particle i test consecutive
+// candidates of each stencil row; a ballot orders the accepted ones, and each
+// is appended (as its LDS index when staged, else its slot) at its place in
+// walk order. Returns the neighbour count, past the cap included.
 template <bool STAGED>
-__device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const uint32_t* __restrict__ starts,
-                             const float4* __restrict__ pos, const float4* __restrict__ vel, const SolveBufs& b,
-                             const int32_t* deltas, const PvLds& L) {
-    const f3 pi = xyz(pos[i]);
-    const f3 vi = xyz(vel[i]);
-    const uint32_t bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
-    const uint32_t by = cell_coord(pi.y, k.box_min[1], k.cell, k.dims[1]);
-    const uint32_t bz = cell_coord(pi.z, k.box_min[2], k.cell, k.dims[2]);
+__device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& id, f3 pi, uint32_t bx, uint32_t by,
+                                                 uint32_t bz, const uint32_t* __restrict__ starts,
+                                                 const float4* __restrict__ pos, const PvLds& L,
+                                                 const RegionS& R, const SolveBufs& b) {
     const Reach st = make_reach(k, pi, bx, by, bz);
-    uint2* __restrict__ list = b.nbr + i;
-    const uint32_t stride = b.stride, cap = b.cap;
-    // Phase 1: distance tests only, appending accepted slots and row counts.
+    const uint32_t shift = __lane_id() & ~(kLanes - 1);
+    const uint32_t below = (1u << id.lane) - 1u;
     uint32_t found = 0;
-    uint64_t rows = 0;
-#pragma unroll 1
+#pragma unroll
     for (uint32_t r = 0; r < kRows; ++r) {
         const int dz = (int)(r / 3) - 1, dy = (int)(r % 3) - 1;
         const int cz = (int)bz + dz, cy = (int)by + dy;
@@ -299,56 +333,63 @@ __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const 
         const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
         const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
         const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
-        const int32_t delta = deltas[r];
-        uint32_t in_row = 0;
-#pragma unroll 4
-        for (uint32_t kk = k0; kk < k1; ++kk) {
-            f3 pj;
-            if (STAGED) {
-                const float4 a = L.pv[kk + delta];
-                pj = mk(a.x, a.y, a.z);
-            } else {
-                pj = xyz(pos[kk]);
-            }
-            const f3 x = sub(pi, pj);
-            const float r2 = dot3(x, x);
-            if (r2 < k.support_sq && kk != i) {
-                if (found < cap) {
-                    list[(size_t)found * stride].x = kk;
-                    in_row++;
+        const int32_t delta = !STAGED ? 0
+                              : r == 0 ? R.d0
+                              : r == 1 ? R.d1
+                              : r == 2 ? R.d2
+                              : r == 3 ? R.d3
+                              : r == 4 ? R.d4
+                              : r == 5 ? R.d5
+                              : r == 6 ? R.d6
+                              : r == 7 ? R.d7
+                                       : R.d8;
+        for (uint32_t kq = k0; kq < k1; kq += kLanes) {
+            const uint32_t kk = kq + id.lane;
+            bool acc = false;
+            const uint32_t j = (uint32_t)((int32_t)kk + delta);
+            if (kk < k1) {
+                f3 pj;
+                if (STAGED) {
+                    const float4 a = L.pv[j];
+                    pj = mk(a.x, a.y, a.z);
+                } else {
+                    pj = xyz(pos[kk]);
                 }
-                found++;
+                const f3 x = sub(pi, pj);
+                acc = dot3(x, x) < k.support_sq && kk != id.p;
             }
-        }
-        rows |= (uint64_t)in_row << (7u * r);
-    }
-    // Phase 2: the sums over the compact list, in list order, in chunks of
-    // kPrefetch entries whose slots are read back together.
-    DensitySums a{k.sigma, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
-    const uint32_t nn = found < cap ? found : cap;
-    uint32_t r = 0, row_end = row_count(rows, 0);
-    for (uint32_t m0 = 0; m0 < nn; m0 += kPrefetch) {
-        uint32_t js[kPrefetch];
-#pragma unroll
-        for (uint32_t t = 0; t < kPrefetch; ++t)
-            js[t] = m0 + t < nn ? list[(size_t)(m0 + t) * stride].x : 0u;
-#pragma unroll
-        for (uint32_t t = 0; t < kPrefetch; ++t) {
-            const uint32_t m = m0 + t;
-            if (m < nn) {
-                seek_row(rows, m, r, row_end);
-                f3 pj, vj;
-                load_pv<STAGED>(L, pos, vel, js[t], deltas[r], pj, vj);
-                const float g = density_pair(k, pi, vi, pj, vj, a);
-                list[(size_t)m * stride].y = __float_as_uint(g);
+            const uint32_t quad = (uint32_t)(__ballot(acc) >> shift) & 0xFu;
+            if (acc) {
+                const uint32_t m = found + __popc(quad & below);
+                if (m < b.cap) b.nbr[list_entry(b.stride, id.p, m)].x = j;
             }
+            found += __popc(quad);
         }
     }
-    if (found > cap) {
-        // Past the cap the list keeps its first `cap` neighbours (the later
-        // sweeps see only those, as in the reference) but the density sums
-        // take every neighbour: walk again for the ones the list dropped.
-        atomicAdd(&b.counters[1], found - cap);
+    return found;
+}
+
+// Phase 2: lane l's share of the pair sums over the compact list (entries l,
+// l + 4, ...), storing each pair's dW/r; past the cap, the neighbours the
+// list dropped are found again by a global walk and summed by the same rule.
+template <bool STAGED>
+__device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, f3 pi, f3 vi, uint32_t found,
+                                             uint32_t bx, uint32_t by, uint32_t bz,
+                                             const uint32_t* __restrict__ starts, const float4* __restrict__ pos,
+                                             const float4* __restrict__ vel, const PvLds& L, const SolveBufs& b,
+                                             DensitySums& a) {
+    const uint32_t nn = found < b.cap ? found : b.cap;
+    const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
+    const size_t step = (size_t)b.stride * kLanes;
+    for (uint32_t t = 0; t < cnt; ++t) {
+        uint2* e = b.nbr + (size_t)t * step + id.gid;
+        const uint32_t j = e->x;
+        f3 pj, vj;
+        load_pv<STAGED>(L, pos, vel, j, pj, vj);
+        const float g = density_pair(k, pi, vi, pj, vj, a);
+        *e = make_uint2(j, __float_as_uint(g));
+    }
+    if (found > b.cap) {
         uint32_t seen = 0;
         for (uint32_t rr = 0; rr < kRows; ++rr) {
             const int dz = (int)(rr / 3) - 1, dy = (int)(rr % 3) - 1;
@@ -361,19 +402,50 @@ __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const 
             for (uint32_t kk = k0; kk < k1; ++kk) {
                 const f3 pj = xyz(pos[kk]);
                 const f3 x = sub(pi, pj);
-                if (dot3(x, x) < k.support_sq && kk != i) {
-                    if (seen >= cap) density_pair(k, pi, vi, pj, xyz(vel[kk]), a);
+                if (dot3(x, x) < k.support_sq && kk != id.p) {
+                    if (seen >= b.cap && seen % kLanes == id.lane) density_pair(k, pi, vi, pj, xyz(vel[kk]), a);
                     seen++;
                 }
             }
         }
     }
+}
+
+template <bool STAGED>
+__device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t n, const LaneId& id,
+                                             const RegionS& R, const uint32_t* __restrict__ starts,
+                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                             const SolveBufs& b, const PvLds& L) {
+    if (STAGED) {
+        stage_pv<kSweepThreads>(R, pos, vel, L);
+        __syncthreads();
+    }
+    f3 pi = mk(0.0f, 0.0f, 0.0f), vi = pi;
+    uint32_t bx = 0, by = 0, bz = 0, found = 0;
+    if (id.live) {
+        pi = xyz(pos[id.p]);
+        vi = xyz(vel[id.p]);
+        bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
+        by = cell_coord(pi.y, k.box_min[1], k.cell, k.dims[1]);
+        bz = cell_coord(pi.z, k.box_min[2], k.cell, k.dims[2]);
+        found = density_walk<STAGED>(k, id, pi, bx, by, bz, starts, pos, L, R, b);
+    }
+    // Phase 1's list entries, written by any lane of the quad, are read back
+    // by lane m % 4 below.
+    __syncthreads();
+    DensitySums a{id.lane == 0 ? k.sigma : 0.0f, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
+    if (id.live) density_sums<STAGED>(k, id, pi, vi, found, bx, by, bz, starts, pos, vel, L, b, a);
+    const float w_sum = fold4(a.w_sum), rn = fold4(a.rn), gsq_sum = fold4(a.gsq), drho_sum = fold4(a.drho);
+    const f3 gsum_sum = fold4(a.gsum);
+    if (!id.live || id.lane != 0) return;
+    const uint32_t i = id.p;
+    const uint32_t nn = found < b.cap ? found : b.cap;
     b.nbr_n[i] = nn;
-    b.nbr_rows[i] = rows;
-    float rho = k.mass * a.w_sum;
-    const float gsq = k.mass * k.mass * a.gsq;
-    float drho = k.mass * a.drho;
-    f3 gsum = scl(k.mass, a.gsum);
+    if (found > b.cap) atomicAdd(&b.counters[1], found - b.cap);
+    float rho = k.mass * w_sum;
+    const float gsq = k.mass * k.mass * gsq_sum;
+    float drho = k.mass * drho_sum;
+    f3 gsum = scl(k.mass, gsum_sum);
     const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h, (pi.z - k.wall_lo[2]) * k.inv_h);
     const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h, (k.wall_hi[2] - pi.z) * k.inv_h);
     float fill = 0.0f;
@@ -386,83 +458,34 @@ __device__ void density_body(const SceneK& k, const StepK& s, uint32_t i, const 
     b.wall_grad[i] = make_float4(wg.x, wg.y, wg.z, 0.0f);
     gsum = add(gsum, wg);
     b.density[i] = rho;
-    b.rho_near[i] = a.rn;
+    b.rho_near[i] = rn;
     const float al = rho / wmax(dot3(gsum, gsum) + gsq, 1e-4f);
     b.alpha[i] = al;
     drho += dot3(vi, wg);
     b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
 }
 
-__global__ __launch_bounds__(kTile) void k_density_div(SceneK k, StepK s, uint32_t n, const uint32_t* __restrict__ starts,
-                                                       const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                                       SolveBufs b) {
+__global__ __launch_bounds__(kSweepThreads) void k_density_div(SceneK k, StepK s, uint32_t n,
+                                                               const uint32_t* __restrict__ starts,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ int32_t s_delta[kRows];
-    const uint32_t tile = tile_index();
-    const RegionS R = load_region(b.regions, tile);
-    const bool staged = R.total <= b.cap_density;
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.cap_density)};
-    load_deltas(b.regions, tile, s_delta);
-    if (staged) stage_pv<kTile>(R, pos, vel, L);
-    __syncthreads();
-    const uint32_t i = tile * kTile + threadIdx.x;
-    if (i >= n) return;
-    if (staged)
-        density_body<true>(k, s, i, starts, pos, vel, b, s_delta, L);
+    const LaneId id = lane_id(n);
+    const RegionS R = load_region(b.regions, id.tile);
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
+    if (R.staged)
+        density_tile<true>(k, s, n, id, R, starts, pos, vel, b, L);
     else
-        density_body<false>(k, s, i, starts, pos, vel, b, s_delta, L);
+        density_tile<false>(k, s, n, id, R, starts, pos, vel, b, L);
 }
 
 // ---- list sweeps -------------------------------------------------------------
-
-struct LaneId {
-    uint32_t tile, p, lane;
-    bool live;
-};
-
-__device__ __forceinline__ LaneId lane_id(uint32_t n) {
-    LaneId id;
-    id.tile = tile_index();
-    id.p = id.tile * kTile + threadIdx.x / kLanes;
-    id.lane = threadIdx.x % kLanes;
-    id.live = id.p < n;
-    return id;
-}
-
-// Walks lane `lane`'s share of particle p's list in order, calling
-// f(slot, dW/r, delta of the slot's row) for each entry.
-template <class F>
-__device__ __forceinline__ void for_list(const SolveBufs& b, const int32_t* deltas, uint32_t p, uint32_t lane, F&& f) {
-    const uint32_t nn = b.nbr_n[p];
-    const uint64_t rows = b.nbr_rows[p];
-    const uint2* __restrict__ list = b.nbr + p;
-    uint32_t r = 0, row_end = row_count(rows, 0);
-    uint2 e[kPrefetch];
-#pragma unroll
-    for (uint32_t t = 0; t < kPrefetch; ++t) {
-        const uint32_t m = lane + t * kLanes;
-        e[t] = m < nn ? list[(size_t)m * b.stride] : make_uint2(0u, 0u);
-    }
-#pragma unroll
-    for (uint32_t t = 0; t < kPrefetch; ++t) {
-        const uint32_t m = lane + t * kLanes;
-        if (m < nn) {
-            seek_row(rows, m, r, row_end);
-            f(e[t].x, __uint_as_float(e[t].y), deltas[r]);
-        }
-    }
-    for (uint32_t m = lane + kPrefetch * kLanes; m < nn; m += kLanes) {
-        seek_row(rows, m, r, row_end);
-        const uint2 q = list[(size_t)m * b.stride];
-        f(q.x, __uint_as_float(q.y), deltas[r]);
-    }
-}
 
 // Stages position and kd (the gathers of div_apply, den_apply and
 // forces_den_apply) as one float4 per slot.
 __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __restrict__ pos,
                                              const float* __restrict__ kd, float4* lds) {
-    constexpr uint32_t IT = 4;
+    constexpr uint32_t IT = 3;
     float4 pp[IT];
     float kk[IT];
     stage_flat<kSweepThreads, IT>(
@@ -477,13 +500,13 @@ __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __r
 // Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
 // its share of the list (kappa_partial, sim_solve.wgsl:596-606).
 template <bool STAGED>
-__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const int32_t* deltas, uint32_t p, uint32_t lane,
-                                            f3 pi, float kdi, const float4* __restrict__ pos, const float4* lds) {
+__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, f3 pi, float kdi,
+                                            const float4* __restrict__ pos, const float4* lds) {
     f3 dv = mk(0.0f, 0.0f, 0.0f);
-    for_list(b, deltas, p, lane, [&](uint32_t j, float g, int32_t delta) {
+    for_list(b, id, [&](uint32_t j, float g) {
         float4 q;
         if (STAGED) {
-            q = lds[j + delta];
+            q = lds[j];
         } else {
             const float4 pj = pos[j];
             q = make_float4(pj.x, pj.y, pj.z, b.kd[j]);
@@ -493,28 +516,34 @@ __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const int32_t* d
     return dv;
 }
 
+template <bool STAGED>
+__device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
+                                          const SolveBufs& b, float4* smem, f3& pi, float& kdi) {
+    if (STAGED) {
+        stage_pos_kd(R, pos, b.kd, smem);
+        __syncthreads();
+    }
+    f3 dv = mk(0.0f, 0.0f, 0.0f);
+    if (id.live) {
+        pi = xyz(pos[id.p]);
+        kdi = b.kd[id.p];
+        dv = kappa_partial<STAGED>(b, id, pi, kdi, pos, smem);
+    }
+    return fold4(dv);
+}
+
 // div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
 // Reads only kd of the neighbours, so the velocity update is in place.
 __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(SceneK k, StepK s, uint32_t n,
                                                                const float4* __restrict__ pos, float4* __restrict__ vel,
                                                                SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
     const RegionS R = load_region(b.regions, id.tile);
-    load_deltas(b.regions, id.tile, s_delta);
-    const bool staged = R.total <= b.cap_kappa;
-    if (staged) stage_pos_kd(R, pos, b.kd, smem);
-    __syncthreads();
-    f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
+    f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
-    if (id.live) {
-        pi = xyz(pos[id.p]);
-        kdi = b.kd[id.p];
-        dv = staged ? kappa_partial<true>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem)
-                    : kappa_partial<false>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem);
-    }
-    const f3 total = fold4(dv);
+    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi)
+                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi);
     if (id.live && id.lane == 0) {
         const float4 wg = b.wall_grad[id.p];
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
@@ -532,19 +561,18 @@ struct ForcesLds {
     float* t;
 };
 
-__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const float4* __restrict__ pos,
-                                                               const float4* __restrict__ vel,
-                                                               const float* __restrict__ temperature,
-                                                               const float* __restrict__ prev_pressure, SolveBufs b) {
-    extern __shared__ float4 smem[];
-    __shared__ int32_t s_delta[kRows];
-    const LaneId id = lane_id(n);
-    const RegionS R = load_region(b.regions, id.tile);
-    load_deltas(b.regions, id.tile, s_delta);
-    const bool staged = R.total <= b.cap_forces;
-    const ForcesLds L{smem, smem + b.cap_forces, reinterpret_cast<float*>(smem + 2 * b.cap_forces)};
-    if (staged) {
-        constexpr uint32_t IT = 2;
+struct ForceSums {
+    f3 visc, blend, nearf, tension;
+    float heat;
+};
+
+template <bool STAGED>
+__device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, const RegionS& R,
+                                            const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                            const float* __restrict__ temperature, const SolveBufs& b,
+                                            const ForcesLds& L, ForceSums& F, f3& pi, float& rho_i) {
+    if (STAGED) {
+        constexpr uint32_t IT = 3;
         float4 pp[IT], vv[IT];
         float dd[IT], nr[IT], tt[IT];
         stage_flat<kSweepThreads, IT>(
@@ -561,61 +589,73 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_
                 L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
                 L.t[idx] = tt[it];
             });
+        __syncthreads();
     }
-    __syncthreads();
-    f3 visc = mk(0.0f, 0.0f, 0.0f), blend = visc, nearf = visc, tension = visc, pi = visc;
-    float heat = 0.0f, rho_i = 0.0f;
-    if (id.live) {
-        const uint32_t i = id.p;
-        pi = xyz(pos[i]);
-        const f3 vi = xyz(vel[i]);
-        const float w_i = b.rho_near[i], temp_i = temperature[i];
-        rho_i = b.density[i];
-        const float rhoi = rho_i;
-        for_list(b, s_delta, i, id.lane, [&](uint32_t j, float gs, int32_t delta) {
-            f3 pj, vj;
-            float rho_j, near_j, temp_j;
-            if (staged) {
-                const uint32_t q = j + delta;
-                const float4 a = L.a[q], c = L.c[q];
-                pj = mk(a.x, a.y, a.z);
-                vj = mk(a.w, c.x, c.y);
-                rho_j = c.z;
-                near_j = c.w;
-                temp_j = L.t[q];
-            } else {
-                pj = xyz(pos[j]);
-                vj = xyz(vel[j]);
-                rho_j = b.density[j];
-                near_j = b.rho_near[j];
-                temp_j = temperature[j];
-            }
-            const f3 x = sub(pi, pj);
-            const float r2 = dot3(x, x);
-            const float rr = sqrtf(r2);
-            const f3 g = scl(gs, x);
-            const float f = dot3(x, g) / (r2 + k.eps_h2);
-            const float pair = k.mass / (rhoi * rho_j) * f;
-            const f3 dv = sub(vi, vj);
-            const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
-            const float rm = wmax(rr, 1e-5f);
-            const float sn = (w_i + near_j) * wn * wn;
-            nearf = add(nearf, mk(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm)));
-            visc = add(visc, scl(k.visc2 * pair, dv));
-            blend = sub(blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
-            heat += k.heat_cond * pair * (temp_i - temp_j);
-            heat -= k.heat_visc * pair * dot3(dv, dv);
-            const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
-            const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
-            const float rg = wmax(rr, 1e-9f);
-            tension = sub(tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
-        });
-    }
-    visc = fold4(visc);
-    blend = fold4(blend);
-    nearf = fold4(nearf);
-    tension = fold4(tension);
-    heat = fold4(heat);
+    if (!id.live) return;
+    const uint32_t i = id.p;
+    pi = xyz(pos[i]);
+    const f3 vi = xyz(vel[i]);
+    const float w_i = b.rho_near[i], temp_i = temperature[i];
+    rho_i = b.density[i];
+    const float rhoi = rho_i;
+    const f3 pii = pi;
+    for_list(b, id, [&](uint32_t j, float gs) {
+        f3 pj, vj;
+        float rho_j, near_j, temp_j;
+        if (STAGED) {
+            const float4 a = L.a[j], c = L.c[j];
+            pj = mk(a.x, a.y, a.z);
+            vj = mk(a.w, c.x, c.y);
+            rho_j = c.z;
+            near_j = c.w;
+            temp_j = L.t[j];
+        } else {
+            pj = xyz(pos[j]);
+            vj = xyz(vel[j]);
+            rho_j = b.density[j];
+            near_j = b.rho_near[j];
+            temp_j = temperature[j];
+        }
+        const f3 x = sub(pii, pj);
+        const float r2 = dot3(x, x);
+        const float rr = sqrtf(r2);
+        const f3 g = scl(gs, x);
+        const float f = dot3(x, g) / (r2 + k.eps_h2);
+        const float pair = k.mass / (rhoi * rho_j) * f;
+        const f3 dv = sub(vi, vj);
+        const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
+        const float rm = wmax(rr, 1e-5f);
+        const float sn = (w_i + near_j) * wn * wn;
+        F.nearf = add(F.nearf, mk(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm)));
+        F.visc = add(F.visc, scl(k.visc2 * pair, dv));
+        F.blend = sub(F.blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
+        F.heat += k.heat_cond * pair * (temp_i - temp_j);
+        F.heat -= k.heat_visc * pair * dot3(dv, dv);
+        const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
+        const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
+        const float rg = wmax(rr, 1e-9f);
+        F.tension = sub(F.tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
+    });
+}
+
+__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const float4* __restrict__ pos,
+                                                               const float4* __restrict__ vel,
+                                                               const float* __restrict__ temperature,
+                                                               const float* __restrict__ prev_pressure, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    const LaneId id = lane_id(n);
+    const RegionS R = load_region(b.regions, id.tile);
+    const ForcesLds L{smem, smem + b.stage_cap, reinterpret_cast<float*>(smem + 2 * b.stage_cap)};
+    const f3 z = mk(0.0f, 0.0f, 0.0f);
+    ForceSums F{z, z, z, z, 0.0f};
+    f3 pi = z;
+    float rho_i = 0.0f;
+    if (R.staged)
+        forces_tile<true>(k, id, R, pos, vel, temperature, b, L, F, pi, rho_i);
+    else
+        forces_tile<false>(k, id, R, pos, vel, temperature, b, L, F, pi, rho_i);
+    const f3 visc = fold4(F.visc), blend = fold4(F.blend), nearf = fold4(F.nearf), tension = fold4(F.tension);
+    const float heat = fold4(F.heat);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const f3 adh = scl(k.beta * k.rho0, wall_adh_sum(k, pi));
@@ -651,22 +691,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, St
                                                                     float4* __restrict__ vel,
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
     const RegionS R = load_region(b.regions, id.tile);
-    load_deltas(b.regions, id.tile, s_delta);
-    const bool staged = R.total <= b.cap_kappa;
-    if (staged) stage_pos_kd(R, pos, b.kd, smem);
-    __syncthreads();
-    f3 dv = mk(0.0f, 0.0f, 0.0f), pi = dv;
+    f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
-    if (id.live) {
-        pi = xyz(pos[id.p]);
-        kdi = b.kd[id.p];
-        dv = staged ? kappa_partial<true>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem)
-                    : kappa_partial<false>(b, s_delta, id.p, id.lane, pi, kdi, pos, smem);
-    }
-    const f3 total = fold4(dv);
+    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi)
+                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const float4 a = b.accel[i];
@@ -688,35 +718,38 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, St
     }
 }
 
+template <bool STAGED>
+__device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
+                                                 const float4* __restrict__ vel, const SolveBufs& b, const PvLds& L,
+                                                 f3& vi) {
+    if (STAGED) {
+        stage_pv<kSweepThreads>(R, pos, vel, L);
+        __syncthreads();
+    }
+    float drho = 0.0f;
+    if (id.live) {
+        const f3 pi = xyz(pos[id.p]);
+        vi = xyz(vel[id.p]);
+        const f3 vii = vi;
+        for_list(b, id, [&](uint32_t j, float g) {
+            f3 pj, vj;
+            load_pv<STAGED>(L, pos, vel, j, pj, vj);
+            drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
+        });
+    }
+    return fold4(drho);
+}
+
 __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(SceneK k, StepK s, uint32_t n,
                                                              const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    __shared__ int32_t s_delta[kRows];
     const LaneId id = lane_id(n);
     const RegionS R = load_region(b.regions, id.tile);
-    load_deltas(b.regions, id.tile, s_delta);
-    const bool staged = R.total <= b.cap_kappa_vel;
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.cap_kappa_vel)};
-    if (staged) stage_pv<kSweepThreads>(R, pos, vel, L);
-    __syncthreads();
-    float drho = 0.0f;
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
-    if (id.live) {
-        const uint32_t i = id.p;
-        const f3 pi = xyz(pos[i]);
-        vi = xyz(vel[i]);
-        const f3 vii = vi;
-        for_list(b, s_delta, i, id.lane, [&](uint32_t j, float g, int32_t delta) {
-            f3 pj, vj;
-            if (staged)
-                load_pv<true>(L, pos, vel, j, delta, pj, vj);
-            else
-                load_pv<false>(L, pos, vel, j, 0, pj, vj);
-            drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
-        });
-    }
-    const float total = fold4(drho);
+    const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi)
+                                 : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const float rho = b.density[i];
@@ -781,10 +814,12 @@ constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
 
 uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 
-// Sizes the sweeps' LDS staging areas from the device's per-workgroup LDS
-// (less 1 KB for the kernels' static LDS) and raises the kernels' dynamic-LDS
-// limit to that size. If the device refuses the larger limit, the areas fit
-// the default 64 KB.
+// Sizes the staging capacity from the device's per-workgroup LDS (less 1 KB
+// for static LDS) and raises the sweeps' dynamic-LDS limit to match. The
+// capacity is one for every sweep, since a staged tile's list holds LDS
+// indices: at most 3,328 slots, which keeps two density and den_kappa
+// workgroups (24 bytes a slot) on a CU; forces_eval (36 bytes) sizes the
+// budget. If the device refuses the larger limit, the capacity fits 64 KB.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
     const void* fns[] = {reinterpret_cast<const void*>(&k_density_div), reinterpret_cast<const void*>(&k_kappa_apply),
@@ -795,17 +830,9 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
             avail = 65536 - 1024;
     (void)hipGetLastError();
-    // Keep capacities multiples of 64 slots; a typical tile region holds about
-    // 9 x 34 cells x 8 slots = 2,448 slots at the configurations' spacing.
-    auto cap = [&](size_t entry, size_t want) {
-        size_t c = avail / entry;
-        if (c > want) c = want;
-        return (uint32_t)(c / 64 * 64);
-    };
-    b.cap_density = cap(kPvEntry, 3072);
-    b.cap_kappa = cap(kKappaEntry, 4096);
-    b.cap_kappa_vel = cap(kPvEntry, 3072);
-    b.cap_forces = cap(kForcesEntry, 3072);
+    size_t c = avail / kForcesEntry;
+    if (c > 3328) c = 3328;
+    b.stage_cap = (uint32_t)(c / 64 * 64);
     return avail;
 }
 
@@ -813,7 +840,7 @@ static void launch_regions(hipStream_t st, const SceneK& k, uint32_t n, const ui
                            const SolveBufs& b) {
     const uint32_t tiles = solve_tiles(n);
     hipLaunchKernelGGL(k_tile_regions, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, k, n, starts, work.pos,
-                       b.regions, tiles);
+                       b.regions, tiles, b.stage_cap);
 }
 
 void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
@@ -821,8 +848,8 @@ void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32
     if (n == 0) return;
     if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
     launch_regions(st, k, n, starts, work, b);
-    hipLaunchKernelGGL(k_density_div, dim3(solve_tiles(n)), dim3(kTile), b.cap_density * kPvEntry, st, k, s, n,
-                       starts, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_density_div, dim3(solve_tiles(n)), dim3(kSweepThreads), b.stage_cap * kPvEntry, st, k, s,
+                       n, starts, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DENSITY_DIV);
 }
 
@@ -831,13 +858,13 @@ void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, c
                   KernelTimer* timer) {
     if (n == 0) return;
     const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
-    const size_t lds_kappa = b.cap_kappa * kKappaEntry;
+    const size_t lds_kappa = b.stage_cap * kKappaEntry;
     launch_density_only(st, k, s, n, starts, work, b, timer);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
     hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.cap_forces * kForcesEntry, st, k, n, work.pos, work.vel,
+    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.stage_cap * kForcesEntry, st, k, n, work.pos, work.vel,
                        work.temperature, work.prev_pressure, b);
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
@@ -846,8 +873,7 @@ void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, c
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
     for (uint32_t r = 0; r < refine; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.cap_kappa_vel * kPvEntry, st, k, s, n, work.pos, work.vel,
-                           b);
+        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.stage_cap * kPvEntry, st, k, s, n, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
         hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);

@@ -68,7 +68,7 @@ int fluid_sim_device_count(void);
  * (render.rs:1468-1612): velocities and previous velocities start at zero,
  * temperature at 293.15 K, previous pressure at zero; positions start at
  * zero until fluid_sim_upload_state. `tracers` packed tracers are allocated
- * (zero = none). `nbr_cap` is the neighbour-list cap (0 = NBR_CAP, 96; at most 127).
+ * (zero = none). `nbr_cap` is the neighbour-list cap (0 = NBR_CAP, 96; a multiple of 4, at most 256).
  * Returns NULL on failure.
  */
 FluidSim *fluid_sim_create(const uint8_t *params64, uint32_t tracers, uint32_t nbr_cap);

@@ -21,11 +21,11 @@
 //   * The reference's sweeps split each particle over 8 lanes and fold
 //     (sim_solve.wgsl:377, 398-402) and append the neighbour list through a
 //     workgroup atomic (sim_solve.wgsl:463). The oracle's density sweep is the
-//     LANES = 1 reading of that code: one serial walk of the 27 cells in the
-//     reference's cell numbering (x fastest, then y, then z,
-//     sim_solve.wgsl:436-438), particles in slot order within a cell, and
-//     the list in that same order. The list sweeps (div_apply, forces_eval,
-//     forces_den_apply, den_kappa, den_apply) split each list over 4 lanes
+//     one serial walk of the 27 cells in the reference's cell numbering (x
+//     fastest, then y, then z, sim_solve.wgsl:436-438), particles in slot
+//     order within a cell, and the list in that same order. Every sweep's
+//     pair sums (density_div, div_apply, forces_eval, forces_den_apply,
+//     den_kappa, den_apply) split the neighbours in that order over 4 lanes
 //     with the reference's strided assignment and xor-butterfly fold
 //     (kListLanes, lane_fold below).
 //   * Per-substep constants the reference evaluates per invocation with
@@ -506,15 +506,21 @@ struct Sim {
             out[a] = (int32_t)std::min(sat_u32(((float)wpos[4 * i + a] - p.box_min[a]) / p.cell), p.dims[a] - 1u);
     }
 
-    // sim_solve.wgsl:411-518 with LANES = 1
+    // sim_solve.wgsl:411-518: the walk in the reference's cell order builds
+    // the list; the pair sums split the neighbours over kListLanes lanes
+    // (neighbour m, past the cap included, to lane m % kListLanes; lane 0
+    // carries the self term, sim_solve.wgsl:426) and fold with lane_fold
+    // (sim_solve.wgsl:470-471).
     void density_div(const StepT<T>& s) {
         uint64_t over = 0;
 #pragma omp parallel for schedule(static) reduction(+ : over)
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t pidx = (uint32_t)pi;
             V3<T> pos_i = wp(pidx), vel_i = wv(pidx);
-            T w_sum = k.sigma, rn = 0, grad_sq = 0, drho = 0;
-            V3<T> grad_sum = v3(T(0), T(0), T(0));
+            T w_l[kListLanes] = {}, rn_l[kListLanes] = {}, gsq_l[kListLanes] = {}, drho_l[kListLanes] = {};
+            V3<T> gsum_l[kListLanes];
+            for (uint32_t l = 0; l < kListLanes; ++l) gsum_l[l] = v3(T(0), T(0), T(0));
+            w_l[0] = k.sigma;
             uint32_t found = 0;
             int32_t base[3];
             cell_base(pidx, base);
@@ -532,19 +538,23 @@ struct Sim {
                     V3<T> x = pos_i - wp(kk);
                     T r2 = dot(x, x);
                     if (r2 >= k.support_sq) continue;
+                    const uint32_t l = found % kListLanes;
                     T r = std::sqrt(r2);
                     T q = r * k.inv_h;
-                    w_sum += kernel_q(k, q);
+                    w_l[l] += kernel_q(k, q);
                     T wn = tmax(T(1) - q, T(0));
-                    rn += wn * wn * wn;
+                    rn_l[l] += wn * wn * wn;
                     V3<T> gw = grad_q(k, x, r, q);
-                    grad_sum = grad_sum + gw;
-                    grad_sq += dot(gw, gw);
-                    drho += dot(vel_i - wv(kk), gw);
+                    gsum_l[l] = gsum_l[l] + gw;
+                    gsq_l[l] += dot(gw, gw);
+                    drho_l[l] += dot(vel_i - wv(kk), gw);
                     if (found < cap) nbr[(size_t)pidx * cap + found] = kk;
                     found++;
                 }
             }
+            const T w_sum = lane_fold(w_l), rn = lane_fold(rn_l);
+            T grad_sq = lane_fold(gsq_l), drho = lane_fold(drho_l);
+            V3<T> grad_sum = lane_fold(gsum_l);
             nbr_n[pidx] = std::min(found, cap);
             if (found > cap) over += found - cap;
             T rho = k.mass * w_sum;

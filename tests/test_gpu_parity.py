@@ -179,3 +179,57 @@ def test_touches_and_rotation_match_oracle():
     o = run_oracle(sc, [step], refine)
     for k, v in compare(g, o).items():
         assert v <= FP32_TOL, k
+
+
+def _bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+@pytest.mark.parametrize("maker", ["small", "dam"])
+def test_one_substep_is_bit_exact_with_the_fp32_oracle(maker):
+    """Same operand order, no contraction, correctly rounded sqrt and divide
+    on both sides: every state word and field word is identical, and so is
+    every neighbour list."""
+    sc = _scene_small(seed=21) if maker == "small" else fa.scenes.dam_break(seed=22)
+    step, refine = _step(sc)
+    with fa.Sim(sc.params) as s:
+        s.upload_state(sc.pos4)
+        s.substeps(step, 1, refine)
+        st = s.state()
+        fields = {k: s.field(k) for k in ("density", "alpha", "kd", "pressure", "rho_near", "accel", "xsph")}
+        nbr, nbr_n = s.neighbours(), s.grid()["nbr_n"]
+    o = OracleSim(sc.params)
+    o.set_state(sc.pos4)
+    o.substep(step, refine)
+    ost = o.state()
+    for k in ("pos", "vel", "prev_vel", "prev_pressure", "temperature"):
+        assert np.array_equal(_bits(st[k]), _bits(ost[k])), k
+    for k, v in fields.items():
+        assert np.array_equal(_bits(v), _bits(o.field(k))), k
+    onbr, on = o.nbr(), o.grid()["nbr_n"]
+    assert np.array_equal(nbr_n, on)
+    for i in range(sc.n):
+        assert np.array_equal(nbr[i, : nbr_n[i]], onbr[i, : on[i]]), i
+
+
+@pytest.mark.parametrize("lds", [8192, 57344])
+def test_global_gather_path_matches_staged_path(lds, monkeypatch):
+    """Tiles whose neighbour rows exceed the staging capacity gather from
+    global memory and their lists hold slots; the results are the same bits.
+    FLUID_LDS_BYTES shrinks the capacity (8 KB: no tile staged; 56 KB: some)."""
+    sc = fa.scenes.dam_break(seed=23)
+    step, refine = _step(sc)
+    ref = run_gpu(sc, [step] * 2, refine)
+    monkeypatch.setenv("FLUID_LDS_BYTES", str(lds))
+    with fa.Sim(sc.params) as s:
+        d = s.describe()
+        assert d["stage_cap"] < 1600, d
+    out = run_gpu(sc, [step] * 2, refine)
+    for k in STATE:
+        assert np.array_equal(_bits(out[k]), _bits(ref[k])), k
+    for k in ("density", "kd", "accel"):
+        assert np.array_equal(_bits(out["fields"][k]), _bits(ref["fields"][k])), k
+    assert np.array_equal(out["grid"]["nbr_n"], ref["grid"]["nbr_n"])
+
+
+STATE = ("pos", "vel", "prev_vel", "prev_pressure", "temperature")

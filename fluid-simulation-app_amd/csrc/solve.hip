@@ -319,20 +319,26 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
     const Reach st = make_reach(k, pi, bx, by, bz);
     const uint32_t shift = __lane_id() & ~(kLanes - 1);
     const uint32_t below = (1u << id.lane) - 1u;
-    uint32_t found = 0;
+    // Every row's candidate range first, so the 18 bound loads overlap.
+    uint32_t k0s[kRows], k1s[kRows];
 #pragma unroll
     for (uint32_t r = 0; r < kRows; ++r) {
         const int dz = (int)(r / 3) - 1, dy = (int)(r % 3) - 1;
         const int cz = (int)bz + dz, cy = (int)by + dy;
-        if (cz < 0 || cz >= (int)k.dims[2] || cy < 0 || cy >= (int)k.dims[1]) continue;
         const float ez = dz < 0 ? st.dl[2] : (dz > 0 ? st.dh[2] : 0.0f);
         const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
         const float e2 = ey * ey + ez * ez;
-        if (e2 > st.lim) continue;
+        const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] && e2 <= st.lim;
         const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.lim) ? bx - 1 : bx;
         const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
-        const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
-        const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
+        const uint32_t row = live_row ? ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0] : 0u;
+        k0s[r] = live_row ? starts[row + x0] : 0u;
+        k1s[r] = live_row ? starts[row + x1 + 1] : 0u;
+    }
+    uint32_t found = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kRows; ++r) {
+        const uint32_t k0 = k0s[r], k1 = k1s[r];
         const int32_t delta = !STAGED ? 0
                               : r == 0 ? R.d0
                               : r == 1 ? R.d1
@@ -624,17 +630,19 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         const float pair = k.mass / (rhoi * rho_j) * f;
         const f3 dv = sub(vi, vj);
         const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
-        const float rm = wmax(rr, 1e-5f);
+        // x / max(r, 1e-5) and x / max(r, 1e-9) (sim_solve.wgsl:559, 567-568)
+        // as x times one reciprocal of r, shared when r >= 1e-5.
+        const float inv_rg = 1.0f / wmax(rr, 1e-9f);
+        const float inv_rm = rr >= 1e-5f ? inv_rg : 1.0f / 1e-5f;
         const float sn = (w_i + near_j) * wn * wn;
-        F.nearf = add(F.nearf, mk(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm)));
+        F.nearf = add(F.nearf, mk(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm)));
         F.visc = add(F.visc, scl(k.visc2 * pair, dv));
         F.blend = sub(F.blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
         F.heat += k.heat_cond * pair * (temp_i - temp_j);
         F.heat -= k.heat_visc * pair * dot3(dv, dv);
         const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
         const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
-        const float rg = wmax(rr, 1e-9f);
-        F.tension = sub(F.tension, mk(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg));
+        F.tension = sub(F.tension, mk(cm * x.x * inv_rg, cm * x.y * inv_rg, cm * x.z * inv_rg));
     });
 }
 

@@ -28,6 +28,10 @@
 //     den_kappa, den_apply) split the neighbours in that order over 4 lanes
 //     with the reference's strided assignment and xor-butterfly fold
 //     (kListLanes, lane_fold below).
+//   * forces_eval divides the pair vector x by max(r, 1e-5) and by
+//     max(r, 1e-9) (sim_solve.wgsl:559, 567-568); the oracle multiplies x by
+//     one reciprocal of r instead (the two maxima agree for r >= 1e-5), so
+//     each quotient is the product with the correctly rounded reciprocal.
 //   * Per-substep constants the reference evaluates per invocation with
 //     exp() (XSPH blend, lens chase, tracer charge decay) are evaluated once
 //     per substep with the host expf/exp.
@@ -648,17 +652,20 @@ struct Sim {
                 T pair = k.mass / (rho_i * rho_j) * f;
                 V3<T> dv = vel_i - wv(j);
                 T wn = tmax(T(1) - r * k.inv_h, T(0));
-                T rm = tmax(r, K(1e-5));
+                // x / max(r, 1e-5) and x / max(r, 1e-9) as x times one
+                // reciprocal of r, shared when r >= 1e-5 (then both maxima
+                // are r).
+                T inv_rg = T(1) / tmax(r, K(1e-9));
+                T inv_rm = r >= K(1e-5) ? inv_rg : T(1) / K(1e-5);
                 T sn = (w_i + rho_near[j]) * wn * wn;
-                nearf = nearf + v3(sn * (x.x / rm), sn * (x.y / rm), sn * (x.z / rm));
+                nearf = nearf + v3(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm));
                 visc = visc + (k.visc2 * pair) * dv;
                 blend = blend - (k.mass / rho_j * kernel_q(k, r * k.inv_h)) * dv;
                 heat += k.heat_cond * pair * (temp_i - wtemp[j]);
                 heat -= k.heat_visc * pair * dot(dv, dv);
                 T corr = T(2) * k.rho0 / (rho_i + rho_j);
                 T cm = corr * k.mass * cohesion(r, k.coh_c, k.coh_k, k.coh_near);
-                T rg = tmax(r, K(1e-9));
-                tension = tension - v3(cm * x.x / rg, cm * x.y / rg, cm * x.z / rg);
+                tension = tension - v3(cm * x.x * inv_rg, cm * x.y * inv_rg, cm * x.z * inv_rg);
             }
             const V3<T> visc = lane_fold(visc_l), blend = lane_fold(blend_l), nearf = lane_fold(near_l),
                         tension = lane_fold(tension_l);

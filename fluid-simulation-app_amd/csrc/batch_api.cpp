@@ -1,0 +1,547 @@
+// The batched DFSPH step: many independent boxes of uneven particle counts
+// stepped together, one launch per kernel per substep.
+//
+// Layout. Every scene's particles sit in one shared SoA set, scene s at slots
+// [begin_s, begin_s + N_s), its range padded to a whole solve tile (256
+// slots) so no workgroup spans two scenes. Its cells sit at cell_offset_s in
+// one shared histogram, followed by one padding cell that the slots past its
+// particles sort into: the one global scan then yields every scene's cell
+// starts already offset to its own slots. A table maps each tile to its scene;
+// the kernels read the scene's SimParams-derived constants and its Step from
+// per-scene device tables.
+//
+// Schedule. A substep is the reference's dispatch order (render.rs:3168-3203)
+// over all scenes at once; each scene uses its own Step (dt, gravity, ...) and
+// its own number of density passes. In the reference schedule a scene runs
+// its refine count; in tolerance mode it runs passes until its mean density
+// error is under the tolerance. The whole substep can be captured once in a
+// hipGraph and replayed: the per-scene Steps reach the device through a
+// copy node from pinned host memory (two buffers, alternating, so the host
+// fills one while the GPU reads the other).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "fluid_common.hpp"
+#include "host_common.hpp"
+
+using namespace fluid;
+
+namespace {
+
+constexpr int kRing = 2;
+
+struct Ring {
+    StepK* host = nullptr;   // pinned
+    StepK* dev = nullptr;
+    hipEvent_t done = nullptr;
+    std::map<uint64_t, hipGraphExec_t> graphs;  // by (passes, tolerance)
+};
+
+}  // namespace
+
+struct FluidBatch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, tiles = 0;
+    std::vector<SceneK> ks;
+    std::vector<uint32_t> padded;
+    SceneK* d_ks = nullptr;
+    uint32_t* d_tile_scene = nullptr;
+    uint32_t* d_pass_limit = nullptr;
+    Ring ring[kRing];
+    uint32_t ring_i = 0;
+    Records rest{}, work{};
+    GridBufs grid{};
+    SolveBufs solve{};
+    StatsBufs stats{};
+    size_t lds_bytes = 0;
+    uint32_t stage_cap_max = 0;
+    bool use_graph = false;
+    Tolerance tol{0.0f, 1, 0};
+    EventTimer timer;
+    DeviceAllocs mem;
+
+    KernelTimer* t() { return timer.mask ? &timer : nullptr; }
+    template <typename P>
+    int alloc(P*& out, size_t count, bool zero = true) {
+        return mem.alloc(out, count, zero);
+    }
+    SceneTable table(const Ring& r) const { return SceneTable{d_ks, r.dev, d_tile_scene, nullptr}; }
+    void drop_graphs() {
+        for (auto& r : ring) {
+            for (auto& g : r.graphs) (void)hipGraphExecDestroy(g.second);
+            r.graphs.clear();
+        }
+    }
+    ~FluidBatch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
+        for (auto& r : ring) {
+            if (r.host) (void)hipHostFree(r.host);
+            if (r.done) (void)hipEventDestroy(r.done);
+        }
+        mem.release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+bool valid(const FluidBatch* b) {
+    if (!b) {
+        set_error("null batch handle");
+        return false;
+    }
+    return true;
+}
+
+bool valid_scene(const FluidBatch* b, uint32_t s) {
+    if (!valid(b)) return false;
+    if (s >= b->scenes) {
+        set_error("scene %u out of range (%u scenes)", s, b->scenes);
+        return false;
+    }
+    return true;
+}
+
+int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t nbr_cap) {
+    B->scenes = scenes;
+    B->cap = nbr_cap ? nbr_cap : FLUID_SIM_NBR_CAP;
+    if (B->cap % kListLanes != 0 || B->cap > 256) {
+        set_error("neighbour cap %u is not a multiple of 4 in [4, 256]", B->cap);
+        return FLUID_ERR_ARG;
+    }
+    uint64_t slots = 0, cells = 0;
+    B->ks.resize(scenes);
+    B->padded.resize(scenes);
+    std::vector<uint32_t> tile_scene;
+    for (uint32_t s = 0; s < scenes; ++s) {
+        SimParamsBlock p;
+        std::memcpy(&p, params + 64 * (size_t)s, sizeof p);
+        if (p.dims[0] < 2 || p.dims[1] < 2 || p.dims[2] < 2) {
+            set_error("scene %u: grid dims %u x %u x %u: each must be at least 2", s, p.dims[0], p.dims[1],
+                      p.dims[2]);
+            return FLUID_ERR_ARG;
+        }
+        SceneK k = make_scene(p);
+        const uint64_t c = (uint64_t)p.dims[0] * p.dims[1] * p.dims[2];
+        const uint64_t pad = ((uint64_t)p.count + kSolveTile - 1) / kSolveTile * kSolveTile;
+        if (slots + pad >= 0x80000000ull || cells + c + 1 >= 0xffffffffull) {
+            set_error("batch exceeds 2^31 slots or 2^32 cells at scene %u", s);
+            return FLUID_ERR_ARG;
+        }
+        k.cell_offset = (uint32_t)cells;
+        k.cells = (uint32_t)c;
+        k.slot_begin = (uint32_t)slots;
+        k.slot_end = (uint32_t)(slots + p.count);
+        B->ks[s] = k;
+        B->padded[s] = (uint32_t)pad;
+        for (uint64_t t = 0; t < pad / kSolveTile; ++t) tile_scene.push_back(s);
+        slots += pad;
+        cells += c + 1;
+    }
+    B->slots = (uint32_t)slots;
+    B->cells = (uint32_t)cells;
+    B->tiles = (uint32_t)tile_scene.size();
+    HIP_TRY(hipGetDevice(&B->device));
+    HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    const size_t n = B->slots, c = B->cells, S = scenes;
+    int rc = FLUID_OK;
+#define A(ptr, cnt)                            \
+    if ((rc = B->alloc(ptr, cnt)) != FLUID_OK) \
+        return rc;
+    for (Records* r : {&B->rest, &B->work}) {
+        A(r->pos, n);
+        A(r->vel, n);
+        A(r->prev_vel, n);
+        A(r->prev_pressure, n);
+        A(r->temperature, n);
+        r->ids = nullptr;
+    }
+    A(B->grid.cellid, n);
+    A(B->grid.rank, n);
+    A(B->grid.counts, c);
+    A(B->grid.starts, c + 1);
+    A(B->grid.perm_tmp, n);
+    A(B->grid.perm, n);
+    A(B->grid.status, scan_tiles(B->cells) + 1);
+    A(B->grid.ticket, 1);
+    SolveBufs& b = B->solve;
+    A(b.density, n);
+    A(b.alpha, n);
+    A(b.kd, n);
+    A(b.pressure, n);
+    A(b.rho_near, n);
+    A(b.accel, n);
+    A(b.xsph, n);
+    A(b.wall_grad, n);
+    b.cap = B->cap;
+    b.stride = std::max<uint32_t>(B->slots, kSolveTile);
+    if ((rc = B->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
+    A(b.nbr_n, n);
+    A(b.regions, std::max<uint32_t>(B->tiles, 1));
+    A(b.counters, 2 * S);
+    A(b.residual, S);
+    A(b.iterations, S);
+    A(B->d_pass_limit, S);
+    A(B->d_ks, S);
+    A(B->d_tile_scene, std::max<uint32_t>(B->tiles, 1));
+    int lds = 0;
+    HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, B->device));
+    B->lds_bytes = (size_t)lds;
+    if (const char* env = getenv("FLUID_LDS_BYTES")) {
+        const long v = strtol(env, nullptr, 10);
+        if (v > 0 && (size_t)v < B->lds_bytes) B->lds_bytes = (size_t)v;
+    }
+    B->lds_bytes = configure_solve(b, B->lds_bytes);
+    B->stage_cap_max = b.stage_cap;
+    B->stats.blocks = 1024;
+    A(B->stats.partials, S * 1024 * 16);
+    A(B->stats.stats, S * 16);
+#undef A
+    for (auto& r : B->ring) {
+        HIP_TRY(hipHostMalloc((void**)&r.host, std::max<size_t>(S, 1) * sizeof(StepK), hipHostMallocDefault));
+        std::memset(r.host, 0, std::max<size_t>(S, 1) * sizeof(StepK));
+        if ((rc = B->alloc(r.dev, S)) != FLUID_OK) return rc;
+        HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
+    }
+    HIP_TRY(hipMemcpy(B->d_ks, B->ks.data(), S * sizeof(SceneK), hipMemcpyHostToDevice));
+    if (B->tiles)
+        HIP_TRY(hipMemcpy(B->d_tile_scene, tile_scene.data(), tile_scene.size() * 4, hipMemcpyHostToDevice));
+    std::vector<float> warm(std::max<size_t>(n, 1), 293.15f);
+    HIP_TRY(hipMemcpy(B->rest.temperature, warm.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    return FLUID_OK;
+}
+
+// The launches of one substep (the copy of the Step table included), on the
+// batch stream; captured into a graph or issued directly.
+void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
+    (void)hipMemcpyAsync(r.dev, r.host, (size_t)B->scenes * sizeof(StepK), hipMemcpyHostToDevice, B->stream);
+    const SceneTable sc = B->table(r);
+    launch_grid_build(B->stream, sc, B->slots, B->cells, B->rest, B->work, B->grid, B->t());
+    if (B->tol.tol > 0.0f)
+        launch_solve_tolerance(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
+                               B->tol, B->d_pass_limit, B->t());
+    else
+        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t());
+}
+
+int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) {
+    Ring& r = B->ring[B->ring_i];
+    // The GPU has finished reading this buffer's previous contents.
+    HIP_TRY(hipEventSynchronize(r.done));
+    uint32_t passes = 0, active = 0;
+    for (uint32_t s = 0; s < B->scenes; ++s) {
+        StepBlock sb;
+        std::memcpy(&sb, steps144 + 144 * (size_t)s, sizeof sb);
+        StepK k = make_step(sb);
+        k.refine = refine ? refine[s] : 0u;
+        // dt = 0 encodes no solve (render.rs:3165-3167).
+        k.active = (sb.dt > 0.0f && B->ks[s].slot_end > B->ks[s].slot_begin) ? 1u : 0u;
+        if (k.active) {
+            passes = std::max(passes, k.refine);
+            ++active;
+        }
+        r.host[s] = k;
+    }
+    if (active == 0) return FLUID_OK;
+    if (B->tol.tol > 0.0f) passes = B->tol.max_passes;
+    if (B->use_graph && !B->t()) {
+        const uint64_t key = ((uint64_t)passes << 1) | (B->tol.tol > 0.0f ? 1u : 0u);
+        auto it = r.graphs.find(key);
+        if (it == r.graphs.end()) {
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+            issue_substep(B, r, passes);
+            HIP_TRY(hipStreamEndCapture(B->stream, &g));
+            hipGraphExec_t exec = nullptr;
+            const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIP_TRY(e);
+            it = r.graphs.emplace(key, exec).first;
+        }
+        HIP_TRY(hipGraphLaunch(it->second, B->stream));
+    } else {
+        issue_substep(B, r, passes);
+    }
+    HIP_TRY(hipEventRecord(r.done, B->stream));
+    B->ring_i = (B->ring_i + 1) % kRing;
+    return check_launch("batch substep");
+}
+
+}  // namespace
+
+extern "C" {
+
+FluidBatch* fluid_batch_create(uint32_t scenes, const uint8_t* params64, uint32_t nbr_cap) {
+    if (!params64 || scenes == 0) {
+        set_error("a batch needs at least one SimParams block");
+        return nullptr;
+    }
+    int devices = 0;
+    if (hipGetDeviceCount(&devices) != hipSuccess || devices < 1) {
+        set_error("no HIP device");
+        return nullptr;
+    }
+    FluidBatch* B = new FluidBatch();
+    if (init_batch(B, scenes, params64, nbr_cap) != FLUID_OK) {
+        delete B;
+        return nullptr;
+    }
+    return B;
+}
+
+void fluid_batch_destroy(FluidBatch* b) { delete b; }
+
+uint32_t fluid_batch_scene_count(const FluidBatch* b) { return b ? b->scenes : 0; }
+uint32_t fluid_batch_slot_count(const FluidBatch* b) { return b ? b->slots : 0; }
+
+int fluid_batch_scene_info(const FluidBatch* b, uint32_t scene, uint32_t* slot_begin, uint32_t* count,
+                           uint32_t* cells) {
+    if (!valid_scene(b, scene)) return FLUID_ERR_ARG;
+    const SceneK& k = b->ks[scene];
+    if (slot_begin) *slot_begin = k.slot_begin;
+    if (count) *count = k.slot_end - k.slot_begin;
+    if (cells) *cells = k.cells;
+    return FLUID_OK;
+}
+
+int fluid_batch_upload_state(FluidBatch* b, uint32_t scene, const float* pos4, const float* vel4,
+                             const float* prev_vel4, const float* prev_pressure, const float* temperature) {
+    if (!valid_scene(b, scene)) return FLUID_ERR_ARG;
+    const SceneK& k = b->ks[scene];
+    const size_t o = k.slot_begin, n = k.slot_end - k.slot_begin;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (pos4) HIP_TRY(hipMemcpy(b->rest.pos + o, pos4, n * 16, hipMemcpyHostToDevice));
+    if (vel4) HIP_TRY(hipMemcpy(b->rest.vel + o, vel4, n * 16, hipMemcpyHostToDevice));
+    if (prev_vel4) HIP_TRY(hipMemcpy(b->rest.prev_vel + o, prev_vel4, n * 16, hipMemcpyHostToDevice));
+    if (prev_pressure) HIP_TRY(hipMemcpy(b->rest.prev_pressure + o, prev_pressure, n * 4, hipMemcpyHostToDevice));
+    if (temperature) HIP_TRY(hipMemcpy(b->rest.temperature + o, temperature, n * 4, hipMemcpyHostToDevice));
+    return FLUID_OK;
+}
+
+int fluid_batch_download_state(FluidBatch* b, uint32_t scene, float* pos4, float* vel4, float* prev_vel4,
+                               float* prev_pressure, float* temperature) {
+    if (!valid_scene(b, scene)) return FLUID_ERR_ARG;
+    const SceneK& k = b->ks[scene];
+    const size_t o = k.slot_begin, n = k.slot_end - k.slot_begin;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (pos4) HIP_TRY(hipMemcpy(pos4, b->rest.pos + o, n * 16, hipMemcpyDeviceToHost));
+    if (vel4) HIP_TRY(hipMemcpy(vel4, b->rest.vel + o, n * 16, hipMemcpyDeviceToHost));
+    if (prev_vel4) HIP_TRY(hipMemcpy(prev_vel4, b->rest.prev_vel + o, n * 16, hipMemcpyDeviceToHost));
+    if (prev_pressure) HIP_TRY(hipMemcpy(prev_pressure, b->rest.prev_pressure + o, n * 4, hipMemcpyDeviceToHost));
+    if (temperature) HIP_TRY(hipMemcpy(temperature, b->rest.temperature + o, n * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_batch_download_field(FluidBatch* b, uint32_t scene, uint32_t field, float* out) {
+    if (!valid_scene(b, scene) || !out) return FLUID_ERR_ARG;
+    const SceneK& k = b->ks[scene];
+    const size_t o = k.slot_begin, n = k.slot_end - k.slot_begin;
+    const float* src = nullptr;
+    size_t per = 1;
+    switch (field) {
+        case FLUID_FIELD_DENSITY: src = b->solve.density; break;
+        case FLUID_FIELD_ALPHA: src = b->solve.alpha; break;
+        case FLUID_FIELD_KD: src = b->solve.kd; break;
+        case FLUID_FIELD_PRESSURE: src = b->solve.pressure; break;
+        case FLUID_FIELD_RHO_NEAR: src = b->solve.rho_near; break;
+        case FLUID_FIELD_ACCEL: src = reinterpret_cast<const float*>(b->solve.accel); per = 4; break;
+        case FLUID_FIELD_XSPH: src = reinterpret_cast<const float*>(b->solve.xsph); per = 4; break;
+        case FLUID_FIELD_WALL_GRAD: src = reinterpret_cast<const float*>(b->solve.wall_grad); per = 4; break;
+        case FLUID_FIELD_WORK_POS: src = reinterpret_cast<const float*>(b->work.pos); per = 4; break;
+        case FLUID_FIELD_WORK_VEL: src = reinterpret_cast<const float*>(b->work.vel); per = 4; break;
+        default:
+            set_error("unknown field %u", field);
+            return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(out, src + o * per, n * per * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+// The scene's grid in its own numbering: cell ids and starts relative to
+// the scene's cells and slots, as a lone scene's grid would read.
+int fluid_batch_download_grid(FluidBatch* b, uint32_t scene, uint32_t* cellid, uint32_t* counts, uint32_t* starts,
+                              uint32_t* perm) {
+    if (!valid_scene(b, scene)) return FLUID_ERR_ARG;
+    const SceneK& k = b->ks[scene];
+    const size_t o = k.slot_begin, n = k.slot_end - k.slot_begin, c = k.cells;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::vector<uint32_t> tmp(std::max(n, c + 1));
+    if (cellid) {
+        HIP_TRY(hipMemcpy(tmp.data(), b->grid.cellid + o, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) cellid[i] = tmp[i] - k.cell_offset;
+    }
+    if (starts || counts) {
+        HIP_TRY(hipMemcpy(tmp.data(), b->grid.starts + k.cell_offset, (c + 1) * 4, hipMemcpyDeviceToHost));
+        if (starts)
+            for (size_t i = 0; i <= c; ++i) starts[i] = tmp[i] - k.slot_begin;
+        if (counts)
+            for (size_t i = 0; i < c; ++i) counts[i] = tmp[i + 1] - tmp[i];
+    }
+    if (perm) {
+        HIP_TRY(hipMemcpy(tmp.data(), b->grid.perm + o, n * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; ++i) perm[i] = tmp[i] - k.slot_begin;
+    }
+    return FLUID_OK;
+}
+
+int fluid_batch_substep(FluidBatch* b, const uint8_t* steps144, const uint32_t* refine) {
+    if (!valid(b) || !steps144) return FLUID_ERR_ARG;
+    return run_substep(b, steps144, refine);
+}
+
+int fluid_batch_set_tolerance(FluidBatch* b, float tol, uint32_t min_passes, uint32_t max_passes) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    if (tol > 0.0f && (max_passes == 0 || min_passes > max_passes)) {
+        set_error("tolerance mode needs 0 < min_passes <= max_passes");
+        return FLUID_ERR_ARG;
+    }
+    b->tol = Tolerance{tol > 0.0f ? tol : 0.0f, std::max<uint32_t>(min_passes, 1), max_passes};
+    return FLUID_OK;
+}
+
+int fluid_batch_iterations(FluidBatch* b, uint32_t* out) {
+    if (!valid(b) || !out) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(out, b->solve.iterations, (size_t)b->scenes * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_batch_use_graph(FluidBatch* b, bool enable) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->use_graph = enable;
+    if (!enable) b->drop_graphs();
+    return FLUID_OK;
+}
+
+int fluid_batch_reduce_stats(FluidBatch* b) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    launch_reduce_stats(b->stream, b->table(b->ring[0]), b->scenes, b->solve.density, b->rest, b->solve.counters,
+                        b->stats, b->t());
+    return check_launch("batch reduce_stats");
+}
+
+int fluid_batch_read_stats(FluidBatch* b, float* out) {
+    if (!valid(b) || !out) return FLUID_ERR_ARG;
+    std::vector<float> all((size_t)b->scenes * 16);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(all.data(), b->stats.stats, all.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t s = 0; s < b->scenes; ++s) std::memcpy(out + 14 * (size_t)s, all.data() + 16 * (size_t)s, 56);
+    return FLUID_OK;
+}
+
+int fluid_batch_counters(FluidBatch* b, uint64_t* out) {
+    if (!valid(b) || !out) return FLUID_ERR_ARG;
+    std::vector<uint32_t> c((size_t)b->scenes * 2);
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(c.data(), b->solve.counters, c.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c.size(); ++i) out[i] = c[i];
+    return FLUID_OK;
+}
+
+int fluid_batch_synchronize(FluidBatch* b) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return FLUID_OK;
+}
+
+int fluid_batch_profile(FluidBatch* b, uint32_t mask) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->timer.reset();
+    b->timer.mask = mask;
+    return FLUID_OK;
+}
+
+int fluid_batch_profile_read(FluidBatch* b, double* ms, uint64_t* launches) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->timer.collect();
+    for (int k = 0; k < KERNEL_COUNT_; ++k) {
+        if (ms) ms[k] = b->timer.ms[k];
+        if (launches) launches[k] = b->timer.launches[k];
+    }
+    return FLUID_OK;
+}
+
+// Picks the sweeps' LDS staging capacity (slots a workgroup stages) by
+// timing `trials` substeps at each candidate, from a saved copy of the
+// resting set, then restores that set: the results do not depend on the
+// capacity (a tile too big to stage gathers the same values from global
+// memory), only the time does. Writes the chosen capacity.
+int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t* refine, uint32_t trials,
+                         uint32_t* chosen) {
+    if (!valid(b) || !steps144) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const size_t n = b->slots;
+    Records save{};
+    DeviceAllocs tmp;
+    int rc = FLUID_OK;
+    if ((rc = tmp.alloc(save.pos, n, false)) || (rc = tmp.alloc(save.vel, n, false)) ||
+        (rc = tmp.alloc(save.prev_vel, n, false)) || (rc = tmp.alloc(save.prev_pressure, n, false)) ||
+        (rc = tmp.alloc(save.temperature, n, false))) {
+        tmp.release();
+        return rc;
+    }
+    auto copy = [&](const Records& from, const Records& to) -> int {
+        HIP_TRY(hipMemcpyAsync(to.pos, from.pos, n * 16, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(to.vel, from.vel, n * 16, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(to.prev_vel, from.prev_vel, n * 16, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(to.prev_pressure, from.prev_pressure, n * 4, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(to.temperature, from.temperature, n * 4, hipMemcpyDeviceToDevice, b->stream));
+        return FLUID_OK;
+    };
+    std::vector<uint32_t> caps;
+    for (uint32_t c : {0u, 1024u, 2048u, 2560u, 3072u, 3328u})
+        if (c <= b->stage_cap_max) caps.push_back(c);
+    if (caps.empty() || caps.back() != b->stage_cap_max) caps.push_back(b->stage_cap_max);
+    const uint32_t keep = b->solve.stage_cap;
+    const bool graph = b->use_graph;
+    b->use_graph = false;  // graphs bake the capacity in
+    b->drop_graphs();
+    double best = 1e300;
+    uint32_t best_cap = keep;
+    if ((rc = copy(b->rest, save))) goto done;
+    for (uint32_t c : caps) {
+        b->solve.stage_cap = c;
+        if ((rc = run_substep(b, steps144, refine))) goto done;  // warm-up
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        const auto t0 = std::chrono::steady_clock::now();
+        for (uint32_t i = 0; i < std::max<uint32_t>(trials, 1); ++i)
+            if ((rc = run_substep(b, steps144, refine))) goto done;
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt < best) {
+            best = dt;
+            best_cap = c;
+        }
+        if ((rc = copy(save, b->rest))) goto done;
+    }
+done:
+    (void)hipStreamSynchronize(b->stream);
+    tmp.release();
+    b->solve.stage_cap = rc == FLUID_OK ? best_cap : keep;
+    b->use_graph = graph;
+    if (chosen) *chosen = b->solve.stage_cap;
+    return rc;
+}
+
+int fluid_batch_describe(const FluidBatch* b, char* buf, size_t size) {
+    if (!valid(b) || !buf || size == 0) return FLUID_ERR_ARG;
+    snprintf(buf, size,
+             "{\"scenes\": %u, \"slots\": %u, \"cells\": %u, \"tiles\": %u, \"nbr_cap\": %u, \"stage_cap\": %u, "
+             "\"lds_bytes\": %zu, \"graph\": %s, \"tolerance\": %g, \"max_passes\": %u}",
+             b->scenes, b->slots, b->cells, b->tiles, b->cap, b->solve.stage_cap, b->lds_bytes,
+             b->use_graph ? "true" : "false", (double)b->tol.tol, b->tol.max_passes);
+    return FLUID_OK;
+}
+
+}  // extern "C"

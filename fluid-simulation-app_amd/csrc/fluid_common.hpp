@@ -72,6 +72,11 @@ struct SceneK {
     float inv_h_wall;  // 1.0 / h as wall_grad_sum and wall_adh_sum evaluate it
     float coh_c, coh_k, coh_near, gamma, beta;
     float visc2, heat_cond, heat_visc, eps_h2;
+    // Where the scene lives in a batch (0, 0, count, C for a lone scene): its
+    // cells are [cell_offset, cell_offset + cells) of the shared histogram,
+    // plus one padding cell after them that the slots past its particles
+    // sort into; its particles are slots [slot_begin, slot_end).
+    uint32_t cell_offset, cells, slot_begin, slot_end;
 };
 
 // Substep constants: the Step block plus the exponentials the shader
@@ -90,6 +95,8 @@ struct StepK {
     float chase;     // 1 - exp(-LENS_RATE dt), sim_solve.wgsl:741
     float decay;     // exp(-dt / T_CHARGE), sim_tracers.wgsl:182
     float dt_guard;  // max(dt, 1e-9), sim_solve.wgsl:516
+    uint32_t refine;    // density-solve passes of this substep (render.rs:77-79)
+    uint32_t active;    // 0: the scene sits this substep out (batched steps)
 };
 
 SceneK make_scene(const SimParamsBlock& p);

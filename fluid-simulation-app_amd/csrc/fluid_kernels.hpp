@@ -9,11 +9,49 @@
 
 namespace fluid {
 
-// Cell arithmetic only (the SimParams head every grid kernel reads).
+// Cell arithmetic only (the SimParams head the tracer splat reads).
 struct GridGeom {
     float box_min[3];
     float cell;
     uint32_t dims[3];
+};
+
+// Where a kernel takes its scene and substep constants from, per solve tile
+// (kSolveTile consecutive slots, never shared by two scenes):
+//   OneScene    the plain step: one scene, by value in the kernel arguments;
+//   SceneTable  the batched step: per-scene tables in device memory and the
+//               scene of each tile.
+// The den_kappa / den_apply pass argument of a launch that runs whatever
+// the scene's pass count (everything but the density passes).
+constexpr uint32_t kEveryPass = 0xffffffffu;
+
+struct OneScene {
+    SceneK k;
+    StepK s;
+    __device__ __forceinline__ uint32_t index(uint32_t) const { return 0; }
+    __device__ __forceinline__ const SceneK& scene(uint32_t) const { return k; }
+    __device__ __forceinline__ const StepK& step(uint32_t) const { return s; }
+    // Whether the tile takes part in a launch of density pass `pass`.
+    __device__ __forceinline__ bool runs(uint32_t, uint32_t pass) const {
+        return s.active != 0u && (pass == kEveryPass || pass < s.refine);
+    }
+};
+struct SceneTable {
+    const SceneK* ks;             // [scenes]
+    const StepK* ss;              // [scenes]
+    const uint32_t* tile_scene;   // [tiles]
+    // [scenes] density passes each scene runs this substep, lowered when it
+    // converges (tolerance mode); null: the Step's refine count.
+    const uint32_t* pass_limit;
+    __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile]; }
+    __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[tile_scene[tile]]; }
+    __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[tile_scene[tile]]; }
+    __device__ __forceinline__ bool runs(uint32_t tile, uint32_t pass) const {
+        const uint32_t i = tile_scene[tile];
+        if (ss[i].active == 0u) return false;
+        if (pass == kEveryPass) return true;
+        return pass < (pass_limit ? pass_limit[i] : ss[i].refine);
+    }
 };
 
 // The five persistent records of one particle set (resting or working),
@@ -70,9 +108,14 @@ struct SolveBufs {
     uint32_t* nbr_n;  // [N]
     uint32_t cap;     // a multiple of 4
     uint32_t stride;  // >= the tiles' slot count
-    uint32_t* counters;  // [0] CFL clamps, [1] neighbour overflow (cumulative)
+    uint32_t* counters;  // per scene: [0] CFL clamps, [1] neighbour overflow (cumulative)
     // Staging capacity (slots) shared by every sweep.
     uint32_t stage_cap;
+    // Tolerance mode (batched): per scene, the fixed-point (2^-32) sum of the
+    // density errors max(rho* - rho0, 0) / rho0 of the current pass, and the
+    // passes run. Null in the reference schedule.
+    unsigned long long* residual;
+    uint32_t* iterations;
     TileRegion* regions;  // [tiles]
 };
 
@@ -111,19 +154,36 @@ struct KernelTimer {
 };
 
 uint32_t scan_tiles(uint32_t cells);
-void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Records& rest, const Records& work,
-                       const GridBufs& b, KernelTimer* timer);
+// The grid build over `slots` resting slots and `cells` histogram cells
+// (a batch's cells include one padding cell per scene).
+void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,
+                       const Records& work, const GridBufs& b, KernelTimer* timer);
+void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uint32_t cells, const Records& rest,
+                       const Records& work, const GridBufs& b, KernelTimer* timer);
 void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,
                       uint32_t* ticket);
 
 // One substep's solve on the working set (after the grid build), ending with
 // integrate writing the resting set: density_div, div_apply, forces_eval,
-// forces_den_apply, refine x (den_kappa, den_apply), integrate.
-void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
-                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t refine,
+// forces_den_apply, passes x (den_kappa, den_apply), integrate. A scene runs
+// min(passes, its refine count) density passes.
+void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts, const Records& work,
+                  const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer);
+void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
+                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
                   KernelTimer* timer);
+// Tolerance mode of a batch: up to max_passes density passes, a scene
+// stopping after the first pass (at least min_passes) whose mean density
+// error is <= tol. sc.pass_limit and b.residual / b.iterations are set.
+struct Tolerance {
+    float tol;
+    uint32_t min_passes, max_passes;
+};
+void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
+                            const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
+                            const Tolerance& tol, uint32_t* pass_limit, KernelTimer* timer);
 // The density sweep alone (the zero-substep path of the reference's tests).
-void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
+void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
                          const Records& work, const SolveBufs& b, KernelTimer* timer);
 
 struct StatsBufs {
@@ -134,6 +194,11 @@ struct StatsBufs {
 uint32_t stats_blocks(uint32_t n);
 void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
                          const uint32_t* counters, const StatsBufs& b, KernelTimer* timer);
+// Per scene of a batch, the same fold over the scene's own slots (so a
+// scene's stats are those of the plain step): partials [scenes][1024][16],
+// stats [scenes][16].
+void launch_reduce_stats(hipStream_t st, const SceneTable& sc, uint32_t scenes, const float* density,
+                         const Records& rest, const uint32_t* counters, const StatsBufs& b, KernelTimer* timer);
 
 struct TracerBufs {
     int32_t* vel_grid;   // [C][4] 16.16 sums (cleared by resolve)

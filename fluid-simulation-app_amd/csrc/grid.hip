@@ -12,7 +12,9 @@
 // Launches (all on the caller's stream, no host synchronisation):
 //   grid_count   cell id per particle; histogram via one atomic per run of
 //                equal cells in a wave; a provisional rank; clears the scan's
-//                tile state for this build.
+//                tile state for this build. In a batch, each scene's cells
+//                sit at its offset in one shared histogram, so one scan
+//                segments every scene at once.
 //   grid_scan    starts[c] = sum of counts below c, starts[C] = N; clears
 //                the counts for the next build.
 //   grid_place   provisional slot starts[c] + rank <- particle index.
@@ -57,7 +59,11 @@ __device__ __forceinline__ void store_status(uint64_t* p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ __launch_bounds__(kBlock) void k_grid_count(GridGeom g, const float4* __restrict__ pos, uint32_t n,
+// Slots past a batched scene's particles (its padding to a whole solve
+// tile) count into the scene's padding cell, the one after its own cells, so
+// the global scan leaves every scene's slots starting at a tile boundary.
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __restrict__ pos, uint32_t n,
                                                        uint32_t* __restrict__ cellid, uint32_t* __restrict__ rank,
                                                        uint32_t* __restrict__ counts, uint64_t* __restrict__ status,
                                                        uint32_t ntiles, uint32_t* __restrict__ ticket) {
@@ -66,13 +72,20 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(GridGeom g, const float4*
     if (i == 0) *ticket = 0;
     bool live = i < n;
     uint32_t c = 0xffffffffu;
-    if (live) {
-        float4 p = pos[i];
-        uint32_t cx = cell_coord(p.x, g.box_min[0], g.cell, g.dims[0]);
-        uint32_t cy = cell_coord(p.y, g.box_min[1], g.cell, g.dims[1]);
-        uint32_t cz = cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]);
-        c = (cz * g.dims[1] + cy) * g.dims[0] + cx;
-        cellid[i] = c;
+    if (blockIdx.x * kBlock < n) {
+        const SceneK& g = sc.scene(blockIdx.x);
+        if (live) {
+            if (i < g.slot_end) {
+                float4 p = pos[i];
+                uint32_t cx = cell_coord(p.x, g.box_min[0], g.cell, g.dims[0]);
+                uint32_t cy = cell_coord(p.y, g.box_min[1], g.cell, g.dims[1]);
+                uint32_t cz = cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]);
+                c = g.cell_offset + (cz * g.dims[1] + cy) * g.dims[0] + cx;
+            } else {
+                c = g.cell_offset + g.cells;
+            }
+            cellid[i] = c;
+        }
     }
     // Lanes that share a cell (neighbours in a cell-sorted resting set mostly
     // do) take one atomic between them; each gets the old count plus its rank
@@ -204,13 +217,13 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32
 
 uint32_t scan_tiles(uint32_t cells) { return (cells + kScanTile - 1) / kScanTile; }
 
-void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Records& rest, const Records& work,
-                       const GridBufs& b, KernelTimer* timer) {
-    const uint32_t cells = g.dims[0] * g.dims[1] * g.dims[2];
+template <class Sc>
+static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells, const Records& rest,
+                       const Records& work, const GridBufs& b, KernelTimer* timer) {
     const uint32_t ntiles = scan_tiles(cells);
     uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
     if (timer) timer->begin(st, KERNEL_GRID_COUNT);
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, g, rest.pos, n,
+    hipLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, sc, rest.pos, n,
                        b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
     if (timer) timer->end(st, KERNEL_GRID_COUNT);
     if (timer) timer->begin(st, KERNEL_GRID_SCAN);
@@ -224,6 +237,15 @@ void launch_grid_build(hipStream_t st, const GridGeom& g, uint32_t n, const Reco
     hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp, b.cellid,
                        b.starts, rest, work, b.perm);
     if (timer) timer->end(st, KERNEL_GRID_SCATTER);
+}
+
+void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,
+                       const Records& work, const GridBufs& b, KernelTimer* timer) {
+    grid_build(st, sc, slots, cells, rest, work, b, timer);
+}
+void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uint32_t cells, const Records& rest,
+                       const Records& work, const GridBufs& b, KernelTimer* timer) {
+    grid_build(st, sc, slots, cells, rest, work, b, timer);
 }
 
 // A stand-alone exclusive scan of `cells` counts (the grid scan, exposed for

@@ -1,0 +1,130 @@
+// Host-side helpers shared by the single-scene and batched C ABI: the
+// error channel, HIP call checking, device allocation and the per-kernel
+// HIP-event timer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "fluid_kernels.hpp"
+#include "../../include/fluid_sim.h"
+
+namespace fluid {
+
+// The last error message of this thread (fluid_sim_last_error); also
+// printed to stderr.
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::fluid::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return FLUID_ERR_HIP;                                                       \
+        }                                                                               \
+    } while (0)
+
+extern const char* kKernelNames[KERNEL_COUNT_];
+
+// FLUID_ERR_HIP with the message if the last launch failed.
+int check_launch(const char* what);
+
+// Device buffers of one handle, freed together.
+struct DeviceAllocs {
+    std::vector<void*> ptrs;
+    template <typename P>
+    int alloc(P*& out, size_t count, bool zero = true) {
+        void* p = nullptr;
+        size_t bytes = std::max<size_t>(count, 1) * sizeof(P);
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+            return FLUID_ERR_NOMEM;
+        }
+        ptrs.push_back(p);
+        if (zero) {
+            e = hipMemset(p, 0, bytes);
+            if (e != hipSuccess) {
+                set_error("hipMemset failed: %s", hipGetErrorString(e));
+                return FLUID_ERR_HIP;
+            }
+        }
+        out = static_cast<P*>(p);
+        return FLUID_OK;
+    }
+    void release() {
+        for (void* p : ptrs) (void)hipFree(p);
+        ptrs.clear();
+    }
+};
+
+class EventTimer final : public KernelTimer {
+  public:
+    uint32_t mask = 0;
+    struct Pair {
+        hipEvent_t a, b;
+        int kernel;
+    };
+    std::vector<Pair> pending;
+    std::vector<hipEvent_t> pool;
+    hipEvent_t open[KERNEL_COUNT_] = {};
+    double ms[KERNEL_COUNT_] = {};
+    uint64_t launches[KERNEL_COUNT_] = {};
+
+    ~EventTimer() override {
+        for (auto& p : pending) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t take() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+    bool wants(int kernel) const { return (mask >> kernel) & 1u; }
+    void begin(hipStream_t st, int kernel) override {
+        if (!wants(kernel)) return;
+        open[kernel] = take();
+        (void)hipEventRecord(open[kernel], st);
+    }
+    void end(hipStream_t st, int kernel) override {
+        if (!wants(kernel) || !open[kernel]) return;
+        hipEvent_t e = take();
+        (void)hipEventRecord(e, st);
+        pending.push_back({open[kernel], e, kernel});
+        open[kernel] = nullptr;
+    }
+    void collect() {
+        for (auto& p : pending) {
+            (void)hipEventSynchronize(p.b);
+            float t = 0.0f;
+            (void)hipEventElapsedTime(&t, p.a, p.b);
+            ms[p.kernel] += t;
+            launches[p.kernel] += 1;
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void reset() {
+        collect();
+        std::fill(ms, ms + KERNEL_COUNT_, 0.0);
+        std::fill(launches, launches + KERNEL_COUNT_, 0);
+    }
+};
+
+
+}  // namespace fluid

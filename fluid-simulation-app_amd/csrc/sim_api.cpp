@@ -17,8 +17,7 @@
 #include <vector>
 
 #include "fluid_common.hpp"
-#include "fluid_kernels.hpp"
-#include "../../include/fluid_sim.h"
+#include "host_common.hpp"
 
 namespace fluid {
 
@@ -55,6 +54,10 @@ SceneK make_scene(const SimParamsBlock& p) {
     k.heat_cond = 2.0f * CONDUCTIVITY / HEAT_CAPACITY;
     k.heat_visc = 0.5f * DYNAMIC_VISCOSITY / HEAT_CAPACITY;
     k.eps_h2 = 0.01f * p.h * p.h;
+    k.cell_offset = 0;
+    k.cells = p.dims[0] * p.dims[1] * p.dims[2];
+    k.slot_begin = 0;
+    k.slot_end = p.count;
     return k;
 }
 
@@ -76,99 +79,11 @@ StepK make_step(const StepBlock& s) {
     k.chase = 1.0f - std::exp(-LENS_RATE * s.dt);
     k.decay = std::exp(-s.dt / T_CHARGE);
     k.dt_guard = std::max(s.dt, 1e-9f);
+    k.refine = 0;
+    k.active = 1;
     return k;
 }
 
-namespace {
-
-thread_local std::string g_error;
-
-void set_error(const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    g_error = buf;
-    fprintf(stderr, "fluid_sim: %s\n", buf);
-}
-
-#define HIP_TRY(expr)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (expr);                                                         \
-        if (e_ != hipSuccess) {                                                         \
-            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-            return FLUID_ERR_HIP;                                                       \
-        }                                                                               \
-    } while (0)
-
-const char* kKernelNames[KERNEL_COUNT_] = {
-    "grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
-    "forces_den_apply", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"};
-
-class EventTimer final : public KernelTimer {
-  public:
-    uint32_t mask = 0;
-    struct Pair {
-        hipEvent_t a, b;
-        int kernel;
-    };
-    std::vector<Pair> pending;
-    std::vector<hipEvent_t> pool;
-    hipEvent_t open[KERNEL_COUNT_] = {};
-    double ms[KERNEL_COUNT_] = {};
-    uint64_t launches[KERNEL_COUNT_] = {};
-
-    ~EventTimer() override {
-        for (auto& p : pending) {
-            (void)hipEventDestroy(p.a);
-            (void)hipEventDestroy(p.b);
-        }
-        for (auto e : pool) (void)hipEventDestroy(e);
-    }
-    hipEvent_t take() {
-        if (!pool.empty()) {
-            hipEvent_t e = pool.back();
-            pool.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        (void)hipEventCreate(&e);
-        return e;
-    }
-    bool wants(int kernel) const { return (mask >> kernel) & 1u; }
-    void begin(hipStream_t st, int kernel) override {
-        if (!wants(kernel)) return;
-        open[kernel] = take();
-        (void)hipEventRecord(open[kernel], st);
-    }
-    void end(hipStream_t st, int kernel) override {
-        if (!wants(kernel) || !open[kernel]) return;
-        hipEvent_t e = take();
-        (void)hipEventRecord(e, st);
-        pending.push_back({open[kernel], e, kernel});
-        open[kernel] = nullptr;
-    }
-    void collect() {
-        for (auto& p : pending) {
-            (void)hipEventSynchronize(p.b);
-            float t = 0.0f;
-            (void)hipEventElapsedTime(&t, p.a, p.b);
-            ms[p.kernel] += t;
-            launches[p.kernel] += 1;
-            pool.push_back(p.a);
-            pool.push_back(p.b);
-        }
-        pending.clear();
-    }
-    void reset() {
-        collect();
-        std::fill(ms, ms + KERNEL_COUNT_, 0.0);
-        std::fill(launches, launches + KERNEL_COUNT_, 0);
-    }
-};
-
-}  // namespace
 }  // namespace fluid
 
 using namespace fluid;
@@ -188,48 +103,23 @@ struct FluidSim {
     bool track_ids = false;
     size_t lds_bytes = 0;
     EventTimer timer;
-    std::vector<void*> allocs;
+    DeviceAllocs mem;
 
     KernelTimer* t() { return timer.mask ? &timer : nullptr; }
 
     template <typename P>
     int alloc(P*& out, size_t count, bool zero = true) {
-        void* p = nullptr;
-        size_t bytes = std::max<size_t>(count, 1) * sizeof(P);
-        hipError_t e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
-            return FLUID_ERR_NOMEM;
-        }
-        allocs.push_back(p);
-        if (zero) {
-            e = hipMemset(p, 0, bytes);
-            if (e != hipSuccess) {
-                set_error("hipMemset failed: %s", hipGetErrorString(e));
-                return FLUID_ERR_HIP;
-            }
-        }
-        out = static_cast<P*>(p);
-        return FLUID_OK;
+        return mem.alloc(out, count, zero);
     }
 
     ~FluidSim() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void* p : allocs) (void)hipFree(p);
+        mem.release();
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
 namespace {
-
-int check_launch(const char* what) {
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) {
-        set_error("%s: kernel launch failed: %s", what, hipGetErrorString(e));
-        return FLUID_ERR_HIP;
-    }
-    return FLUID_OK;
-}
 
 int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_cap) {
     std::memcpy(&s->params, params64, sizeof s->params);
@@ -338,9 +228,10 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
     // dt = 0 encodes no solve: kappa divides by dt^2 (render.rs:3165-3167).
     if (!(sb.dt > 0.0f) || nsub == 0) return FLUID_OK;
     const StepK k = make_step(sb);
+    const OneScene sc{s->scene, k};
     for (uint32_t i = 0; i < nsub; ++i) {
-        launch_grid_build(s->stream, s->geom, s->n, s->rest, s->work, s->grid, s->t());
-        launch_solve(s->stream, s->scene, k, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
+        launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
+        launch_solve(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
     }
     return check_launch("substep");
 }
@@ -349,7 +240,7 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
 
 extern "C" {
 
-const char* fluid_sim_last_error(void) { return g_error.c_str(); }
+const char* fluid_sim_last_error(void) { return last_error(); }
 
 int fluid_sim_device_count(void) {
     int n = 0;
@@ -523,9 +414,9 @@ int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
     if (!valid(s) || !step144) return FLUID_ERR_ARG;
     StepBlock sb;
     std::memcpy(&sb, step144, sizeof sb);
-    const StepK k = make_step(sb);
-    launch_grid_build(s->stream, s->geom, s->n, s->rest, s->work, s->grid, s->t());
-    launch_density_only(s->stream, s->scene, k, s->n, s->grid.starts, s->work, s->solve, s->t());
+    const OneScene sc{s->scene, make_step(sb)};
+    launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
+    launch_density_only(s->stream, sc, s->n, s->grid.starts, s->work, s->solve, s->t());
     return check_launch("density_only");
 }
 

@@ -55,17 +55,21 @@ __device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
     return (cz * k.dims[1] + cy) * k.dims[0] + cx;
 }
 
-// One thread a tile: the tile's nine row ranges.
-__global__ __launch_bounds__(kBlock) void k_tile_regions(SceneK k, uint32_t n, const uint32_t* __restrict__ starts,
+// One thread a tile: the tile's nine row ranges (cells of the tile's own
+// scene; starts indexed at the scene's cell offset).
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_tile_regions(Sc sc, const uint32_t* __restrict__ starts,
                                                          const float4* __restrict__ pos, TileRegion* __restrict__ out,
                                                          uint32_t tiles, uint32_t stage_cap) {
     const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
     if (t >= tiles) return;
+    const SceneK& k = sc.scene(t);
     const uint32_t s0 = t * kTile;
-    const uint32_t s1 = s0 + kTile < n ? s0 + kTile : n;
+    const uint32_t s1 = s0 + kTile < k.slot_end ? s0 + kTile : k.slot_end;
     const int64_t c_first = cell_of_pos(k, pos[s0]);
     const int64_t c_last = cell_of_pos(k, pos[s1 - 1]);
-    const int64_t cells = (int64_t)k.dims[0] * k.dims[1] * k.dims[2];
+    const int64_t cells = k.cells;
+    const uint32_t* __restrict__ sst = starts + k.cell_offset;
     const int64_t plane = (int64_t)k.dims[0] * k.dims[1];
     TileRegion R;
     uint32_t total = 0;
@@ -78,8 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_tile_regions(SceneK k, uint32_t n, c
         if (hi > cells - 1) hi = cells - 1;
         uint32_t a = 0, e = 0;
         if (lo <= hi) {
-            a = starts[lo];
-            e = starts[hi + 1];
+            a = sst[lo];
+            e = sst[hi + 1];
         }
         R.delta[r] = (int32_t)total - (int32_t)a;
         R.base[r] = total;
@@ -180,15 +184,18 @@ struct LaneId {
     bool live;
 };
 
-__device__ __forceinline__ LaneId lane_id(uint32_t n) {
+// The particle and lane of this thread; `live` when the slot holds one of
+// the scene's particles (slots past them pad the scene to whole tiles).
+__device__ __forceinline__ LaneId lane_id(uint32_t tile, uint32_t slot_end) {
     LaneId id;
-    id.tile = tile_index();
+    id.tile = tile;
     id.gid = id.tile * kSweepThreads + threadIdx.x;  // = p * kLanes + lane
     id.p = id.tile * kTile + threadIdx.x / kLanes;
     id.lane = threadIdx.x % kLanes;
-    id.live = id.p < n;
+    id.live = id.p < slot_end;
     return id;
 }
+
 
 // Entry m of particle p.
 __device__ __forceinline__ size_t list_entry(uint32_t stride, uint32_t p, uint32_t m) {
@@ -331,7 +338,7 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
         const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] && e2 <= st.lim;
         const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.lim) ? bx - 1 : bx;
         const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
-        const uint32_t row = live_row ? ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0] : 0u;
+        const uint32_t row = live_row ? k.cell_offset + ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0] : 0u;
         k0s[r] = live_row ? starts[row + x0] : 0u;
         k1s[r] = live_row ? starts[row + x1 + 1] : 0u;
     }
@@ -403,7 +410,7 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
             if (cz < 0 || cz >= (int)k.dims[2] || cy < 0 || cy >= (int)k.dims[1]) continue;
             const uint32_t x0 = bx > 0 ? bx - 1 : bx;
             const uint32_t x1 = bx + 1 < k.dims[0] ? bx + 1 : bx;
-            const uint32_t row = ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
+            const uint32_t row = k.cell_offset + ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0];
             const uint32_t k0 = starts[row + x0], k1 = starts[row + x1 + 1];
             for (uint32_t kk = k0; kk < k1; ++kk) {
                 const f3 pj = xyz(pos[kk]);
@@ -418,7 +425,7 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
 }
 
 template <bool STAGED>
-__device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t n, const LaneId& id,
+__device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                              const RegionS& R, const uint32_t* __restrict__ starts,
                                              const float4* __restrict__ pos, const float4* __restrict__ vel,
                                              const SolveBufs& b, const PvLds& L) {
@@ -447,7 +454,7 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     const uint32_t i = id.p;
     const uint32_t nn = found < b.cap ? found : b.cap;
     b.nbr_n[i] = nn;
-    if (found > b.cap) atomicAdd(&b.counters[1], found - b.cap);
+    if (found > b.cap) atomicAdd(&counters[1], found - b.cap);
     float rho = k.mass * w_sum;
     const float gsq = k.mass * k.mass * gsq_sum;
     float drho = k.mass * drho_sum;
@@ -471,18 +478,23 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
 }
 
-__global__ __launch_bounds__(kSweepThreads) void k_density_div(SceneK k, StepK s, uint32_t n,
-                                                               const uint32_t* __restrict__ starts,
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint32_t* __restrict__ starts,
                                                                const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const LaneId id = lane_id(n);
+    const uint32_t tile = tile_index();
+    const SceneK& k = sc.scene(tile);
+    const StepK& s = sc.step(tile);
+    if (!sc.runs(tile, kEveryPass)) return;
+    const LaneId id = lane_id(tile, k.slot_end);
+    uint32_t* counters = b.counters + 2 * sc.index(tile);
     const RegionS R = load_region(b.regions, id.tile);
     const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
     if (R.staged)
-        density_tile<true>(k, s, n, id, R, starts, pos, vel, b, L);
+        density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L);
     else
-        density_tile<false>(k, s, n, id, R, starts, pos, vel, b, L);
+        density_tile<false>(k, s, counters, id, R, starts, pos, vel, b, L);
 }
 
 // ---- list sweeps -------------------------------------------------------------
@@ -540,11 +552,15 @@ __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, co
 
 // div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
 // Reads only kd of the neighbours, so the velocity update is in place.
-__global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(SceneK k, StepK s, uint32_t n,
-                                                               const float4* __restrict__ pos, float4* __restrict__ vel,
-                                                               SolveBufs b) {
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t pass, const float4* __restrict__ pos,
+                                                               float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const LaneId id = lane_id(n);
+    const uint32_t tile = tile_index();
+    const SceneK& k = sc.scene(tile);
+    const StepK& s = sc.step(tile);
+    if (!sc.runs(tile, pass)) return;
+    const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
@@ -646,12 +662,16 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     });
 }
 
-__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(SceneK k, uint32_t n, const float4* __restrict__ pos,
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel,
                                                                const float* __restrict__ temperature,
                                                                const float* __restrict__ prev_pressure, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const LaneId id = lane_id(n);
+    const uint32_t tile = tile_index();
+    const SceneK& k = sc.scene(tile);
+    if (!sc.runs(tile, kEveryPass)) return;
+    const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
     const ForcesLds L{smem, smem + b.stage_cap, reinterpret_cast<float*>(smem + 2 * b.stage_cap)};
     const f3 z = mk(0.0f, 0.0f, 0.0f);
@@ -694,12 +714,16 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
     return mk(wmix(v.x, pullx / weight, blend), wmix(v.y, pully / weight, blend), v.z);
 }
 
-__global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, StepK s, uint32_t n,
-                                                                    const float4* __restrict__ pos,
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const float4* __restrict__ pos,
                                                                     float4* __restrict__ vel,
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const LaneId id = lane_id(n);
+    const uint32_t tile = tile_index();
+    const SceneK& k = sc.scene(tile);
+    const StepK& s = sc.step(tile);
+    if (!sc.runs(tile, kEveryPass)) return;
+    const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
@@ -726,6 +750,23 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(SceneK k, St
     }
 }
 
+// The workgroup's sum of the particles' fixed-point density errors: a wave
+// sum, the waves' sums through LDS, one atomic. Integer sums, so the total
+// is the same in any order.
+__device__ __forceinline__ void block_residual(unsigned long long v, unsigned long long* out) {
+    __shared__ unsigned long long s_wave[kSweepThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (unsigned long long)__shfl_xor((long long)v, o);
+    if (__lane_id() == 0) s_wave[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kSweepThreads / 64; ++w) t += s_wave[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
 template <bool STAGED>
 __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel, const SolveBufs& b, const PvLds& L,
@@ -748,34 +789,47 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
     return fold4(drho);
 }
 
-__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(SceneK k, StepK s, uint32_t n,
-                                                             const float4* __restrict__ pos,
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const LaneId id = lane_id(n);
+    const uint32_t tile = tile_index();
+    const SceneK& k = sc.scene(tile);
+    const StepK& s = sc.step(tile);
+    if (!sc.runs(tile, pass)) return;
+    const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
     const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
     const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi)
                                  : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi);
+    unsigned long long err = 0;
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const float rho = b.density[i];
         const float4 wg = b.wall_grad[i];
         const float d = k.mass * total + dot3(vi, xyz(wg));
         const float rho_star = rho + s.dt * d;
-        const float kap = wmax(rho_star - k.rho0, 0.0f) * b.alpha[i] / (s.dt * s.dt);
+        const float excess = wmax(rho_star - k.rho0, 0.0f);
+        const float kap = excess * b.alpha[i] / (s.dt * s.dt);
         b.kd[i] = kap / rho;
         b.pressure[i] = b.pressure[i] + kap * rho;
+        if (b.residual) err = (unsigned long long)(excess / k.rho0 * 4294967296.0f);
     }
+    if (b.residual) block_residual(err, b.residual + sc.index(tile));
 }
 
 // Reads the working set, writes the resting set at the same slot.
-__global__ __launch_bounds__(kBlock) void k_integrate(SceneK k, StepK s, uint32_t n, Records work, Records rest,
-                                                      SolveBufs b) {
-    const uint32_t i = xcd_tile(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_integrate(Sc sc, Records work, Records rest, SolveBufs b) {
+    static_assert(kBlock == kTile, "integrate takes one solve tile a workgroup");
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const SceneK& k = sc.scene(tile);
+    const StepK& s = sc.step(tile);
+    if (!sc.runs(tile, kEveryPass)) return;
+    const uint32_t i = tile * kBlock + threadIdx.x;
     bool clamped_speed = false;
-    if (i < n) {
+    if (i < k.slot_end) {
         const float4 was = work.vel[i];
         f3 v = xyz(was);
         const float speed = sqrtf(dot3(v, v));
@@ -811,7 +865,35 @@ __global__ __launch_bounds__(kBlock) void k_integrate(SceneK k, StepK s, uint32_
     // One atomic per wave for the clamp counter.
     const uint64_t m = __ballot(clamped_speed);
     if (m && __lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-        atomicAdd(&b.counters[0], (uint32_t)__popcll(m));
+        atomicAdd(&b.counters[2 * sc.index(tile)], (uint32_t)__popcll(m));
+}
+
+// Tolerance mode, once per scene after den_kappa of `pass`: the mean
+// density error of the pass; a scene at or below tol (after at least
+// min_passes) runs this pass's den_apply and no further pass.
+__global__ __launch_bounds__(kBlock) void k_converge(SceneTable sc, uint32_t scenes, uint32_t pass, float tol,
+                                                     uint32_t min_passes, unsigned long long* __restrict__ residual,
+                                                     uint32_t* __restrict__ pass_limit,
+                                                     uint32_t* __restrict__ iterations) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= scenes || sc.ss[i].active == 0u || pass >= pass_limit[i]) return;
+    const SceneK& k = sc.ks[i];
+    const uint32_t n = k.slot_end - k.slot_begin;
+    const double mean = (double)residual[i] * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+    residual[i] = 0;
+    iterations[i] = pass + 1;
+    if (mean <= (double)tol && pass + 1 >= min_passes) pass_limit[i] = pass + 1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_reset_passes(uint32_t scenes, uint32_t max_passes,
+                                                         unsigned long long* __restrict__ residual,
+                                                         uint32_t* __restrict__ pass_limit,
+                                                         uint32_t* __restrict__ iterations) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= scenes) return;
+    residual[i] = 0;
+    pass_limit[i] = max_passes;
+    iterations[i] = 0;
 }
 
 constexpr size_t kPvEntry = sizeof(float4) + sizeof(float2);          // 24
@@ -830,10 +912,16 @@ uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 // budget. If the device refuses the larger limit, the capacity fits 64 KB.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
-    const void* fns[] = {reinterpret_cast<const void*>(&k_density_div), reinterpret_cast<const void*>(&k_kappa_apply),
-                         reinterpret_cast<const void*>(&k_forces_eval),
-                         reinterpret_cast<const void*>(&k_forces_den_apply),
-                         reinterpret_cast<const void*>(&k_den_kappa)};
+    const void* fns[] = {reinterpret_cast<const void*>(&k_density_div<OneScene>),
+                         reinterpret_cast<const void*>(&k_kappa_apply<OneScene>),
+                         reinterpret_cast<const void*>(&k_forces_eval<OneScene>),
+                         reinterpret_cast<const void*>(&k_forces_den_apply<OneScene>),
+                         reinterpret_cast<const void*>(&k_den_kappa<OneScene>),
+                         reinterpret_cast<const void*>(&k_density_div<SceneTable>),
+                         reinterpret_cast<const void*>(&k_kappa_apply<SceneTable>),
+                         reinterpret_cast<const void*>(&k_forces_eval<SceneTable>),
+                         reinterpret_cast<const void*>(&k_forces_den_apply<SceneTable>),
+                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
             avail = 65536 - 1024;
@@ -844,52 +932,93 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     return avail;
 }
 
-static void launch_regions(hipStream_t st, const SceneK& k, uint32_t n, const uint32_t* starts, const Records& work,
-                           const SolveBufs& b) {
-    const uint32_t tiles = solve_tiles(n);
-    hipLaunchKernelGGL(k_tile_regions, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, k, n, starts, work.pos,
-                       b.regions, tiles, b.stage_cap);
-}
-
-void launch_density_only(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
-                         const Records& work, const SolveBufs& b, KernelTimer* timer) {
+template <class Sc>
+static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
+                         const SolveBufs& b, KernelTimer* timer) {
     if (n == 0) return;
+    const uint32_t tiles = solve_tiles(n);
     if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
-    launch_regions(st, k, n, starts, work, b);
-    hipLaunchKernelGGL(k_density_div, dim3(solve_tiles(n)), dim3(kSweepThreads), b.stage_cap * kPvEntry, st, k, s,
-                       n, starts, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, sc, starts, work.pos,
+                       b.regions, tiles, b.stage_cap);
+    hipLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), b.stage_cap * kPvEntry, st, sc, starts,
+                       work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DENSITY_DIV);
 }
 
-void launch_solve(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, const uint32_t* starts,
-                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t refine,
-                  KernelTimer* timer) {
+struct NoCheck {
+    void operator()(hipStream_t, uint32_t) const {}
+};
+
+template <class Sc, class Check = NoCheck>
+static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
+                  const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
+                  const Check& check = Check()) {
     if (n == 0) return;
     const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
     const size_t lds_kappa = b.stage_cap * kKappaEntry;
-    launch_density_only(st, k, s, n, starts, work, b, timer);
+    density_only(st, sc, n, starts, work, b, timer);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
-    hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval, tiles, sweep, b.stage_cap * kForcesEntry, st, k, n, work.pos, work.vel,
+    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, b.stage_cap * kForcesEntry, st, sc, work.pos, work.vel,
                        work.temperature, work.prev_pressure, b);
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
-    hipLaunchKernelGGL(k_forces_den_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel,
+    hipLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, sc, work.pos, work.vel,
                        work.temperature, b);
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
-    for (uint32_t r = 0; r < refine; ++r) {
+    for (uint32_t r = 0; r < passes; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa, tiles, sweep, b.stage_cap * kPvEntry, st, k, s, n, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, b.stage_cap * kPvEntry, st, sc, r, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
+        check(st, r);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
-        hipLaunchKernelGGL(k_kappa_apply, tiles, sweep, lds_kappa, st, k, s, n, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, r, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_APPLY);
     }
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
-    hipLaunchKernelGGL(k_integrate, dim3(blocks_for(n)), dim3(kBlock), 0, st, k, s, n, work, rest, b);
+    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kBlock), 0, st, sc, work, rest, b);
     if (timer) timer->end(st, KERNEL_INTEGRATE);
+}
+
+void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
+                         const Records& work, const SolveBufs& b, KernelTimer* timer) {
+    density_only(st, sc, slots, starts, work, b, timer);
+}
+
+void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts, const Records& work,
+                  const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer) {
+    OneScene one = sc;
+    one.s.refine = passes;
+    SolveBufs fixed = b;
+    fixed.residual = nullptr;
+    solve(st, one, slots, starts, work, rest, fixed, passes, timer);
+}
+
+void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
+                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
+                  KernelTimer* timer) {
+    SolveBufs fixed = b;
+    fixed.residual = nullptr;
+    SceneTable t = sc;
+    t.pass_limit = nullptr;
+    solve(st, t, slots, starts, work, rest, fixed, passes, timer);
+}
+
+void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
+                            const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
+                            const Tolerance& tol, uint32_t* pass_limit, KernelTimer* timer) {
+    SceneTable t = sc;
+    t.pass_limit = pass_limit;
+    const dim3 grid(blocks_for(scenes));
+    hipLaunchKernelGGL(k_reset_passes, grid, dim3(kBlock), 0, st, scenes, tol.max_passes, b.residual, pass_limit,
+                       b.iterations);
+    auto check = [&](hipStream_t s2, uint32_t pass) {
+        hipLaunchKernelGGL(k_converge, grid, dim3(kBlock), 0, s2, t, scenes, pass, tol.tol, tol.min_passes,
+                           b.residual, pass_limit, b.iterations);
+    };
+    solve(st, t, slots, starts, work, rest, b, tol.max_passes, timer, check);
 }
 
 }  // namespace fluid

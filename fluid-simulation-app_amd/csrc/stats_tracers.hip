@@ -20,17 +20,55 @@ namespace {
 
 constexpr int kStatsMaxBlocks = 1024;
 
+// Workgroups of the stats fold for n particles (at most kStatsMaxBlocks).
+__host__ __device__ __forceinline__ uint32_t stats_block_count(uint32_t n) {
+    uint32_t b = (n + kBlock - 1) / kBlock;
+    if (b < 1) b = 1;
+    return b < (uint32_t)kStatsMaxBlocks ? b : (uint32_t)kStatsMaxBlocks;
+}
+
 __device__ __forceinline__ float lane_min(float a, float b) { return wmin(a, b); }
 __device__ __forceinline__ float lane_max(float a, float b) { return wmax(a, b); }
+
+// Workgroup b of `blocks` folds slots base + b*256 + t, base + (b + blocks)*256
+// + t, ... below base + n into partials[b]. The batched form runs one row of
+// workgroups per scene (blockIdx.y), each over its scene's own slots with the
+// block count the plain step would take for that particle count.
+__device__ __forceinline__ void stats_partial(float rho0, uint32_t base, uint32_t n, uint32_t blocks, uint32_t b,
+                                              const float* __restrict__ density, const float4* __restrict__ vel,
+                                              const float4* __restrict__ prev_vel,
+                                              const float* __restrict__ temperature, float* __restrict__ out);
 
 __global__ __launch_bounds__(kBlock) void k_stats_partial(float rho0, uint32_t n, const float* __restrict__ density,
                                                           const float4* __restrict__ vel,
                                                           const float4* __restrict__ prev_vel,
                                                           const float* __restrict__ temperature,
                                                           float* __restrict__ partials) {
+    stats_partial(rho0, 0, n, gridDim.x, blockIdx.x, density, vel, prev_vel, temperature,
+                  partials + (size_t)blockIdx.x * 16);
+}
+
+__global__ __launch_bounds__(kBlock) void k_stats_partial_many(SceneTable sc, const float* __restrict__ density,
+                                                               const float4* __restrict__ vel,
+                                                               const float4* __restrict__ prev_vel,
+                                                               const float* __restrict__ temperature,
+                                                               float* __restrict__ partials) {
+    const SceneK& k = sc.ks[blockIdx.y];
+    const uint32_t n = k.slot_end - k.slot_begin;
+    const uint32_t blocks = stats_block_count(n);
+    if (blockIdx.x >= blocks) return;
+    stats_partial(k.rho0, k.slot_begin, n, blocks, blockIdx.x, density, vel, prev_vel, temperature,
+                  partials + ((size_t)blockIdx.y * kStatsMaxBlocks + blockIdx.x) * 16);
+}
+
+__device__ __forceinline__ void stats_partial(float rho0, uint32_t base, uint32_t n, uint32_t blocks, uint32_t b,
+                                              const float* __restrict__ density, const float4* __restrict__ vel,
+                                              const float4* __restrict__ prev_vel,
+                                              const float* __restrict__ temperature, float* __restrict__ out) {
     float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const uint32_t stride = blocks * kBlock;
+    for (uint32_t li = b * kBlock + threadIdx.x; li < n; li += stride) {
+        const uint32_t i = base + li;
         const float d = density[i];
         const float c = d / rho0 - 1.0f;
         acc[0] += wmax(c, 0.0f);
@@ -67,15 +105,33 @@ __global__ __launch_bounds__(kBlock) void k_stats_partial(float rho0, uint32_t n
         }
         __syncthreads();
     }
-    if (threadIdx.x < 12) partials[(size_t)blockIdx.x * 16 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 12) out[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Folds the per-workgroup partials in a fixed order (thread t takes
 // partials t, t + 256, ..., then a tree over the threads), so the result is
 // the same on every run for a given particle count.
+__device__ __forceinline__ void stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
+                                            const uint32_t* __restrict__ counters, float* __restrict__ stats);
+
 __global__ __launch_bounds__(kBlock) void k_stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
                                                         const uint32_t* __restrict__ counters,
                                                         float* __restrict__ stats) {
+    stats_final(n, blocks, partials, counters, stats);
+}
+
+// One workgroup a scene: that scene's partials, counters and stats row.
+__global__ __launch_bounds__(kBlock) void k_stats_final_many(SceneTable sc, const float* __restrict__ partials,
+                                                             const uint32_t* __restrict__ counters,
+                                                             float* __restrict__ stats) {
+    const SceneK& k = sc.ks[blockIdx.x];
+    const uint32_t n = k.slot_end - k.slot_begin;
+    stats_final(n, stats_block_count(n), partials + (size_t)blockIdx.x * kStatsMaxBlocks * 16, counters + 2 * blockIdx.x,
+                stats + (size_t)blockIdx.x * 16);
+}
+
+__device__ __forceinline__ void stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
+                                            const uint32_t* __restrict__ counters, float* __restrict__ stats) {
     float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
     for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
         const float* p = partials + (size_t)b * 16;
@@ -317,11 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_advect(SceneK k, StepK s, uint32_t n
 
 }  // namespace
 
-uint32_t stats_blocks(uint32_t n) {
-    uint32_t b = blocks_for(n);
-    if (b < 1) b = 1;
-    return b < (uint32_t)kStatsMaxBlocks ? b : (uint32_t)kStatsMaxBlocks;
-}
+uint32_t stats_blocks(uint32_t n) { return stats_block_count(n); }
 
 void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
                          const uint32_t* counters, const StatsBufs& b, KernelTimer* timer) {
@@ -329,6 +381,16 @@ void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* de
     hipLaunchKernelGGL(k_stats_partial, dim3(b.blocks), dim3(kBlock), 0, st, rho0, n, density, rest.vel,
                        rest.prev_vel, rest.temperature, b.partials);
     hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, st, n, b.blocks, b.partials, counters, b.stats);
+    if (timer) timer->end(st, KERNEL_REDUCE_STATS);
+}
+
+void launch_reduce_stats(hipStream_t st, const SceneTable& sc, uint32_t scenes, const float* density,
+                         const Records& rest, const uint32_t* counters, const StatsBufs& b, KernelTimer* timer) {
+    if (scenes == 0) return;
+    if (timer) timer->begin(st, KERNEL_REDUCE_STATS);
+    hipLaunchKernelGGL(k_stats_partial_many, dim3(b.blocks, scenes), dim3(kBlock), 0, st, sc, density, rest.vel,
+                       rest.prev_vel, rest.temperature, b.partials);
+    hipLaunchKernelGGL(k_stats_final_many, dim3(scenes), dim3(kBlock), 0, st, sc, b.partials, counters, b.stats);
     if (timer) timer->end(st, KERNEL_REDUCE_STATS);
 }
 

@@ -58,6 +58,27 @@ def _declare(L):
         "fluid_sim_kernel_name": ([u32], C.c_char_p),
         "fluid_sim_describe": ([vp, C.c_char_p, C.c_size_t], i32),
         "fluid_scan_u32": ([P(u32), u32, P(u32)], i32),
+        "fluid_batch_create": ([u32, u8p, u32], vp),
+        "fluid_batch_destroy": ([vp], None),
+        "fluid_batch_scene_count": ([vp], u32),
+        "fluid_batch_slot_count": ([vp], u32),
+        "fluid_batch_scene_info": ([vp, u32, P(u32), P(u32), P(u32)], i32),
+        "fluid_batch_upload_state": ([vp, u32] + [P(f32)] * 5, i32),
+        "fluid_batch_download_state": ([vp, u32] + [P(f32)] * 5, i32),
+        "fluid_batch_download_field": ([vp, u32, u32, P(f32)], i32),
+        "fluid_batch_download_grid": ([vp, u32] + [P(u32)] * 4, i32),
+        "fluid_batch_substep": ([vp, u8p, P(u32)], i32),
+        "fluid_batch_set_tolerance": ([vp, f32, u32, u32], i32),
+        "fluid_batch_iterations": ([vp, P(u32)], i32),
+        "fluid_batch_use_graph": ([vp, C.c_bool], i32),
+        "fluid_batch_reduce_stats": ([vp], i32),
+        "fluid_batch_read_stats": ([vp, P(f32)], i32),
+        "fluid_batch_counters": ([vp, P(u64)], i32),
+        "fluid_batch_synchronize": ([vp], i32),
+        "fluid_batch_profile": ([vp, u32], i32),
+        "fluid_batch_profile_read": ([vp, P(f64), P(u64)], i32),
+        "fluid_batch_autotune": ([vp, u8p, P(u32), u32, P(u32)], i32),
+        "fluid_batch_describe": ([vp, C.c_char_p, C.c_size_t], i32),
         "fluid_hash": ([u32], u32),
         "fluid_grid_new": ([P(f32), f32, vp], None),
         "fluid_grid_cell_of": ([vp, P(f32)], u32),

@@ -128,3 +128,35 @@ def small_block(lattice=(8, 8, 8), seed=0, h_factor=1.2, spacing=None, half=(0.1
     d = spacing if spacing is not None else 0.01
     return block_scene(f"small_{lattice[0]}x{lattice[1]}x{lattice[2]}", lattice, d, half, place=place,
                        jitter=jitter, seed=seed, h_factor=h_factor, rho0=rho0, tracers=tracers)
+
+
+def fill_block(name, n, fill, half=(0.5, 0.5, 0.5), seed=0, h_factor=1.0, rho0=1000.0, jitter=0.01) -> Scene:
+    """Exactly n particles filling `fill` of the box volume: a 2:1:2 lattice
+    block on the floor (its top layer partly filled), spacing
+    d = (fill * volume / n)^(1/3)."""
+    vol = 8.0 * half[0] * half[1] * half[2]
+    d = (fill * vol / n) ** (1.0 / 3.0)
+    nx = max(1, int(round((2.0 * n) ** (1.0 / 3.0))))
+    ny = int(math.ceil(n / (nx * nx)))
+    rng = np.random.default_rng(seed)
+    corner = (-0.5 * nx * d, -half[1], -0.5 * nx * d)
+    pos4 = _lattice((nx, ny, nx), d, corner, rng, jitter)
+    keep = np.sort(np.argsort(pos4[:, 1], kind="stable")[:n])
+    pos4 = np.ascontiguousarray(pos4[keep])
+    h = h_factor * d
+    grid = Grid.new(half, 2.0 * h)
+    params = pack_sim_params(grid, n, h, rho0 * d * d * d, rho0)
+    phase = float(rng.uniform(0.0, 2.0 * math.pi))
+    return Scene(name, tuple(half), d, h, rho0, (nx, ny, nx), pos4, grid, params, np.zeros((0, 2), np.uint32),
+                 phase, seed)
+
+
+def ensemble(scenes=256, seed=0, n_min=2048, n_max=65536, fill=(0.2, 0.5), h_factor=1.0) -> list:
+    """The batched configuration: `scenes` independent unit boxes, particle
+    counts log-uniform in [n_min, n_max], fill fractions uniform in `fill`,
+    each with its own seeded tilt phase."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), size=scenes))).astype(int)
+    fills = rng.uniform(fill[0], fill[1], size=scenes)
+    return [fill_block(f"ensemble_{i}", int(counts[i]), float(fills[i]), seed=seed * 100003 + i, h_factor=h_factor)
+            for i in range(scenes)]

@@ -298,3 +298,137 @@ class Sim:
         check(self.L.fluid_sim_profile_read(self.h, ms.ctypes.data_as(C.POINTER(C.c_double)),
                                             cnt.ctypes.data_as(C.POINTER(C.c_uint64))), "profile_read")
         return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNELS) if cnt[i]}
+
+
+class Batch:
+    """Many scenes stepped together behind the batched C ABI
+    (include/fluid_sim.h, fluid_batch_*). Scene s's state, fields and grid
+    read as a lone Sim's would."""
+
+    def __init__(self, params: list, nbr_cap: int = 0):
+        blob = b"".join(params)
+        assert len(blob) == 64 * len(params) and params
+        self.L = lib()
+        self.h = self.L.fluid_batch_create(len(params), _u8p(blob), int(nbr_cap))
+        if not self.h:
+            raise RuntimeError("fluid_batch_create failed: " + self.L.fluid_sim_last_error().decode())
+        self.scenes = int(self.L.fluid_batch_scene_count(self.h))
+        self.slots = int(self.L.fluid_batch_slot_count(self.h))
+        self.info = []
+        for s in range(self.scenes):
+            a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+            check(self.L.fluid_batch_scene_info(self.h, s, C.byref(a), C.byref(b), C.byref(c)), "scene_info")
+            self.info.append((int(a.value), int(b.value), int(c.value)))
+
+    def count(self, s: int) -> int:
+        return self.info[s][1]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.fluid_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_state(self, s, pos4=None, vel4=None, prev_vel4=None, prev_pressure=None, temperature=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32)
+                for a in (pos4, vel4, prev_vel4, prev_pressure, temperature)]
+        check(self.L.fluid_batch_upload_state(self.h, int(s), *[_f32p(a) for a in arrs]), "batch upload_state")
+
+    def state(self, s) -> dict:
+        n = self.count(s)
+        out = dict(pos=np.empty((n, 4), np.float32), vel=np.empty((n, 4), np.float32),
+                   prev_vel=np.empty((n, 4), np.float32), prev_pressure=np.empty(n, np.float32),
+                   temperature=np.empty(n, np.float32))
+        check(self.L.fluid_batch_download_state(self.h, int(s), *[_f32p(out[k]) for k in
+                                                                  ("pos", "vel", "prev_vel", "prev_pressure",
+                                                                   "temperature")]), "batch download_state")
+        return out
+
+    def field(self, s, name: str) -> np.ndarray:
+        which, per = FIELDS[name]
+        n = self.count(s)
+        out = np.empty(n * per, np.float32)
+        check(self.L.fluid_batch_download_field(self.h, int(s), which, _f32p(out)), "batch download_field")
+        return out.reshape(n, per) if per > 1 else out
+
+    def grid(self, s) -> dict:
+        n, c = self.count(s), self.info[s][2]
+        out = dict(cellid=np.empty(n, np.uint32), counts=np.empty(c, np.uint32),
+                   starts=np.empty(c + 1, np.uint32), perm=np.empty(n, np.uint32))
+        check(self.L.fluid_batch_download_grid(self.h, int(s), *[_u32p(out[k]) for k in
+                                                                 ("cellid", "counts", "starts", "perm")]),
+              "batch download_grid")
+        return out
+
+    def substep(self, steps144: list, refine):
+        blob = b"".join(steps144)
+        assert len(blob) == 144 * self.scenes
+        r = np.ascontiguousarray(refine, np.uint32)
+        assert r.size == self.scenes
+        check(self.L.fluid_batch_substep(self.h, _u8p(blob), _u32p(r)), "batch substep")
+
+    def set_tolerance(self, tol: float, min_passes: int = 1, max_passes: int = 0):
+        check(self.L.fluid_batch_set_tolerance(self.h, float(tol), int(min_passes), int(max_passes)),
+              "batch set_tolerance")
+
+    def iterations(self) -> np.ndarray:
+        out = np.empty(self.scenes, np.uint32)
+        check(self.L.fluid_batch_iterations(self.h, _u32p(out)), "batch iterations")
+        return out
+
+    def use_graph(self, enable=True):
+        check(self.L.fluid_batch_use_graph(self.h, bool(enable)), "batch use_graph")
+
+    def reduce_stats(self):
+        check(self.L.fluid_batch_reduce_stats(self.h), "batch reduce_stats")
+
+    def read_stats(self) -> np.ndarray:
+        out = np.empty((self.scenes, STATS), np.float32)
+        check(self.L.fluid_batch_read_stats(self.h, _f32p(out)), "batch read_stats")
+        return out
+
+    def counters(self) -> np.ndarray:
+        out = np.empty((self.scenes, 2), np.uint64)
+        check(self.L.fluid_batch_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_uint64))), "batch counters")
+        return out
+
+    def synchronize(self):
+        check(self.L.fluid_batch_synchronize(self.h), "batch synchronize")
+
+    def profile(self, kernels=None):
+        mask = 0
+        for k in (kernels or []):
+            mask |= 1 << (KERNELS.index(k) if isinstance(k, str) else int(k))
+        check(self.L.fluid_batch_profile(self.h, mask), "batch profile")
+
+    def profile_read(self) -> dict:
+        ms = np.zeros(len(KERNELS), np.float64)
+        cnt = np.zeros(len(KERNELS), np.uint64)
+        check(self.L.fluid_batch_profile_read(self.h, ms.ctypes.data_as(C.POINTER(C.c_double)),
+                                              cnt.ctypes.data_as(C.POINTER(C.c_uint64))), "batch profile_read")
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNELS) if cnt[i]}
+
+    def autotune(self, steps144: list, refine, trials: int = 3) -> int:
+        blob = b"".join(steps144)
+        r = np.ascontiguousarray(refine, np.uint32)
+        out = C.c_uint32()
+        check(self.L.fluid_batch_autotune(self.h, _u8p(blob), _u32p(r), int(trials), C.byref(out)),
+              "batch autotune")
+        return int(out.value)
+
+    def describe(self) -> dict:
+        import json
+        buf = C.create_string_buffer(512)
+        check(self.L.fluid_batch_describe(self.h, buf, 512), "batch describe")
+        return json.loads(buf.value.decode())

@@ -218,6 +218,65 @@ uint32_t fluid_substeps_for(float dt, float v_max, float spacing, uint32_t cap);
 float fluid_spacing_for(float scale, const float extent[2]);                      /* render.rs:1362-1380 */
 float fluid_slab_depth(void);                                                     /* sim.rs:10 */
 
+/* ---- batched step ---------------------------------------------------------
+ *
+ * Many independent boxes of uneven particle counts stepped together, one
+ * launch per kernel per substep. The reference steps one box per Sim
+ * (render.rs:1382-1612, 3168-3203); a batch is the same step over a list of
+ * SimParams blocks, each scene with its own Step and density-pass count. A
+ * scene's results are those of a lone FluidSim on the same inputs, bit for
+ * bit. Scenes are indexed 0..scenes-1; state and fields are per scene, in
+ * the scene's slot order. Same conventions as FluidSim.
+ */
+typedef struct FluidBatch FluidBatch;
+
+/* `scenes` SimParams blocks of 64 bytes each, back to back (count at byte 28
+ * of each is that scene's particle count). NULL on failure. */
+FluidBatch *fluid_batch_create(uint32_t scenes, const uint8_t *params64, uint32_t nbr_cap);
+void fluid_batch_destroy(FluidBatch *batch);
+uint32_t fluid_batch_scene_count(const FluidBatch *batch);
+/* Slots of the shared particle set (each scene padded to 256 slots). */
+uint32_t fluid_batch_slot_count(const FluidBatch *batch);
+int fluid_batch_scene_info(const FluidBatch *batch, uint32_t scene, uint32_t *slot_begin, uint32_t *count,
+                           uint32_t *cells);
+/* Per scene, as fluid_sim_upload_state / fluid_sim_download_state. */
+int fluid_batch_upload_state(FluidBatch *batch, uint32_t scene, const float *pos4, const float *vel4,
+                             const float *prev_vel4, const float *prev_pressure, const float *temperature);
+int fluid_batch_download_state(FluidBatch *batch, uint32_t scene, float *pos4, float *vel4, float *prev_vel4,
+                               float *prev_pressure, float *temperature);
+int fluid_batch_download_field(FluidBatch *batch, uint32_t scene, uint32_t field, float *out);
+/* The scene's grid in its own numbering (cell ids from 0, starts from 0). */
+int fluid_batch_download_grid(FluidBatch *batch, uint32_t scene, uint32_t *cellid, uint32_t *counts,
+                              uint32_t *starts, uint32_t *perm);
+/* One substep of every scene: `steps144` holds one 144-byte Step per scene;
+ * `refine` one density-pass count per scene (render.rs:77-79). A scene
+ * whose Step has dt <= 0 sits the substep out (render.rs:3165-3167). */
+int fluid_batch_substep(FluidBatch *batch, const uint8_t *steps144, const uint32_t *refine);
+/* Tolerance mode (tol > 0): each scene runs density passes until the mean
+ * of max(rho* - rho0, 0) / rho0 over its particles is <= tol, at least
+ * min_passes and at most max_passes; `refine` is then ignored. tol <= 0
+ * restores the reference schedule. */
+int fluid_batch_set_tolerance(FluidBatch *batch, float tol, uint32_t min_passes, uint32_t max_passes);
+/* Density passes each scene ran in the last tolerance-mode substep. */
+int fluid_batch_iterations(FluidBatch *batch, uint32_t *out);
+/* Capture each substep shape once in a hipGraph and replay it. */
+int fluid_batch_use_graph(FluidBatch *batch, bool enable);
+/* Per scene, as fluid_sim_reduce_stats / fluid_sim_read_stats: out holds
+ * scenes x 14 floats. */
+int fluid_batch_reduce_stats(FluidBatch *batch);
+int fluid_batch_read_stats(FluidBatch *batch, float *out);
+/* scenes x (clamps, overflow). */
+int fluid_batch_counters(FluidBatch *batch, uint64_t *out);
+int fluid_batch_synchronize(FluidBatch *batch);
+int fluid_batch_profile(FluidBatch *batch, uint32_t kernel_mask);
+int fluid_batch_profile_read(FluidBatch *batch, double *ms, uint64_t *launches);
+/* Times `trials` substeps at each candidate LDS staging capacity from a
+ * saved copy of the state (restored afterwards; results do not depend on
+ * the capacity) and keeps the fastest; writes it to *chosen. */
+int fluid_batch_autotune(FluidBatch *batch, const uint8_t *steps144, const uint32_t *refine, uint32_t trials,
+                         uint32_t *chosen);
+int fluid_batch_describe(const FluidBatch *batch, char *buf, size_t size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,77 @@ def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=8):
                       f"substep, {dt:.1f} s"}
 
 
+def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=4, seed=0, autotune=True):
+    """The batched configuration: `scenes` ensemble boxes stepped as one
+    batch (one launch per kernel per substep, replayed from a hipGraph),
+    against looping the plain step over the same scenes. Each scene takes its
+    own reference schedule (dt_sub, R) and tilt; both legs run the same
+    substeps, from the same initial state, inputs resident on the GPU."""
+    ens = fa.scenes.ensemble(scenes, seed=seed)
+    total = sum(sc.n for sc in ens)
+    sched = [sc.schedule() for sc in ens]
+    plan = []
+    for k in range(warmup + max(steps, plain_rounds) + 1):
+        plan.append(([sc.step_block(k * sd[1], sd[1], sd[3], seed=k) for sc, sd in zip(ens, sched)],
+                     [sd[2] for sd in sched]))
+    b = fa.Batch([sc.params for sc in ens])
+    for i, sc in enumerate(ens):
+        b.upload_state(i, sc.pos4)
+    tuned = b.autotune(plan[0][0], plan[0][1], trials=2) if autotune else None
+    b.use_graph(True)
+    for k in range(warmup):
+        b.substep(*plan[k])
+    b.synchronize()
+    t0 = time.perf_counter()
+    for k in range(warmup, warmup + steps):
+        b.substep(*plan[k])
+    b.synchronize()
+    t_batch = time.perf_counter() - t0
+    b.use_graph(False)
+    b.profile(fa.KERNELS)
+    b.substep(*plan[warmup + steps])
+    prof = b.profile_read()
+    b.profile([])
+    desc = b.describe()
+    b.close()
+    sims = []
+    for sc in ens:
+        s = fa.Sim(sc.params)
+        s.upload_state(sc.pos4)
+        sims.append(s)
+    for s, st, r in zip(sims, *plan[0]):
+        s.substeps(st, 1, r)
+    for s in sims:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for k in range(1, 1 + plain_rounds):
+        for s, st, r in zip(sims, *plan[k]):
+            s.substeps(st, 1, r)
+    for s in sims:
+        s.synchronize()
+    t_plain = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    batched = total * steps / t_batch
+    plain = total * plain_rounds / t_plain
+    canon = sum(canonical_bytes(sc.n, sc.grid.cell_count(), sd[2]) for sc, sd in zip(ens, sched))
+    return {
+        "metric": "DFSPH particle-steps/s, batched ensemble",
+        "value": batched, "unit": "particle-substeps/s",
+        "ms_per_step": t_batch * 1e3 / steps, "steps": steps, "warmup": warmup,
+        "config": {"workload": f"ensemble_{scenes}", "scenes": scenes, "particles": total,
+                   "counts": "log-uniform [2048, 65536]", "fill": "U(0.2, 0.5)", "seed": seed,
+                   "refine": sorted(set(sd[2] for sd in sched)), "graph": True},
+        "plain_loop": {"value": plain, "ms_per_step": t_plain * 1e3 / plain_rounds, "rounds": plain_rounds,
+                       "what": "fluid_sim_substeps over each scene in turn, one stream, same substeps"},
+        "batched_over_plain": batched / plain,
+        "step_roofline": {"canonical_bytes": canon, "achieved_gbs": canon / (t_batch / steps) / 1e9,
+                          "frac": canon / (t_batch / steps) / 1e9 / HBM_PEAK_GBS},
+        "per_kernel_ms": {k: v[0] for k, v in prof.items()},
+        "launch": desc, "autotuned_stage_cap": tuned,
+    }
+
+
 def copy_calibration(nbytes: int, reps: int = 4) -> None:
     """Runs `reps` device-to-device copies of `nbytes` (HIP's own copy kernel)
     so that a counter run can read what FETCH_SIZE / WRITE_SIZE report for a
@@ -102,6 +173,9 @@ def main():
     ap.add_argument("--workload", default="block_1m", choices=["block_1m", "block_8m", "dam_break_16k"])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (counter calibration) and exit")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--batched-only", action="store_true", help="print only the batched-ensemble line")
+    ap.add_argument("--scenes", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
@@ -115,6 +189,9 @@ def main():
     if fa.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.batched_only:
+        print(json.dumps(batched_ensemble(fa, scenes=args.scenes, seed=args.seed)), flush=True)
+        return
     maker = {"block_1m": fa.scenes.block_1m, "block_8m": fa.scenes.block_8m,
              "dam_break_16k": fa.scenes.dam_break}[args.workload]
     sc = maker(seed=args.seed)
@@ -197,6 +274,8 @@ def main():
         "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
                   "compression_avg": float(stats[0])},
     }
+    if not args.no_batched:
+        out["batched"] = batched_ensemble(fa, scenes=args.scenes, seed=args.seed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, steps[0], refine)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -87,23 +87,26 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __re
             cellid[i] = c;
         }
     }
-    // Lanes that share a cell (neighbours in a cell-sorted resting set mostly
-    // do) take one atomic between them; each gets the old count plus its rank
-    // among the equal lanes below it.
-    uint64_t active = __ballot(live);
-    uint32_t lane = lane_id();
-    uint32_t mine = 0;
-    while (active) {
-        int leader = __ffsll((unsigned long long)active) - 1;
-        uint32_t lc = (uint32_t)__shfl((int)c, leader);
-        uint64_t m = __ballot(c == lc) & active;
-        uint32_t base = 0;
-        if ((int)lane == leader) base = atomicAdd(&counts[lc], (uint32_t)__popcll(m));
-        base = (uint32_t)__shfl((int)base, leader);
-        if ((m >> lane) & 1ull) mine = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        active &= ~m;
+    // Consecutive lanes that share a cell (a cell-sorted resting set gives
+    // runs of them) take one atomic between them, all runs of the wave at
+    // once; each lane gets the old count plus its offset in its run. Live
+    // lanes are a prefix of the wave.
+    const uint32_t lane = lane_id();
+    const uint32_t prev = (uint32_t)__shfl((int)c, lane > 0 ? (int)lane - 1 : 0);
+    const bool leader = live && (lane == 0 || prev != c);
+    const uint64_t lead = __ballot(leader);
+    const uint32_t nlive = (uint32_t)__popcll(__ballot(live));
+    uint32_t base = 0;
+    if (leader) {
+        const uint64_t above = lane == 63 ? 0ull : lead & ~((2ull << lane) - 1ull);
+        uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
+        end = end < nlive ? end : nlive;
+        base = atomicAdd(&counts[c], end - lane);
     }
-    if (live) rank[i] = mine;
+    const uint64_t upto = lane == 63 ? lead : lead & ((2ull << lane) - 1ull);
+    const uint32_t my_leader = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
+    base = (uint32_t)__shfl((int)base, (int)my_leader);
+    if (live) rank[i] = base + (lane - my_leader);
 }
 
 // One tile of kScanTile counts per workgroup, tiles taken in ticket order so

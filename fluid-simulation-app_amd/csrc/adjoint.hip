@@ -1,0 +1,455 @@
+// The adjoint of the DFSPH substep for gfx950: reverse sweeps that carry the
+// gradient of a loss on the final state back through one substep.
+//
+// What is differentiated (the adjoint's differentiable profile: near
+// pressure, cohesion, adhesion and XSPH switched off, no touches, no
+// rotation; the grid permutation and neighbour lists held fixed per
+// substep, as recorded by the taped forward):
+//   density_div   rho, alpha and the divergence kappa (sim_solve.wgsl:411-518)
+//                 including the wall fill and wall gradient (:212-328)
+//   div_apply     the divergence-free correction (:594-621, 665-668)
+//   forces_eval   Morris viscosity (:554-560) and the warm start (:586-590)
+//   forces_den_apply  gravity + viscosity, then the warm-start correction
+//   den_kappa / den_apply   the constant-density Jacobi passes (:670-703),
+//                 reversed pass by pass: each transposed Jacobi pass is a
+//                 neighbour sweep of the same shape as the forward one
+//   integrate     the speed clamp and the wall clamp (:705-756)
+// Temperature, the lens EMAs and the stats do not feed back into the motion
+// and carry no adjoint.
+//
+// Pair geometry: r = x_i - x_j, rho = |r|, grad W_ij = g r with g the list's
+// dW/r, and the Hessian H_ij = g I + hh r r^T with hh = (dw'(q)/h - g) / rho^2.
+// The lists are symmetric (the squared distance is bit-identical both ways),
+// so every pair term's adjoint is gathered by both particles from their own
+// lists; each sweep is one thread a particle over its list.
+#include "fluid_common.hpp"
+#include "fluid_kernels.hpp"
+#include "adjoint.hpp"
+
+namespace fluid {
+
+namespace {
+
+constexpr uint32_t kAdjBlock = 256;
+
+__device__ __forceinline__ f3 ld3(const float4* a, uint32_t i) { return xyz(a[i]); }
+__device__ __forceinline__ void add3(float4* a, uint32_t i, f3 v) {
+    float4 w = a[i];
+    a[i] = make_float4(w.x + v.x, w.y + v.y, w.z + v.z, w.w);
+}
+
+struct PairGeom {
+    f3 r;
+    float g, hh;
+};
+
+// dw'(q): the q-derivative of the kernel-gradient magnitude of grad_scale
+// (sim_solve.wgsl:184-196).
+__device__ __forceinline__ PairGeom pair_geom(const SceneK& k, f3 xi, f3 xj, float g) {
+    PairGeom P;
+    P.r = sub(xi, xj);
+    const float r2 = dot3(P.r, P.r);
+    const float rho = sqrtf(r2);
+    const float q = rho * k.inv_h;
+    const float dwp = q < 1.0f ? k.sigma_h * (-3.0f + 4.5f * q) : k.sigma_h * (1.5f * (2.0f - q));
+    P.g = g;
+    P.hh = rho > 1e-9f ? (dwp * k.inv_h - g) / r2 : 0.0f;
+    return P;
+}
+__device__ __forceinline__ f3 hess(const PairGeom& P, f3 u) { return add(scl(P.g, u), scl(P.hh * dot3(P.r, u), P.r)); }
+
+__device__ __forceinline__ uint32_t entry(const SolveBufs& b, uint32_t p, uint32_t m) {
+    return b.nbr[((size_t)(m / kListLanes) * b.stride + p) * kListLanes + m % kListLanes].x;
+}
+__device__ __forceinline__ float entry_g(const SolveBufs& b, uint32_t p, uint32_t m) {
+    return __uint_as_float(b.nbr[((size_t)(m / kListLanes) * b.stride + p) * kListLanes + m % kListLanes].y);
+}
+
+// Transposed 3x3 wall-gradient Jacobian times u: J rows are d wg / dx_a
+// stored as columns a (J[a] = d wg / d x_a), so J^T u = (J[0].u, J[1].u, J[2].u).
+__device__ __forceinline__ f3 jt(const float4* J, uint32_t i, f3 u) {
+    return mk(dot3(xyz(J[3 * i]), u), dot3(xyz(J[3 * i + 1]), u), dot3(xyz(J[3 * i + 2]), u));
+}
+
+// Per particle: the gradient of the wall fill (grad F) and the Jacobian of
+// wall_grad_sum, by dual evaluation of the same polynomials.
+__global__ __launch_bounds__(kAdjBlock) void k_wall_jacobians(SceneK k, uint32_t n, const float4* __restrict__ pos,
+                                                              float4* __restrict__ dfill, float4* __restrict__ J) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const f3 p = xyz(pos[i]);
+    float df[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const Dn px(p.x, a == 0 ? 1.0f : 0.0f), py(p.y, a == 1 ? 1.0f : 0.0f), pz(p.z, a == 2 ? 1.0f : 0.0f);
+        df[a] = wall_fill_t(k, px, py, pz).d;
+        Dn gx, gy, gz;
+        wall_grad_sum_t(k, px, py, pz, gx, gy, gz);
+        J[3 * i + a] = make_float4(gx.d, gy.d, gz.d, 0.0f);
+    }
+    dfill[i] = make_float4(df[0], df[1], df[2], 0.0f);
+}
+
+// integrate (sim_solve.wgsl:709-756) in reverse. lam_x, lam_v arrive as the
+// adjoints of the resting position and velocity, which integrate writes at
+// the working slot; lam_p is the adjoint of the pressure it stores as the
+// next substep's previous pressure. Outputs, in place: lam_x of the working
+// position and lam_v of the velocity entering integrate.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_integrate(SceneK k, StepK s, uint32_t n,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vin,
+                                                             const float* __restrict__ density,
+                                                             float4* __restrict__ lam_x, float4* __restrict__ lam_v) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    f3 v = xyz(vin[i]);
+    const float speed = sqrtf(dot3(v, v));
+    const float rho = density[i];
+    const float ceiling = rho < 0.5f * k.rho0 ? 3.0f * s.v_clamp : s.v_clamp;
+    const bool clamped = speed > ceiling;
+    if (clamped) v = scl(ceiling / speed, v);
+    const f3 p = xyz(pos[i]);
+    const float in[3] = {p.x + s.dt * v.x, p.y + s.dt * v.y, p.z + s.dt * v.z};
+    const f3 lx = xyz(lam_x[i]), lvo = xyz(lam_v[i]);
+    const float lxa[3] = {lx.x, lx.y, lx.z}, lva[3] = {lvo.x, lvo.y, lvo.z};
+    float ox[3], ov[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float c = wmin(wmax(in[a], k.wall_lo[a]), k.wall_hi[a]);
+        const bool hit = c != in[a];
+        ox[a] = hit ? 0.0f : lxa[a];
+        ov[a] = hit ? 0.0f : lva[a] + s.dt * lxa[a];
+    }
+    f3 lv = mk(ov[0], ov[1], ov[2]);
+    if (clamped) {
+        const f3 u = scl(1.0f / speed, xyz(vin[i]));
+        lv = scl(ceiling / speed, sub(lv, scl(dot3(u, lv), u)));
+    }
+    lam_x[i] = make_float4(ox[0], ox[1], ox[2], 0.0f);
+    lam_v[i] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+}
+
+// An apply sweep in reverse (div_apply, the warm start of forces_den_apply,
+// den_apply): v_out = v_in - dt (m sum (kd_i + kd_j) gradW_ij + kd_i wg_i).
+// lam (the adjoint of v_out) passes to v_in unchanged; this sweep yields
+// lam_kd and adds the position terms to lam_x.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_apply(SceneK k, float dt, uint32_t n, SolveBufs b,
+                                                         const float4* __restrict__ pos,
+                                                         const float* __restrict__ kd,
+                                                         const float4* __restrict__ wg, const float4* __restrict__ J,
+                                                         const float4* __restrict__ lam, float* __restrict__ lam_kd,
+                                                         float4* __restrict__ lam_x) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const f3 xi = ld3(pos, i), li = ld3(lam, i);
+    const float ki = kd[i];
+    float s_lk = 0.0f;
+    f3 s_x = mk(0.0f, 0.0f, 0.0f);
+    const uint32_t nn = b.nbr_n[i];
+    for (uint32_t m = 0; m < nn; ++m) {
+        const uint32_t j = entry(b, i, m);
+        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
+        const f3 dl = sub(li, ld3(lam, j));
+        s_lk += dot3(dl, scl(P.g, P.r));
+        s_x = add(s_x, scl(ki + kd[j], hess(P, dl)));
+    }
+    lam_kd[i] = -dt * (k.mass * s_lk + dot3(li, ld3(wg, i)));
+    const f3 w = jt(J, i, li);
+    add3(lam_x, i, scl(-dt, add(scl(k.mass, s_x), scl(ki, w))));
+}
+
+// den_kappa in reverse, per particle: from lam_kd of the pass (and the
+// pressure adjoint lam_p, p += kappa rho) to lam_D, adding to lam_rho and
+// lam_alpha. kappa = max(rho + dt D - rho0, 0) alpha / dt^2, kd = kappa / rho.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_den_kappa_pre(StepK s, uint32_t n, const float* __restrict__ rho,
+                                                                 const float* __restrict__ alpha,
+                                                                 const float* __restrict__ kd,
+                                                                 const float* __restrict__ lam_kd,
+                                                                 const float* __restrict__ lam_p,
+                                                                 float* __restrict__ lam_D,
+                                                                 float* __restrict__ lam_rho,
+                                                                 float* __restrict__ lam_alpha) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const float r = rho[i], a = alpha[i], k = kd[i], lk = lam_kd[i], lp = lam_p[i];
+    const float kap = k * r;
+    const float lkap = lk / r + lp * r;
+    float lr = -lk * k / r + lp * kap;
+    float lD = 0.0f;
+    if (k > 0.0f) {
+        const float dt2 = s.dt * s.dt;
+        const float le = lkap * a / dt2;
+        lam_alpha[i] += lkap * k * r / a;  // lkap * e / dt^2, e = kd rho dt^2 / alpha
+        lr += le;
+        lD = s.dt * le;
+    }
+    lam_rho[i] += lr;
+    lam_D[i] = lD;
+}
+
+// The sweep of a divergence-type term D_i = m sum (v_i - v_j).gradW_ij +
+// v_i.wg_i in reverse (den_kappa, and density_div's D): lam_v_i = m sum
+// (lam_D_i + lam_D_j) gradW_ij + lam_D_i wg_i, and the position terms.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_divergence(SceneK k, uint32_t n, SolveBufs b,
+                                                              const float4* __restrict__ pos,
+                                                              const float4* __restrict__ vel,
+                                                              const float4* __restrict__ wg,
+                                                              const float4* __restrict__ J,
+                                                              const float* __restrict__ lam_D,
+                                                              float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const f3 xi = ld3(pos, i), vi = ld3(vel, i);
+    const float di = lam_D[i];
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
+    const uint32_t nn = b.nbr_n[i];
+    for (uint32_t m = 0; m < nn; ++m) {
+        const uint32_t j = entry(b, i, m);
+        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
+        const float dj = lam_D[j];
+        // D_j holds (v_j - v_i).gradW_ji = v_i.gradW_ij + ..., so both ends add
+        s_v = add(s_v, scl(di + dj, scl(P.g, P.r)));
+        s_x = add(s_x, scl(di + dj, hess(P, sub(vi, ld3(vel, j)))));
+    }
+    add3(lam_v, i, add(scl(k.mass, s_v), scl(di, ld3(wg, i))));
+    add3(lam_x, i, add(scl(k.mass, s_x), scl(di, jt(J, i, vi))));
+}
+
+// forces_den_apply's first half in reverse: v2 = v1 + dt (g + a). lam_v (the
+// adjoint of v2) stays as v1's; lam_a = dt lam_v; the gravity adjoint of the
+// substep is dt sum lam_v (one atomic per block). The warm start kd_w =
+// 0.5 pp / rho^2 and p = 0.5 pp: lam_pp and lam_rho.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_forces_den(StepK s, uint32_t n, const float* __restrict__ rho,
+                                                              const float* __restrict__ pp,
+                                                              const float4* __restrict__ lam_v,
+                                                              const float* __restrict__ lam_kdw,
+                                                              const float* __restrict__ lam_p,
+                                                              float4* __restrict__ lam_a, float* __restrict__ lam_pp,
+                                                              float* __restrict__ lam_rho, float* __restrict__ lam_g) {
+    __shared__ float red[3][kAdjBlock];
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    f3 lv = mk(0.0f, 0.0f, 0.0f);
+    if (i < n) {
+        lv = ld3(lam_v, i);
+        lam_a[i] = make_float4(s.dt * lv.x, s.dt * lv.y, s.dt * lv.z, 0.0f);
+        const float r = rho[i], lk = lam_kdw[i];
+        lam_pp[i] = 0.5f * lk / (r * r) + 0.5f * lam_p[i];
+        lam_rho[i] += -lk * pp[i] / (r * r * r);
+    }
+    red[0][threadIdx.x] = lv.x;
+    red[1][threadIdx.x] = lv.y;
+    red[2][threadIdx.x] = lv.z;
+    __syncthreads();
+    for (uint32_t o = kAdjBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) atomicAdd(&lam_g[threadIdx.x], s.dt * red[threadIdx.x][0]);
+}
+
+// forces_eval's viscosity in reverse: a_i = visc2 sum P_ij (v_i - v_j), P_ij
+// = m f(|r|) / (rho_i rho_j), f = g r^2 / (r^2 + eps). lam_v, lam_rho, lam_x,
+// and d loss / d mu = sum lam_a . a / mu (one atomic per block).
+__global__ __launch_bounds__(kAdjBlock) void k_rev_viscosity(SceneK k, float mu, uint32_t n, SolveBufs b,
+                                                             const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel,
+                                                             const float* __restrict__ rho,
+                                                             const float4* __restrict__ accel,
+                                                             const float4* __restrict__ lam_a,
+                                                             float4* __restrict__ lam_v, float* __restrict__ lam_rho,
+                                                             float4* __restrict__ lam_x, float* __restrict__ lam_mu) {
+    __shared__ float red[kAdjBlock];
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    float dmu = 0.0f;
+    if (i < n) {
+        const f3 xi = ld3(pos, i), vi = ld3(vel, i), la = ld3(lam_a, i);
+        const float ri = rho[i];
+        f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
+        float s_r = 0.0f;
+        const uint32_t nn = b.nbr_n[i];
+        for (uint32_t m = 0; m < nn; ++m) {
+            const uint32_t j = entry(b, i, m);
+            const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
+            const float r2 = dot3(P.r, P.r);
+            const float den = r2 + k.eps_h2;
+            const float f = P.g * r2 / den;
+            const float rj = rho[j];
+            const float pair = k.mass / (ri * rj) * f;
+            const f3 dl = sub(la, ld3(lam_a, j));
+            const f3 dv = sub(vi, ld3(vel, j));
+            const float dldv = dot3(dl, dv);
+            s_v = add(s_v, scl(pair, dl));
+            s_r += pair * dldv;
+            const float fp_over_r = P.hh * r2 / den + 2.0f * P.g * k.eps_h2 / (den * den);
+            s_x = add(s_x, scl(k.mass / (ri * rj) * fp_over_r * dldv, P.r));
+        }
+        add3(lam_v, i, scl(k.visc2, s_v));
+        lam_rho[i] += -k.visc2 / ri * s_r;
+        add3(lam_x, i, scl(k.visc2, s_x));
+        dmu = dot3(la, ld3(accel, i)) / mu;
+    }
+    red[threadIdx.x] = dmu;
+    __syncthreads();
+    for (uint32_t o = kAdjBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(lam_mu, red[0]);
+}
+
+// density_div in reverse, part 1 (per particle, with a sweep recomputing G =
+// m sum gradW + wg and gsq = m^2 sum |gradW|^2): kd = max(D, 0) alpha / dt /
+// rho and alpha = rho / max(|G|^2 + gsq, 1e-4) give lam_D, lam_G, lam_gsq and
+// the final lam_rho.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_density_pre(SceneK k, StepK s, uint32_t n, SolveBufs b,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ wg,
+                                                               const float* __restrict__ rho,
+                                                               const float* __restrict__ alpha,
+                                                               const float* __restrict__ kd,
+                                                               const float* __restrict__ lam_kd,
+                                                               float* __restrict__ lam_alpha,
+                                                               float* __restrict__ lam_rho, float* __restrict__ lam_D,
+                                                               float4* __restrict__ lam_G) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const f3 xi = ld3(pos, i);
+    f3 G = mk(0.0f, 0.0f, 0.0f);
+    float gsq = 0.0f;
+    const uint32_t nn = b.nbr_n[i];
+    for (uint32_t m = 0; m < nn; ++m) {
+        const uint32_t j = entry(b, i, m);
+        const f3 gw = scl(entry_g(b, i, m), sub(xi, ld3(pos, j)));
+        G = add(G, gw);
+        gsq += dot3(gw, gw);
+    }
+    G = add(scl(k.mass, G), ld3(wg, i));
+    gsq *= k.mass * k.mass;
+    const float r = rho[i], a = alpha[i], kk = kd[i], lk = lam_kd[i];
+    float la = lam_alpha[i], lr = lam_rho[i];
+    float lD = 0.0f;
+    if (kk > 0.0f) {
+        lD = lk * a / (s.dt_guard * r);
+        la += lk * kk / a;
+    }
+    lr += -lk * kk / r;
+    const float q = dot3(G, G) + gsq;
+    const float den = wmax(q, 1e-4f);
+    lr += la / den;
+    float lden = 0.0f;
+    if (q > 1e-4f) lden = -la * a / den;
+    lam_rho[i] = lr;
+    lam_D[i] = lD;
+    lam_G[i] = make_float4(2.0f * lden * G.x, 2.0f * lden * G.y, 2.0f * lden * G.z, lden);  // w: lam_gsq
+}
+
+// density_div in reverse, part 2 (the sweep): lam_v from D, lam_x from D, G,
+// gsq, rho and the walls. rho = m (sigma + sum W) + rho0 fill(x).
+__global__ __launch_bounds__(kAdjBlock) void k_rev_density(SceneK k, uint32_t n, SolveBufs b,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel,
+                                                           const float4* __restrict__ wg,
+                                                           const float4* __restrict__ J,
+                                                           const float4* __restrict__ dfill,
+                                                           const float* __restrict__ lam_rho,
+                                                           const float* __restrict__ lam_D,
+                                                           const float4* __restrict__ lam_G,
+                                                           float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
+    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i >= n) return;
+    const f3 xi = ld3(pos, i), vi = ld3(vel, i);
+    const float di = lam_D[i], ri = lam_rho[i];
+    const float4 Gi4 = lam_G[i];
+    const f3 Gi = xyz(Gi4);
+    const float si = Gi4.w;
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
+    const uint32_t nn = b.nbr_n[i];
+    for (uint32_t m = 0; m < nn; ++m) {
+        const uint32_t j = entry(b, i, m);
+        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
+        const f3 gw = scl(P.g, P.r);
+        const float dj = lam_D[j];
+        const float4 Gj4 = lam_G[j];
+        s_v = add(s_v, scl(di + dj, gw));
+        f3 t = scl(di + dj, hess(P, sub(vi, ld3(vel, j))));
+        t = add(t, hess(P, sub(Gi, xyz(Gj4))));
+        t = add(t, scl(2.0f * k.mass * (si + Gj4.w), hess(P, gw)));
+        t = add(t, scl(ri + lam_rho[j], gw));
+        s_x = add(s_x, t);
+    }
+    add3(lam_v, i, add(scl(k.mass, s_v), scl(di, ld3(wg, i))));
+    f3 w = add(scl(di, jt(J, i, vi)), jt(J, i, Gi));
+    w = add(w, scl(k.rho0 * ri, ld3(dfill, i)));
+    add3(lam_x, i, add(scl(k.mass, s_x), w));
+}
+
+// The working set came from the resting set through the permutation: the
+// adjoints go back to the previous substep's resting order.
+__global__ __launch_bounds__(kAdjBlock) void k_rev_gather(uint32_t n, const uint32_t* __restrict__ perm,
+                                                          const float4* __restrict__ lx, const float4* __restrict__ lv,
+                                                          const float* __restrict__ lpp, float4* __restrict__ ox,
+                                                          float4* __restrict__ ov, float* __restrict__ opp) {
+    const uint32_t f = blockIdx.x * kAdjBlock + threadIdx.x;
+    if (f >= n) return;
+    const uint32_t i = perm[f];
+    ox[i] = lx[f];
+    ov[i] = lv[f];
+    opp[i] = lpp[f];
+}
+
+__global__ __launch_bounds__(kAdjBlock) void k_fill_zero(float* __restrict__ a, size_t count) {
+    const size_t i = (size_t)blockIdx.x * kAdjBlock + threadIdx.x;
+    if (i < count) a[i] = 0.0f;
+}
+
+}  // namespace
+
+static dim3 adj_grid(uint32_t n) { return dim3((n + kAdjBlock - 1) / kAdjBlock); }
+
+void adj_zero(hipStream_t st, float* a, size_t count) {
+    if (count) hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((count + kAdjBlock - 1) / kAdjBlock)),
+                                  dim3(kAdjBlock), 0, st, a, count);
+}
+
+// One substep in reverse. On entry L.x, L.v hold the adjoints of the resting
+// position and velocity integrate wrote (working slots), L.p that of the
+// pressure it carried over; on exit L.x, L.v, L.pp hold the adjoints of the
+// previous resting set (its order), and L.g / L.mu have this substep's
+// gravity and viscosity terms added.
+void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, float mu, uint32_t n, uint32_t passes,
+                            const Tape& T, const SolveBufs& b, const AdjointBufs& L) {
+    if (n == 0) return;
+    const dim3 g = adj_grid(n), blk(kAdjBlock);
+    hipLaunchKernelGGL(k_wall_jacobians, g, blk, 0, st, k, n, T.pos, L.dfill, L.J);
+    adj_zero(st, L.rho, n);
+    adj_zero(st, L.alpha, n);
+    hipLaunchKernelGGL(k_rev_integrate, g, blk, 0, st, k, s, n, T.pos, T.vel[passes + 2], T.density, L.x, L.v);
+    for (uint32_t r = passes; r-- > 0;) {
+        // den_apply r: kd of pass r, lam of its output = L.v
+        hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[2 + r], T.wall_grad, L.J, L.v,
+                           L.kd, L.x);
+        // den_kappa r at the velocities entering pass r
+        hipLaunchKernelGGL(k_rev_den_kappa_pre, g, blk, 0, st, s, n, T.density, T.alpha, T.kd[2 + r], L.kd, L.p,
+                           L.D, L.rho, L.alpha);
+        hipLaunchKernelGGL(k_rev_divergence, g, blk, 0, st, k, n, b, T.pos, T.vel[2 + r], T.wall_grad, L.J, L.D,
+                           L.v, L.x);
+    }
+    // forces_den_apply: the warm-start correction, then v2 = v1 + dt (g + a)
+    hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[1], T.wall_grad, L.J, L.v, L.kd, L.x);
+    hipLaunchKernelGGL(k_rev_forces_den, g, blk, 0, st, s, n, T.density, T.pp, L.v, L.kd, L.p, L.a, L.pp, L.rho,
+                       L.g);
+    // forces_eval: viscosity at the velocities after div_apply
+    hipLaunchKernelGGL(k_rev_viscosity, g, blk, 0, st, k, mu, n, b, T.pos, T.vel[1], T.density, T.accel, L.a, L.v,
+                       L.rho, L.x, L.mu);
+    // div_apply with the divergence kappa
+    hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[0], T.wall_grad, L.J, L.v, L.kd, L.x);
+    // density_div at the substep's first velocities
+    hipLaunchKernelGGL(k_rev_density_pre, g, blk, 0, st, k, s, n, b, T.pos, T.wall_grad, T.density, T.alpha,
+                       T.kd[0], L.kd, L.alpha, L.rho, L.D, L.G);
+    hipLaunchKernelGGL(k_rev_density, g, blk, 0, st, k, n, b, T.pos, T.vel[0], T.wall_grad, L.J, L.dfill, L.rho,
+                       L.D, L.G, L.v, L.x);
+    hipLaunchKernelGGL(k_rev_gather, g, blk, 0, st, n, T.perm, L.x, L.v, L.pp, L.x_out, L.v_out, L.pp_out);
+}
+
+}  // namespace fluid

@@ -63,6 +63,7 @@ struct FluidBatch {
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
     Tolerance tol{0.0f, 1, 0};
+    Physics phys{};
     EventTimer timer;
     DeviceAllocs mem;
 
@@ -239,7 +240,7 @@ int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) 
     for (uint32_t s = 0; s < B->scenes; ++s) {
         StepBlock sb;
         std::memcpy(&sb, steps144 + 144 * (size_t)s, sizeof sb);
-        StepK k = make_step(sb);
+        StepK k = make_step(sb, B->phys);
         k.refine = refine ? refine[s] : 0u;
         // dt = 0 encodes no solve (render.rs:3165-3167).
         k.active = (sb.dt > 0.0f && B->ks[s].slot_end > B->ks[s].slot_begin) ? 1u : 0u;
@@ -532,6 +533,43 @@ done:
     b->use_graph = graph;
     if (chosen) *chosen = b->solve.stage_cap;
     return rc;
+}
+
+int fluid_batch_set_physics(FluidBatch* b, const float* phys5) {
+    if (!valid(b) || !phys5) return FLUID_ERR_ARG;
+    Physics ph;
+    ph.mu = phys5[0];
+    ph.k_near = phys5[1];
+    ph.tension_scale = phys5[2];
+    ph.adhesion_scale = phys5[3];
+    ph.xsph_rate = phys5[4];
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (SceneK& k : b->ks) {
+        const SceneK old = k;
+        SimParamsBlock p{};
+        for (int a = 0; a < 3; ++a) {
+            p.box_min[a] = old.box_min[a];
+            p.dims[a] = old.dims[a];
+        }
+        p.cell = old.cell;
+        p.count = old.count;
+        p.h = old.h;
+        p.mass = old.mass;
+        p.rho0 = old.rho0;
+        p.inv_h = old.inv_h;
+        p.sigma = old.sigma;
+        p.sigma_h = old.sigma_h;
+        p.support_sq = old.support_sq;
+        k = make_scene(p, ph);
+        k.cell_offset = old.cell_offset;
+        k.cells = old.cells;
+        k.slot_begin = old.slot_begin;
+        k.slot_end = old.slot_end;
+    }
+    b->phys = ph;
+    b->drop_graphs();
+    HIP_TRY(hipMemcpy(b->d_ks, b->ks.data(), b->ks.size() * sizeof(SceneK), hipMemcpyHostToDevice));
+    return FLUID_OK;
 }
 
 int fluid_batch_describe(const FluidBatch* b, char* buf, size_t size) {

@@ -59,6 +59,19 @@ constexpr float HUGE_F = 1e30f;
 
 // ---- what the kernels take by value --------------------------------------
 
+// Physical coefficients the reference fixes at compile time
+// (sim_solve.wgsl:135-137, 161-172, 657), as runtime values: the dynamic
+// viscosity (a parameter of the adjoint), and switches for the terms the
+// adjoint's differentiable profile leaves out (near pressure, cohesion,
+// adhesion, XSPH). The defaults are the reference's.
+struct Physics {
+    float mu = DYNAMIC_VISCOSITY;
+    float k_near = K_NEAR;
+    float tension_scale = 1.0f;   // multiplies tension_gamma
+    float adhesion_scale = 1.0f;  // multiplies adhesion_beta
+    float xsph_rate = XSPH_RATE;
+};
+
 // Scene constants: the SimParams block plus the pair-loop constants the
 // shader hoists (sim_solve.wgsl:161-172, 534-537), evaluated once on the host
 // in single precision in the shader's operand order.
@@ -72,6 +85,7 @@ struct SceneK {
     float inv_h_wall;  // 1.0 / h as wall_grad_sum and wall_adh_sum evaluate it
     float coh_c, coh_k, coh_near, gamma, beta;
     float visc2, heat_cond, heat_visc, eps_h2;
+    float k_near;
     // Where the scene lives in a batch (0, 0, count, C for a lone scene): its
     // cells are [cell_offset, cell_offset + cells) of the shared histogram,
     // plus one padding cell after them that the slots past its particles
@@ -99,8 +113,8 @@ struct StepK {
     uint32_t active;    // 0: the scene sits this substep out (batched steps)
 };
 
-SceneK make_scene(const SimParamsBlock& p);
-StepK make_step(const StepBlock& s);
+SceneK make_scene(const SimParamsBlock& p, const Physics& phys = Physics());
+StepK make_step(const StepBlock& s, const Physics& phys = Physics());
 
 // ---- device math (the reference's WGSL helpers) --------------------------
 
@@ -184,71 +198,131 @@ __device__ __forceinline__ float cohesion(float r, float c, float kk, float near
     if (2.0f * r > c) return kk * a;
     return kk * (2.0f * a - nearc);
 }
+// A dual number (value, derivative) for the adjoint's derivatives of the
+// wall polynomials: the templates below evaluate at float for the step and
+// at Dn for the step's derivative, the same operations in the same order.
+struct Dn {
+    float v, d;
+    __device__ __forceinline__ Dn(float x = 0.0f, float dx = 0.0f) : v(x), d(dx) {}
+};
+__device__ __forceinline__ Dn operator+(Dn a, Dn b) { return Dn(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dn operator-(Dn a, Dn b) { return Dn(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dn operator*(Dn a, Dn b) { return Dn(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dn operator/(Dn a, float b) { return Dn(a.v / b, a.d / b); }
+__device__ __forceinline__ Dn operator+(float a, Dn b) { return Dn(a + b.v, b.d); }
+__device__ __forceinline__ Dn operator-(float a, Dn b) { return Dn(a - b.v, -b.d); }
+__device__ __forceinline__ Dn operator*(float a, Dn b) { return Dn(a * b.v, a * b.d); }
+__device__ __forceinline__ Dn operator+(Dn a, float b) { return Dn(a.v + b, a.d); }
+__device__ __forceinline__ Dn operator-(Dn a, float b) { return Dn(a.v - b, a.d); }
+__device__ __forceinline__ Dn operator*(Dn a, float b) { return Dn(a.v * b, a.d * b); }
+__device__ __forceinline__ Dn& operator+=(Dn& a, Dn b) { return a = a + b; }
+__device__ __forceinline__ Dn& operator-=(Dn& a, Dn b) { return a = a - b; }
+__device__ __forceinline__ bool operator>=(Dn a, float b) { return a.v >= b; }
+__device__ __forceinline__ bool operator<(Dn a, float b) { return a.v < b; }
+__device__ __forceinline__ float value(float x) { return x; }
+__device__ __forceinline__ float value(Dn x) { return x.v; }
+
 // sim_solve.wgsl:213-222
-__device__ __forceinline__ float wall_density(float t) {
-    if (t >= 2.0f) return 0.0f;
-    float t3 = t * t * t;
+template <class N>
+__device__ __forceinline__ N wall_density_t(N t) {
+    if (t >= 2.0f) return N(0.0f);
+    N t3 = t * t * t;
     if (t < 1.0f) return 0.5f - 0.7f * t + t3 * ((float)(1.0 / 3.0) + t * t * (-0.15f + 0.05f * t));
     return (float)(8.0 / 15.0) - 0.8f * t + t3 * ((float)(2.0 / 3.0) + t * (-0.5f + t * (0.15f - t / 60.0f)));
 }
+__device__ __forceinline__ float wall_density(float t) { return wall_density_t<float>(t); }
 // sim_solve.wgsl:226-235
-__device__ __forceinline__ float wall_gradient(float t) {
-    if (t >= 2.0f) return 0.0f;
-    float t2 = t * t;
+template <class N>
+__device__ __forceinline__ N wall_gradient_t(N t) {
+    if (t >= 2.0f) return N(0.0f);
+    N t2 = t * t;
     if (t < 1.0f) return 0.7f + t2 * (-1.0f + t2 * (0.75f - t * 0.3f));
     return 0.8f + t2 * (-2.0f + t * (2.0f + t * (-0.75f + t * 0.1f)));
 }
+__device__ __forceinline__ float wall_gradient(float t) { return wall_gradient_t<float>(t); }
 // sim_solve.wgsl:249-270 — the reference's fitted coefficient table.
-__device__ __forceinline__ float wedge(float t1, float t2) {
-    if (t1 * t1 + t2 * t2 >= 4.0f) return 0.0f;
-    float c0 = 2.5005807e-01f + t2 * (-3.4681153e-01f + t2 * (-4.0986882e-02f + t2 * (3.0301620e-01f + t2 * (-1.9202736e-01f + t2 * (4.6947582e-02f + t2 * (-3.8543515e-03f))))));
-    float c1 = -3.4681153e-01f + t2 * (4.8587248e-01f + t2 * (1.0092180e-02f + t2 * (-2.9978363e-01f + t2 * (1.5320335e-01f + t2 * (-2.2544282e-02f)))));
-    float c2 = -4.0986882e-02f + t2 * (1.0092180e-02f + t2 * (-5.4145610e-02f + t2 * (7.9597369e-02f + t2 * (-2.3350373e-02f))));
-    float c3 = 3.0301620e-01f + t2 * (-2.9978363e-01f + t2 * (7.9597369e-02f + t2 * (-8.2431946e-03f)));
-    float c4 = -1.9202736e-01f + t2 * (1.5320335e-01f + t2 * (-2.3350373e-02f));
-    float c5 = 4.6947582e-02f + t2 * (-2.2544282e-02f);
+template <class N>
+__device__ __forceinline__ N wedge_t(N t1, N t2) {
+    if (t1 * t1 + t2 * t2 >= 4.0f) return N(0.0f);
+    N c0 = 2.5005807e-01f + t2 * (-3.4681153e-01f + t2 * (-4.0986882e-02f + t2 * (3.0301620e-01f + t2 * (-1.9202736e-01f + t2 * (4.6947582e-02f + t2 * (-3.8543515e-03f))))));
+    N c1 = -3.4681153e-01f + t2 * (4.8587248e-01f + t2 * (1.0092180e-02f + t2 * (-2.9978363e-01f + t2 * (1.5320335e-01f + t2 * (-2.2544282e-02f)))));
+    N c2 = -4.0986882e-02f + t2 * (1.0092180e-02f + t2 * (-5.4145610e-02f + t2 * (7.9597369e-02f + t2 * (-2.3350373e-02f))));
+    N c3 = 3.0301620e-01f + t2 * (-2.9978363e-01f + t2 * (7.9597369e-02f + t2 * (-8.2431946e-03f)));
+    N c4 = -1.9202736e-01f + t2 * (1.5320335e-01f + t2 * (-2.3350373e-02f));
+    N c5 = 4.6947582e-02f + t2 * (-2.2544282e-02f);
     return c0 + t1 * (c1 + t1 * (c2 + t1 * (c3 + t1 * (c4 + t1 * (c5 + t1 * (-3.8543515e-03f))))));
 }
+__device__ __forceinline__ float wedge(float t1, float t2) { return wedge_t<float>(t1, t2); }
 // sim_solve.wgsl:272-293
-__device__ __forceinline__ float wedge_d(float t1, float t2) {
-    if (t1 * t1 + t2 * t2 >= 4.0f) return 0.0f;
-    float c0 = -3.4611994e-01f + t2 * (4.4757673e-01f + t2 * (1.1932939e-01f + t2 * (-4.6460101e-01f + t2 * (2.7637253e-01f + t2 * (-6.2846025e-02f + t2 * (4.3871652e-03f))))));
-    float c1 = -6.3622823e-02f + t2 * (2.0521006e-01f + t2 * (-3.7311339e-01f + t2 * (3.7508155e-01f + t2 * (-1.6629713e-01f + t2 * (2.5030129e-02f)))));
-    float c2 = 7.8352497e-01f + t2 * (-1.2658344e+00f + t2 * (5.8898945e-01f + t2 * (-1.0772903e-01f + t2 * (1.3535777e-02f))));
-    float c3 = -4.9470084e-01f + t2 * (9.1302071e-01f + t2 * (-3.4330355e-01f + t2 * (2.4034609e-02f)));
-    float c4 = -2.5605481e-02f + t2 * (-2.0067981e-01f + t2 * (5.9854468e-02f));
-    float c5 = 8.8898623e-02f + t2 * (3.9914838e-03f);
+template <class N>
+__device__ __forceinline__ N wedge_d_t(N t1, N t2) {
+    if (t1 * t1 + t2 * t2 >= 4.0f) return N(0.0f);
+    N c0 = -3.4611994e-01f + t2 * (4.4757673e-01f + t2 * (1.1932939e-01f + t2 * (-4.6460101e-01f + t2 * (2.7637253e-01f + t2 * (-6.2846025e-02f + t2 * (4.3871652e-03f))))));
+    N c1 = -6.3622823e-02f + t2 * (2.0521006e-01f + t2 * (-3.7311339e-01f + t2 * (3.7508155e-01f + t2 * (-1.6629713e-01f + t2 * (2.5030129e-02f)))));
+    N c2 = 7.8352497e-01f + t2 * (-1.2658344e+00f + t2 * (5.8898945e-01f + t2 * (-1.0772903e-01f + t2 * (1.3535777e-02f))));
+    N c3 = -4.9470084e-01f + t2 * (9.1302071e-01f + t2 * (-3.4330355e-01f + t2 * (2.4034609e-02f)));
+    N c4 = -2.5605481e-02f + t2 * (-2.0067981e-01f + t2 * (5.9854468e-02f));
+    N c5 = 8.8898623e-02f + t2 * (3.9914838e-03f);
     return c0 + t1 * (c1 + t1 * (c2 + t1 * (c3 + t1 * (c4 + t1 * (c5 + t1 * (-1.7788321e-02f))))));
 }
+__device__ __forceinline__ float wedge_d(float t1, float t2) { return wedge_d_t<float>(t1, t2); }
 // sim_solve.wgsl:334-340
 __device__ __forceinline__ float wall_adhesion(float t) {
     float u = wmin(wmax(2.0f - t, 0.0f), 2.0f);
     return u * u * (1.7847015e-3f + u * (3.3392196e-3f + u * (-4.3394677e-3f + u * (1.6711868e-3f + u * (-2.1788750e-4f)))));
 }
-__device__ __forceinline__ float wedge_dsum(float t, float a, float b, float c, float d) {
-    return wedge_d(t, a) + wedge_d(t, b) + wedge_d(t, c) + wedge_d(t, d);
+template <class N>
+__device__ __forceinline__ N wedge_dsum_t(N t, N a, N b, N c, N d) {
+    return wedge_d_t(t, a) + wedge_d_t(t, b) + wedge_d_t(t, c) + wedge_d_t(t, d);
 }
-// sim_solve.wgsl:299-304
-__device__ __forceinline__ float wedge_sum(f3 tl, f3 th) {
-    float w = wedge(tl.x, tl.y) + wedge(tl.x, th.y) + wedge(th.x, tl.y) + wedge(th.x, th.y);
-    w += wedge(tl.x, tl.z) + wedge(tl.x, th.z) + wedge(th.x, tl.z) + wedge(th.x, th.z);
-    w += wedge(tl.y, tl.z) + wedge(tl.y, th.z) + wedge(th.y, tl.z) + wedge(th.y, th.z);
+__device__ __forceinline__ float wedge_dsum(float t, float a, float b, float c, float d) {
+    return wedge_dsum_t<float>(t, a, b, c, d);
+}
+// sim_solve.wgsl:299-304, components (lx, ly, lz) = tl, (hx, hy, hz) = th
+template <class N>
+__device__ __forceinline__ N wedge_sum_t(N lx, N ly, N lz, N hx, N hy, N hz) {
+    N w = wedge_t(lx, ly) + wedge_t(lx, hy) + wedge_t(hx, ly) + wedge_t(hx, hy);
+    w += wedge_t(lx, lz) + wedge_t(lx, hz) + wedge_t(hx, lz) + wedge_t(hx, hz);
+    w += wedge_t(ly, lz) + wedge_t(ly, hz) + wedge_t(hy, lz) + wedge_t(hy, hz);
     return w;
 }
+__device__ __forceinline__ float wedge_sum(f3 tl, f3 th) {
+    return wedge_sum_t<float>(tl.x, tl.y, tl.z, th.x, th.y, th.z);
+}
 // sim_solve.wgsl:310-328
+template <class N>
+__device__ __forceinline__ void wall_grad_sum_t(const SceneK& k, N px, N py, N pz, N& gx, N& gy, N& gz) {
+    const float ih = k.inv_h_wall;
+    const N lx = (px - k.wall_lo[0]) * ih, ly = (py - k.wall_lo[1]) * ih, lz = (pz - k.wall_lo[2]) * ih;
+    const N hx = (k.wall_hi[0] - px) * ih, hy = (k.wall_hi[1] - py) * ih, hz = (k.wall_hi[2] - pz) * ih;
+    const float scale = k.rho0 * ih;
+    gx = N(0.0f);
+    gy = N(0.0f);
+    gz = N(0.0f);
+    gx -= scale * (wall_gradient_t(lx) + wedge_dsum_t(lx, ly, hy, lz, hz));
+    gx += scale * (wall_gradient_t(hx) + wedge_dsum_t(hx, ly, hy, lz, hz));
+    gy -= scale * (wall_gradient_t(ly) + wedge_dsum_t(ly, lx, hx, lz, hz));
+    gy += scale * (wall_gradient_t(hy) + wedge_dsum_t(hy, lx, hx, lz, hz));
+    gz -= scale * (wall_gradient_t(lz) + wedge_dsum_t(lz, lx, hx, ly, hy));
+    gz += scale * (wall_gradient_t(hz) + wedge_dsum_t(hz, lx, hx, ly, hy));
+}
 __device__ __forceinline__ f3 wall_grad_sum(const SceneK& k, f3 pos) {
-    float ih = k.inv_h_wall;
-    f3 tl = mk((pos.x - k.wall_lo[0]) * ih, (pos.y - k.wall_lo[1]) * ih, (pos.z - k.wall_lo[2]) * ih);
-    f3 th = mk((k.wall_hi[0] - pos.x) * ih, (k.wall_hi[1] - pos.y) * ih, (k.wall_hi[2] - pos.z) * ih);
-    float scale = k.rho0 * ih;
-    f3 g = mk(0.0f, 0.0f, 0.0f);
-    g.x -= scale * (wall_gradient(tl.x) + wedge_dsum(tl.x, tl.y, th.y, tl.z, th.z));
-    g.x += scale * (wall_gradient(th.x) + wedge_dsum(th.x, tl.y, th.y, tl.z, th.z));
-    g.y -= scale * (wall_gradient(tl.y) + wedge_dsum(tl.y, tl.x, th.x, tl.z, th.z));
-    g.y += scale * (wall_gradient(th.y) + wedge_dsum(th.y, tl.x, th.x, tl.z, th.z));
-    g.z -= scale * (wall_gradient(tl.z) + wedge_dsum(tl.z, tl.x, th.x, tl.y, th.y));
-    g.z += scale * (wall_gradient(th.z) + wedge_dsum(th.z, tl.x, th.x, tl.y, th.y));
+    f3 g;
+    wall_grad_sum_t<float>(k, pos.x, pos.y, pos.z, g.x, g.y, g.z);
     return g;
+}
+// The wall fill of density_div (sim_solve.wgsl:485-493): the kernel mass
+// behind the six walls, less the doubly counted wedges.
+template <class N>
+__device__ __forceinline__ N wall_fill_t(const SceneK& k, N px, N py, N pz) {
+    const N lx = (px - k.wall_lo[0]) * k.inv_h, ly = (py - k.wall_lo[1]) * k.inv_h, lz = (pz - k.wall_lo[2]) * k.inv_h;
+    const N hx = (k.wall_hi[0] - px) * k.inv_h, hy = (k.wall_hi[1] - py) * k.inv_h, hz = (k.wall_hi[2] - pz) * k.inv_h;
+    N fill = N(0.0f);
+    fill += wall_density_t(lx) + wall_density_t(hx);
+    fill += wall_density_t(ly) + wall_density_t(hy);
+    fill += wall_density_t(lz) + wall_density_t(hz);
+    fill -= wedge_sum_t(lx, ly, lz, hx, hy, hz);
+    return fill;
 }
 // sim_solve.wgsl:349-361
 __device__ __forceinline__ f3 wall_adh_sum(const SceneK& k, f3 pos) {

@@ -21,7 +21,7 @@
 
 namespace fluid {
 
-SceneK make_scene(const SimParamsBlock& p) {
+SceneK make_scene(const SimParamsBlock& p, const Physics& phys) {
     SceneK k;
     for (int a = 0; a < 3; ++a) {
         k.box_min[a] = p.box_min[a];
@@ -47,12 +47,14 @@ SceneK make_scene(const SimParamsBlock& p) {
     k.coh_near = c3 * c3 / 64.0f;
     // tension_gamma and adhesion_beta, sim_solve.wgsl:161-172
     const float c = 2.0f * p.h;
-    k.gamma = (SIGMA_WATER * 7040.0f) / (21.0f * LATTICE_SIGMA * p.rho0 * p.rho0 * c * c);
-    k.beta = (SIGMA_WATER * (1.0f + COS_CONTACT) * 1558.4f) / (LATTICE_SIGMA * p.rho0 * p.rho0 * c * c);
+    k.gamma = (SIGMA_WATER * 7040.0f) / (21.0f * LATTICE_SIGMA * p.rho0 * p.rho0 * c * c) * phys.tension_scale;
+    k.beta = (SIGMA_WATER * (1.0f + COS_CONTACT) * 1558.4f) / (LATTICE_SIGMA * p.rho0 * p.rho0 * c * c) *
+             phys.adhesion_scale;
     // the constant factors of forces_eval's pair terms, sim_solve.wgsl:554-565
-    k.visc2 = 2.0f * DYNAMIC_VISCOSITY;
+    k.visc2 = 2.0f * phys.mu;
     k.heat_cond = 2.0f * CONDUCTIVITY / HEAT_CAPACITY;
-    k.heat_visc = 0.5f * DYNAMIC_VISCOSITY / HEAT_CAPACITY;
+    k.heat_visc = 0.5f * phys.mu / HEAT_CAPACITY;
+    k.k_near = phys.k_near;
     k.eps_h2 = 0.01f * p.h * p.h;
     k.cell_offset = 0;
     k.cells = p.dims[0] * p.dims[1] * p.dims[2];
@@ -61,7 +63,7 @@ SceneK make_scene(const SimParamsBlock& p) {
     return k;
 }
 
-StepK make_step(const StepBlock& s) {
+StepK make_step(const StepBlock& s, const Physics& phys) {
     StepK k;
     for (int a = 0; a < 3; ++a) {
         k.force[a] = s.force[a];
@@ -75,7 +77,7 @@ StepK make_step(const StepBlock& s) {
     // The shader's touches array has five slots (sim_solve.wgsl:40).
     k.touch_n = s.touch_n < 5u ? s.touch_n : 5u;
     std::memcpy(k.touches, s.touches, sizeof k.touches);
-    k.c_xsph = 1.0f - std::exp(-XSPH_RATE * s.dt);
+    k.c_xsph = 1.0f - std::exp(-phys.xsph_rate * s.dt);
     k.chase = 1.0f - std::exp(-LENS_RATE * s.dt);
     k.decay = std::exp(-s.dt / T_CHARGE);
     k.dt_guard = std::max(s.dt, 1e-9f);
@@ -93,6 +95,7 @@ struct FluidSim {
     hipStream_t stream = nullptr;
     SimParamsBlock params{};
     SceneK scene{};
+    Physics phys{};
     GridGeom geom{};
     uint32_t n = 0, cells = 0, cap = 0, ntr = 0;
     Records rest{}, work{};
@@ -227,7 +230,7 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
     std::memcpy(&sb, step144, sizeof sb);
     // dt = 0 encodes no solve: kappa divides by dt^2 (render.rs:3165-3167).
     if (!(sb.dt > 0.0f) || nsub == 0) return FLUID_OK;
-    const StepK k = make_step(sb);
+    const StepK k = make_step(sb, s->phys);
     const OneScene sc{s->scene, k};
     for (uint32_t i = 0; i < nsub; ++i) {
         launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
@@ -414,7 +417,7 @@ int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
     if (!valid(s) || !step144) return FLUID_ERR_ARG;
     StepBlock sb;
     std::memcpy(&sb, step144, sizeof sb);
-    const OneScene sc{s->scene, make_step(sb)};
+    const OneScene sc{s->scene, make_step(sb, s->phys)};
     launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
     launch_density_only(s->stream, sc, s->n, s->grid.starts, s->work, s->solve, s->t());
     return check_launch("density_only");
@@ -439,7 +442,7 @@ int fluid_sim_tracers(FluidSim* s, const uint8_t* step144) {
     if (!valid(s) || !step144) return FLUID_ERR_ARG;
     StepBlock sb;
     std::memcpy(&sb, step144, sizeof sb);
-    launch_tracers(s->stream, s->scene, make_step(sb), s->n, s->rest, s->tracers, s->t());
+    launch_tracers(s->stream, s->scene, make_step(sb, s->phys), s->n, s->rest, s->tracers, s->t());
     return check_launch("tracers");
 }
 
@@ -476,6 +479,20 @@ int fluid_sim_counters(FluidSim* s, uint64_t* clamps, uint64_t* overflow) {
     HIP_TRY(hipMemcpy(c, s->solve.counters, sizeof c, hipMemcpyDeviceToHost));
     if (clamps) *clamps = c[0];
     if (overflow) *overflow = c[1];
+    return FLUID_OK;
+}
+
+int fluid_sim_set_physics(FluidSim* s, const float* phys5) {
+    if (!valid(s) || !phys5) return FLUID_ERR_ARG;
+    Physics ph;
+    ph.mu = phys5[0];
+    ph.k_near = phys5[1];
+    ph.tension_scale = phys5[2];
+    ph.adhesion_scale = phys5[3];
+    ph.xsph_rate = phys5[4];
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->phys = ph;
+    s->scene = make_scene(s->params, ph);
     return FLUID_OK;
 }
 

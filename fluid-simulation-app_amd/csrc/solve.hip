@@ -687,7 +687,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const f3 adh = scl(k.beta * k.rho0, wall_adh_sum(k, pi));
-        const f3 acc = add(add(add(visc, scl(K_NEAR, nearf)), scl(k.gamma, tension)), adh);
+        const f3 acc = add(add(add(visc, scl(k.k_near, nearf)), scl(k.gamma, tension)), adh);
         b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
         b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
         const float half = 0.5f * prev_pressure[i];
@@ -948,18 +948,23 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
 struct NoCheck {
     void operator()(hipStream_t, uint32_t) const {}
 };
+struct NoStage {
+    void operator()(hipStream_t, StageId, uint32_t) const {}
+};
 
-template <class Sc, class Check = NoCheck>
+template <class Sc, class Check = NoCheck, class Stage = NoStage>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
-                  const Check& check = Check()) {
+                  const Check& check = Check(), const Stage& stage = Stage()) {
     if (n == 0) return;
     const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
     const size_t lds_kappa = b.stage_cap * kKappaEntry;
     density_only(st, sc, n, starts, work, b, timer);
+    stage(st, STAGE_DENSITY, 0);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
     hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
+    stage(st, STAGE_DIV_APPLY, 0);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
     hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, b.stage_cap * kForcesEntry, st, sc, work.pos, work.vel,
                        work.temperature, work.prev_pressure, b);
@@ -968,14 +973,17 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     hipLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, sc, work.pos, work.vel,
                        work.temperature, b);
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
+    stage(st, STAGE_FORCES_DEN_APPLY, 0);
     for (uint32_t r = 0; r < passes; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
         hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, b.stage_cap * kPvEntry, st, sc, r, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         check(st, r);
+        stage(st, STAGE_DEN_KAPPA, r);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
         hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, r, work.pos, work.vel, b);
         if (timer) timer->end(st, KERNEL_DEN_APPLY);
+        stage(st, STAGE_DEN_APPLY, r);
     }
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
     hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kBlock), 0, st, sc, work, rest, b);
@@ -1004,6 +1012,17 @@ void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const ui
     SceneTable t = sc;
     t.pass_limit = nullptr;
     solve(st, t, slots, starts, work, rest, fixed, passes, timer);
+}
+
+void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
+                         const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
+                         const StageFn& fn) {
+    OneScene one = sc;
+    one.s.refine = passes;
+    SolveBufs fixed = b;
+    fixed.residual = nullptr;
+    auto stage = [&](hipStream_t s2, StageId id, uint32_t r) { fn(s2, id, r); };
+    solve(st, one, slots, starts, work, rest, fixed, passes, nullptr, NoCheck(), stage);
 }
 
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,

@@ -58,6 +58,14 @@ def _declare(L):
         "fluid_sim_kernel_name": ([u32], C.c_char_p),
         "fluid_sim_describe": ([vp, C.c_char_p, C.c_size_t], i32),
         "fluid_scan_u32": ([P(u32), u32, P(u32)], i32),
+        "fluid_sim_set_physics": ([vp, P(f32)], i32),
+        "fluid_batch_set_physics": ([vp, P(f32)], i32),
+        "fluid_adjoint_create": ([u8p, u32, u32, u32], vp),
+        "fluid_adjoint_destroy": ([vp], None),
+        "fluid_adjoint_set_physics": ([vp, P(f32)], i32),
+        "fluid_adjoint_forward": ([vp, P(f32), P(f32), u8p, P(u32), P(f32), P(f32)], i32),
+        "fluid_adjoint_backward": ([vp, P(f32), P(f32), P(f32), P(f32), P(f32), P(f32)], i32),
+        "fluid_adjoint_timing": ([vp, P(f64), P(f64)], i32),
         "fluid_batch_create": ([u32, u8p, u32], vp),
         "fluid_batch_destroy": ([vp], None),
         "fluid_batch_scene_count": ([vp], u32),

@@ -280,6 +280,10 @@ class Sim:
     def synchronize(self):
         check(self.L.fluid_sim_synchronize(self.h), "synchronize")
 
+    def set_physics(self, mu=1.002e-3, k_near=3000.0, tension_scale=1.0, adhesion_scale=1.0, xsph_rate=6.0):
+        arr = (C.c_float * 5)(mu, k_near, tension_scale, adhesion_scale, xsph_rate)
+        check(self.L.fluid_sim_set_physics(self.h, arr), "set_physics")
+
     def describe(self) -> dict:
         import json
         buf = C.create_string_buffer(512)
@@ -426,6 +430,10 @@ class Batch:
         check(self.L.fluid_batch_autotune(self.h, _u8p(blob), _u32p(r), int(trials), C.byref(out)),
               "batch autotune")
         return int(out.value)
+
+    def set_physics(self, mu=1.002e-3, k_near=3000.0, tension_scale=1.0, adhesion_scale=1.0, xsph_rate=6.0):
+        arr = (C.c_float * 5)(mu, k_near, tension_scale, adhesion_scale, xsph_rate)
+        check(self.L.fluid_batch_set_physics(self.h, arr), "batch set_physics")
 
     def describe(self) -> dict:
         import json

@@ -218,6 +218,15 @@ uint32_t fluid_substeps_for(float dt, float v_max, float spacing, uint32_t cap);
 float fluid_spacing_for(float scale, const float extent[2]);                      /* render.rs:1362-1380 */
 float fluid_slab_depth(void);                                                     /* sim.rs:10 */
 
+/* The reference's compile-time physical coefficients as runtime values,
+ * phys5 = {mu, k_near, tension_scale, adhesion_scale, xsph_rate}: dynamic
+ * viscosity (DYNAMIC_VISCOSITY, sim_solve.wgsl:135), the near-pressure
+ * stiffness (K_NEAR, :137), factors on tension_gamma and adhesion_beta
+ * (:161-172) and the XSPH rate (:657). Defaults {1.002e-3, 3000, 1, 1, 6}
+ * are the reference's; the adjoint's differentiable profile is
+ * {mu, 0, 0, 0, 0}. No counterpart in the reference (compile-time there). */
+int fluid_sim_set_physics(FluidSim *sim, const float *phys5);
+
 /* ---- batched step ---------------------------------------------------------
  *
  * Many independent boxes of uneven particle counts stepped together, one
@@ -276,6 +285,43 @@ int fluid_batch_profile_read(FluidBatch *batch, double *ms, uint64_t *launches);
 int fluid_batch_autotune(FluidBatch *batch, const uint8_t *steps144, const uint32_t *refine, uint32_t trials,
                          uint32_t *chosen);
 int fluid_batch_describe(const FluidBatch *batch, char *buf, size_t size);
+/* As fluid_sim_set_physics, for every scene of the batch. */
+int fluid_batch_set_physics(FluidBatch *batch, const float *phys5);
+
+/* ---- adjoint ----------------------------------------------------------------
+ *
+ * The gradient of a loss on the final state of `steps` substeps with respect
+ * to the initial velocities and positions, the dynamic viscosity and each
+ * substep's gravity. The reference has no adjoint; this is the hand-written
+ * reverse of the step above in its differentiable profile (set it with
+ * fluid_adjoint_set_physics: {mu, 0, 0, 0, 0} switches off near pressure,
+ * cohesion, adhesion and XSPH; Steps must have no touches and no rotation).
+ * The grid permutation and neighbour lists are held fixed per substep. The
+ * forward keeps a checkpoint of the resting set every `checkpoint_every`
+ * substeps; the backward replays each segment with a tape and runs the
+ * reverse sweeps (the density solve's passes as transposed Jacobi sweeps).
+ */
+typedef struct FluidAdjoint FluidAdjoint;
+
+FluidAdjoint *fluid_adjoint_create(const uint8_t *params64, uint32_t steps, uint32_t checkpoint_every,
+                                   uint32_t nbr_cap);
+void fluid_adjoint_destroy(FluidAdjoint *adj);
+int fluid_adjoint_set_physics(FluidAdjoint *adj, const float *phys5);
+/* Runs the substeps from positions pos4 and velocities vel4 (NULL: zero),
+ * previous pressures zero, temperature 293.15 K. steps144 holds one Step per
+ * substep (dt > 0), refine one density-pass count per substep (<= 8). Writes
+ * the final resting positions and velocities (NULL: skipped); their order is
+ * the final resting order. Fails if a neighbour list overflowed its cap. */
+int fluid_adjoint_forward(FluidAdjoint *adj, const float *pos4, const float *vel4, const uint8_t *steps144,
+                          const uint32_t *refine, float *pos_out4, float *vel_out4);
+/* Given d loss / d final position (and velocity; NULL: zero), in the order
+ * fluid_adjoint_forward returned them, writes d loss / d initial velocity and
+ * position (input order; NULL: skipped), d loss / d gravity of each substep
+ * (steps x 3) and d loss / d mu. */
+int fluid_adjoint_backward(FluidAdjoint *adj, const float *dpos4, const float *dvel4, float *dvel0_4,
+                           float *dpos0_4, float *dgravity3, float *dmu);
+/* GPU time of the last forward and backward (HIP events on the stream). */
+int fluid_adjoint_timing(const FluidAdjoint *adj, double *forward_ms, double *backward_ms);
 
 #ifdef __cplusplus
 }

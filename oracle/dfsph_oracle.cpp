@@ -56,7 +56,65 @@
 #include <omp.h>
 #endif
 
+// A forward-mode dual number in double precision: a value and its
+// derivative along one seeded direction. Comparisons and branches follow the
+// value, so a dual run takes exactly the double run's path and its tangent is
+// the exact derivative of that code (the FP64 reference gradient along the
+// seed: initial velocities, viscosity or a substep's gravity).
+struct Dual {
+    double v = 0.0, d = 0.0;
+    Dual() = default;
+    Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants and parameters
+    Dual(double x, double dx) : v(x), d(dx) {}
+    explicit operator float() const { return (float)v; }
+    explicit operator double() const { return v; }
+    Dual& operator+=(const Dual& b) { d += b.d; v += b.v; return *this; }
+    Dual& operator-=(const Dual& b) { d -= b.d; v -= b.v; return *this; }
+    Dual& operator*=(const Dual& b) { d = d * b.v + v * b.d; v *= b.v; return *this; }
+    Dual& operator/=(const Dual& b) {
+        const double q = v / b.v;
+        d = (d - q * b.d) / b.v;
+        v = q;
+        return *this;
+    }
+};
+inline Dual operator+(Dual a, const Dual& b) { return a += b; }
+inline Dual operator-(Dual a, const Dual& b) { return a -= b; }
+inline Dual operator*(Dual a, const Dual& b) { return a *= b; }
+inline Dual operator/(Dual a, const Dual& b) { return a /= b; }
+inline Dual operator-(const Dual& a) { return Dual(-a.v, -a.d); }
+inline bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+inline bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+inline bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+inline bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+inline bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+inline bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+inline Dual dsqrt(const Dual& a) {
+    const double r = std::sqrt(a.v);
+    return Dual(r, r > 0.0 ? a.d / (2.0 * r) : 0.0);
+}
+inline Dual dexp(const Dual& a) {
+    const double e = std::exp(a.v);
+    return Dual(e, e * a.d);
+}
+
 namespace {
+
+// Sets the tangent of a dual value (nothing at float or double).
+inline void seed(float&, double) {}
+inline void seed(double&, double) {}
+inline void seed(Dual& x, double d) { x.d = d; }
+inline double tangent(float) { return 0.0; }
+inline double tangent(double) { return 0.0; }
+inline double tangent(const Dual& x) { return x.d; }
+
+// sqrt and exp at precision T (the dual forms carry the derivative).
+inline float tsqrt(float x) { return std::sqrt(x); }
+inline double tsqrt(double x) { return std::sqrt(x); }
+inline Dual tsqrt(const Dual& x) { return dsqrt(x); }
+inline float texp(float x) { return std::exp(x); }
+inline double texp(double x) { return std::exp(x); }
+inline Dual texp(const Dual& x) { return dexp(x); }
 
 struct Params {  // SimParams, sim.rs:325-350 / sim_solve.wgsl:11-23
     float box_min[3];
@@ -207,6 +265,18 @@ template <typename T> inline T tmax(T a, T b) { return a > b ? a : b; }  // WGSL
 template <typename T> inline T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T> inline T mix(T a, T b, T t) { return a * (T(1) - t) + b * t; }
 
+// The reference's compile-time physical coefficients as runtime values
+// (sim_solve.wgsl:135-137, 161-172, 657): viscosity, and switches for the
+// terms the adjoint's differentiable profile leaves out. Defaults are the
+// reference's.
+struct Physics {
+    float mu = DYNAMIC_VISCOSITY;
+    float k_near = K_NEAR;
+    float tension_scale = 1.0f;
+    float adhesion_scale = 1.0f;
+    float xsph_rate = XSPH_RATE;
+};
+
 // Constants of the method at precision T: parameter values come from the
 // f32 SimParams block; const-expressions of the shader are folded at T.
 template <typename T>
@@ -214,8 +284,8 @@ struct Consts {
     T box_min[3], cell, h, mass, rho0, inv_h, sigma, sigma_h, support_sq;
     T wall_lo[3], wall_hi[3];
     T coh_c, coh_k, coh_near, gamma, beta;
-    T visc2, heat_cond, heat_visc, eps_h2;
-    explicit Consts(const Params& p) {
+    T visc2, heat_cond, heat_visc, eps_h2, k_near;
+    explicit Consts(const Params& p, const Physics& phys = Physics()) {
         for (int i = 0; i < 3; ++i) box_min[i] = (T)p.box_min[i];
         cell = (T)p.cell;
         h = (T)p.h;
@@ -236,13 +306,15 @@ struct Consts {
         coh_near = c3 * c3 / T(64);
         // tension_gamma / adhesion_beta, sim_solve.wgsl:161-172
         T c = T(2) * h;
-        gamma = (T(SIGMA_WATER) * T(7040)) / (T(21) * T(LATTICE_SIGMA) * rho0 * rho0 * c * c);
+        gamma = (T(SIGMA_WATER) * T(7040)) / (T(21) * T(LATTICE_SIGMA) * rho0 * rho0 * c * c) *
+                T(phys.tension_scale);
         beta = (T(SIGMA_WATER) * (T(1) + T(COS_CONTACT)) * K(1558.4)) /
-               (T(LATTICE_SIGMA) * rho0 * rho0 * c * c);
+               (T(LATTICE_SIGMA) * rho0 * rho0 * c * c) * T(phys.adhesion_scale);
         // sim_solve.wgsl:559-565 constant factors
-        visc2 = T(2) * T(DYNAMIC_VISCOSITY);
+        visc2 = T(2) * T(phys.mu);
         heat_cond = T(2) * T(CONDUCTIVITY) / T(HEAT_CAPACITY);
-        heat_visc = K(0.5) * T(DYNAMIC_VISCOSITY) / T(HEAT_CAPACITY);
+        heat_visc = K(0.5) * T(phys.mu) / T(HEAT_CAPACITY);
+        k_near = T(phys.k_near);
         eps_h2 = K(0.01) * h * h;
     }
 };
@@ -375,7 +447,7 @@ struct StepT {
     uint32_t touch_n, seed;
     T touches[5][4];
     T c_xsph, chase;
-    explicit StepT(const StepBlock& s) {
+    explicit StepT(const StepBlock& s, const Physics& phys = Physics()) {
         force = v3((T)s.force[0], (T)s.force[1], (T)s.force[2]);
         omega = v3((T)s.omega[0], (T)s.omega[1], (T)s.omega[2]);
         domega = v3((T)s.domega[0], (T)s.domega[1], (T)s.domega[2]);
@@ -386,8 +458,8 @@ struct StepT {
         seed = s.seed;
         for (int i = 0; i < 5; ++i)
             for (int j = 0; j < 4; ++j) touches[i][j] = (T)s.touches[i][j];
-        c_xsph = T(1) - std::exp(-T(XSPH_RATE) * dt);  // sim_solve.wgsl:657
-        chase = T(1) - std::exp(-T(LENS_RATE) * dt);   // sim_solve.wgsl:741
+        c_xsph = T(1) - texp(-T(phys.xsph_rate) * dt);  // sim_solve.wgsl:657
+        chase = T(1) - texp(-T(LENS_RATE) * dt);   // sim_solve.wgsl:741
     }
 };
 
@@ -404,6 +476,7 @@ inline float exp_fixed(float x) {
     return std::ldexp(p, (int)k);
 }
 inline double exp_fixed(double x) { return std::exp(x); }
+inline Dual exp_fixed(const Dual& x) { return dexp(x); }
 
 // sim_solve.wgsl:103-127
 template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v) {
@@ -444,6 +517,18 @@ struct Sim {
     float stats[14] = {0};
     // tracers
     std::vector<int32_t> vel_grid, vel_flat;
+
+    Physics phys;
+    void set_physics(const Physics& ph) {
+        phys = ph;
+        k = Consts<T>(p, ph);
+    }
+    // Tangent seeds of the dual run: the next substep's gravity, viscosity.
+    double dforce[3] = {0.0, 0.0, 0.0};
+    void seed_mu(double dmu) {
+        seed(k.visc2, 2.0 * dmu);
+        seed(k.heat_visc, 0.5 * dmu / (double)HEAT_CAPACITY);
+    }
 
     Sim(const Params& prm, uint32_t nbr_cap) : p(prm), k(prm) {
         n = p.count;
@@ -543,7 +628,7 @@ struct Sim {
                     T r2 = dot(x, x);
                     if (r2 >= k.support_sq) continue;
                     const uint32_t l = found % kListLanes;
-                    T r = std::sqrt(r2);
+                    T r = tsqrt(r2);
                     T q = r * k.inv_h;
                     w_l[l] += kernel_q(k, q);
                     T wn = tmax(T(1) - q, T(0));
@@ -594,7 +679,7 @@ struct Sim {
     // Gradient of the pair (i, j), recomputed exactly as density_div did.
     inline V3<T> pair_grad(V3<T> pos_i, uint32_t j) const {
         V3<T> x = pos_i - wp(j);
-        T r = std::sqrt(dot(x, x));
+        T r = tsqrt(dot(x, x));
         return grad_q(k, x, r, r * k.inv_h);
     }
 
@@ -645,7 +730,7 @@ struct Sim {
                 T& heat = heat_l[l];
                 V3<T> x = pos_i - wp(j);
                 T r2 = dot(x, x);
-                T r = std::sqrt(r2);
+                T r = tsqrt(r2);
                 V3<T> g = grad_q(k, x, r, r * k.inv_h);
                 T rho_j = density[j];
                 T f = dot(x, g) / (r2 + k.eps_h2);
@@ -671,7 +756,7 @@ struct Sim {
                         tension = lane_fold(tension_l);
             const T heat = lane_fold(heat_l);
             V3<T> adh = (k.beta * k.rho0) * wall_adh_sum(k, pos_i);
-            V3<T> acc = visc + T(K_NEAR) * nearf + k.gamma * tension + adh;
+            V3<T> acc = visc + k.k_near * nearf + k.gamma * tension + adh;
             accel[4 * i] = acc.x;
             accel[4 * i + 1] = acc.y;
             accel[4 * i + 2] = acc.z;
@@ -749,7 +834,7 @@ struct Sim {
             uint32_t i = (uint32_t)pi;
             V3<T> v = wv(i);
             T was_w = wvel[4 * i + 3];
-            T speed = std::sqrt(dot(v, v));
+            T speed = tsqrt(dot(v, v));
             T ceiling = density[i] < K(0.5) * k.rho0 ? T(3) * s.v_clamp : s.v_clamp;
             if (speed > ceiling) {
                 v = (ceiling / speed) * v;
@@ -757,7 +842,7 @@ struct Sim {
             }
             V3<T> pv = v3(wpvel[4 * i], wpvel[4 * i + 1], wpvel[4 * i + 2]);
             V3<T> dvv = v - pv;
-            T acc = std::sqrt(dot(dvv, dvv)) / s.dt;
+            T acc = tsqrt(dot(dvv, dvv)) / s.dt;
             V3<T> np = wp(i) + s.dt * v;
             T out[3];
             T in[3] = {np.x, np.y, np.z};
@@ -790,7 +875,10 @@ struct Sim {
 
     // One substep in the reference dispatch order, render.rs:3175-3203.
     void substep(const StepBlock& sb, uint32_t refine) {
-        StepT<T> s(sb);
+        StepT<T> s(sb, phys);
+        seed(s.force.x, dforce[0]);
+        seed(s.force.y, dforce[1]);
+        seed(s.force.z, dforce[2]);
         grid_build();
         density_div(s);
         kappa_apply(s);       // div_apply
@@ -806,7 +894,7 @@ struct Sim {
     // The zero-substep path of the reference's test driver (render.rs:3902-3937):
     // rebuild plus the density sweep alone.
     void density_only(const StepBlock& sb) {
-        StepT<T> s(sb);
+        StepT<T> s(sb, phys);
         grid_build();
         density_div(s);
     }
@@ -963,13 +1051,15 @@ void getd(const std::vector<T>& src, double* dst, size_t count) {
 }
 
 struct Handle {
-    int precision;  // 32 or 64
+    int precision;  // 32, 64, or 128 (the dual-number FP64 run)
     Sim<float>* f;
     Sim<double>* d;
+    Sim<Dual>* a;
 };
 
 template <typename F>
 auto dispatch(Handle* h, F&& fn) {
+    if (h->precision == 128) return fn(*h->a);
     if (h->precision == 64) return fn(*h->d);
     return fn(*h->f);
 }
@@ -982,8 +1072,10 @@ extern "C" {
 void* oracle_create(const void* params64, uint32_t nbr_cap, int precision) {
     Params p;
     std::memcpy(&p, params64, 64);
-    Handle* h = new Handle{precision, nullptr, nullptr};
-    if (precision == 64)
+    Handle* h = new Handle{precision, nullptr, nullptr, nullptr};
+    if (precision == 128)
+        h->a = new Sim<Dual>(p, nbr_cap);
+    else if (precision == 64)
         h->d = new Sim<double>(p, nbr_cap);
     else
         h->f = new Sim<float>(p, nbr_cap);
@@ -994,7 +1086,58 @@ void oracle_destroy(void* hv) {
     Handle* h = (Handle*)hv;
     delete h->f;
     delete h->d;
+    delete h->a;
     delete h;
+}
+
+// {mu, k_near, tension_scale, adhesion_scale, xsph_rate}
+void oracle_set_physics(void* hv, const float* phys5) {
+    Physics ph;
+    ph.mu = phys5[0];
+    ph.k_near = phys5[1];
+    ph.tension_scale = phys5[2];
+    ph.adhesion_scale = phys5[3];
+    ph.xsph_rate = phys5[4];
+    dispatch((Handle*)hv, [&](auto& s) -> int {
+        s.set_physics(ph);
+        return 0;
+    });
+}
+
+// Dual runs (precision 128): tangents of the resting set's positions (0,
+// n x 4), velocities (1, n x 4) or previous pressures (2, n).
+void oracle_set_tangent(void* hv, int which, const double* d) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        auto* v = which == 0 ? &s.pos : which == 1 ? &s.vel : &s.pp;
+        const size_t cnt = which == 2 ? s.n : (size_t)s.n * 4;
+        for (size_t i = 0; i < cnt; ++i) seed((*v)[i], d[i]);
+        return 0;
+    });
+}
+
+void oracle_get_tangent(void* hv, int which, double* out) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        const auto* v = which == 0 ? &s.pos : which == 1 ? &s.vel : &s.pp;
+        const size_t cnt = which == 2 ? s.n : (size_t)s.n * 4;
+        for (size_t i = 0; i < cnt; ++i) out[i] = tangent((*v)[i]);
+        return 0;
+    });
+}
+
+// The tangent of the next substeps' gravity (kept until changed).
+void oracle_seed_force(void* hv, const double* d3) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        for (int a = 0; a < 3; ++a) s.dforce[a] = d3[a];
+        return 0;
+    });
+}
+
+// The tangent of the viscosity (after oracle_set_physics).
+void oracle_seed_mu(void* hv, double dmu) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        s.seed_mu(dmu);
+        return 0;
+    });
 }
 
 void oracle_set_threads(int n) {
@@ -1130,7 +1273,7 @@ int oracle_kernel(void* hv, const char* name, const void* step144) {
         using S = std::remove_reference_t<decltype(s)>;
         using T = std::remove_reference_t<decltype(s.density[0])>;
         (void)sizeof(S);
-        StepT<T> st(sb);
+        StepT<T> st(sb, s.phys);
         if (k == "grid") s.grid_build();
         else if (k == "density_div") s.density_div(st);
         else if (k == "div_apply" || k == "den_apply") s.kappa_apply(st);

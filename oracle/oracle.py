@@ -54,6 +54,11 @@ def lib():
         L.oracle_create.restype = vp
         L.oracle_destroy.argtypes = [vp]
         L.oracle_set_threads.argtypes = [i32]
+        L.oracle_set_physics.argtypes = [vp, P(f32)]
+        L.oracle_set_tangent.argtypes = [vp, i32, P(f64)]
+        L.oracle_get_tangent.argtypes = [vp, i32, P(f64)]
+        L.oracle_seed_force.argtypes = [vp, P(f64)]
+        L.oracle_seed_mu.argtypes = [vp, f64]
         L.oracle_max_threads.restype = i32
         L.oracle_set_state.argtypes = [vp] + [P(f32)] * 5
         L.oracle_get_state.argtypes = [vp] + [P(f32)] * 5 + [P(u32)]
@@ -126,6 +131,29 @@ class OracleSim:
             self.close()
         except Exception:
             pass
+
+    def set_physics(self, mu=1.002e-3, k_near=3000.0, tension_scale=1.0, adhesion_scale=1.0, xsph_rate=6.0):
+        arr = (C.c_float * 5)(mu, k_near, tension_scale, adhesion_scale, xsph_rate)
+        self.L.oracle_set_physics(self.h, arr)
+
+    # Dual runs (precision=128): forward-mode derivatives in FP64.
+    def set_tangent(self, which: str, d):
+        w = {"pos": 0, "vel": 1, "prev_pressure": 2}[which]
+        a = np.ascontiguousarray(d, np.float64)
+        self.L.oracle_set_tangent(self.h, w, a.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def tangent(self, which: str):
+        w = {"pos": 0, "vel": 1, "prev_pressure": 2}[which]
+        out = np.empty(self.n * (1 if w == 2 else 4), np.float64)
+        self.L.oracle_get_tangent(self.h, w, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out if w == 2 else out.reshape(self.n, 4)
+
+    def seed_force(self, d3):
+        a = np.ascontiguousarray(d3, np.float64)
+        self.L.oracle_seed_force(self.h, a.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def seed_mu(self, dmu: float):
+        self.L.oracle_seed_mu(self.h, float(dmu))
 
     def set_state(self, pos4=None, vel4=None, prev_vel4=None, prev_pressure=None, temperature=None):
         arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float32)

@@ -143,6 +143,39 @@ def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=4, seed=0,
     }
 
 
+def adjoint_leg(fa, n=65536, steps=64, every=8, seed=0):
+    """The adjoint configuration: one 65,536-particle scene, 64 substeps with
+    a seeded random-walk tilt of gravity, a checkpoint every 8 substeps; the
+    forward run and the backward run (segment replays + reverse sweeps),
+    each timed with HIP events on the adjoint's stream. The loss seed is the
+    centroid term's gradient (the cost does not depend on the seed)."""
+    from fluid_amd import adjoint as fad
+    sc = fa.scenes.fill_block("adjoint_64k", n, 0.3, seed=seed)
+    _, dt, refine, vc = sc.schedule()
+    rng = np.random.default_rng(seed)
+    th = np.cumsum(rng.normal(0.0, 0.05, steps))
+    grav = np.stack([9.81 * np.sin(th), -9.81 * np.cos(th), np.zeros(steps)], 1)
+    v0 = np.zeros((sc.n, 4), np.float32)
+    v0[:, :3] = rng.normal(0.0, 0.05, (sc.n, 3))
+    adj = fad.Adjoint(sc.params, steps, every)
+    adj.set_mu(1.002e-3)
+    blocks = fad.steps_for(grav, dt, vc)
+    pos, _ = adj.forward(sc.pos4, v0, blocks, [refine] * steps)  # warm-up
+    c_t = pos[:, :3].mean(0) + 0.01
+    seed4 = np.zeros((sc.n, 4), np.float32)
+    seed4[:, :3] = 2.0 * (pos[:, :3].mean(0) - c_t) / sc.n
+    adj.backward(seed4)
+    pos, _ = adj.forward(sc.pos4, v0, blocks, [refine] * steps)
+    adj.backward(seed4)
+    fwd_ms, bwd_ms = adj.timing()
+    adj.close()
+    return {"config": {"workload": "adjoint_64k", "particles": sc.n, "substeps": steps, "checkpoint_every": every,
+                       "refine": refine, "profile": "differentiable (viscosity, gravity; near/cohesion/adhesion/"
+                                                    "XSPH off)"},
+            "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adjoint_over_forward": bwd_ms / fwd_ms,
+            "forward_particle_substeps_per_s": sc.n * steps / (fwd_ms * 1e-3)}
+
+
 def copy_calibration(nbytes: int, reps: int = 4) -> None:
     """Runs `reps` device-to-device copies of `nbytes` (HIP's own copy kernel)
     so that a counter run can read what FETCH_SIZE / WRITE_SIZE report for a
@@ -176,6 +209,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
     ap.add_argument("--batched-only", action="store_true", help="print only the batched-ensemble line")
     ap.add_argument("--scenes", type=int, default=256)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint line")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
@@ -189,6 +224,9 @@ def main():
     if fa.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.adjoint_only:
+        print(json.dumps(adjoint_leg(fa, seed=args.seed)), flush=True)
+        return
     if args.batched_only:
         print(json.dumps(batched_ensemble(fa, scenes=args.scenes, seed=args.seed)), flush=True)
         return
@@ -276,6 +314,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_ensemble(fa, scenes=args.scenes, seed=args.seed)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_leg(fa, seed=args.seed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, steps[0], refine)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

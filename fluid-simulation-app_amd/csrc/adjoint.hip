@@ -21,7 +21,9 @@
 // dW/r, and the Hessian H_ij = g I + hh r r^T with hh = (dw'(q)/h - g) / rho^2.
 // The lists are symmetric (the squared distance is bit-identical both ways),
 // so every pair term's adjoint is gathered by both particles from their own
-// lists; each sweep is one thread a particle over its list.
+// lists. The sweeps run four lanes a particle as the forward does (lane l
+// takes list entries l, l + 4, ...; the partials fold across the quad), the
+// per-particle work one thread a particle.
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 #include "adjoint.hpp"
@@ -31,6 +33,33 @@ namespace fluid {
 namespace {
 
 constexpr uint32_t kAdjBlock = 256;
+constexpr uint32_t kQuadBlock = 256 * kListLanes;  // 256 particles a workgroup
+
+// A particle's lane in the four-lane sweeps.
+struct Quad {
+    uint32_t p, lane, gid;
+    bool live;
+};
+__device__ __forceinline__ Quad quad(uint32_t n) {
+    Quad q;
+    q.gid = blockIdx.x * kQuadBlock + threadIdx.x;
+    q.p = q.gid / kListLanes;
+    q.lane = q.gid % kListLanes;
+    q.live = q.p < n;
+    return q;
+}
+// Calls f(neighbour slot, dW/r) for the lane's share of particle p's list:
+// entry 4t + lane sits at quad row t, offset gid.
+template <class F>
+__device__ __forceinline__ void for_lane(const SolveBufs& b, const Quad& q, F&& f) {
+    const uint32_t nn = b.nbr_n[q.p];
+    const size_t step = (size_t)b.stride * kListLanes;
+    uint32_t t = 0;
+    for (uint32_t m = q.lane; m < nn; m += kListLanes, ++t) {
+        const uint2 e = (b.nbr + (size_t)t * step)[q.gid];
+        f(e.x, __uint_as_float(e.y));
+    }
+}
 
 __device__ __forceinline__ f3 ld3(const float4* a, uint32_t i) { return xyz(a[i]); }
 __device__ __forceinline__ void add3(float4* a, uint32_t i, f3 v) {
@@ -58,12 +87,6 @@ __device__ __forceinline__ PairGeom pair_geom(const SceneK& k, f3 xi, f3 xj, flo
 }
 __device__ __forceinline__ f3 hess(const PairGeom& P, f3 u) { return add(scl(P.g, u), scl(P.hh * dot3(P.r, u), P.r)); }
 
-__device__ __forceinline__ uint32_t entry(const SolveBufs& b, uint32_t p, uint32_t m) {
-    return b.nbr[((size_t)(m / kListLanes) * b.stride + p) * kListLanes + m % kListLanes].x;
-}
-__device__ __forceinline__ float entry_g(const SolveBufs& b, uint32_t p, uint32_t m) {
-    return __uint_as_float(b.nbr[((size_t)(m / kListLanes) * b.stride + p) * kListLanes + m % kListLanes].y);
-}
 
 // Transposed 3x3 wall-gradient Jacobian times u: J rows are d wg / dx_a
 // stored as columns a (J[a] = d wg / d x_a), so J^T u = (J[0].u, J[1].u, J[2].u).
@@ -73,11 +96,20 @@ __device__ __forceinline__ f3 jt(const float4* J, uint32_t i, f3 u) {
 
 // Per particle: the gradient of the wall fill (grad F) and the Jacobian of
 // wall_grad_sum, by dual evaluation of the same polynomials.
-__global__ __launch_bounds__(kAdjBlock) void k_wall_jacobians(SceneK k, uint32_t n, const float4* __restrict__ pos,
-                                                              float4* __restrict__ dfill, float4* __restrict__ J) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
+__device__ __forceinline__ void wall_jacobians(const SceneK& k, uint32_t i, const float4* __restrict__ pos,
+                                               float4* __restrict__ dfill, float4* __restrict__ J) {
     const f3 p = xyz(pos[i]);
+    // Every wall term vanishes when all six wall distances are at least the
+    // support (t >= 2): wall_density, wall_gradient, and the wedges (which
+    // need two of them below 2).
+    const float ih = fminf(k.inv_h, k.inv_h_wall);
+    float dmin = fminf(fminf(p.x - k.wall_lo[0], k.wall_hi[0] - p.x), fminf(p.y - k.wall_lo[1], k.wall_hi[1] - p.y));
+    dmin = fminf(dmin, fminf(p.z - k.wall_lo[2], k.wall_hi[2] - p.z));
+    if (dmin * ih >= 2.0f) {
+        dfill[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        for (int a = 0; a < 3; ++a) J[3 * i + a] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        return;
+    }
     float df[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -95,13 +127,21 @@ __global__ __launch_bounds__(kAdjBlock) void k_wall_jacobians(SceneK k, uint32_t
 // the working slot; lam_p is the adjoint of the pressure it stores as the
 // next substep's previous pressure. Outputs, in place: lam_x of the working
 // position and lam_v of the velocity entering integrate.
+// Also, per particle: the wall Jacobians of the substep and zeroed
+// lam_rho / lam_alpha accumulators.
 __global__ __launch_bounds__(kAdjBlock) void k_rev_integrate(SceneK k, StepK s, uint32_t n,
                                                              const float4* __restrict__ pos,
                                                              const float4* __restrict__ vin,
                                                              const float* __restrict__ density,
-                                                             float4* __restrict__ lam_x, float4* __restrict__ lam_v) {
+                                                             float4* __restrict__ lam_x, float4* __restrict__ lam_v,
+                                                             float4* __restrict__ dfill, float4* __restrict__ J,
+                                                             float* __restrict__ lam_rho,
+                                                             float* __restrict__ lam_alpha) {
     const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
     if (i >= n) return;
+    lam_rho[i] = 0.0f;
+    lam_alpha[i] = 0.0f;
+    wall_jacobians(k, i, pos, dfill, J);
     f3 v = xyz(vin[i]);
     const float speed = sqrtf(dot3(v, v));
     const float rho = density[i];
@@ -133,45 +173,61 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_integrate(SceneK k, StepK s, 
 // den_apply): v_out = v_in - dt (m sum (kd_i + kd_j) gradW_ij + kd_i wg_i).
 // lam (the adjoint of v_out) passes to v_in unchanged; this sweep yields
 // lam_kd and adds the position terms to lam_x.
-__global__ __launch_bounds__(kAdjBlock) void k_rev_apply(SceneK k, float dt, uint32_t n, SolveBufs b,
-                                                         const float4* __restrict__ pos,
-                                                         const float* __restrict__ kd,
-                                                         const float4* __restrict__ wg, const float4* __restrict__ J,
-                                                         const float4* __restrict__ lam, float* __restrict__ lam_kd,
-                                                         float4* __restrict__ lam_x) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const f3 xi = ld3(pos, i), li = ld3(lam, i);
-    const float ki = kd[i];
+// The per-particle reverse of den_kappa (below) that follows den_apply's.
+__device__ __forceinline__ void den_kappa_pre(const StepK& s, uint32_t i, float lk, const float* __restrict__ rho,
+                                              const float* __restrict__ alpha, const float* __restrict__ kd,
+                                              const float* __restrict__ lam_p, float* __restrict__ lam_D,
+                                              float* __restrict__ lam_rho, float* __restrict__ lam_alpha);
+
+// With DEN, the reverse of den_kappa's per-particle part follows at once
+// (its kd is this apply's kd), giving lam_D for the divergence sweep.
+template <bool DEN>
+__global__ __launch_bounds__(kQuadBlock) void k_rev_apply(SceneK k, StepK st, uint32_t n, SolveBufs b,
+                                                          const float4* __restrict__ pos,
+                                                          const float* __restrict__ kd,
+                                                          const float4* __restrict__ wg, const float4* __restrict__ J,
+                                                          const float4* __restrict__ lam, float* __restrict__ lam_kd,
+                                                          float4* __restrict__ lam_x, const float* __restrict__ rho,
+                                                          const float* __restrict__ alpha,
+                                                          const float* __restrict__ lam_p, float* __restrict__ lam_D,
+                                                          float* __restrict__ lam_rho, float* __restrict__ lam_alpha) {
+    const float dt = st.dt;
+    const Quad q = quad(n);
     float s_lk = 0.0f;
-    f3 s_x = mk(0.0f, 0.0f, 0.0f);
-    const uint32_t nn = b.nbr_n[i];
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = entry(b, i, m);
-        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
-        const f3 dl = sub(li, ld3(lam, j));
-        s_lk += dot3(dl, scl(P.g, P.r));
-        s_x = add(s_x, scl(ki + kd[j], hess(P, dl)));
+    f3 s_x = mk(0.0f, 0.0f, 0.0f), li = s_x;
+    float ki = 0.0f;
+    if (q.live) {
+        const f3 xi = ld3(pos, q.p);
+        li = ld3(lam, q.p);
+        ki = kd[q.p];
+        for_lane(b, q, [&](uint32_t j, float g) {
+            const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
+            const f3 dl = sub(li, ld3(lam, j));
+            s_lk += dot3(dl, scl(P.g, P.r));
+            s_x = add(s_x, scl(ki + kd[j], hess(P, dl)));
+        });
     }
-    lam_kd[i] = -dt * (k.mass * s_lk + dot3(li, ld3(wg, i)));
+    s_lk = fold4(s_lk);
+    s_x = fold4(s_x);
+    if (!q.live || q.lane != 0) return;
+    const uint32_t i = q.p;
+    const float lk = -dt * (k.mass * s_lk + dot3(li, ld3(wg, i)));
     const f3 w = jt(J, i, li);
     add3(lam_x, i, scl(-dt, add(scl(k.mass, s_x), scl(ki, w))));
+    if (DEN)
+        den_kappa_pre(st, i, lk, rho, alpha, kd, lam_p, lam_D, lam_rho, lam_alpha);
+    else
+        lam_kd[i] = lk;
 }
 
 // den_kappa in reverse, per particle: from lam_kd of the pass (and the
 // pressure adjoint lam_p, p += kappa rho) to lam_D, adding to lam_rho and
 // lam_alpha. kappa = max(rho + dt D - rho0, 0) alpha / dt^2, kd = kappa / rho.
-__global__ __launch_bounds__(kAdjBlock) void k_rev_den_kappa_pre(StepK s, uint32_t n, const float* __restrict__ rho,
-                                                                 const float* __restrict__ alpha,
-                                                                 const float* __restrict__ kd,
-                                                                 const float* __restrict__ lam_kd,
-                                                                 const float* __restrict__ lam_p,
-                                                                 float* __restrict__ lam_D,
-                                                                 float* __restrict__ lam_rho,
-                                                                 float* __restrict__ lam_alpha) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const float r = rho[i], a = alpha[i], k = kd[i], lk = lam_kd[i], lp = lam_p[i];
+__device__ __forceinline__ void den_kappa_pre(const StepK& s, uint32_t i, float lk, const float* __restrict__ rho,
+                                              const float* __restrict__ alpha, const float* __restrict__ kd,
+                                              const float* __restrict__ lam_p, float* __restrict__ lam_D,
+                                              float* __restrict__ lam_rho, float* __restrict__ lam_alpha) {
+    const float r = rho[i], a = alpha[i], k = kd[i], lp = lam_p[i];
     const float kap = k * r;
     const float lkap = lk / r + lp * r;
     float lr = -lk * k / r + lp * kap;
@@ -190,27 +246,33 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_den_kappa_pre(StepK s, uint32
 // The sweep of a divergence-type term D_i = m sum (v_i - v_j).gradW_ij +
 // v_i.wg_i in reverse (den_kappa, and density_div's D): lam_v_i = m sum
 // (lam_D_i + lam_D_j) gradW_ij + lam_D_i wg_i, and the position terms.
-__global__ __launch_bounds__(kAdjBlock) void k_rev_divergence(SceneK k, uint32_t n, SolveBufs b,
-                                                              const float4* __restrict__ pos,
-                                                              const float4* __restrict__ vel,
-                                                              const float4* __restrict__ wg,
-                                                              const float4* __restrict__ J,
-                                                              const float* __restrict__ lam_D,
-                                                              float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const f3 xi = ld3(pos, i), vi = ld3(vel, i);
-    const float di = lam_D[i];
-    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
-    const uint32_t nn = b.nbr_n[i];
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = entry(b, i, m);
-        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
-        const float dj = lam_D[j];
-        // D_j holds (v_j - v_i).gradW_ji = v_i.gradW_ij + ..., so both ends add
-        s_v = add(s_v, scl(di + dj, scl(P.g, P.r)));
-        s_x = add(s_x, scl(di + dj, hess(P, sub(vi, ld3(vel, j)))));
+__global__ __launch_bounds__(kQuadBlock) void k_rev_divergence(SceneK k, uint32_t n, SolveBufs b,
+                                                               const float4* __restrict__ pos,
+                                                               const float4* __restrict__ vel,
+                                                               const float4* __restrict__ wg,
+                                                               const float4* __restrict__ J,
+                                                               const float* __restrict__ lam_D,
+                                                               float4* __restrict__ lam_v,
+                                                               float4* __restrict__ lam_x) {
+    const Quad q = quad(n);
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, vi = s_v;
+    float di = 0.0f;
+    if (q.live) {
+        const f3 xi = ld3(pos, q.p);
+        vi = ld3(vel, q.p);
+        di = lam_D[q.p];
+        for_lane(b, q, [&](uint32_t j, float g) {
+            const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
+            const float dj = lam_D[j];
+            // D_j holds (v_j - v_i).gradW_ji = v_i.gradW_ij + ..., so both ends add
+            s_v = add(s_v, scl(di + dj, scl(P.g, P.r)));
+            s_x = add(s_x, scl(di + dj, hess(P, sub(vi, ld3(vel, j)))));
+        });
     }
+    s_v = fold4(s_v);
+    s_x = fold4(s_x);
+    if (!q.live || q.lane != 0) return;
+    const uint32_t i = q.p;
     add3(lam_v, i, add(scl(k.mass, s_v), scl(di, ld3(wg, i))));
     add3(lam_x, i, add(scl(k.mass, s_x), scl(di, jt(J, i, vi))));
 }
@@ -251,79 +313,89 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_forces_den(StepK s, uint32_t 
 // forces_eval's viscosity in reverse: a_i = visc2 sum P_ij (v_i - v_j), P_ij
 // = m f(|r|) / (rho_i rho_j), f = g r^2 / (r^2 + eps). lam_v, lam_rho, lam_x,
 // and d loss / d mu = sum lam_a . a / mu (one atomic per block).
-__global__ __launch_bounds__(kAdjBlock) void k_rev_viscosity(SceneK k, float mu, uint32_t n, SolveBufs b,
-                                                             const float4* __restrict__ pos,
-                                                             const float4* __restrict__ vel,
-                                                             const float* __restrict__ rho,
-                                                             const float4* __restrict__ accel,
-                                                             const float4* __restrict__ lam_a,
-                                                             float4* __restrict__ lam_v, float* __restrict__ lam_rho,
-                                                             float4* __restrict__ lam_x, float* __restrict__ lam_mu) {
-    __shared__ float red[kAdjBlock];
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    float dmu = 0.0f;
-    if (i < n) {
-        const f3 xi = ld3(pos, i), vi = ld3(vel, i), la = ld3(lam_a, i);
-        const float ri = rho[i];
-        f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
-        float s_r = 0.0f;
-        const uint32_t nn = b.nbr_n[i];
-        for (uint32_t m = 0; m < nn; ++m) {
-            const uint32_t j = entry(b, i, m);
-            const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
+__global__ __launch_bounds__(kQuadBlock) void k_rev_viscosity(SceneK k, float mu, uint32_t n, SolveBufs b,
+                                                              const float4* __restrict__ pos,
+                                                              const float4* __restrict__ vel,
+                                                              const float* __restrict__ rho,
+                                                              const float4* __restrict__ accel,
+                                                              const float4* __restrict__ lam_a,
+                                                              float4* __restrict__ lam_v, float* __restrict__ lam_rho,
+                                                              float4* __restrict__ lam_x, float* __restrict__ lam_mu) {
+    __shared__ float red[kQuadBlock / 64];
+    const Quad q = quad(n);
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, la = s_v;
+    float s_r = 0.0f, ri = 1.0f;
+    if (q.live) {
+        const f3 xi = ld3(pos, q.p), vi = ld3(vel, q.p);
+        la = ld3(lam_a, q.p);
+        ri = rho[q.p];
+        for_lane(b, q, [&](uint32_t j, float g) {
+            const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
             const float r2 = dot3(P.r, P.r);
             const float den = r2 + k.eps_h2;
             const float f = P.g * r2 / den;
             const float rj = rho[j];
             const float pair = k.mass / (ri * rj) * f;
             const f3 dl = sub(la, ld3(lam_a, j));
-            const f3 dv = sub(vi, ld3(vel, j));
-            const float dldv = dot3(dl, dv);
+            const float dldv = dot3(dl, sub(vi, ld3(vel, j)));
             s_v = add(s_v, scl(pair, dl));
             s_r += pair * dldv;
             const float fp_over_r = P.hh * r2 / den + 2.0f * P.g * k.eps_h2 / (den * den);
             s_x = add(s_x, scl(k.mass / (ri * rj) * fp_over_r * dldv, P.r));
-        }
+        });
+    }
+    s_v = fold4(s_v);
+    s_x = fold4(s_x);
+    s_r = fold4(s_r);
+    float dmu = 0.0f;
+    if (q.live && q.lane == 0) {
+        const uint32_t i = q.p;
         add3(lam_v, i, scl(k.visc2, s_v));
         lam_rho[i] += -k.visc2 / ri * s_r;
         add3(lam_x, i, scl(k.visc2, s_x));
         dmu = dot3(la, ld3(accel, i)) / mu;
     }
-    red[threadIdx.x] = dmu;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) dmu += __shfl_xor(dmu, o);
+    if (__lane_id() == 0) red[threadIdx.x / 64] = dmu;
     __syncthreads();
-    for (uint32_t o = kAdjBlock / 2; o > 0; o >>= 1) {
-        if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.0f;
+        for (uint32_t w = 0; w < kQuadBlock / 64; ++w) t += red[w];
+        atomicAdd(lam_mu, t);
     }
-    if (threadIdx.x == 0) atomicAdd(lam_mu, red[0]);
 }
 
 // density_div in reverse, part 1 (per particle, with a sweep recomputing G =
 // m sum gradW + wg and gsq = m^2 sum |gradW|^2): kd = max(D, 0) alpha / dt /
 // rho and alpha = rho / max(|G|^2 + gsq, 1e-4) give lam_D, lam_G, lam_gsq and
 // the final lam_rho.
-__global__ __launch_bounds__(kAdjBlock) void k_rev_density_pre(SceneK k, StepK s, uint32_t n, SolveBufs b,
-                                                               const float4* __restrict__ pos,
-                                                               const float4* __restrict__ wg,
-                                                               const float* __restrict__ rho,
-                                                               const float* __restrict__ alpha,
-                                                               const float* __restrict__ kd,
-                                                               const float* __restrict__ lam_kd,
-                                                               float* __restrict__ lam_alpha,
-                                                               float* __restrict__ lam_rho, float* __restrict__ lam_D,
-                                                               float4* __restrict__ lam_G) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const f3 xi = ld3(pos, i);
+__global__ __launch_bounds__(kQuadBlock) void k_rev_density_pre(SceneK k, StepK s, uint32_t n, SolveBufs b,
+                                                                const float4* __restrict__ pos,
+                                                                const float4* __restrict__ wg,
+                                                                const float* __restrict__ rho,
+                                                                const float* __restrict__ alpha,
+                                                                const float* __restrict__ kd,
+                                                                const float* __restrict__ lam_kd,
+                                                                float* __restrict__ lam_alpha,
+                                                                float* __restrict__ lam_rho,
+                                                                float* __restrict__ lam_D,
+                                                                float4* __restrict__ lam_G) {
+    const Quad q = quad(n);
     f3 G = mk(0.0f, 0.0f, 0.0f);
     float gsq = 0.0f;
-    const uint32_t nn = b.nbr_n[i];
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = entry(b, i, m);
-        const f3 gw = scl(entry_g(b, i, m), sub(xi, ld3(pos, j)));
-        G = add(G, gw);
-        gsq += dot3(gw, gw);
+    if (q.live) {
+        const f3 xi = ld3(pos, q.p);
+        for_lane(b, q, [&](uint32_t j, float g) {
+            const f3 gw = scl(g, sub(xi, ld3(pos, j)));
+            G = add(G, gw);
+            gsq += dot3(gw, gw);
+        });
     }
+    G = fold4(G);
+    gsq = fold4(gsq);
+    if (!q.live || q.lane != 0) return;
+    const uint32_t i = q.p;
     G = add(scl(k.mass, G), ld3(wg, i));
     gsq *= k.mass * k.mass;
     const float r = rho[i], a = alpha[i], kk = kd[i], lk = lam_kd[i];
@@ -334,11 +406,11 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_density_pre(SceneK k, StepK s
         la += lk * kk / a;
     }
     lr += -lk * kk / r;
-    const float q = dot3(G, G) + gsq;
-    const float den = wmax(q, 1e-4f);
+    const float qq = dot3(G, G) + gsq;
+    const float den = wmax(qq, 1e-4f);
     lr += la / den;
     float lden = 0.0f;
-    if (q > 1e-4f) lden = -la * a / den;
+    if (qq > 1e-4f) lden = -la * a / den;
     lam_rho[i] = lr;
     lam_D[i] = lD;
     lam_G[i] = make_float4(2.0f * lden * G.x, 2.0f * lden * G.y, 2.0f * lden * G.z, lden);  // w: lam_gsq
@@ -346,38 +418,44 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_density_pre(SceneK k, StepK s
 
 // density_div in reverse, part 2 (the sweep): lam_v from D, lam_x from D, G,
 // gsq, rho and the walls. rho = m (sigma + sum W) + rho0 fill(x).
-__global__ __launch_bounds__(kAdjBlock) void k_rev_density(SceneK k, uint32_t n, SolveBufs b,
-                                                           const float4* __restrict__ pos,
-                                                           const float4* __restrict__ vel,
-                                                           const float4* __restrict__ wg,
-                                                           const float4* __restrict__ J,
-                                                           const float4* __restrict__ dfill,
-                                                           const float* __restrict__ lam_rho,
-                                                           const float* __restrict__ lam_D,
-                                                           const float4* __restrict__ lam_G,
-                                                           float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
-    const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    if (i >= n) return;
-    const f3 xi = ld3(pos, i), vi = ld3(vel, i);
-    const float di = lam_D[i], ri = lam_rho[i];
-    const float4 Gi4 = lam_G[i];
-    const f3 Gi = xyz(Gi4);
-    const float si = Gi4.w;
-    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v;
-    const uint32_t nn = b.nbr_n[i];
-    for (uint32_t m = 0; m < nn; ++m) {
-        const uint32_t j = entry(b, i, m);
-        const PairGeom P = pair_geom(k, xi, ld3(pos, j), entry_g(b, i, m));
-        const f3 gw = scl(P.g, P.r);
-        const float dj = lam_D[j];
-        const float4 Gj4 = lam_G[j];
-        s_v = add(s_v, scl(di + dj, gw));
-        f3 t = scl(di + dj, hess(P, sub(vi, ld3(vel, j))));
-        t = add(t, hess(P, sub(Gi, xyz(Gj4))));
-        t = add(t, scl(2.0f * k.mass * (si + Gj4.w), hess(P, gw)));
-        t = add(t, scl(ri + lam_rho[j], gw));
-        s_x = add(s_x, t);
+__global__ __launch_bounds__(kQuadBlock) void k_rev_density(SceneK k, uint32_t n, SolveBufs b,
+                                                            const float4* __restrict__ pos,
+                                                            const float4* __restrict__ vel,
+                                                            const float4* __restrict__ wg,
+                                                            const float4* __restrict__ J,
+                                                            const float4* __restrict__ dfill,
+                                                            const float* __restrict__ lam_rho,
+                                                            const float* __restrict__ lam_D,
+                                                            const float4* __restrict__ lam_G,
+                                                            float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
+    const Quad q = quad(n);
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, vi = s_v, Gi = s_v;
+    float di = 0.0f, ri = 0.0f;
+    if (q.live) {
+        const f3 xi = ld3(pos, q.p);
+        vi = ld3(vel, q.p);
+        di = lam_D[q.p];
+        ri = lam_rho[q.p];
+        const float4 Gi4 = lam_G[q.p];
+        Gi = xyz(Gi4);
+        const float si = Gi4.w;
+        for_lane(b, q, [&](uint32_t j, float g) {
+            const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
+            const f3 gw = scl(P.g, P.r);
+            const float dj = lam_D[j];
+            const float4 Gj4 = lam_G[j];
+            s_v = add(s_v, scl(di + dj, gw));
+            f3 t = scl(di + dj, hess(P, sub(vi, ld3(vel, j))));
+            t = add(t, hess(P, sub(Gi, xyz(Gj4))));
+            t = add(t, scl(2.0f * k.mass * (si + Gj4.w), hess(P, gw)));
+            t = add(t, scl(ri + lam_rho[j], gw));
+            s_x = add(s_x, t);
+        });
     }
+    s_v = fold4(s_v);
+    s_x = fold4(s_x);
+    if (!q.live || q.lane != 0) return;
+    const uint32_t i = q.p;
     add3(lam_v, i, add(scl(k.mass, s_v), scl(di, ld3(wg, i))));
     f3 w = add(scl(di, jt(J, i, vi)), jt(J, i, Gi));
     w = add(w, scl(k.rho0 * ri, ld3(dfill, i)));
@@ -406,6 +484,7 @@ __global__ __launch_bounds__(kAdjBlock) void k_fill_zero(float* __restrict__ a, 
 }  // namespace
 
 static dim3 adj_grid(uint32_t n) { return dim3((n + kAdjBlock - 1) / kAdjBlock); }
+static dim3 quad_grid(uint32_t n) { return dim3((n * kListLanes + kQuadBlock - 1) / kQuadBlock); }
 
 void adj_zero(hipStream_t st, float* a, size_t count) {
     if (count) hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((count + kAdjBlock - 1) / kAdjBlock)),
@@ -420,34 +499,33 @@ void adj_zero(hipStream_t st, float* a, size_t count) {
 void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, float mu, uint32_t n, uint32_t passes,
                             const Tape& T, const SolveBufs& b, const AdjointBufs& L) {
     if (n == 0) return;
-    const dim3 g = adj_grid(n), blk(kAdjBlock);
-    hipLaunchKernelGGL(k_wall_jacobians, g, blk, 0, st, k, n, T.pos, L.dfill, L.J);
-    adj_zero(st, L.rho, n);
-    adj_zero(st, L.alpha, n);
-    hipLaunchKernelGGL(k_rev_integrate, g, blk, 0, st, k, s, n, T.pos, T.vel[passes + 2], T.density, L.x, L.v);
+    const dim3 g = adj_grid(n), blk(kAdjBlock), gq = quad_grid(n), bq(kQuadBlock);
+    hipLaunchKernelGGL(k_rev_integrate, g, blk, 0, st, k, s, n, T.pos, T.vel[passes + 2], T.density, L.x, L.v,
+                       L.dfill, L.J, L.rho, L.alpha);
     for (uint32_t r = passes; r-- > 0;) {
-        // den_apply r: kd of pass r, lam of its output = L.v
-        hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[2 + r], T.wall_grad, L.J, L.v,
-                           L.kd, L.x);
-        // den_kappa r at the velocities entering pass r
-        hipLaunchKernelGGL(k_rev_den_kappa_pre, g, blk, 0, st, s, n, T.density, T.alpha, T.kd[2 + r], L.kd, L.p,
-                           L.D, L.rho, L.alpha);
-        hipLaunchKernelGGL(k_rev_divergence, g, blk, 0, st, k, n, b, T.pos, T.vel[2 + r], T.wall_grad, L.J, L.D,
+        // den_apply r (kd of pass r, lam of its output = L.v), then den_kappa
+        // r's per-particle reverse and its sweep at the velocities entering
+        // pass r
+        hipLaunchKernelGGL(k_rev_apply<true>, gq, bq, 0, st, k, s, n, b, T.pos, T.kd[2 + r], T.wall_grad, L.J, L.v,
+                           L.kd, L.x, T.density, T.alpha, L.p, L.D, L.rho, L.alpha);
+        hipLaunchKernelGGL(k_rev_divergence, gq, bq, 0, st, k, n, b, T.pos, T.vel[2 + r], T.wall_grad, L.J, L.D,
                            L.v, L.x);
     }
     // forces_den_apply: the warm-start correction, then v2 = v1 + dt (g + a)
-    hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[1], T.wall_grad, L.J, L.v, L.kd, L.x);
+    hipLaunchKernelGGL(k_rev_apply<false>, gq, bq, 0, st, k, s, n, b, T.pos, T.kd[1], T.wall_grad, L.J, L.v, L.kd,
+                       L.x, T.density, T.alpha, L.p, L.D, L.rho, L.alpha);
     hipLaunchKernelGGL(k_rev_forces_den, g, blk, 0, st, s, n, T.density, T.pp, L.v, L.kd, L.p, L.a, L.pp, L.rho,
                        L.g);
     // forces_eval: viscosity at the velocities after div_apply
-    hipLaunchKernelGGL(k_rev_viscosity, g, blk, 0, st, k, mu, n, b, T.pos, T.vel[1], T.density, T.accel, L.a, L.v,
+    hipLaunchKernelGGL(k_rev_viscosity, gq, bq, 0, st, k, mu, n, b, T.pos, T.vel[1], T.density, T.accel, L.a, L.v,
                        L.rho, L.x, L.mu);
     // div_apply with the divergence kappa
-    hipLaunchKernelGGL(k_rev_apply, g, blk, 0, st, k, s.dt, n, b, T.pos, T.kd[0], T.wall_grad, L.J, L.v, L.kd, L.x);
+    hipLaunchKernelGGL(k_rev_apply<false>, gq, bq, 0, st, k, s, n, b, T.pos, T.kd[0], T.wall_grad, L.J, L.v, L.kd,
+                       L.x, T.density, T.alpha, L.p, L.D, L.rho, L.alpha);
     // density_div at the substep's first velocities
-    hipLaunchKernelGGL(k_rev_density_pre, g, blk, 0, st, k, s, n, b, T.pos, T.wall_grad, T.density, T.alpha,
+    hipLaunchKernelGGL(k_rev_density_pre, gq, bq, 0, st, k, s, n, b, T.pos, T.wall_grad, T.density, T.alpha,
                        T.kd[0], L.kd, L.alpha, L.rho, L.D, L.G);
-    hipLaunchKernelGGL(k_rev_density, g, blk, 0, st, k, n, b, T.pos, T.vel[0], T.wall_grad, L.J, L.dfill, L.rho,
+    hipLaunchKernelGGL(k_rev_density, gq, bq, 0, st, k, n, b, T.pos, T.vel[0], T.wall_grad, L.J, L.dfill, L.rho,
                        L.D, L.G, L.v, L.x);
     hipLaunchKernelGGL(k_rev_gather, g, blk, 0, st, n, T.perm, L.x, L.v, L.pp, L.x_out, L.v_out, L.pp_out);
 }

@@ -7,7 +7,7 @@
 namespace fluid {
 
 // The most density passes a taped substep records.
-constexpr uint32_t kMaxTapePasses = 8;
+constexpr uint32_t kMaxTapePasses = kMaxStagePasses;
 
 // One substep of the taped forward, all in that substep's working (slot)
 // order. vel[0] enters div_apply, vel[1] leaves it, vel[2] leaves

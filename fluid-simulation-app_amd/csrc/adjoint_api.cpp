@@ -2,8 +2,9 @@
 // keeps a checkpoint of the resting set every k substeps, and a backward run
 // that, segment by segment from the last, restores the checkpoint, replays
 // the segment's substeps with a tape (the working set, the velocities and kd
-// after each stage, the neighbour lists, the permutation) and runs the
-// reverse sweeps substep by substep.
+// of each stage written to their own arrays by the kernels themselves, the
+// neighbour lists, the permutation) and runs the reverse sweeps substep by
+// substep.
 //
 // The taped replay runs with no LDS staging, so its lists hold slots; its
 // results are bit-identical to the staged forward's (the staging only moves
@@ -168,6 +169,7 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         A->tape_solve[t] = tb;
         Records w = A->work;
         w.pos = T.pos;
+        w.vel = T.vel[0];
         w.prev_pressure = T.pp;
         A->tape_work[t] = w;
     }
@@ -193,39 +195,6 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
     return FLUID_OK;
 }
 
-// Copies the working velocity and kd after each stage into the tape.
-struct TapeHook final : StageFn {
-    const Tape* T;
-    const Records* wk;
-    const SolveBufs* b;
-    size_t n;
-    void operator()(hipStream_t st, StageId id, uint32_t r) const override {
-        auto vel = [&](float4* dst) {
-            (void)hipMemcpyAsync(dst, wk->vel, n * 16, hipMemcpyDeviceToDevice, st);
-        };
-        auto kd = [&](float* dst) { (void)hipMemcpyAsync(dst, b->kd, n * 4, hipMemcpyDeviceToDevice, st); };
-        switch (id) {
-            case STAGE_DENSITY:
-                vel(T->vel[0]);
-                kd(T->kd[0]);
-                break;
-            case STAGE_DIV_APPLY:
-                vel(T->vel[1]);
-                break;
-            case STAGE_FORCES_DEN_APPLY:
-                vel(T->vel[2]);
-                kd(T->kd[1]);
-                break;
-            case STAGE_DEN_KAPPA:
-                kd(T->kd[2 + r]);
-                break;
-            case STAGE_DEN_APPLY:
-                vel(T->vel[3 + r]);
-                break;
-        }
-    }
-};
-
 int run_plain(FluidAdjoint* A, uint32_t s) {
     const OneScene sc{A->scene, make_step(A->blocks[s], A->phys)};
     launch_grid_build(A->stream, sc, A->n, A->cells, A->rest, A->work, A->grid, nullptr);
@@ -239,12 +208,10 @@ int run_taped(FluidAdjoint* A, uint32_t s, uint32_t t) {
     gb.perm = A->tapes[t].perm;
     const Records& wk = A->tape_work[t];
     launch_grid_build(A->stream, sc, A->n, A->cells, A->rest, wk, gb, nullptr);
-    TapeHook hook;
-    hook.T = &A->tapes[t];
-    hook.wk = &wk;
-    hook.b = &A->tape_solve[t];
-    hook.n = A->n;
-    launch_solve_staged(A->stream, sc, A->n, A->grid.starts, wk, A->rest, A->tape_solve[t], A->refine[s], hook);
+    StageBufs stages;
+    for (uint32_t i = 0; i < kMaxTapePasses + 3; ++i) stages.vel[i] = A->tapes[t].vel[i];
+    for (uint32_t i = 0; i < kMaxTapePasses + 2; ++i) stages.kd[i] = A->tapes[t].kd[i];
+    launch_solve_staged(A->stream, sc, A->n, A->grid.starts, wk, A->rest, A->tape_solve[t], A->refine[s], stages);
     return check_launch("adjoint taped forward");
 }
 

@@ -337,6 +337,19 @@ __device__ __forceinline__ f3 wall_adh_sum(const SceneK& k, f3 pos) {
     return a;
 }
 
+// lane_fold (sim_solve.wgsl:398-402) over the 4 lanes of a quad: xor 2, then
+// xor 1 (every lane ends with (p0 + p2) + (p1 + p3)). DPP quad permutations,
+// no LDS; every lane of the wave must take part.
+template <int CTRL>
+__device__ __forceinline__ float quad_perm(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float fold4(float v) {
+    const float t = v + quad_perm<0x4E>(v);  // [2, 3, 0, 1]: lane ^ 2
+    return t + quad_perm<0xB1>(t);           // [1, 0, 3, 2]: lane ^ 1
+}
+__device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
+
 // ---- launch geometry -----------------------------------------------------
 
 constexpr int kBlock = 256;

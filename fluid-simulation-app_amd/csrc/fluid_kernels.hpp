@@ -172,22 +172,19 @@ void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
                   KernelTimer* timer);
-// The plain solve with a callback after each stage (the adjoint's taped
-// forward: it copies the stage's velocities and kd away).
-enum StageId {
-    STAGE_DENSITY = 0,          // density_div done (kd = the divergence kappa)
-    STAGE_DIV_APPLY,            // div_apply done
-    STAGE_FORCES_DEN_APPLY,     // forces_eval + forces_den_apply done
-    STAGE_DEN_KAPPA,            // den_kappa of pass r done
-    STAGE_DEN_APPLY,            // den_apply of pass r done
-};
-struct StageFn {
-    virtual ~StageFn() = default;
-    virtual void operator()(hipStream_t st, StageId id, uint32_t pass) const = 0;
+// The plain solve writing each stage's velocities and kd to arrays of their
+// own (the adjoint's taped forward): vel[0] enters div_apply (the working
+// velocities the grid build wrote), vel[1] leaves it, vel[2] leaves
+// forces_den_apply, vel[3 + r] leaves den_apply of pass r; kd[0] is the
+// divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
+constexpr uint32_t kMaxStagePasses = 8;
+struct StageBufs {
+    float4* vel[kMaxStagePasses + 3];
+    float* kd[kMaxStagePasses + 2];
 };
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
                          const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                         const StageFn& fn);
+                         const StageBufs& T);
 // Tolerance mode of a batch: up to max_passes density passes, a scene
 // stopping after the first pass (at least min_passes) whose mean density
 // error is <= tol. sc.pass_limit and b.residual / b.iterations are set.

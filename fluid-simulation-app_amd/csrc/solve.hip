@@ -165,17 +165,6 @@ __device__ __forceinline__ void stage_flat(const RegionS& R, Load&& load, Store&
     }
 }
 
-// lane_fold over the kLanes = 4 lanes of a quad: xor 2, then xor 1 (every
-// lane ends with (p0 + p2) + (p1 + p3)). DPP quad permutations, no LDS.
-template <int CTRL>
-__device__ __forceinline__ float quad_perm(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-__device__ __forceinline__ float fold4(float v) {
-    const float t = v + quad_perm<0x4E>(v);  // [2, 3, 0, 1]: lane ^ 2
-    return t + quad_perm<0xB1>(t);           // [1, 0, 3, 2]: lane ^ 1
-}
-__device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
 
 __device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
 
@@ -551,10 +540,11 @@ __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, co
 }
 
 // div_apply / den_apply: v -= dt (m sum (kd_i + kd_j) gradW + kd_i wall_grad).
-// Reads only kd of the neighbours, so the velocity update is in place.
+// Reads only kd of the neighbours, so the velocity update is in place
+// (vel_out = vel_in) unless the caller keeps both (the taped forward).
 template <class Sc>
 __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t pass, const float4* __restrict__ pos,
-                                                               float4* __restrict__ vel, SolveBufs b) {
+                                                               const float4* vel_in, float4* vel_out, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index();
     const SceneK& k = sc.scene(tile);
@@ -569,8 +559,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
     if (id.live && id.lane == 0) {
         const float4 wg = b.wall_grad[id.p];
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
-        const float4 w = vel[id.p];
-        vel[id.p] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
+        const float4 w = vel_in[id.p];
+        vel_out[id.p] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
     }
 }
 
@@ -716,7 +706,7 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
 
 template <class Sc>
 __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const float4* __restrict__ pos,
-                                                                    float4* __restrict__ vel,
+                                                                    const float4* vel_in, float4* vel_out,
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index();
@@ -732,7 +722,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const float4 a = b.accel[i];
-        const float4 was = vel[i];
+        const float4 was = vel_in[i];
         const f3 v = xyz(was);
         const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
         const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
@@ -746,7 +736,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
         temperature[i] = temperature[i] + s.dt * a.w;
         const float4 wg = b.wall_grad[i];
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
-        vel[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z, was.w);
+        vel_out[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z,
+                                 was.w);
     }
 }
 
@@ -948,45 +939,53 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
 struct NoCheck {
     void operator()(hipStream_t, uint32_t) const {}
 };
-struct NoStage {
-    void operator()(hipStream_t, StageId, uint32_t) const {}
-};
-
-template <class Sc, class Check = NoCheck, class Stage = NoStage>
+// The substep's launches. With stage buffers (the adjoint's taped forward)
+// each stage's velocities and kd go to arrays of their own instead of being
+// updated in place: vel[0] enters div_apply, vel[1] leaves it, vel[2] leaves
+// forces_den_apply, vel[3 + r] leaves den_apply of pass r; kd[0] is the
+// divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
+template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
-                  const Check& check = Check(), const Stage& stage = Stage()) {
+                  const Check& check = Check(), const StageBufs* T = nullptr) {
     if (n == 0) return;
     const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
     const size_t lds_kappa = b.stage_cap * kKappaEntry;
-    density_only(st, sc, n, starts, work, b, timer);
-    stage(st, STAGE_DENSITY, 0);
+    auto V = [&](uint32_t i) { return T ? T->vel[i] : work.vel; };
+    auto KB = [&](uint32_t i) {
+        SolveBufs c = b;
+        if (T) c.kd = T->kd[i];
+        return c;
+    };
+    Records w0 = work;
+    w0.vel = V(0);
+    density_only(st, sc, n, starts, w0, KB(0), timer);
     if (timer) timer->begin(st, KERNEL_DIV_APPLY);
-    hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, work.vel, b);
+    hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, V(0), V(1), KB(0));
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
-    stage(st, STAGE_DIV_APPLY, 0);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, b.stage_cap * kForcesEntry, st, sc, work.pos, work.vel,
-                       work.temperature, work.prev_pressure, b);
+    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, b.stage_cap * kForcesEntry, st, sc, work.pos, V(1),
+                       work.temperature, work.prev_pressure, KB(1));
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
-    hipLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, sc, work.pos, work.vel,
-                       work.temperature, b);
+    hipLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, sc, work.pos, V(1), V(2),
+                       work.temperature, KB(1));
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
-    stage(st, STAGE_FORCES_DEN_APPLY, 0);
     for (uint32_t r = 0; r < passes; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, b.stage_cap * kPvEntry, st, sc, r, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, b.stage_cap * kPvEntry, st, sc, r, work.pos, V(2 + r),
+                           KB(2 + r));
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         check(st, r);
-        stage(st, STAGE_DEN_KAPPA, r);
         if (timer) timer->begin(st, KERNEL_DEN_APPLY);
-        hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, r, work.pos, work.vel, b);
+        hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, r, work.pos, V(2 + r), V(3 + r),
+                           KB(2 + r));
         if (timer) timer->end(st, KERNEL_DEN_APPLY);
-        stage(st, STAGE_DEN_APPLY, r);
     }
+    Records wl = work;
+    wl.vel = V(passes + 2);
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
-    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kBlock), 0, st, sc, work, rest, b);
+    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kBlock), 0, st, sc, wl, rest, b);
     if (timer) timer->end(st, KERNEL_INTEGRATE);
 }
 
@@ -1016,13 +1015,12 @@ void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const ui
 
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
                          const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                         const StageFn& fn) {
+                         const StageBufs& T) {
     OneScene one = sc;
     one.s.refine = passes;
     SolveBufs fixed = b;
     fixed.residual = nullptr;
-    auto stage = [&](hipStream_t s2, StageId id, uint32_t r) { fn(s2, id, r); };
-    solve(st, one, slots, starts, work, rest, fixed, passes, nullptr, NoCheck(), stage);
+    solve(st, one, slots, starts, work, rest, fixed, passes, nullptr, NoCheck(), &T);
 }
 
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,

@@ -8,7 +8,9 @@
 //
 // The taped replay runs with no LDS staging, so its lists hold slots; its
 // results are bit-identical to the staged forward's (the staging only moves
-// where the same values are read from).
+// where the same values are read from). It runs on a second stream into one
+// of two tape sets while the reverse sweeps read the other, so segment c - 1
+// replays while segment c runs in reverse.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -21,9 +23,15 @@
 
 using namespace fluid;
 
+// Tape sets: the backward replays one segment into one set on the replay
+// stream while the reverse sweeps read the other on the main stream.
+constexpr int kTapeSets = 2;
+
 struct FluidAdjoint {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // forward, reverse sweeps
+    hipStream_t replay = nullptr;   // segment replays of the backward
+    hipEvent_t ready[kTapeSets] = {}, done[kTapeSets] = {};
     SimParamsBlock params{};
     SceneK scene{};
     Physics phys{};
@@ -32,9 +40,9 @@ struct FluidAdjoint {
     GridBufs grid{};
     SolveBufs solve{};
     std::vector<Records> checkpoints;
-    std::vector<Tape> tapes;
-    std::vector<SolveBufs> tape_solve;
-    std::vector<Records> tape_work;
+    std::vector<Tape> tapes[kTapeSets];
+    std::vector<SolveBufs> tape_solve[kTapeSets];
+    std::vector<Records> tape_work[kTapeSets];
     AdjointBufs L{};
     float* grad_g = nullptr;
     float* grad_mu = nullptr;
@@ -51,10 +59,16 @@ struct FluidAdjoint {
     }
     ~FluidAdjoint() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (replay) (void)hipStreamSynchronize(replay);
         for (auto e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (int i = 0; i < kTapeSets; ++i) {
+            if (ready[i]) (void)hipEventDestroy(ready[i]);
+            if (done[i]) (void)hipEventDestroy(done[i]);
+        }
         mem.release();
         if (stream) (void)hipStreamDestroy(stream);
+        if (replay) (void)hipStreamDestroy(replay);
     }
 };
 
@@ -106,7 +120,12 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
     A->segments = (steps + A->every - 1) / A->every;
     HIP_TRY(hipGetDevice(&A->device));
     HIP_TRY(hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking));
+    HIP_TRY(hipStreamCreateWithFlags(&A->replay, hipStreamNonBlocking));
     for (auto& e : A->ev) HIP_TRY(hipEventCreate(&e));
+    for (int i = 0; i < kTapeSets; ++i) {
+        HIP_TRY(hipEventCreateWithFlags(&A->ready[i], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&A->done[i], hipEventDisableTiming));
+    }
     const size_t n = A->n, c = A->cells;
     int rc = FLUID_OK;
 #define A_(ptr, cnt)                           \
@@ -144,11 +163,12 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         if ((rc = alloc_records(A, r, n))) return rc;
     // Tapes: each owns its working position and previous pressure, its
     // stage snapshots, its density-side fields and its lists.
-    A->tapes.resize(A->every);
-    A->tape_solve.resize(A->every);
-    A->tape_work.resize(A->every);
+    for (int set = 0; set < kTapeSets; ++set) {
+    A->tapes[set].resize(A->every);
+    A->tape_solve[set].resize(A->every);
+    A->tape_work[set].resize(A->every);
     for (uint32_t t = 0; t < A->every; ++t) {
-        Tape& T = A->tapes[t];
+        Tape& T = A->tapes[set][t];
         A_(T.pos, n);
         A_(T.pp, n);
         for (auto& v : T.vel) A_(v, n);
@@ -166,12 +186,13 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         if ((rc = A->alloc(tb.nbr, (size_t)b.cap * b.stride, false))) return rc;
         A_(tb.nbr_n, n);
         tb.stage_cap = 0;
-        A->tape_solve[t] = tb;
+        A->tape_solve[set][t] = tb;
         Records w = A->work;
         w.pos = T.pos;
         w.vel = T.vel[0];
         w.prev_pressure = T.pp;
-        A->tape_work[t] = w;
+        A->tape_work[set][t] = w;
+    }
     }
     AdjointBufs& L = A->L;
     A_(L.x, n);
@@ -202,17 +223,33 @@ int run_plain(FluidAdjoint* A, uint32_t s) {
     return check_launch("adjoint forward");
 }
 
-int run_taped(FluidAdjoint* A, uint32_t s, uint32_t t) {
+// Substep s replayed on the replay stream into tape t of `set`.
+int run_taped(FluidAdjoint* A, int set, uint32_t s, uint32_t t) {
     const OneScene sc{A->scene, make_step(A->blocks[s], A->phys)};
     GridBufs gb = A->grid;
-    gb.perm = A->tapes[t].perm;
-    const Records& wk = A->tape_work[t];
-    launch_grid_build(A->stream, sc, A->n, A->cells, A->rest, wk, gb, nullptr);
+    gb.perm = A->tapes[set][t].perm;
+    const Records& wk = A->tape_work[set][t];
+    launch_grid_build(A->replay, sc, A->n, A->cells, A->rest, wk, gb, nullptr);
     StageBufs stages;
-    for (uint32_t i = 0; i < kMaxTapePasses + 3; ++i) stages.vel[i] = A->tapes[t].vel[i];
-    for (uint32_t i = 0; i < kMaxTapePasses + 2; ++i) stages.kd[i] = A->tapes[t].kd[i];
-    launch_solve_staged(A->stream, sc, A->n, A->grid.starts, wk, A->rest, A->tape_solve[t], A->refine[s], stages);
+    for (uint32_t i = 0; i < kMaxTapePasses + 3; ++i) stages.vel[i] = A->tapes[set][t].vel[i];
+    for (uint32_t i = 0; i < kMaxTapePasses + 2; ++i) stages.kd[i] = A->tapes[set][t].kd[i];
+    launch_solve_staged(A->replay, sc, A->n, A->grid.starts, wk, A->rest, A->tape_solve[set][t], A->refine[s],
+                        stages);
     return check_launch("adjoint taped forward");
+}
+
+// Segment c replayed into tape set c % 2 once the reverse sweeps that last
+// read that set are done; `ready` marks the set filled.
+int replay_segment(FluidAdjoint* A, uint32_t c) {
+    const int set = (int)(c % kTapeSets);
+    const uint32_t s0 = c * A->every, s1 = std::min(s0 + A->every, A->steps);
+    HIP_TRY(hipStreamWaitEvent(A->replay, A->done[set], 0));
+    int rc;
+    if ((rc = copy_records(A->replay, A->checkpoints[c], A->rest, A->n))) return rc;
+    for (uint32_t s = s0; s < s1; ++s)
+        if ((rc = run_taped(A, set, s, s - s0))) return rc;
+    HIP_TRY(hipEventRecord(A->ready[set], A->replay));
+    return FLUID_OK;
 }
 
 int upload_f4(hipStream_t st, float4* dst, const float* src, size_t n) {
@@ -327,21 +364,28 @@ int fluid_adjoint_backward(FluidAdjoint* A, const float* dpos4, const float* dve
     adj_zero(A->stream, A->grad_mu, 1);
     L.mu = A->grad_mu;
     HIP_TRY(hipEventRecord(A->ev[2], A->stream));
+    // The replay stream starts after this call's uploads; the tape sets are
+    // free (nothing reads them yet).
+    HIP_TRY(hipStreamWaitEvent(A->replay, A->ev[2], 0));
+    for (int set = 0; set < kTapeSets; ++set) HIP_TRY(hipEventRecord(A->done[set], A->stream));
+    if ((rc = replay_segment(A, A->segments - 1))) return rc;
     for (uint32_t c = A->segments; c-- > 0;) {
+        // the next segment back replays while this one runs in reverse
+        if (c > 0 && (rc = replay_segment(A, c - 1))) return rc;
+        const int set = (int)(c % kTapeSets);
         const uint32_t s0 = c * A->every, s1 = std::min(s0 + A->every, A->steps);
-        if ((rc = copy_records(A->stream, A->checkpoints[c], A->rest, n))) return rc;
-        for (uint32_t s = s0; s < s1; ++s)
-            if ((rc = run_taped(A, s, s - s0))) return rc;
+        HIP_TRY(hipStreamWaitEvent(A->stream, A->ready[set], 0));
         for (uint32_t s = s1; s-- > s0;) {
             const uint32_t t = s - s0;
             L.g = A->grad_g + 3 * (size_t)s;
             launch_reverse_substep(A->stream, A->scene, make_step(A->blocks[s], A->phys), A->phys.mu, A->n,
-                                   A->refine[s], A->tapes[t], A->tape_solve[t], L);
+                                   A->refine[s], A->tapes[set][t], A->tape_solve[set][t], L);
             std::swap(L.x, L.x_out);
             std::swap(L.v, L.v_out);
             std::swap(L.p, L.pp_out);
             if ((rc = check_launch("adjoint reverse"))) return rc;
         }
+        HIP_TRY(hipEventRecord(A->done[set], A->stream));
     }
     HIP_TRY(hipEventRecord(A->ev[3], A->stream));
     HIP_TRY(hipStreamSynchronize(A->stream));

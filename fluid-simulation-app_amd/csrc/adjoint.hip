@@ -49,15 +49,16 @@ __device__ __forceinline__ Quad quad(uint32_t n) {
     return q;
 }
 // Calls f(neighbour slot, dW/r) for the lane's share of particle p's list:
-// entry 4t + lane sits at quad row t, offset gid.
+// entry 4t + lane sits at quad row t, offset gid. The taped forward stages
+// nothing, so its lists hold slots.
 template <class F>
 __device__ __forceinline__ void for_lane(const SolveBufs& b, const Quad& q, F&& f) {
     const uint32_t nn = b.nbr_n[q.p];
     const size_t step = (size_t)b.stride * kListLanes;
     uint32_t t = 0;
     for (uint32_t m = q.lane; m < nn; m += kListLanes, ++t) {
-        const uint2 e = (b.nbr + (size_t)t * step)[q.gid];
-        f(e.x, __uint_as_float(e.y));
+        const size_t e = (size_t)t * step + q.gid;
+        f(b.nbr32[e], b.nbrg[e]);
     }
 }
 

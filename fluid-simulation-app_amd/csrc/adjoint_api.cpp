@@ -151,7 +151,7 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
     A_(b.wall_grad, n);
     b.cap = A->cap;
     b.stride = std::max<uint32_t>(solve_tiles(A->n), 1) * kSolveTile;
-    if ((rc = A->alloc(b.nbr, (size_t)b.cap * b.stride, false))) return rc;
+    if ((rc = A->mem.alloc_list(b))) return rc;
     A_(b.nbr_n, n);
     A_(b.regions, std::max<uint32_t>(solve_tiles(A->n), 1));
     A_(b.counters, 4);
@@ -183,7 +183,7 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         tb.alpha = T.alpha;
         tb.wall_grad = T.wall_grad;
         tb.accel = T.accel;
-        if ((rc = A->alloc(tb.nbr, (size_t)b.cap * b.stride, false))) return rc;
+        if ((rc = A->mem.alloc_list(tb))) return rc;
         A_(tb.nbr_n, n);
         tb.stage_cap = 0;
         A->tape_solve[set][t] = tb;

@@ -183,7 +183,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.wall_grad, n);
     b.cap = B->cap;
     b.stride = std::max<uint32_t>(B->slots, kSolveTile);
-    if ((rc = B->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
+    if ((rc = B->mem.alloc_list(b)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
     A(b.regions, std::max<uint32_t>(B->tiles, 1));
     A(b.counters, 2 * S);

@@ -99,12 +99,14 @@ struct SolveBufs {
     float4* accel;    // xyz; w = dT/dt
     float4* xsph;
     float4* wall_grad;
-    // Per pair (neighbour, bits of dW/r), quad-major: entry m of particle p at
-    // ((m / 4) * stride + p) * 4 + m % 4, so the four list lanes of a particle
-    // read one 32-byte quad and a wave's 64 lanes one 512-byte line. The
-    // neighbour is its LDS index when the particle's tile is staged, else its
-    // slot.
-    uint2* nbr;
+    // Per pair, quad-major: entry m of particle p at ((m / 4) * stride + p) *
+    // 4 + m % 4, so the four list lanes of a particle read one quad and a
+    // wave's 64 lanes one contiguous run. The neighbour is its LDS index
+    // (16 bits) when the particle's tile is staged, else its slot (32 bits,
+    // a separate array only unstaged tiles touch); dW/r in its own array.
+    uint16_t* nbr16;
+    uint32_t* nbr32;
+    float* nbrg;
     uint32_t* nbr_n;  // [N]
     uint32_t cap;     // a multiple of 4
     uint32_t stride;  // >= the tiles' slot count

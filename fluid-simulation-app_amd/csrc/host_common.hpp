@@ -58,6 +58,14 @@ struct DeviceAllocs {
         out = static_cast<P*>(p);
         return FLUID_OK;
     }
+    // The neighbour list of `cap` entries for each of `stride` slots.
+    int alloc_list(SolveBufs& b) {
+        const size_t e = (size_t)b.cap * b.stride;
+        int rc;
+        if ((rc = alloc(b.nbr16, e, false)) || (rc = alloc(b.nbr32, e, false)) || (rc = alloc(b.nbrg, e, false)))
+            return rc;
+        return FLUID_OK;
+    }
     void release() {
         for (void* p : ptrs) (void)hipFree(p);
         ptrs.clear();

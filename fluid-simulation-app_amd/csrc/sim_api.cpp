@@ -190,7 +190,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.wall_grad, n);
     b.cap = s->cap;
     b.stride = std::max<uint32_t>(solve_tiles(s->n), 1) * kSolveTile;
-    if ((rc = s->alloc(b.nbr, (size_t)b.cap * b.stride, false)) != FLUID_OK) return rc;
+    if ((rc = s->mem.alloc_list(b)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
     A(b.regions, std::max<uint32_t>(solve_tiles(s->n), 1));
     A(b.counters, 4);
@@ -348,10 +348,12 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     const size_t n = s->n, cap = s->cap, stride = s->solve.stride;
     if (n == 0) return FLUID_OK;
-    std::vector<uint2> raw(cap * stride);
+    std::vector<uint16_t> raw16(cap * stride);
+    std::vector<uint32_t> raw32(cap * stride);
     std::vector<uint32_t> cnt(n);
     std::vector<TileRegion> reg(solve_tiles(s->n));
-    HIP_TRY(hipMemcpy(raw.data(), s->solve.nbr, raw.size() * sizeof(uint2), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(raw16.data(), s->solve.nbr16, raw16.size() * 2, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(raw32.data(), s->solve.nbr32, raw32.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(reg.data(), s->solve.regions, reg.size() * sizeof(TileRegion), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; ++i) {
@@ -359,12 +361,16 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
         for (size_t m = 0; m < cap; ++m) {
             uint32_t j = 0;
             if (m < cnt[i]) {
-                j = raw[((m / kListLanes) * stride + i) * kListLanes + m % kListLanes].x;
+                const size_t e = ((m / kListLanes) * stride + i) * kListLanes + m % kListLanes;
                 if (R.staged) {
-                    // An LDS index: back to the slot through its stencil row.
+                    // An LDS index: back to the slot through the last stencil
+                    // row that starts at or before it.
+                    j = raw16[e];
                     int r = 8;
-                    while (r > 0 && (j < R.base[r] || R.base[r] == (r < 8 ? R.base[r + 1] : R.total))) --r;
+                    while (r > 0 && j < R.base[r]) --r;
                     j = (uint32_t)((int32_t)j - R.delta[r]);
+                } else {
+                    j = raw32[e];
                 }
             }
             nbr[i * cap + m] = j;

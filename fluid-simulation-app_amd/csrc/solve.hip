@@ -193,29 +193,43 @@ __device__ __forceinline__ size_t list_entry(uint32_t stride, uint32_t p, uint32
 
 // Lane `lane` of particle p walks its share of the list in order (entries
 // lane, lane + 4, ...: quad t of the list holds entry 4t + lane at quad
-// offset gid), calling f(neighbour, dW/r). The quad's address is uniform
-// across the wave; kPrefetch quads are loaded before they are used.
-template <class F>
+// offset gid), calling f(neighbour, dW/r): the neighbour is an LDS index in
+// a staged tile, a slot otherwise. The quad's address is uniform across the
+// wave; kPrefetch quads are loaded before they are used.
+template <bool STAGED>
+__device__ __forceinline__ uint32_t list_at(const SolveBufs& b, size_t e) {
+    return STAGED ? (uint32_t)b.nbr16[e] : b.nbr32[e];
+}
+template <bool STAGED, class F>
 __device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, F&& f) {
     const uint32_t nn = b.nbr_n[id.p];
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
     const size_t step = (size_t)b.stride * kLanes;
     uint32_t t = 0;
     for (; t + kPrefetch <= cnt; t += kPrefetch) {
-        uint2 e[kPrefetch];
+        uint32_t j[kPrefetch];
+        float g[kPrefetch];
 #pragma unroll
-        for (uint32_t u = 0; u < kPrefetch; ++u) e[u] = (b.nbr + (size_t)(t + u) * step)[id.gid];
+        for (uint32_t u = 0; u < kPrefetch; ++u) {
+            const size_t e = (size_t)(t + u) * step + id.gid;
+            j[u] = list_at<STAGED>(b, e);
+            g[u] = b.nbrg[e];
+        }
 #pragma unroll
-        for (uint32_t u = 0; u < kPrefetch; ++u) f(e[u].x, __uint_as_float(e[u].y));
+        for (uint32_t u = 0; u < kPrefetch; ++u) f(j[u], g[u]);
     }
     if (t < cnt) {
-        uint2 e[kPrefetch - 1];
+        uint32_t j[kPrefetch - 1];
+        float g[kPrefetch - 1];
+#pragma unroll
+        for (uint32_t u = 0; u < kPrefetch - 1; ++u) {
+            const size_t e = (size_t)(t + u) * step + id.gid;
+            j[u] = t + u < cnt ? list_at<STAGED>(b, e) : 0u;
+            g[u] = t + u < cnt ? b.nbrg[e] : 0.0f;
+        }
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch - 1; ++u)
-            e[u] = t + u < cnt ? (b.nbr + (size_t)(t + u) * step)[id.gid] : make_uint2(0u, 0u);
-#pragma unroll
-        for (uint32_t u = 0; u < kPrefetch - 1; ++u)
-            if (t + u < cnt) f(e[u].x, __uint_as_float(e[u].y));
+            if (t + u < cnt) f(j[u], g[u]);
     }
 }
 
@@ -363,7 +377,13 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
             const uint32_t quad = (uint32_t)(__ballot(acc) >> shift) & 0xFu;
             if (acc) {
                 const uint32_t m = found + __popc(quad & below);
-                if (m < b.cap) b.nbr[list_entry(b.stride, id.p, m)].x = j;
+                if (m < b.cap) {
+                    const size_t e = list_entry(b.stride, id.p, m);
+                    if (STAGED)
+                        b.nbr16[e] = (uint16_t)j;
+                    else
+                        b.nbr32[e] = j;
+                }
             }
             found += __popc(quad);
         }
@@ -384,12 +404,11 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
     const size_t step = (size_t)b.stride * kLanes;
     for (uint32_t t = 0; t < cnt; ++t) {
-        uint2* e = b.nbr + (size_t)t * step + id.gid;
-        const uint32_t j = e->x;
+        const size_t e = (size_t)t * step + id.gid;
+        const uint32_t j = list_at<STAGED>(b, e);
         f3 pj, vj;
         load_pv<STAGED>(L, pos, vel, j, pj, vj);
-        const float g = density_pair(k, pi, vi, pj, vj, a);
-        *e = make_uint2(j, __float_as_uint(g));
+        b.nbrg[e] = density_pair(k, pi, vi, pj, vj, a);
     }
     if (found > b.cap) {
         uint32_t seen = 0;
@@ -510,7 +529,7 @@ template <bool STAGED>
 __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, f3 pi, float kdi,
                                             const float4* __restrict__ pos, const float4* lds) {
     f3 dv = mk(0.0f, 0.0f, 0.0f);
-    for_list(b, id, [&](uint32_t j, float g) {
+    for_list<STAGED>(b, id, [&](uint32_t j, float g) {
         float4 q;
         if (STAGED) {
             q = lds[j];
@@ -611,7 +630,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     rho_i = b.density[i];
     const float rhoi = rho_i;
     const f3 pii = pi;
-    for_list(b, id, [&](uint32_t j, float gs) {
+    for_list<STAGED>(b, id, [&](uint32_t j, float gs) {
         f3 pj, vj;
         float rho_j, near_j, temp_j;
         if (STAGED) {
@@ -771,7 +790,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
         const f3 pi = xyz(pos[id.p]);
         vi = xyz(vel[id.p]);
         const f3 vii = vi;
-        for_list(b, id, [&](uint32_t j, float g) {
+        for_list<STAGED>(b, id, [&](uint32_t j, float g) {
             f3 pj, vj;
             load_pv<STAGED>(L, pos, vel, j, pj, vj);
             drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));

@@ -54,10 +54,10 @@ __device__ __forceinline__ Quad quad(uint32_t n) {
 template <class F>
 __device__ __forceinline__ void for_lane(const SolveBufs& b, const Quad& q, F&& f) {
     const uint32_t nn = b.nbr_n[q.p];
-    const size_t step = (size_t)b.stride * kListLanes;
+    const uint32_t step = b.stride * kListLanes;  // list indices fit 32 bits
     uint32_t t = 0;
     for (uint32_t m = q.lane; m < nn; m += kListLanes, ++t) {
-        const size_t e = (size_t)t * step + q.gid;
+        const uint32_t e = t * step + q.gid;
         f(b.nbr32[e], b.nbrg[e]);
     }
 }

@@ -501,9 +501,11 @@ int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t*
         return FLUID_OK;
     };
     std::vector<uint32_t> caps;
-    for (uint32_t c : {0u, 1024u, 2048u, 2560u, 3072u, 3328u})
-        if (c <= b->stage_cap_max) caps.push_back(c);
-    if (caps.empty() || caps.back() != b->stage_cap_max) caps.push_back(b->stage_cap_max);
+    // 0 (no staging) and 4/13 .. 13/13 of the largest capacity, in 64s.
+    for (uint32_t f : {0u, 4u, 8u, 10u, 12u, 13u}) {
+        const uint32_t c = b->stage_cap_max * f / 13u / 64u * 64u;
+        if (caps.empty() || caps.back() != c) caps.push_back(c);
+    }
     const uint32_t keep = b->solve.stage_cap;
     const bool graph = b->use_graph;
     b->use_graph = false;  // graphs bake the capacity in

@@ -125,8 +125,14 @@ struct SolveBufs {
 uint32_t solve_tiles(uint32_t n);
 // List lanes a particle in the sweeps (entry m belongs to lane m % 4).
 constexpr uint32_t kListLanes = 4;
-// Slots a solve tile holds.
-constexpr uint32_t kSolveTile = 256;
+// Slots a solve tile holds (64, 128 or 256, fixed at build time). Smaller
+// tiles stage a little more halo per particle but put more independent
+// workgroups on a CU, which hides the sweeps' memory latency better.
+#ifndef FLUID_SOLVE_TILE
+#define FLUID_SOLVE_TILE 128
+#endif
+constexpr uint32_t kSolveTile = FLUID_SOLVE_TILE;
+static_assert(kSolveTile == 64 || kSolveTile == 128 || kSolveTile == 256, "solve tile of 64, 128 or 256 slots");
 
 // Sets the sweeps' staging capacity for the device's per-workgroup LDS (and
 // raises their dynamic-LDS limit); returns the dynamic LDS budget used.

@@ -72,8 +72,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __re
     if (i == 0) *ticket = 0;
     bool live = i < n;
     uint32_t c = 0xffffffffu;
-    if (blockIdx.x * kBlock < n) {
-        const SceneK& g = sc.scene(blockIdx.x);
+    // A wave's 64 slots lie in one solve tile, so in one scene.
+    const uint32_t first = blockIdx.x * kBlock + (threadIdx.x & ~63u);
+    if (first < n) {
+        const SceneK& g = sc.scene(first / kSolveTile);
         if (live) {
             if (i < g.slot_end) {
                 float4 p = pos[i];

@@ -61,6 +61,11 @@ struct DeviceAllocs {
     // The neighbour list of `cap` entries for each of `stride` slots.
     int alloc_list(SolveBufs& b) {
         const size_t e = (size_t)b.cap * b.stride;
+        // The kernels index the list with 32-bit offsets.
+        if (e >= ((size_t)1 << 32)) {
+            set_error("neighbour list of %zu entries (cap %u x %u slots) exceeds 2^32", e, b.cap, b.stride);
+            return FLUID_ERR_ARG;
+        }
         int rc;
         if ((rc = alloc(b.nbr16, e, false)) || (rc = alloc(b.nbr32, e, false)) || (rc = alloc(b.nbrg, e, false)))
             return rc;

@@ -13,7 +13,7 @@
 //
 // Layout: every array is in slot (cell) order after the grid build. A
 // workgroup owns a tile of kTile consecutive slots, kLanes threads a particle
-// (kTile * kLanes = 1024 threads). All neighbours of a tile lie in nine slot
+// (kTile * kLanes threads). All neighbours of a tile lie in nine slot
 // ranges, one per stencil row (dy, dz): in cell-id space, row (dy, dz) of
 // every particle of the tile falls in [first cell + off(dy, dz) - 1, last
 // cell + off(dy, dz) + 1]. k_tile_regions records those ranges once per
@@ -42,9 +42,13 @@ namespace {
 
 constexpr uint32_t kTile = kSolveTile;              // particles a workgroup
 constexpr uint32_t kLanes = kListLanes;             // lanes a particle
-constexpr uint32_t kSweepThreads = kTile * kLanes;  // 1024
+constexpr uint32_t kSweepThreads = kTile * kLanes;
 constexpr uint32_t kRows = 9;
 constexpr uint32_t kPrefetch = 4;                   // list quads a lane loads ahead
+// LDS slots a staged array keeps past the staging capacity, so that a read up
+// to 7 slots past any row's end stays inside the array.
+constexpr uint32_t kStagePad = 8;
+__host__ __device__ __forceinline__ uint32_t stage_slots(const SolveBufs& b) { return b.stage_cap + kStagePad; }
 
 __device__ __forceinline__ uint32_t tile_index() { return xcd_tile(blockIdx.x, gridDim.x); }
 
@@ -148,22 +152,49 @@ __device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) 
 
 // Copies `total` staged slots into LDS: NT threads, ITERS slots each, all
 // loads issued before the stores. `load(slot, it)` fetches into registers,
-// `store(idx, it)` writes them to LDS.
-template <uint32_t NT, uint32_t ITERS, class Load, class Store>
-__device__ __forceinline__ void stage_flat(const RegionS& R, Load&& load, Store&& store) {
-    for (uint32_t chunk = 0; chunk < R.total; chunk += NT * ITERS) {
+// `store(idx, it)` writes them to LDS. `early()` runs once, after the first
+// chunk's loads are issued: the sweep's own loads placed there are in flight
+// while the stores and the barrier complete.
+//
+// Every load and store is issued unconditionally, so the compiler knows how
+// many loads are in flight and waits for the staging loads alone before the
+// stores. A thread past the region re-reads the region's last slot and
+// stores it at its own index, which the tile leaves unused (distinct
+// addresses, so no bank serialisation); an index past the array's `slots`
+// stores into its last slot.
+template <uint32_t NT, uint32_t ITERS, class Load, class Store, class Early>
+__device__ __forceinline__ void stage_flat(const RegionS& R, uint32_t slots, Load&& load, Store&& store,
+                                           Early&& early) {
+    if (R.total == 0) {
+        early();
+        return;
+    }
+    auto chunk_loads = [&](uint32_t chunk) {
 #pragma unroll
         for (uint32_t it = 0; it < ITERS; ++it) {
             const uint32_t idx = chunk + it * NT + threadIdx.x;
-            if (idx < R.total) load((uint32_t)((int32_t)idx - region_delta(R, idx)), it);
+            const uint32_t ic = idx < R.total ? idx : R.total - 1u;
+            load((uint32_t)((int32_t)ic - region_delta(R, ic)), it);
         }
+    };
+    auto chunk_stores = [&](uint32_t chunk) {
 #pragma unroll
         for (uint32_t it = 0; it < ITERS; ++it) {
             const uint32_t idx = chunk + it * NT + threadIdx.x;
-            if (idx < R.total) store(idx, it);
+            store(idx < slots ? idx : slots - 1u, it);
         }
+    };
+    chunk_loads(0);
+    early();
+    chunk_stores(0);
+    for (uint32_t chunk = NT * ITERS; chunk < R.total; chunk += NT * ITERS) {
+        chunk_loads(chunk);
+        chunk_stores(chunk);
     }
 }
+struct NoEarly {
+    __device__ void operator()() const {}
+};
 
 
 __device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
@@ -186,49 +217,63 @@ __device__ __forceinline__ LaneId lane_id(uint32_t tile, uint32_t slot_end) {
 }
 
 
-// Entry m of particle p.
-__device__ __forceinline__ size_t list_entry(uint32_t stride, uint32_t p, uint32_t m) {
-    return ((size_t)(m / kLanes) * stride + p) * kLanes + m % kLanes;
-}
-
 // Lane `lane` of particle p walks its share of the list in order (entries
 // lane, lane + 4, ...: quad t of the list holds entry 4t + lane at quad
 // offset gid), calling f(neighbour, dW/r): the neighbour is an LDS index in
 // a staged tile, a slot otherwise. The quad's address is uniform across the
 // wave; kPrefetch quads are loaded before they are used.
 template <bool STAGED>
-__device__ __forceinline__ uint32_t list_at(const SolveBufs& b, size_t e) {
+__device__ __forceinline__ uint32_t list_at(const SolveBufs& b, uint32_t e) {
     return STAGED ? (uint32_t)b.nbr16[e] : b.nbr32[e];
 }
+// The slot whose own fields a thread reads: its particle, or for a lane past
+// the scene's end the tile's first slot (a tile that runs holds one), so the
+// sweeps' own loads need no branch.
+__device__ __forceinline__ uint32_t own_slot(const LaneId& id) { return id.live ? id.p : id.tile * kTile; }
+
+// The particle's entry count and the lane's first kHead quads. A sweep loads
+// them before it stages its tile, so their latency overlaps the staging. The
+// loads are unconditional: quads past the count, and those of a lane past the
+// scene's end, lie inside the list and are never used.
+constexpr uint32_t kHead = 8;
+struct ListHead {
+    uint32_t nn;  // the particle's entries (its lane's share is taken later)
+    uint32_t j[kHead];
+    float g[kHead];
+};
+template <bool STAGED>
+__device__ __forceinline__ ListHead list_head(const SolveBufs& b, const LaneId& id) {
+    ListHead h;
+    h.nn = b.nbr_n[own_slot(id)];
+    const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
+#pragma unroll
+    for (uint32_t u = 0; u < kHead; ++u) {
+        const uint32_t e = (u < last ? u : last) * step + id.gid;
+        h.j[u] = list_at<STAGED>(b, e);
+        h.g[u] = b.nbrg[e];
+    }
+    return h;
+}
+// Calls f(j, g) over the lane's entries in order: the head, then any further
+// quads kPrefetch at a time.
 template <bool STAGED, class F>
-__device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, F&& f) {
-    const uint32_t nn = b.nbr_n[id.p];
-    const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
-    const size_t step = (size_t)b.stride * kLanes;
-    uint32_t t = 0;
-    for (; t + kPrefetch <= cnt; t += kPrefetch) {
+__device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, const ListHead& h, F&& f) {
+    const uint32_t cnt = id.live && h.nn > id.lane ? (h.nn - id.lane + kLanes - 1) / kLanes : 0u;
+#pragma unroll
+    for (uint32_t u = 0; u < kHead; ++u)
+        if (u < cnt) f(h.j[u], h.g[u]);
+    const uint32_t step = b.stride * kLanes;
+    for (uint32_t t = kHead; t < cnt; t += kPrefetch) {
         uint32_t j[kPrefetch];
         float g[kPrefetch];
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch; ++u) {
-            const size_t e = (size_t)(t + u) * step + id.gid;
+            const uint32_t e = (t + u < cnt ? t + u : cnt - 1u) * step + id.gid;
             j[u] = list_at<STAGED>(b, e);
             g[u] = b.nbrg[e];
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kPrefetch; ++u) f(j[u], g[u]);
-    }
-    if (t < cnt) {
-        uint32_t j[kPrefetch - 1];
-        float g[kPrefetch - 1];
-#pragma unroll
-        for (uint32_t u = 0; u < kPrefetch - 1; ++u) {
-            const size_t e = (size_t)(t + u) * step + id.gid;
-            j[u] = t + u < cnt ? list_at<STAGED>(b, e) : 0u;
-            g[u] = t + u < cnt ? b.nbrg[e] : 0.0f;
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < kPrefetch - 1; ++u)
+        for (uint32_t u = 0; u < kPrefetch; ++u)
             if (t + u < cnt) f(j[u], g[u]);
     }
 }
@@ -300,13 +345,14 @@ __device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict
     }
 }
 
-template <uint32_t NT>
+template <uint32_t NT, class Early = NoEarly>
 __device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restrict__ pos,
-                                         const float4* __restrict__ vel, const PvLds& L) {
+                                         const float4* __restrict__ vel, const PvLds& L, uint32_t slots,
+                                         Early&& early = Early()) {
     constexpr uint32_t IT = 3;
     float4 pp[IT], vv[IT];
     stage_flat<NT, IT>(
-        R,
+        R, slots,
         [&](uint32_t slot, uint32_t it) {
             pp[it] = pos[slot];
             vv[it] = vel[slot];
@@ -314,7 +360,8 @@ __device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restr
         [&](uint32_t idx, uint32_t it) {
             L.pv[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
             L.vw[idx] = make_float2(vv[it].y, vv[it].z);
-        });
+        },
+        early);
 }
 
 // Phase 1 of density_div: the four lanes of particle i test consecutive
@@ -345,6 +392,17 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
         k0s[r] = live_row ? starts[row + x0] : 0u;
         k1s[r] = live_row ? starts[row + x1 + 1] : 0u;
     }
+    // List indices fit 32 bits (cap x stride < 2^32, checked at allocation).
+    const uint32_t lbase = id.p * kLanes, lstep = b.stride * kLanes;
+    auto append = [&](bool acc, uint32_t m, uint32_t j) {
+        if (acc && m < b.cap) {
+            const uint32_t e = (m / kLanes) * lstep + lbase + m % kLanes;
+            if (STAGED)
+                b.nbr16[e] = (uint16_t)j;
+            else
+                b.nbr32[e] = j;
+        }
+    };
     uint32_t found = 0;
 #pragma unroll
     for (uint32_t r = 0; r < kRows; ++r) {
@@ -359,33 +417,33 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
                               : r == 6 ? R.d6
                               : r == 7 ? R.d7
                                        : R.d8;
-        for (uint32_t kq = k0; kq < k1; kq += kLanes) {
-            const uint32_t kk = kq + id.lane;
-            bool acc = false;
-            const uint32_t j = (uint32_t)((int32_t)kk + delta);
-            if (kk < k1) {
-                f3 pj;
-                if (STAGED) {
-                    const float4 a = L.pv[j];
-                    pj = mk(a.x, a.y, a.z);
-                } else {
-                    pj = xyz(pos[kk]);
-                }
-                const f3 x = sub(pi, pj);
-                acc = dot3(x, x) < k.support_sq && kk != id.p;
+        // Two quads of candidates a trip (kq + lane, kq + 4 + lane), tested
+        // branch-free: a lane past the row's end re-reads its last slot and
+        // is masked out of the ballot.
+        const uint32_t last = k1 - 1u;
+        for (uint32_t kq = k0; kq < k1; kq += 2 * kLanes) {
+            const uint32_t ka = kq + id.lane, kb = ka + kLanes;
+            const bool ina = ka < k1, inb = kb < k1;
+            const uint32_t ja = (uint32_t)((int32_t)(ina ? ka : last) + delta);
+            const uint32_t jb = (uint32_t)((int32_t)(inb ? kb : last) + delta);
+            f3 pa, pb;
+            if (STAGED) {
+                const float4 a = L.pv[ja], c = L.pv[jb];
+                pa = mk(a.x, a.y, a.z);
+                pb = mk(c.x, c.y, c.z);
+            } else {
+                pa = xyz(pos[ja]);
+                pb = xyz(pos[jb]);
             }
-            const uint32_t quad = (uint32_t)(__ballot(acc) >> shift) & 0xFu;
-            if (acc) {
-                const uint32_t m = found + __popc(quad & below);
-                if (m < b.cap) {
-                    const size_t e = list_entry(b.stride, id.p, m);
-                    if (STAGED)
-                        b.nbr16[e] = (uint16_t)j;
-                    else
-                        b.nbr32[e] = j;
-                }
-            }
-            found += __popc(quad);
+            const f3 xa = sub(pi, pa), xb = sub(pi, pb);
+            const bool acc_a = ina & (dot3(xa, xa) < k.support_sq) & (ka != id.p);
+            const bool acc_b = inb & (dot3(xb, xb) < k.support_sq) & (kb != id.p);
+            const uint32_t qa = (uint32_t)(__ballot(acc_a) >> shift) & 0xFu;
+            const uint32_t qb = (uint32_t)(__ballot(acc_b) >> shift) & 0xFu;
+            const uint32_t fb = found + __popc(qa);
+            append(acc_a, found + __popc(qa & below), ja);
+            append(acc_b, fb + __popc(qb & below), jb);
+            found = fb + __popc(qb);
         }
     }
     return found;
@@ -402,9 +460,9 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
                                              DensitySums& a) {
     const uint32_t nn = found < b.cap ? found : b.cap;
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
-    const size_t step = (size_t)b.stride * kLanes;
+    const uint32_t step = b.stride * kLanes;
     for (uint32_t t = 0; t < cnt; ++t) {
-        const size_t e = (size_t)t * step + id.gid;
+        const uint32_t e = t * step + id.gid;
         const uint32_t j = list_at<STAGED>(b, e);
         f3 pj, vj;
         load_pv<STAGED>(L, pos, vel, j, pj, vj);
@@ -438,7 +496,7 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
                                              const float4* __restrict__ pos, const float4* __restrict__ vel,
                                              const SolveBufs& b, const PvLds& L) {
     if (STAGED) {
-        stage_pv<kSweepThreads>(R, pos, vel, L);
+        stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b));
         __syncthreads();
     }
     f3 pi = mk(0.0f, 0.0f, 0.0f), vi = pi;
@@ -498,7 +556,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint
     const LaneId id = lane_id(tile, k.slot_end);
     uint32_t* counters = b.counters + 2 * sc.index(tile);
     const RegionS R = load_region(b.regions, id.tile);
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
     if (R.staged)
         density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L);
     else
@@ -509,27 +567,29 @@ __global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint
 
 // Stages position and kd (the gathers of div_apply, den_apply and
 // forces_den_apply) as one float4 per slot.
+template <class Early>
 __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __restrict__ pos,
-                                             const float* __restrict__ kd, float4* lds) {
+                                             const float* __restrict__ kd, float4* lds, uint32_t slots,
+                                             Early&& early) {
     constexpr uint32_t IT = 3;
     float4 pp[IT];
     float kk[IT];
     stage_flat<kSweepThreads, IT>(
-        R,
+        R, slots,
         [&](uint32_t slot, uint32_t it) {
             pp[it] = pos[slot];
             kk[it] = kd[slot];
         },
-        [&](uint32_t idx, uint32_t it) { lds[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, kk[it]); });
+        [&](uint32_t idx, uint32_t it) { lds[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, kk[it]); }, early);
 }
 
 // Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
 // its share of the list (kappa_partial, sim_solve.wgsl:596-606).
 template <bool STAGED>
-__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, f3 pi, float kdi,
-                                            const float4* __restrict__ pos, const float4* lds) {
+__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, const ListHead& h, f3 pi,
+                                            float kdi, const float4* __restrict__ pos, const float4* lds) {
     f3 dv = mk(0.0f, 0.0f, 0.0f);
-    for_list<STAGED>(b, id, [&](uint32_t j, float g) {
+    for_list<STAGED>(b, id, h, [&](uint32_t j, float g) {
         float4 q;
         if (STAGED) {
             q = lds[j];
@@ -545,15 +605,25 @@ __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id
 template <bool STAGED>
 __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                           const SolveBufs& b, float4* smem, f3& pi, float& kdi) {
+    ListHead h;
+    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float kd0 = 0.0f;
+    auto early = [&] {
+        h = list_head<STAGED>(b, id);
+        p4 = pos[own_slot(id)];
+        kd0 = b.kd[own_slot(id)];
+    };
     if (STAGED) {
-        stage_pos_kd(R, pos, b.kd, smem);
+        stage_pos_kd(R, pos, b.kd, smem, stage_slots(b), early);
         __syncthreads();
+    } else {
+        early();
     }
     f3 dv = mk(0.0f, 0.0f, 0.0f);
     if (id.live) {
-        pi = xyz(pos[id.p]);
-        kdi = b.kd[id.p];
-        dv = kappa_partial<STAGED>(b, id, pi, kdi, pos, smem);
+        pi = xyz(p4);
+        kdi = kd0;
+        dv = kappa_partial<STAGED>(b, id, h, pi, kdi, pos, smem);
     }
     return fold4(dv);
 }
@@ -602,12 +672,24 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
                                             const float* __restrict__ temperature, const SolveBufs& b,
                                             const ForcesLds& L, ForceSums& F, f3& pi, float& rho_i) {
+    ListHead h;
+    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v4 = p4;
+    float w_i = 0.0f, temp_i = 0.0f;
+    auto early = [&] {
+        const uint32_t o = own_slot(id);
+        h = list_head<STAGED>(b, id);
+        p4 = pos[o];
+        v4 = vel[o];
+        w_i = b.rho_near[o];
+        temp_i = temperature[o];
+        rho_i = b.density[o];
+    };
     if (STAGED) {
         constexpr uint32_t IT = 3;
         float4 pp[IT], vv[IT];
         float dd[IT], nr[IT], tt[IT];
         stage_flat<kSweepThreads, IT>(
-            R,
+            R, stage_slots(b),
             [&](uint32_t slot, uint32_t it) {
                 pp[it] = pos[slot];
                 vv[it] = vel[slot];
@@ -619,18 +701,18 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
                 L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
                 L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
                 L.t[idx] = tt[it];
-            });
+            },
+            early);
         __syncthreads();
+    } else {
+        early();
     }
     if (!id.live) return;
-    const uint32_t i = id.p;
-    pi = xyz(pos[i]);
-    const f3 vi = xyz(vel[i]);
-    const float w_i = b.rho_near[i], temp_i = temperature[i];
-    rho_i = b.density[i];
+    pi = xyz(p4);
+    const f3 vi = xyz(v4);
     const float rhoi = rho_i;
     const f3 pii = pi;
-    for_list<STAGED>(b, id, [&](uint32_t j, float gs) {
+    for_list<STAGED>(b, id, h, [&](uint32_t j, float gs) {
         f3 pj, vj;
         float rho_j, near_j, temp_j;
         if (STAGED) {
@@ -682,7 +764,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
     if (!sc.runs(tile, kEveryPass)) return;
     const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
-    const ForcesLds L{smem, smem + b.stage_cap, reinterpret_cast<float*>(smem + 2 * b.stage_cap)};
+    const ForcesLds L{smem, smem + stage_slots(b), reinterpret_cast<float*>(smem + 2 * stage_slots(b))};
     const f3 z = mk(0.0f, 0.0f, 0.0f);
     ForceSums F{z, z, z, z, 0.0f};
     f3 pi = z;
@@ -781,16 +863,25 @@ template <bool STAGED>
 __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel, const SolveBufs& b, const PvLds& L,
                                                  f3& vi) {
+    ListHead h;
+    float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v4 = p4;
+    auto early = [&] {
+        h = list_head<STAGED>(b, id);
+        p4 = pos[own_slot(id)];
+        v4 = vel[own_slot(id)];
+    };
     if (STAGED) {
-        stage_pv<kSweepThreads>(R, pos, vel, L);
+        stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b), early);
         __syncthreads();
+    } else {
+        early();
     }
     float drho = 0.0f;
     if (id.live) {
-        const f3 pi = xyz(pos[id.p]);
-        vi = xyz(vel[id.p]);
+        const f3 pi = xyz(p4);
+        vi = xyz(v4);
         const f3 vii = vi;
-        for_list<STAGED>(b, id, [&](uint32_t j, float g) {
+        for_list<STAGED>(b, id, h, [&](uint32_t j, float g) {
             f3 pj, vj;
             load_pv<STAGED>(L, pos, vel, j, pj, vj);
             drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
@@ -809,7 +900,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
     if (!sc.runs(tile, pass)) return;
     const LaneId id = lane_id(tile, k.slot_end);
     const RegionS R = load_region(b.regions, id.tile);
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + b.stage_cap)};
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
     const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi)
                                  : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi);
@@ -831,13 +922,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
 
 // Reads the working set, writes the resting set at the same slot.
 template <class Sc>
-__global__ __launch_bounds__(kBlock) void k_integrate(Sc sc, Records work, Records rest, SolveBufs b) {
-    static_assert(kBlock == kTile, "integrate takes one solve tile a workgroup");
-    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Records rest, SolveBufs b) {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);  // one tile a workgroup
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, kEveryPass)) return;
-    const uint32_t i = tile * kBlock + threadIdx.x;
+    const uint32_t i = tile * kTile + threadIdx.x;
     bool clamped_speed = false;
     if (i < k.slot_end) {
         const float4 was = work.vel[i];
@@ -909,6 +999,10 @@ __global__ __launch_bounds__(kBlock) void k_reset_passes(uint32_t scenes, uint32
 constexpr size_t kPvEntry = sizeof(float4) + sizeof(float2);          // 24
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
 constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
+// A tile within one cell row of its scene stages nine rows of about kTile +
+// 16 slots; 13 tiles' worth leaves room for ragged rows and keeps two 256-slot
+// (four 128-slot) density and den_kappa workgroups' LDS on a CU.
+constexpr uint32_t kStageMax = 13 * kTile;
 
 }  // namespace
 
@@ -917,8 +1011,7 @@ uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 // Sizes the staging capacity from the device's per-workgroup LDS (less 1 KB
 // for static LDS) and raises the sweeps' dynamic-LDS limit to match. The
 // capacity is one for every sweep, since a staged tile's list holds LDS
-// indices: at most 3,328 slots, which keeps two density and den_kappa
-// workgroups (24 bytes a slot) on a CU; forces_eval (36 bytes) sizes the
+// indices: at most kStageMax slots; forces_eval (36 bytes a slot) sizes the
 // budget. If the device refuses the larger limit, the capacity fits 64 KB.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
@@ -937,7 +1030,8 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
             avail = 65536 - 1024;
     (void)hipGetLastError();
     size_t c = avail / kForcesEntry;
-    if (c > 3328) c = 3328;
+    c = c > kStagePad ? c - kStagePad : 0;
+    if (c > kStageMax) c = kStageMax;
     b.stage_cap = (uint32_t)(c / 64 * 64);
     return avail;
 }
@@ -950,7 +1044,7 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
     if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
     hipLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, sc, starts, work.pos,
                        b.regions, tiles, b.stage_cap);
-    hipLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), b.stage_cap * kPvEntry, st, sc, starts,
+    hipLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), stage_slots(b) * kPvEntry, st, sc, starts,
                        work.pos, work.vel, b);
     if (timer) timer->end(st, KERNEL_DENSITY_DIV);
 }
@@ -969,7 +1063,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
                   const Check& check = Check(), const StageBufs* T = nullptr) {
     if (n == 0) return;
     const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
-    const size_t lds_kappa = b.stage_cap * kKappaEntry;
+    const size_t lds_kappa = stage_slots(b) * kKappaEntry;
     auto V = [&](uint32_t i) { return T ? T->vel[i] : work.vel; };
     auto KB = [&](uint32_t i) {
         SolveBufs c = b;
@@ -983,7 +1077,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, V(0), V(1), KB(0));
     if (timer) timer->end(st, KERNEL_DIV_APPLY);
     if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, b.stage_cap * kForcesEntry, st, sc, work.pos, V(1),
+    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, sc, work.pos, V(1),
                        work.temperature, work.prev_pressure, KB(1));
     if (timer) timer->end(st, KERNEL_FORCES_EVAL);
     if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
@@ -992,7 +1086,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
     for (uint32_t r = 0; r < passes; ++r) {
         if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, b.stage_cap * kPvEntry, st, sc, r, work.pos, V(2 + r),
+        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, sc, r, work.pos, V(2 + r),
                            KB(2 + r));
         if (timer) timer->end(st, KERNEL_DEN_KAPPA);
         check(st, r);
@@ -1004,7 +1098,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     Records wl = work;
     wl.vel = V(passes + 2);
     if (timer) timer->begin(st, KERNEL_INTEGRATE);
-    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kBlock), 0, st, sc, wl, rest, b);
+    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kTile), 0, st, sc, wl, rest, b);
     if (timer) timer->end(st, KERNEL_INTEGRATE);
 }
 

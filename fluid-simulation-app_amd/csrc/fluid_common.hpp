@@ -349,6 +349,20 @@ __device__ __forceinline__ float fold4(float v) {
     return t + quad_perm<0xB1>(t);           // [1, 0, 3, 2]: lane ^ 1
 }
 __device__ __forceinline__ f3 fold4(f3 v) { return mk(fold4(v.x), fold4(v.y), fold4(v.z)); }
+// Lane K's value in every lane of its quad (quad_perm [K, K, K, K]).
+template <int K>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return quad_perm<K * 0x55>(v);
+}
+// A float4 spread over a quad (lane l holds component l), whole in every
+// lane. The four lanes of a particle share the loads of its own fields this
+// way: one register a lane where a whole float4 would take four.
+__device__ __forceinline__ float4 quad_float4(float v) {
+    return make_float4(quad_bcast<0>(v), quad_bcast<1>(v), quad_bcast<2>(v), quad_bcast<3>(v));
+}
+__device__ __forceinline__ float lane_part(const float4* a, uint32_t i, uint32_t lane) {
+    return reinterpret_cast<const float*>(a)[(size_t)i * 4 + lane];
+}
 
 // ---- launch geometry -----------------------------------------------------
 

@@ -602,16 +602,19 @@ __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id
     return dv;
 }
 
-template <bool STAGED>
+// `own(slot)` loads the caller's own fields with the list head.
+template <bool STAGED, class Own>
 __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
-                                          const SolveBufs& b, float4* smem, f3& pi, float& kdi) {
+                                          const SolveBufs& b, float4* smem, f3& pi, float& kdi, Own&& own) {
     ListHead h;
     float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float kd0 = 0.0f;
     auto early = [&] {
+        const uint32_t o = own_slot(id);
         h = list_head<STAGED>(b, id);
-        p4 = pos[own_slot(id)];
-        kd0 = b.kd[own_slot(id)];
+        p4 = pos[o];
+        kd0 = b.kd[o];
+        own(o);
     };
     if (STAGED) {
         stage_pos_kd(R, pos, b.kd, smem, stage_slots(b), early);
@@ -643,12 +646,16 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
     const RegionS R = load_region(b.regions, id.tile);
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
-    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi)
-                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi);
+    float wv = 0.0f, gv = 0.0f;  // component `lane` of vel_in and wall_grad
+    auto own = [&](uint32_t o) {
+        wv = lane_part(vel_in, o, id.lane);
+        gv = lane_part(b.wall_grad, o, id.lane);
+    };
+    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
+                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi, own);
+    const float4 w = quad_float4(wv), wg = quad_float4(gv);
     if (id.live && id.lane == 0) {
-        const float4 wg = b.wall_grad[id.p];
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
-        const float4 w = vel_in[id.p];
         vel_out[id.p] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
     }
 }
@@ -670,8 +677,9 @@ struct ForceSums {
 template <bool STAGED>
 __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, const RegionS& R,
                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                            const float* __restrict__ temperature, const SolveBufs& b,
-                                            const ForcesLds& L, ForceSums& F, f3& pi, float& rho_i) {
+                                            const float* __restrict__ temperature,
+                                            const float* __restrict__ prev_pressure, const SolveBufs& b,
+                                            const ForcesLds& L, ForceSums& F, f3& pi, float& rho_i, float& prev_p) {
     ListHead h;
     float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v4 = p4;
     float w_i = 0.0f, temp_i = 0.0f;
@@ -683,6 +691,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         w_i = b.rho_near[o];
         temp_i = temperature[o];
         rho_i = b.density[o];
+        prev_p = prev_pressure[o];
     };
     if (STAGED) {
         constexpr uint32_t IT = 3;
@@ -768,11 +777,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
     const f3 z = mk(0.0f, 0.0f, 0.0f);
     ForceSums F{z, z, z, z, 0.0f};
     f3 pi = z;
-    float rho_i = 0.0f;
+    float rho_i = 0.0f, prev_p = 0.0f;
     if (R.staged)
-        forces_tile<true>(k, id, R, pos, vel, temperature, b, L, F, pi, rho_i);
+        forces_tile<true>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, pi, rho_i, prev_p);
     else
-        forces_tile<false>(k, id, R, pos, vel, temperature, b, L, F, pi, rho_i);
+        forces_tile<false>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, pi, rho_i, prev_p);
     const f3 visc = fold4(F.visc), blend = fold4(F.blend), nearf = fold4(F.nearf), tension = fold4(F.tension);
     const float heat = fold4(F.heat);
     if (id.live && id.lane == 0) {
@@ -781,7 +790,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
         const f3 acc = add(add(add(visc, scl(k.k_near, nearf)), scl(k.gamma, tension)), adh);
         b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
         b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
-        const float half = 0.5f * prev_pressure[i];
+        const float half = 0.5f * prev_p;
         b.kd[i] = half / (rho_i * rho_i);
         b.pressure[i] = half;
     }
@@ -818,12 +827,20 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
     const RegionS R = load_region(b.regions, id.tile);
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
-    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi)
-                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi);
+    // Component `lane` of accel, vel_in, xsph and wall_grad, and temperature.
+    float av = 0.0f, wv = 0.0f, xv = 0.0f, gv = 0.0f, t0 = 0.0f;
+    auto own = [&](uint32_t o) {
+        av = lane_part(b.accel, o, id.lane);
+        wv = lane_part(vel_in, o, id.lane);
+        xv = lane_part(b.xsph, o, id.lane);
+        gv = lane_part(b.wall_grad, o, id.lane);
+        t0 = temperature[o];
+    };
+    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
+                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi, own);
+    const float4 a = quad_float4(av), was = quad_float4(wv), xs = quad_float4(xv), wg = quad_float4(gv);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
-        const float4 a = b.accel[i];
-        const float4 was = vel_in[i];
         const f3 v = xyz(was);
         const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
         const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
@@ -832,10 +849,8 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
         const f3 c3 = scl(2.0f, cross3(om, v));
         const f3 fict = sub(sub(mk(-c1.x, -c1.y, -c1.z), c2), c3);
         const f3 fsum = add(add(mk(s.force[0], s.force[1], s.force[2]), fict), xyz(a));
-        const float4 xs = b.xsph[i];
         const f3 forced = fingers(s, pi, add(add(v, scl(s.dt, fsum)), scl(s.c_xsph, xyz(xs))));
-        temperature[i] = temperature[i] + s.dt * a.w;
-        const float4 wg = b.wall_grad[i];
+        temperature[i] = t0 + s.dt * a.w;
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
         vel_out[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z,
                                  was.w);
@@ -859,16 +874,27 @@ __device__ __forceinline__ void block_residual(unsigned long long v, unsigned lo
     }
 }
 
+// The particle's own fields den_kappa ends with, loaded with the list head.
+struct DenOwn {
+    float rho, alpha, pressure;
+    float4 wg;
+};
+
 template <bool STAGED>
 __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel, const SolveBufs& b, const PvLds& L,
-                                                 f3& vi) {
+                                                 f3& vi, DenOwn& own) {
     ListHead h;
     float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v4 = p4;
     auto early = [&] {
+        const uint32_t o = own_slot(id);
         h = list_head<STAGED>(b, id);
-        p4 = pos[own_slot(id)];
-        v4 = vel[own_slot(id)];
+        p4 = pos[o];
+        v4 = vel[o];
+        own.rho = b.density[o];
+        own.alpha = b.alpha[o];
+        own.pressure = b.pressure[o];
+        own.wg = b.wall_grad[o];
     };
     if (STAGED) {
         stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b), early);
@@ -902,19 +928,19 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
     const RegionS R = load_region(b.regions, id.tile);
     const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
-    const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi)
-                                 : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi);
+    DenOwn own;
+    const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi, own)
+                                 : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi, own);
     unsigned long long err = 0;
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
-        const float rho = b.density[i];
-        const float4 wg = b.wall_grad[i];
-        const float d = k.mass * total + dot3(vi, xyz(wg));
+        const float rho = own.rho;
+        const float d = k.mass * total + dot3(vi, xyz(own.wg));
         const float rho_star = rho + s.dt * d;
         const float excess = wmax(rho_star - k.rho0, 0.0f);
-        const float kap = excess * b.alpha[i] / (s.dt * s.dt);
+        const float kap = excess * own.alpha / (s.dt * s.dt);
         b.kd[i] = kap / rho;
-        b.pressure[i] = b.pressure[i] + kap * rho;
+        b.pressure[i] = own.pressure + kap * rho;
         if (b.residual) err = (unsigned long long)(excess / k.rho0 * 4294967296.0f);
     }
     if (b.residual) block_residual(err, b.residual + sc.index(tile));

@@ -418,14 +418,16 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
                               : r == 7 ? R.d7
                                        : R.d8;
         // Two quads of candidates a trip (kq + lane, kq + 4 + lane), tested
-        // branch-free: a lane past the row's end re-reads its last slot and
-        // is masked out of the ballot.
+        // branch-free, a lane past the row's end masked out of the ballot.
+        // Staged, it reads at most 7 slots past the row's LDS range, inside
+        // the array's spare slots; unstaged, it re-reads the row's last slot.
+        // Only the centre row holds the particle itself.
         const uint32_t last = k1 - 1u;
         for (uint32_t kq = k0; kq < k1; kq += 2 * kLanes) {
             const uint32_t ka = kq + id.lane, kb = ka + kLanes;
             const bool ina = ka < k1, inb = kb < k1;
-            const uint32_t ja = (uint32_t)((int32_t)(ina ? ka : last) + delta);
-            const uint32_t jb = (uint32_t)((int32_t)(inb ? kb : last) + delta);
+            const uint32_t ja = STAGED ? (uint32_t)((int32_t)ka + delta) : (ina ? ka : last);
+            const uint32_t jb = STAGED ? (uint32_t)((int32_t)kb + delta) : (inb ? kb : last);
             f3 pa, pb;
             if (STAGED) {
                 const float4 a = L.pv[ja], c = L.pv[jb];
@@ -436,10 +438,14 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
                 pb = xyz(pos[jb]);
             }
             const f3 xa = sub(pi, pa), xb = sub(pi, pb);
-            const bool acc_a = ina & (dot3(xa, xa) < k.support_sq) & (ka != id.p);
-            const bool acc_b = inb & (dot3(xb, xb) < k.support_sq) & (kb != id.p);
-            const uint32_t qa = (uint32_t)(__ballot(acc_a) >> shift) & 0xFu;
-            const uint32_t qb = (uint32_t)(__ballot(acc_b) >> shift) & 0xFu;
+            bool acc_a = ina & (dot3(xa, xa) < k.support_sq);
+            bool acc_b = inb & (dot3(xb, xb) < k.support_sq);
+            if (r == 4) {
+                acc_a &= ka != id.p;
+                acc_b &= kb != id.p;
+            }
+            const uint32_t qa = (uint32_t)(__builtin_amdgcn_ballot_w64(acc_a) >> shift) & 0xFu;
+            const uint32_t qb = (uint32_t)(__builtin_amdgcn_ballot_w64(acc_b) >> shift) & 0xFu;
             const uint32_t fb = found + __popc(qa);
             append(acc_a, found + __popc(qa & below), ja);
             append(acc_b, fb + __popc(qb & below), jb);
@@ -460,14 +466,21 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
                                              DensitySums& a) {
     const uint32_t nn = found < b.cap ? found : b.cap;
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
-    const uint32_t step = b.stride * kLanes;
-    for (uint32_t t = 0; t < cnt; ++t) {
-        const uint32_t e = t * step + id.gid;
-        const uint32_t j = list_at<STAGED>(b, e);
+    const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
+    // The lane's first kHead entries load together (unconditionally: quads
+    // past the count lie inside the list and are not used).
+    uint32_t jh[kHead];
+#pragma unroll
+    for (uint32_t u = 0; u < kHead; ++u) jh[u] = list_at<STAGED>(b, (u < last ? u : last) * step + id.gid);
+    auto pair = [&](uint32_t t, uint32_t j) {
         f3 pj, vj;
         load_pv<STAGED>(L, pos, vel, j, pj, vj);
-        b.nbrg[e] = density_pair(k, pi, vi, pj, vj, a);
-    }
+        b.nbrg[t * step + id.gid] = density_pair(k, pi, vi, pj, vj, a);
+    };
+#pragma unroll
+    for (uint32_t u = 0; u < kHead; ++u)
+        if (u < cnt) pair(u, jh[u]);
+    for (uint32_t t = kHead; t < cnt; ++t) pair(t, list_at<STAGED>(b, t * step + id.gid));
     if (found > b.cap) {
         uint32_t seen = 0;
         for (uint32_t rr = 0; rr < kRows; ++rr) {

@@ -2,6 +2,7 @@
 // translation units (grid.hip, solve.hip, stats_tracers.hip).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -25,9 +26,11 @@ struct GridGeom {
 // the scene's pass count (everything but the density passes).
 constexpr uint32_t kEveryPass = 0xffffffffu;
 
+// `flip` reverses a launch's tile order (see tile_index in solve.hip).
 struct OneScene {
     SceneK k;
     StepK s;
+    uint32_t flip = 0;
     __device__ __forceinline__ uint32_t index(uint32_t) const { return 0; }
     __device__ __forceinline__ const SceneK& scene(uint32_t) const { return k; }
     __device__ __forceinline__ const StepK& step(uint32_t) const { return s; }
@@ -43,6 +46,7 @@ struct SceneTable {
     // [scenes] density passes each scene runs this substep, lowered when it
     // converges (tolerance mode); null: the Step's refine count.
     const uint32_t* pass_limit;
+    uint32_t flip = 0;
     __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile]; }
     __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[tile_scene[tile]]; }
     __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[tile_scene[tile]]; }
@@ -155,11 +159,28 @@ enum KernelId {
 };
 
 // HIP-event timing of kernel classes on the launch stream (profiling mode).
+// begin/end record events between launches; start/stop hand out the events
+// that hipExtLaunchKernelGGL records with a dispatch itself (start with a
+// class's first launch, stop with its last), so no event packet sits
+// between the kernels of a substep. Null when the class is not timed.
 struct KernelTimer {
     virtual ~KernelTimer() = default;
     virtual void begin(hipStream_t st, int kernel) = 0;
     virtual void end(hipStream_t st, int kernel) = 0;
+    virtual hipEvent_t start(int kernel) = 0;
+    virtual hipEvent_t stop(int kernel) = 0;
 };
+struct Span {
+    hipEvent_t a = nullptr, z = nullptr;
+};
+inline Span span(KernelTimer* t, int kernel) {
+    Span s;
+    if (t) {
+        s.a = t->start(kernel);
+        s.z = t->stop(kernel);
+    }
+    return s;
+}
 
 uint32_t scan_tiles(uint32_t cells);
 // The grid build over `slots` resting slots and `cells` histogram cells

@@ -227,21 +227,18 @@ static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells,
                        const Records& work, const GridBufs& b, KernelTimer* timer) {
     const uint32_t ntiles = scan_tiles(cells);
     uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
-    if (timer) timer->begin(st, KERNEL_GRID_COUNT);
-    hipLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, sc, rest.pos, n,
-                       b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
-    if (timer) timer->end(st, KERNEL_GRID_COUNT);
-    if (timer) timer->begin(st, KERNEL_GRID_SCAN);
-    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, b.counts, b.starts, cells, b.status,
-                       b.ticket);
-    if (timer) timer->end(st, KERNEL_GRID_SCAN);
+    const Span tc = span(timer, KERNEL_GRID_COUNT);
+    hipExtLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, tc.a, tc.z, 0, sc,
+                          rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
+    const Span ts = span(timer, KERNEL_GRID_SCAN);
+    hipExtLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, ts.a, ts.z, 0, b.counts, b.starts, cells,
+                          b.status, b.ticket);
     if (n == 0) return;
-    if (timer) timer->begin(st, KERNEL_GRID_SCATTER);
-    hipLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.cellid, b.rank, b.starts,
-                       b.perm_tmp);
-    hipLaunchKernelGGL(k_grid_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp, b.cellid,
+    const Span tg = span(timer, KERNEL_GRID_SCATTER);
+    hipExtLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, tg.a, nullptr, 0, n, b.cellid,
+                          b.rank, b.starts, b.perm_tmp);
+    hipExtLaunchKernelGGL(k_grid_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, nullptr, tg.z, 0, n, b.perm_tmp, b.cellid,
                        b.starts, rest, work, b.perm);
-    if (timer) timer->end(st, KERNEL_GRID_SCATTER);
 }
 
 void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,

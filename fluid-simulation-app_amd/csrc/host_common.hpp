@@ -120,6 +120,18 @@ class EventTimer final : public KernelTimer {
         pending.push_back({open[kernel], e, kernel});
         open[kernel] = nullptr;
     }
+    hipEvent_t start(int kernel) override {
+        if (!wants(kernel)) return nullptr;
+        open[kernel] = take();
+        return open[kernel];
+    }
+    hipEvent_t stop(int kernel) override {
+        if (!wants(kernel) || !open[kernel]) return nullptr;
+        hipEvent_t e = take();
+        pending.push_back({open[kernel], e, kernel});
+        open[kernel] = nullptr;
+        return e;
+    }
     void collect() {
         for (auto& p : pending) {
             (void)hipEventSynchronize(p.b);

@@ -50,7 +50,17 @@ constexpr uint32_t kPrefetch = 4;                   // list quads a lane loads a
 constexpr uint32_t kStagePad = 8;
 __host__ __device__ __forceinline__ uint32_t stage_slots(const SolveBufs& b) { return b.stage_cap + kStagePad; }
 
-__device__ __forceinline__ uint32_t tile_index() { return xcd_tile(blockIdx.x, gridDim.x); }
+// The workgroup's tile. Consecutive list sweeps run in opposite tile orders:
+// one sweep's working set (the neighbour list alone is ~170 bytes a
+// particle) is about the size of the 256 MB memory-side cache, and a sweep
+// that starts where the previous one ended finds its first tiles' data still
+// there, where the same order every time would evict each line just before
+// it is needed again.
+template <class Sc>
+__device__ __forceinline__ uint32_t tile_index(const Sc& sc) {
+    const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
+    return sc.flip ? gridDim.x - 1u - t : t;
+}
 
 __device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
     const uint32_t cx = cell_coord(p.x, k.box_min[0], k.cell, k.dims[0]);
@@ -562,7 +572,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint
                                                                const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(sc);
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, kEveryPass)) return;
@@ -651,7 +661,7 @@ template <class Sc>
 __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t pass, const float4* __restrict__ pos,
                                                                const float4* vel_in, float4* vel_out, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(sc);
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, pass)) return;
@@ -781,7 +791,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
                                                                const float* __restrict__ temperature,
                                                                const float* __restrict__ prev_pressure, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(sc);
     const SceneK& k = sc.scene(tile);
     if (!sc.runs(tile, kEveryPass)) return;
     const LaneId id = lane_id(tile, k.slot_end);
@@ -832,7 +842,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
                                                                     const float4* vel_in, float4* vel_out,
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(sc);
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, kEveryPass)) return;
@@ -933,7 +943,7 @@ template <class Sc>
 __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = tile_index(sc);
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, pass)) return;
@@ -1080,12 +1090,11 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
                          const SolveBufs& b, KernelTimer* timer) {
     if (n == 0) return;
     const uint32_t tiles = solve_tiles(n);
-    if (timer) timer->begin(st, KERNEL_DENSITY_DIV);
-    hipLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, sc, starts, work.pos,
-                       b.regions, tiles, b.stage_cap);
-    hipLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), stage_slots(b) * kPvEntry, st, sc, starts,
-                       work.pos, work.vel, b);
-    if (timer) timer->end(st, KERNEL_DENSITY_DIV);
+    const Span t = span(timer, KERNEL_DENSITY_DIV);
+    hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, t.a, nullptr, 0, sc, starts,
+                          work.pos, b.regions, tiles, b.stage_cap);
+    hipExtLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), stage_slots(b) * kPvEntry, st, nullptr,
+                          t.z, 0, sc, starts, work.pos, work.vel, b);
 }
 
 struct NoCheck {
@@ -1112,33 +1121,30 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     Records w0 = work;
     w0.vel = V(0);
     density_only(st, sc, n, starts, w0, KB(0), timer);
-    if (timer) timer->begin(st, KERNEL_DIV_APPLY);
-    hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, kEveryPass, work.pos, V(0), V(1), KB(0));
-    if (timer) timer->end(st, KERNEL_DIV_APPLY);
-    if (timer) timer->begin(st, KERNEL_FORCES_EVAL);
-    hipLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, sc, work.pos, V(1),
-                       work.temperature, work.prev_pressure, KB(1));
-    if (timer) timer->end(st, KERNEL_FORCES_EVAL);
-    if (timer) timer->begin(st, KERNEL_FORCES_DEN_APPLY);
-    hipLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, sc, work.pos, V(1), V(2),
-                       work.temperature, KB(1));
-    if (timer) timer->end(st, KERNEL_FORCES_DEN_APPLY);
+    Sc rev = sc;  // the reverse-order launches
+    rev.flip = 1;
+    const Span td = span(timer, KERNEL_DIV_APPLY);
+    hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass, work.pos,
+                          V(0), V(1), KB(0));
+    const Span tf = span(timer, KERNEL_FORCES_EVAL);
+    hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z, 0, sc,
+                          work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
+    const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
+    hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev, work.pos, V(1),
+                          V(2), work.temperature, KB(1));
     for (uint32_t r = 0; r < passes; ++r) {
-        if (timer) timer->begin(st, KERNEL_DEN_KAPPA);
-        hipLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, sc, r, work.pos, V(2 + r),
-                           KB(2 + r));
-        if (timer) timer->end(st, KERNEL_DEN_KAPPA);
+        const Span tk = span(timer, KERNEL_DEN_KAPPA);
+        hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, tk.a, tk.z, 0, sc, r,
+                              work.pos, V(2 + r), KB(2 + r));
         check(st, r);
-        if (timer) timer->begin(st, KERNEL_DEN_APPLY);
-        hipLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, sc, r, work.pos, V(2 + r), V(3 + r),
-                           KB(2 + r));
-        if (timer) timer->end(st, KERNEL_DEN_APPLY);
+        const Span tp = span(timer, KERNEL_DEN_APPLY);
+        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r, work.pos,
+                              V(2 + r), V(3 + r), KB(2 + r));
     }
     Records wl = work;
     wl.vel = V(passes + 2);
-    if (timer) timer->begin(st, KERNEL_INTEGRATE);
-    hipLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kTile), 0, st, sc, wl, rest, b);
-    if (timer) timer->end(st, KERNEL_INTEGRATE);
+    const Span ti = span(timer, KERNEL_INTEGRATE);
+    hipExtLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kTile), 0, st, ti.a, ti.z, 0, sc, wl, rest, b);
 }
 
 void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,

@@ -51,7 +51,111 @@ def kernel_bytes(name, n, cells):
     return None
 
 
-def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=8):
+# The committed counter summary `roofline.traffic` comes from (written by
+# --pmc-summary from rocprofv3 runs of this bench; see its header).
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "pmc_block_1m.md")
+COPY_BYTES = 268435456
+
+
+def _kernel_classes(names):
+    """Labels a substep's dispatches (in dispatch order) with the bench's
+    kernel classes. kappa_apply serves div_apply (after density_div) and
+    den_apply (after den_kappa); grid_scatter is place + gather and
+    density_div includes its tile_regions launch, as the sim's own timer
+    groups them. Returns (class, primary) per dispatch: `primary` marks the
+    launch that closes one occurrence of the class."""
+    import re
+    out, after = [], None
+    for name in names:
+        m = re.search(r"\b(k_\w+?)(?:<|\(|$)", name)
+        k = m.group(1) if m else name
+        cls = {"k_grid_count": "grid_count", "k_grid_scan": "grid_scan", "k_grid_place": "grid_scatter",
+               "k_grid_gather": "grid_scatter", "k_tile_regions": "density_div", "k_density_div": "density_div",
+               "k_forces_eval": "forces_eval", "k_forces_den_apply": "forces_den_apply",
+               "k_den_kappa": "den_kappa", "k_integrate": "integrate"}.get(k)
+        if k == "k_kappa_apply":
+            cls = "div_apply" if after == "k_density_div" else "den_apply"
+        primary = k not in ("k_grid_place", "k_tile_regions")
+        if k in ("k_density_div", "k_den_kappa"):
+            after = k
+        out.append((cls, primary))
+    return out
+
+
+def pmc_summary(base):
+    """Prints the markdown summary of four rocprofv3 --pmc runs under `base`:
+    pmc_F / pmc_W (FETCH_SIZE, WRITE_SIZE of a short 1M bench run) and
+    pmc_copyF / pmc_copyW (the same counters over --copy-calibration). Each
+    counter is scaled by what it reads for the plain copy of known size."""
+    import csv
+
+    def rows(d):
+        f = os.path.join(base, d, "pmc_counter_collection.csv")
+        return sorted(csv.DictReader(open(f)), key=lambda r: int(r["Dispatch_Id"]))
+
+    def copy_factor(d):
+        vals = [float(r["Counter_Value"]) for r in rows(d) if "copyBuffer" in r["Kernel_Name"]]
+        return (COPY_BYTES / 1024.0) / (sum(vals) / len(vals)), sum(vals) / len(vals)
+
+    ff, fcopy = copy_factor("pmc_copyF")
+    fw, wcopy = copy_factor("pmc_copyW")
+    per = {}
+    for d, key in (("pmc_F", "fetch"), ("pmc_W", "write")):
+        rs = [r for r in rows(d) if "k_" in r["Kernel_Name"]]
+        for r, (cls, primary) in zip(rs, _kernel_classes([r["Kernel_Name"] for r in rs])):
+            if cls is None:
+                continue
+            e = per.setdefault(cls, {"fetch": 0.0, "write": 0.0, "launches_fetch": 0, "launches_write": 0})
+            e[key] += float(r["Counter_Value"])
+            if primary:
+                e["launches_" + key] += 1
+    cmd = "python3 -u bench.py --steps 2 --warmup 2 --no-cpu --no-batched --no-adjoint"
+    print("# HBM traffic per launch, block_1m (1,048,576 particles, R = 5)\n")
+    print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
+    print("```")
+    for c, d in (("FETCH_SIZE", "pmc_F"), ("WRITE_SIZE", "pmc_W")):
+        print(f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/{d} -o pmc -- {cmd}")
+    for c, d in (("FETCH_SIZE", "pmc_copyF"), ("WRITE_SIZE", "pmc_copyW")):
+        print(f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/{d} -o pmc -- "
+              f"python3 -u bench.py --copy-calibration {COPY_BYTES}")
+    print("python3 bench.py --pmc-summary gpurun_out > profiles/pmc_block_1m.md")
+    print("```\n")
+    print(f"Calibration: a {COPY_BYTES >> 20} MiB device copy (HIP's copy kernel) reads FETCH_SIZE = "
+          f"{fcopy:.0f} KiB and WRITE_SIZE = {wcopy:.0f} KiB, so traffic = {ff:.3f} x FETCH_SIZE + "
+          f"{fw:.3f} x WRITE_SIZE (KiB x 1024 bytes).\n")
+    print("| kernel | launches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic MB | algorithmic MB |")
+    print("|---|---|---|---|---|---|")
+    n, cells = 1048576, 357911
+    for cls in ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
+                "forces_den_apply", "den_kappa", "den_apply", "integrate"]:
+        e = per.get(cls)
+        if not e or not e["launches_fetch"] or not e["launches_write"]:
+            continue
+        fk = e["fetch"] / e["launches_fetch"]
+        wk = e["write"] / e["launches_write"]
+        traffic = (ff * fk + fw * wk) * 1024.0
+        alg = kernel_bytes(cls, n, cells)
+        print(f"| {cls} | {e['launches_fetch']} | {fk:.0f} | {wk:.0f} | {traffic / 1e6:.1f} | "
+              f"{alg / 1e6:.1f} |")
+
+
+def load_traffic(path=PMC_SUMMARY):
+    """kernel class -> measured HBM bytes per launch from the committed
+    counter summary, or {} when there is none."""
+    out = {}
+    if not os.path.exists(path):
+        return out
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and (cells[0] in BYTES_PER_PARTICLE or cells[0] in BYTES_PER_CELL):
+            try:
+                out[cells[0]] = float(cells[4]) * 1e6
+            except ValueError:
+                pass
+    return out
+
+
+def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=64):
     """Times the oracle's FP32 step on this host's cores on a bounded sample."""
     import oracle
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "1024")))
@@ -199,8 +303,8 @@ def copy_calibration(nbytes: int, reps: int = 4) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
     ap.add_argument("--workload", default="block_1m", choices=["block_1m", "block_8m", "dam_break_16k"])
@@ -211,7 +315,12 @@ def main():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint line")
+    ap.add_argument("--pmc-summary", metavar="DIR", default="",
+                    help="print the counter summary of rocprofv3 --pmc runs under DIR and exit")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
               "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
@@ -240,23 +349,26 @@ def main():
     if ntr:
         sim.upload_tracers(sc.tracers)
 
-    total = args.warmup + max(args.steps, 3)  # the kernel profile below takes 3 substeps
+    # A bench step is one substep of the reference schedule from the seeded
+    # block: `warmup` untimed substeps, `steps` timed ones, then 3 profiled.
+    total = args.warmup + args.steps + 3
     steps = [sc.step_block(k * dt_sub, dt_sub, v_clamp) for k in range(total)]
     frame_blocks = [fa.pack_step(force=sc.gravity(k * dt_sub), dt=dt_sub * n_sub, seed=k) for k in range(total)]
 
-    # warm-up substeps, then a short all-kernel profile to name the dominant kernel
+    # warm-up substeps; the last 3 of them profile every kernel class to name
+    # the dominant one
     for k in range(args.warmup):
+        if k == args.warmup - 3:
+            sim.synchronize()
+            sim.profile(fa.KERNELS)
         sim.substeps(steps[k], 1, refine)
-    sim.synchronize()
-    sim.profile(fa.KERNELS)
-    for k in range(3):
-        sim.substeps(steps[args.warmup + k], 1, refine)
-    prof = sim.profile_read()
+    pre = sim.profile_read()
     sim.profile([])
-    per_kernel_ms = {k: v[0] / 3.0 for k, v in prof.items()}
-    dominant = max((k for k in per_kernel_ms if kernel_bytes(k, sc.n, sim.cells)), key=lambda k: per_kernel_ms[k])
+    dominant = max((k for k in pre if kernel_bytes(k, sc.n, sim.cells)), key=lambda k: pre[k][0])
 
-    # the timed region: exactly `steps` substeps, events on the dominant kernel only
+    # the timed region: exactly `steps` substeps, the dominant kernel timed by
+    # start/stop events carried on its own dispatches (no extra packets)
+    clamps0, overflow0 = sim.counters()
     sim.profile([dominant])
     sim.synchronize()
     t0 = time.perf_counter()
@@ -267,6 +379,16 @@ def main():
     dom = sim.profile_read()[dominant]
     sim.profile([])
     dom_ms = dom[0] / max(dom[1], 1)
+    clamps1, overflow1 = sim.counters()
+
+    # every kernel class over 3 substeps just after the timed region
+    sim.profile(fa.KERNELS)
+    for k in range(3):
+        sim.substeps(steps[args.warmup + args.steps + k], 1, refine)
+    prof = sim.profile_read()
+    sim.profile([])
+    per_kernel_ms = {k: v[0] / 3.0 for k, v in prof.items()}
+    launch_us = {k: v[0] * 1e3 / max(v[1], 1) for k, v in prof.items()}
 
     # per-frame work, timed on its own: reduce_stats + tracers
     sim.synchronize()
@@ -285,10 +407,18 @@ def main():
     b_dom = kernel_bytes(dominant, sc.n, sim.cells)
     achieved = b_dom / (dom_ms * 1e-3) / 1e9
     step_bytes = canonical_bytes(sc.n, sim.cells, refine)
+    traffic = load_traffic() if sc.name == "block_1m" else {}
+    kernels = {}
+    for k, us in launch_us.items():
+        kb = kernel_bytes(k, sc.n, sim.cells)
+        kernels[k] = {"avg_launch_us": us, "launches_per_substep": prof[k][1] / 3.0,
+                      "algorithmic_bytes": kb, "achieved_gbs": kb / (us * 1e-6) / 1e9 if kb else None,
+                      "traffic_bytes": traffic.get(k),
+                      "traffic_gbs": traffic[k] / (us * 1e-6) / 1e9 if k in traffic else None}
     out = {
-        "metric": "DFSPH particle-steps/s @1M (plain)",
+        "metric": "DFSPH particle-steps/s @1M (plain vs batched) + adjoint/forward cost ratio",
         "value": value,
-        "unit": "particle-substeps/s",
+        "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -302,13 +432,17 @@ def main():
                    "substep_dt_s": dt_sub, "substeps_per_frame": n_sub, "h_over_spacing": sc.h / sc.spacing,
                    "tracers": ntr, "schedule": "reference fixed (1 divergence + warm start + R density)"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic.get(dominant),
+                     "traffic_gbs": traffic[dominant] / (dom_ms * 1e-3) / 1e9 if dominant in traffic else None,
+                     "traffic_source": os.path.relpath(PMC_SUMMARY, ROOT) if dominant in traffic else None,
                      "avg_launch_us": dom_ms * 1e3, "algorithmic_bytes": b_dom},
+        "kernels": kernels,
         "step_roofline": {"canonical_bytes": step_bytes, "achieved_gbs": step_bytes / (ms_per_step * 1e-3) / 1e9,
                           "frac": step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "per_kernel_ms": per_kernel_ms,
         "launch": sim.describe(),
         "frame_extra_ms": frame_extra_ms,
+        "timed_counters": {"speed_clamps": clamps1 - clamps0, "list_overflow_pairs": overflow1 - overflow0},
         "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
                   "compression_avg": float(stats[0])},
     }

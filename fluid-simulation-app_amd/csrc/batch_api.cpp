@@ -187,6 +187,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.nbr_n, n);
     A(b.regions, std::max<uint32_t>(B->tiles, 1));
     A(b.counters, 2 * S);
+    A(b.clamps, (size_t)kClampWords * S);
     A(b.residual, S);
     A(b.iterations, S);
     A(B->d_pass_limit, S);
@@ -426,7 +427,7 @@ int fluid_batch_use_graph(FluidBatch* b, bool enable) {
 int fluid_batch_reduce_stats(FluidBatch* b) {
     if (!valid(b)) return FLUID_ERR_ARG;
     launch_reduce_stats(b->stream, b->table(b->ring[0]), b->scenes, b->solve.density, b->rest, b->solve.counters,
-                        b->stats, b->t());
+                        b->solve.clamps, b->stats, b->t());
     return check_launch("batch reduce_stats");
 }
 
@@ -441,10 +442,16 @@ int fluid_batch_read_stats(FluidBatch* b, float* out) {
 
 int fluid_batch_counters(FluidBatch* b, uint64_t* out) {
     if (!valid(b) || !out) return FLUID_ERR_ARG;
-    std::vector<uint32_t> c((size_t)b->scenes * 2);
+    std::vector<uint32_t> c((size_t)b->scenes * 2), spread((size_t)b->scenes * kClampWords);
     HIP_TRY(hipStreamSynchronize(b->stream));
     HIP_TRY(hipMemcpy(c.data(), b->solve.counters, c.size() * 4, hipMemcpyDeviceToHost));
-    for (size_t i = 0; i < c.size(); ++i) out[i] = c[i];
+    HIP_TRY(hipMemcpy(spread.data(), b->solve.clamps, spread.size() * 4, hipMemcpyDeviceToHost));
+    for (size_t s = 0; s < b->scenes; ++s) {
+        uint64_t clamped = 0;
+        for (uint32_t w = 0; w < kClampSlots; ++w) clamped += spread[s * kClampWords + w * kClampStride];
+        out[2 * s] = clamped;
+        out[2 * s + 1] = c[2 * s + 1];
+    }
     return FLUID_OK;
 }
 

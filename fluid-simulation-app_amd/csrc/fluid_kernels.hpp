@@ -114,7 +114,11 @@ struct SolveBufs {
     uint32_t* nbr_n;  // [N]
     uint32_t cap;     // a multiple of 4
     uint32_t stride;  // >= the tiles' slot count
-    uint32_t* counters;  // per scene: [0] CFL clamps, [1] neighbour overflow (cumulative)
+    uint32_t* counters;  // per scene: [1] neighbour overflow pairs (cumulative); [0] unused
+    // Per scene, the speed-clamp count (cumulative) as kClampSlots words
+    // kClampStride apart, one per cache line, whose sum is the count: many
+    // workgroups add to it each substep, and one word would serialise them.
+    uint32_t* clamps;
     // Staging capacity (slots) shared by every sweep.
     uint32_t stage_cap;
     // Tolerance mode (batched): per scene, the fixed-point (2^-32) sum of the
@@ -124,6 +128,10 @@ struct SolveBufs {
     uint32_t* iterations;
     TileRegion* regions;  // [tiles]
 };
+
+constexpr uint32_t kClampSlots = 16;
+constexpr uint32_t kClampStride = 32;
+constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
 
 // Tiles of the solve (consecutive slots a workgroup owns).
 uint32_t solve_tiles(uint32_t n);
@@ -235,12 +243,13 @@ struct StatsBufs {
 };
 uint32_t stats_blocks(uint32_t n);
 void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
-                         const uint32_t* counters, const StatsBufs& b, KernelTimer* timer);
+                         const uint32_t* counters, const uint32_t* clamps, const StatsBufs& b, KernelTimer* timer);
 // Per scene of a batch, the same fold over the scene's own slots (so a
 // scene's stats are those of the plain step): partials [scenes][1024][16],
 // stats [scenes][16].
 void launch_reduce_stats(hipStream_t st, const SceneTable& sc, uint32_t scenes, const float* density,
-                         const Records& rest, const uint32_t* counters, const StatsBufs& b, KernelTimer* timer);
+                         const Records& rest, const uint32_t* counters, const uint32_t* clamps, const StatsBufs& b,
+                         KernelTimer* timer);
 
 struct TracerBufs {
     int32_t* vel_grid;   // [C][4] 16.16 sums (cleared by resolve)

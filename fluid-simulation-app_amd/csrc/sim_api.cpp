@@ -194,6 +194,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.nbr_n, n);
     A(b.regions, std::max<uint32_t>(solve_tiles(s->n), 1));
     A(b.counters, 4);
+    A(b.clamps, kClampWords);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_bytes = (size_t)lds;
@@ -432,7 +433,8 @@ int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
 int fluid_sim_reduce_stats(FluidSim* s) {
     if (!valid(s)) return FLUID_ERR_ARG;
     if (s->n == 0) return FLUID_OK;
-    launch_reduce_stats(s->stream, s->params.rho0, s->n, s->solve.density, s->rest, s->solve.counters, s->stats,
+    launch_reduce_stats(s->stream, s->params.rho0, s->n, s->solve.density, s->rest, s->solve.counters,
+                        s->solve.clamps, s->stats,
                         s->t());
     return check_launch("reduce_stats");
 }
@@ -481,9 +483,13 @@ int fluid_sim_frame(FluidSim* s, const uint8_t* step144, uint32_t n, uint32_t re
 int fluid_sim_counters(FluidSim* s, uint64_t* clamps, uint64_t* overflow) {
     if (!valid(s)) return FLUID_ERR_ARG;
     uint32_t c[4];
+    std::vector<uint32_t> spread(kClampWords);
     HIP_TRY(hipStreamSynchronize(s->stream));
     HIP_TRY(hipMemcpy(c, s->solve.counters, sizeof c, hipMemcpyDeviceToHost));
-    if (clamps) *clamps = c[0];
+    HIP_TRY(hipMemcpy(spread.data(), s->solve.clamps, kClampWords * 4, hipMemcpyDeviceToHost));
+    uint64_t clamped = 0;
+    for (uint32_t w = 0; w < kClampSlots; ++w) clamped += spread[w * kClampStride];
+    if (clamps) *clamps = clamped;
     if (overflow) *overflow = c[1];
     return FLUID_OK;
 }

@@ -1011,10 +1011,12 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
         rest.temperature[i] = t + t * EXPANSION / (rho * HEAT_CAPACITY) * dp;
         if (rest.ids) rest.ids[i] = work.ids[i];
     }
-    // One atomic per wave for the clamp counter.
+    // One atomic per wave for the clamp count, into one of the scene's
+    // spread words (by tile).
     const uint64_t m = __ballot(clamped_speed);
     if (m && __lane_id() == (uint32_t)(__ffsll((unsigned long long)m) - 1))
-        atomicAdd(&b.counters[2 * sc.index(tile)], (uint32_t)__popcll(m));
+        atomicAdd(&b.clamps[sc.index(tile) * kClampWords + (tile % kClampSlots) * kClampStride],
+                  (uint32_t)__popcll(m));
 }
 
 // Tolerance mode, once per scene after den_kappa of `pass`: the mean

@@ -112,26 +112,30 @@ __device__ __forceinline__ void stats_partial(float rho0, uint32_t base, uint32_
 // partials t, t + 256, ..., then a tree over the threads), so the result is
 // the same on every run for a given particle count.
 __device__ __forceinline__ void stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
-                                            const uint32_t* __restrict__ counters, float* __restrict__ stats);
+                                            const uint32_t* __restrict__ counters,
+                                            const uint32_t* __restrict__ clamps, float* __restrict__ stats);
 
 __global__ __launch_bounds__(kBlock) void k_stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
                                                         const uint32_t* __restrict__ counters,
+                                                        const uint32_t* __restrict__ clamps,
                                                         float* __restrict__ stats) {
-    stats_final(n, blocks, partials, counters, stats);
+    stats_final(n, blocks, partials, counters, clamps, stats);
 }
 
 // One workgroup a scene: that scene's partials, counters and stats row.
 __global__ __launch_bounds__(kBlock) void k_stats_final_many(SceneTable sc, const float* __restrict__ partials,
                                                              const uint32_t* __restrict__ counters,
+                                                             const uint32_t* __restrict__ clamps,
                                                              float* __restrict__ stats) {
     const SceneK& k = sc.ks[blockIdx.x];
     const uint32_t n = k.slot_end - k.slot_begin;
     stats_final(n, stats_block_count(n), partials + (size_t)blockIdx.x * kStatsMaxBlocks * 16, counters + 2 * blockIdx.x,
-                stats + (size_t)blockIdx.x * 16);
+                clamps + (size_t)blockIdx.x * kClampWords, stats + (size_t)blockIdx.x * 16);
 }
 
 __device__ __forceinline__ void stats_final(uint32_t n, uint32_t blocks, const float* __restrict__ partials,
-                                            const uint32_t* __restrict__ counters, float* __restrict__ stats) {
+                                            const uint32_t* __restrict__ counters,
+                                            const uint32_t* __restrict__ clamps, float* __restrict__ stats) {
     float acc[12] = {0.0f, 0.0f, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F, HUGE_F, -HUGE_F};
     for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) {
         const float* p = partials + (size_t)b * 16;
@@ -170,7 +174,9 @@ __device__ __forceinline__ void stats_final(uint32_t n, uint32_t blocks, const f
     stats[6] = red[5][0];
     stats[7] = red[10][0];
     stats[8] = red[11][0];
-    stats[9] = (float)counters[0];
+    uint32_t clamped = 0;
+    for (uint32_t w = 0; w < kClampSlots; ++w) clamped += clamps[w * kClampStride];
+    stats[9] = (float)clamped;
     stats[10] = (float)counters[1];
     stats[11] = red[4][0];
     stats[12] = red[6][0];
@@ -376,21 +382,24 @@ __global__ __launch_bounds__(kBlock) void k_advect(SceneK k, StepK s, uint32_t n
 uint32_t stats_blocks(uint32_t n) { return stats_block_count(n); }
 
 void launch_reduce_stats(hipStream_t st, float rho0, uint32_t n, const float* density, const Records& rest,
-                         const uint32_t* counters, const StatsBufs& b, KernelTimer* timer) {
+                         const uint32_t* counters, const uint32_t* clamps, const StatsBufs& b, KernelTimer* timer) {
     if (timer) timer->begin(st, KERNEL_REDUCE_STATS);
     hipLaunchKernelGGL(k_stats_partial, dim3(b.blocks), dim3(kBlock), 0, st, rho0, n, density, rest.vel,
                        rest.prev_vel, rest.temperature, b.partials);
-    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, st, n, b.blocks, b.partials, counters, b.stats);
+    hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, st, n, b.blocks, b.partials, counters, clamps,
+                       b.stats);
     if (timer) timer->end(st, KERNEL_REDUCE_STATS);
 }
 
 void launch_reduce_stats(hipStream_t st, const SceneTable& sc, uint32_t scenes, const float* density,
-                         const Records& rest, const uint32_t* counters, const StatsBufs& b, KernelTimer* timer) {
+                         const Records& rest, const uint32_t* counters, const uint32_t* clamps, const StatsBufs& b,
+                         KernelTimer* timer) {
     if (scenes == 0) return;
     if (timer) timer->begin(st, KERNEL_REDUCE_STATS);
     hipLaunchKernelGGL(k_stats_partial_many, dim3(b.blocks, scenes), dim3(kBlock), 0, st, sc, density, rest.vel,
                        rest.prev_vel, rest.temperature, b.partials);
-    hipLaunchKernelGGL(k_stats_final_many, dim3(scenes), dim3(kBlock), 0, st, sc, b.partials, counters, b.stats);
+    hipLaunchKernelGGL(k_stats_final_many, dim3(scenes), dim3(kBlock), 0, st, sc, b.partials, counters, clamps,
+                       b.stats);
     if (timer) timer->end(st, KERNEL_REDUCE_STATS);
 }
 

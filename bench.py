@@ -109,7 +109,7 @@ def pmc_summary(base):
             e[key] += float(r["Counter_Value"])
             if primary:
                 e["launches_" + key] += 1
-    cmd = "python3 -u bench.py --steps 2 --warmup 2 --no-cpu --no-batched --no-adjoint"
+    cmd = "python3 -u bench.py --no-cpu --no-batched --no-adjoint"
     print("# HBM traffic per launch, block_1m (1,048,576 particles, R = 5)\n")
     print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
     print("```")

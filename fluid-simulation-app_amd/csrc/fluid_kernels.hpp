@@ -38,6 +38,15 @@ struct OneScene {
     __device__ __forceinline__ bool runs(uint32_t, uint32_t pass) const {
         return s.active != 0u && (pass == kEveryPass || pass < s.refine);
     }
+    // The same from a tile's region record (scene fields of TileRegion);
+    // here the kernel arguments already hold them.
+    __device__ __forceinline__ uint32_t index_at(uint32_t) const { return 0; }
+    __device__ __forceinline__ const SceneK& scene_at(uint32_t) const { return k; }
+    __device__ __forceinline__ const StepK& step_at(uint32_t) const { return s; }
+    __device__ __forceinline__ uint32_t slot_end_at(uint32_t) const { return k.slot_end; }
+    __device__ __forceinline__ bool runs_at(uint32_t, uint32_t, uint32_t, uint32_t pass) const {
+        return runs(0, pass);
+    }
 };
 struct SceneTable {
     const SceneK* ks;             // [scenes]
@@ -55,6 +64,15 @@ struct SceneTable {
         if (ss[i].active == 0u) return false;
         if (pass == kEveryPass) return true;
         return pass < (pass_limit ? pass_limit[i] : ss[i].refine);
+    }
+    __device__ __forceinline__ uint32_t index_at(uint32_t scene) const { return scene; }
+    __device__ __forceinline__ const SceneK& scene_at(uint32_t scene) const { return ks[scene]; }
+    __device__ __forceinline__ const StepK& step_at(uint32_t scene) const { return ss[scene]; }
+    __device__ __forceinline__ uint32_t slot_end_at(uint32_t slot_end) const { return slot_end; }
+    __device__ __forceinline__ bool runs_at(uint32_t scene, uint32_t active, uint32_t refine, uint32_t pass) const {
+        if (active == 0u) return false;
+        if (pass == kEveryPass) return true;
+        return pass < (pass_limit ? pass_limit[scene] : refine);
     }
 };
 
@@ -91,6 +109,10 @@ struct TileRegion {
     uint32_t base[9];  // LDS offset of row r's first slot (prefix of the lengths)
     uint32_t total;    // slots in the nine ranges
     uint32_t staged;   // total <= the staging capacity
+    // The tile's scene and what decides whether it runs this substep, copied
+    // here once a substep: a sweep reads them with its region in one go
+    // instead of a chain of dependent loads through the scene tables.
+    uint32_t scene, slot_end, active, refine;
 };
 
 // Per-substep scratch of the solver, all in slot (cell) order.

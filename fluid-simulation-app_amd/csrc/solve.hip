@@ -105,6 +105,11 @@ __global__ __launch_bounds__(kBlock) void k_tile_regions(Sc sc, const uint32_t* 
     }
     R.total = total;
     R.staged = total <= stage_cap ? 1u : 0u;
+    const StepK& st = sc.step(t);
+    R.scene = sc.index(t);
+    R.slot_end = k.slot_end;
+    R.active = st.active;
+    R.refine = st.refine;
     out[t] = R;
 }
 
@@ -114,6 +119,7 @@ struct RegionS {
     uint32_t b1, b2, b3, b4, b5, b6, b7, b8;
     uint32_t total;
     bool staged;
+    uint32_t scene, slot_end, active, refine;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
@@ -139,6 +145,10 @@ __device__ __forceinline__ RegionS load_region(const TileRegion* __restrict__ re
     s.b7 = uni(R->base[7]);
     s.b8 = uni(R->base[8]);
     s.total = uni(R->total);
+    s.scene = uni(R->scene);
+    s.slot_end = uni(R->slot_end);
+    s.active = uni(R->active);
+    s.refine = uni(R->refine);
     s.staged = uni(R->staged) != 0u;
     return s;
 }
@@ -573,12 +583,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint
                                                                const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
-    const SceneK& k = sc.scene(tile);
-    const StepK& s = sc.step(tile);
-    if (!sc.runs(tile, kEveryPass)) return;
-    const LaneId id = lane_id(tile, k.slot_end);
-    uint32_t* counters = b.counters + 2 * sc.index(tile);
-    const RegionS R = load_region(b.regions, id.tile);
+    const RegionS R = load_region(b.regions, tile);
+    if (!sc.runs_at(R.scene, R.active, R.refine, kEveryPass)) return;
+    const SceneK& k = sc.scene_at(R.scene);
+    const StepK& s = sc.step_at(R.scene);
+    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
+    uint32_t* counters = b.counters + 2 * sc.index_at(R.scene);
     const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
     if (R.staged)
         density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L);
@@ -662,11 +672,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
                                                                const float4* vel_in, float4* vel_out, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
-    const SceneK& k = sc.scene(tile);
-    const StepK& s = sc.step(tile);
-    if (!sc.runs(tile, pass)) return;
-    const LaneId id = lane_id(tile, k.slot_end);
-    const RegionS R = load_region(b.regions, id.tile);
+    const RegionS R = load_region(b.regions, tile);
+    if (!sc.runs_at(R.scene, R.active, R.refine, pass)) return;
+    const SceneK& k = sc.scene_at(R.scene);
+    const StepK& s = sc.step_at(R.scene);
+    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
     float wv = 0.0f, gv = 0.0f;  // component `lane` of vel_in and wall_grad
@@ -792,10 +802,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
                                                                const float* __restrict__ prev_pressure, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
-    const SceneK& k = sc.scene(tile);
-    if (!sc.runs(tile, kEveryPass)) return;
-    const LaneId id = lane_id(tile, k.slot_end);
-    const RegionS R = load_region(b.regions, id.tile);
+    const RegionS R = load_region(b.regions, tile);
+    if (!sc.runs_at(R.scene, R.active, R.refine, kEveryPass)) return;
+    const SceneK& k = sc.scene_at(R.scene);
+    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
     const ForcesLds L{smem, smem + stage_slots(b), reinterpret_cast<float*>(smem + 2 * stage_slots(b))};
     const f3 z = mk(0.0f, 0.0f, 0.0f);
     ForceSums F{z, z, z, z, 0.0f};
@@ -843,11 +853,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
                                                                     float* __restrict__ temperature, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
-    const SceneK& k = sc.scene(tile);
-    const StepK& s = sc.step(tile);
-    if (!sc.runs(tile, kEveryPass)) return;
-    const LaneId id = lane_id(tile, k.slot_end);
-    const RegionS R = load_region(b.regions, id.tile);
+    const RegionS R = load_region(b.regions, tile);
+    if (!sc.runs_at(R.scene, R.active, R.refine, kEveryPass)) return;
+    const SceneK& k = sc.scene_at(R.scene);
+    const StepK& s = sc.step_at(R.scene);
+    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
     f3 pi = mk(0.0f, 0.0f, 0.0f);
     float kdi = 0.0f;
     // Component `lane` of accel, vel_in, xsph and wall_grad, and temperature.
@@ -944,11 +954,11 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
-    const SceneK& k = sc.scene(tile);
-    const StepK& s = sc.step(tile);
-    if (!sc.runs(tile, pass)) return;
-    const LaneId id = lane_id(tile, k.slot_end);
-    const RegionS R = load_region(b.regions, id.tile);
+    const RegionS R = load_region(b.regions, tile);
+    if (!sc.runs_at(R.scene, R.active, R.refine, pass)) return;
+    const SceneK& k = sc.scene_at(R.scene);
+    const StepK& s = sc.step_at(R.scene);
+    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
     const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
     DenOwn own;
@@ -966,7 +976,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
         b.pressure[i] = own.pressure + kap * rho;
         if (b.residual) err = (unsigned long long)(excess / k.rho0 * 4294967296.0f);
     }
-    if (b.residual) block_residual(err, b.residual + sc.index(tile));
+    if (b.residual) block_residual(err, b.residual + sc.index_at(R.scene));
 }
 
 // Reads the working set, writes the resting set at the same slot.

@@ -188,6 +188,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.regions, std::max<uint32_t>(B->tiles, 1));
     A(b.counters, 2 * S);
     A(b.clamps, (size_t)kClampWords * S);
+    A(b.zeros, kZeroRows);
     A(b.residual, S);
     A(b.iterations, S);
     A(B->d_pass_limit, S);

@@ -113,6 +113,9 @@ struct TileRegion {
     // here once a substep: a sweep reads them with its region in one go
     // instead of a chain of dependent loads through the scene tables.
     uint32_t scene, slot_end, active, refine;
+    // Whether a particle of the tile has a nonzero wall gradient (set by
+    // density_div): the later sweeps read wall_grad only for such tiles.
+    uint32_t walls;
 };
 
 // Per-substep scratch of the solver, all in slot (cell) order.
@@ -141,6 +144,10 @@ struct SolveBufs {
     // kClampStride apart, one per cache line, whose sum is the count: many
     // workgroups add to it each substep, and one word would serialise them.
     uint32_t* clamps;
+    // kZeroRows zero float4s: what a sweep reads in place of wall_grad for a
+    // tile with no particle near a wall (the same values, from lines that
+    // stay in cache instead of HBM).
+    const float4* zeros;
     // Staging capacity (slots) shared by every sweep.
     uint32_t stage_cap;
     // Tolerance mode (batched): per scene, the fixed-point (2^-32) sum of the
@@ -151,6 +158,7 @@ struct SolveBufs {
     TileRegion* regions;  // [tiles]
 };
 
+constexpr uint32_t kZeroRows = 4096;
 constexpr uint32_t kClampSlots = 16;
 constexpr uint32_t kClampStride = 32;
 constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene

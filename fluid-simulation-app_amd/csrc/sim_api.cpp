@@ -195,6 +195,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.regions, std::max<uint32_t>(solve_tiles(s->n), 1));
     A(b.counters, 4);
     A(b.clamps, kClampWords);
+    A(b.zeros, kZeroRows);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_bytes = (size_t)lds;

@@ -110,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_regions(Sc sc, const uint32_t* 
     R.slot_end = k.slot_end;
     R.active = st.active;
     R.refine = st.refine;
+    R.walls = 0;
     out[t] = R;
 }
 
@@ -119,7 +120,7 @@ struct RegionS {
     uint32_t b1, b2, b3, b4, b5, b6, b7, b8;
     uint32_t total;
     bool staged;
-    uint32_t scene, slot_end, active, refine;
+    uint32_t scene, slot_end, active, refine, walls;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
@@ -149,6 +150,7 @@ __device__ __forceinline__ RegionS load_region(const TileRegion* __restrict__ re
     s.slot_end = uni(R->slot_end);
     s.active = uni(R->active);
     s.refine = uni(R->refine);
+    s.walls = uni(R->walls);
     s.staged = uni(R->staged) != 0u;
     return s;
 }
@@ -168,6 +170,18 @@ __device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) 
     d += (int32_t)(idx >= R.b7) * (R.d7 - R.d6);
     d += (int32_t)(idx >= R.b8) * (R.d8 - R.d7);
     return d;
+}
+
+// Slot o's wall gradient (component `lane`, or whole): wall_grad for a tile
+// with a particle near a wall, else the zero rows. A pointer select, not a
+// branch, so the load is issued like the others.
+__device__ __forceinline__ float wall_part(const SolveBufs& b, const RegionS& R, uint32_t o, uint32_t lane) {
+    const float4* src = R.walls ? b.wall_grad : b.zeros;
+    return lane_part(src, R.walls ? o : (o & (kZeroRows - 1u)), lane);
+}
+__device__ __forceinline__ float4 wall_grad_of(const SolveBufs& b, const RegionS& R, uint32_t o) {
+    const float4* src = R.walls ? b.wall_grad : b.zeros;
+    return src[R.walls ? o : (o & (kZeroRows - 1u))];
 }
 
 // Copies `total` staged slots into LDS: NT threads, ITERS slots each, all
@@ -523,6 +537,39 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
     }
 }
 
+// Lane 0's end of density_div for particle i: the wall terms, ρ, α and the
+// divergence κ/ρ. Returns whether the particle's wall gradient is nonzero.
+__device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
+                                               f3 pi, f3 vi, uint32_t found, float w_sum, float rn, float gsq_sum,
+                                               float drho_sum, f3 gsum_sum, const SolveBufs& b) {
+    const uint32_t i = id.p;
+    const uint32_t nn = found < b.cap ? found : b.cap;
+    b.nbr_n[i] = nn;
+    if (found > b.cap) atomicAdd(&counters[1], found - b.cap);
+    float rho = k.mass * w_sum;
+    const float gsq = k.mass * k.mass * gsq_sum;
+    float drho = k.mass * drho_sum;
+    f3 gsum = scl(k.mass, gsum_sum);
+    const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h, (pi.z - k.wall_lo[2]) * k.inv_h);
+    const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h, (k.wall_hi[2] - pi.z) * k.inv_h);
+    float fill = 0.0f;
+    fill += wall_density(tl.x) + wall_density(th.x);
+    fill += wall_density(tl.y) + wall_density(th.y);
+    fill += wall_density(tl.z) + wall_density(th.z);
+    fill -= wedge_sum(tl, th);
+    rho += k.rho0 * fill;
+    const f3 wg = wall_grad_sum(k, pi);
+    b.wall_grad[i] = make_float4(wg.x, wg.y, wg.z, 0.0f);
+    gsum = add(gsum, wg);
+    b.density[i] = rho;
+    b.rho_near[i] = rn;
+    const float al = rho / wmax(dot3(gsum, gsum) + gsq, 1e-4f);
+    b.alpha[i] = al;
+    drho += dot3(vi, wg);
+    b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
+    return wg.x != 0.0f || wg.y != 0.0f || wg.z != 0.0f;
+}
+
 template <bool STAGED>
 __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                              const RegionS& R, const uint32_t* __restrict__ starts,
@@ -550,32 +597,12 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     const float w_sum = fold4(a.w_sum), rn = fold4(a.rn), gsq_sum = fold4(a.gsq), drho_sum = fold4(a.drho);
     const f3 gsum_sum = fold4(a.gsum);
     if (!id.live || id.lane != 0) return;
-    const uint32_t i = id.p;
-    const uint32_t nn = found < b.cap ? found : b.cap;
-    b.nbr_n[i] = nn;
-    if (found > b.cap) atomicAdd(&counters[1], found - b.cap);
-    float rho = k.mass * w_sum;
-    const float gsq = k.mass * k.mass * gsq_sum;
-    float drho = k.mass * drho_sum;
-    f3 gsum = scl(k.mass, gsum_sum);
-    const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h, (pi.z - k.wall_lo[2]) * k.inv_h);
-    const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h, (k.wall_hi[2] - pi.z) * k.inv_h);
-    float fill = 0.0f;
-    fill += wall_density(tl.x) + wall_density(th.x);
-    fill += wall_density(tl.y) + wall_density(th.y);
-    fill += wall_density(tl.z) + wall_density(th.z);
-    fill -= wedge_sum(tl, th);
-    rho += k.rho0 * fill;
-    const f3 wg = wall_grad_sum(k, pi);
-    b.wall_grad[i] = make_float4(wg.x, wg.y, wg.z, 0.0f);
-    gsum = add(gsum, wg);
-    b.density[i] = rho;
-    b.rho_near[i] = rn;
-    const float al = rho / wmax(dot3(gsum, gsum) + gsq, 1e-4f);
-    b.alpha[i] = al;
-    drho += dot3(vi, wg);
-    b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
+    // A particle near a wall marks its tile (k_tile_regions cleared the mark
+    // this substep; any number of lanes may store the same 1).
+    if (density_finish(k, s, counters, id, pi, vi, found, w_sum, rn, gsq_sum, drho_sum, gsum_sum, b))
+        b.regions[id.tile].walls = 1u;
 }
+
 
 template <class Sc>
 __global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint32_t* __restrict__ starts,
@@ -682,7 +709,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
     float wv = 0.0f, gv = 0.0f;  // component `lane` of vel_in and wall_grad
     auto own = [&](uint32_t o) {
         wv = lane_part(vel_in, o, id.lane);
-        gv = lane_part(b.wall_grad, o, id.lane);
+        gv = wall_part(b, R, o, id.lane);
     };
     const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
                               : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi, own);
@@ -866,7 +893,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
         av = lane_part(b.accel, o, id.lane);
         wv = lane_part(vel_in, o, id.lane);
         xv = lane_part(b.xsph, o, id.lane);
-        gv = lane_part(b.wall_grad, o, id.lane);
+        gv = wall_part(b, R, o, id.lane);
         t0 = temperature[o];
     };
     const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
@@ -927,7 +954,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
         own.rho = b.density[o];
         own.alpha = b.alpha[o];
         own.pressure = b.pressure[o];
-        own.wg = b.wall_grad[o];
+        own.wg = wall_grad_of(b, R, o);
     };
     if (STAGED) {
         stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b), early);
@@ -1063,7 +1090,10 @@ constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
 // A tile within one cell row of its scene stages nine rows of about kTile +
 // 16 slots; 13 tiles' worth leaves room for ragged rows and keeps two 256-slot
 // (four 128-slot) density and den_kappa workgroups' LDS on a CU.
-constexpr uint32_t kStageMax = 13 * kTile;
+#ifndef FLUID_STAGE_MAX_HALF_TILES
+#define FLUID_STAGE_MAX_HALF_TILES 26
+#endif
+constexpr uint32_t kStageMax = FLUID_STAGE_MAX_HALF_TILES * kTile / 2;
 
 }  // namespace
 

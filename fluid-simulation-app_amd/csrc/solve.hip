@@ -604,8 +604,11 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
 }
 
 
+// At most 80 VGPRs (six waves a SIMD: three 512-thread workgroups a CU); the
+// scene-table form would take a few more and drop to two, which costs it
+// 16% against a 16-byte spill.
 template <class Sc>
-__global__ __launch_bounds__(kSweepThreads) void k_density_div(Sc sc, const uint32_t* __restrict__ starts,
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_density_div(Sc sc, const uint32_t* __restrict__ starts,
                                                                const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];

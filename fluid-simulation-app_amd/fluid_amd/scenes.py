@@ -34,14 +34,21 @@ class Scene:
     tracers: np.ndarray = field(default_factory=lambda: np.zeros((0, 2), np.uint32))
     phase: float = 0.0
     seed: int = 0
+    # A random-walk tilt: the angle of each frame (None: the sinusoid).
+    walk: np.ndarray | None = None
 
     @property
     def n(self) -> int:
         return int(self.pos4.shape[0])
 
     def gravity(self, t: float):
-        """g(t) = 9.81 (sin th, -cos th, 0), th = (pi/4) sin(2 pi t / 2 s + phase)."""
-        th = (math.pi / 4.0) * math.sin(2.0 * math.pi * t / 2.0 + self.phase)
+        """g(t) = 9.81 (sin th, -cos th, 0): th = (pi/4) sin(2 pi t / 2 s +
+        phase), or with a random walk th of the frame that holds t."""
+        if self.walk is not None:
+            k = min(int(t / FRAME_DT), len(self.walk) - 1)
+            th = float(self.walk[k])
+        else:
+            th = (math.pi / 4.0) * math.sin(2.0 * math.pi * t / 2.0 + self.phase)
         return (GRAVITY * math.sin(th), -GRAVITY * math.cos(th), 0.0)
 
     def schedule(self, v_max: float = 0.0, cap: int = 16, frame_dt: float = FRAME_DT):
@@ -114,11 +121,21 @@ def block_1m(seed=0, h_factor=1.0, tracers=4096) -> Scene:
                        tracers=tracers)
 
 
-def block_8m(seed=0, h_factor=1.0) -> Scene:
-    """8,388,608 particles (256 x 128 x 256) filling 30% of the unit box."""
+def tilt_walk(frames: int, sigma: float, seed: int) -> np.ndarray:
+    """A seeded random-walk tilt, one angle a frame: th_0 = 0, th_k+1 = th_k +
+    N(0, sigma^2)."""
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 0x7417]))
+    return np.concatenate([[0.0], np.cumsum(rng.normal(0.0, sigma, frames - 1))])
+
+
+def block_8m(seed=0, h_factor=1.0, frames=10000) -> Scene:
+    """8,388,608 particles (256 x 128 x 256) filling 30% of the unit box,
+    with a random-walk tilt (step sigma 0.05 rad a frame)."""
     n = 256 * 128 * 256
     d = (0.3 / n) ** (1.0 / 3.0)
-    return block_scene("block_8m", (256, 128, 256), d, (0.5, 0.5, 0.5), seed=seed, h_factor=h_factor)
+    sc = block_scene("block_8m", (256, 128, 256), d, (0.5, 0.5, 0.5), seed=seed, h_factor=h_factor)
+    sc.walk = tilt_walk(frames, 0.05, seed)
+    return sc
 
 
 def small_block(lattice=(8, 8, 8), seed=0, h_factor=1.2, spacing=None, half=(0.1, 0.1, 0.1), jitter=0.1,

@@ -358,12 +358,29 @@ __device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, 
     return st;
 }
 
+// A 16-bit LDS word, typed so that its loads and stores stay LDS instructions.
+typedef __attribute__((address_space(3))) uint16_t LdsU16;
+
 // LDS layout of density_div and den_kappa: per slot the position and the
-// velocity, 24 bytes.
+// velocity, 24 bytes. density_div adds its walk list after them.
 struct PvLds {
     float4* pv;  // x, y, z, vx
     float2* vw;  // vy, vz
+    LdsU16* list;    // density_div: kWalkList entries a particle of the tile
 };
+constexpr size_t kPvEntry = sizeof(float4) + sizeof(float2);  // 24
+
+// A staged tile's walk writes each particle's first kWalkList entries (LDS
+// indices) to LDS, at the particle's base plus the entry's place, and the
+// pair sums move them to the list in its quad layout; entries past them go
+// to the list directly. Developed flow at 1M particles holds at most 42.
+constexpr uint32_t kWalkList = 48;
+__host__ __device__ __forceinline__ uint32_t walk_list(const SolveBufs& b) {
+    return b.cap < kWalkList ? b.cap : kWalkList;
+}
+__host__ __device__ __forceinline__ size_t density_lds(const SolveBufs& b) {
+    return stage_slots(b) * kPvEntry + (size_t)kTile * walk_list(b) * sizeof(uint16_t);
+}
 
 template <bool STAGED>
 __device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict__ pos,
@@ -428,13 +445,17 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
     }
     // List indices fit 32 bits (cap x stride < 2^32, checked at allocation).
     const uint32_t lbase = id.p * kLanes, lstep = b.stride * kLanes;
+    const uint32_t T = walk_list(b);
+    LdsU16* const mine = L.list + (threadIdx.x / kLanes) * T;
     auto append = [&](bool acc, uint32_t m, uint32_t j) {
-        if (acc && m < b.cap) {
-            const uint32_t e = (m / kLanes) * lstep + lbase + m % kLanes;
-            if (STAGED)
-                b.nbr16[e] = (uint16_t)j;
-            else
-                b.nbr32[e] = j;
+        if (!acc) return;
+        if (STAGED) {
+            if (m < T)
+                mine[m] = (uint16_t)j;
+            else if (m < b.cap)
+                b.nbr16[(m / kLanes) * lstep + lbase + m % kLanes] = (uint16_t)j;
+        } else if (m < b.cap) {
+            b.nbr32[(m / kLanes) * lstep + lbase + m % kLanes] = j;
         }
     };
     uint32_t found = 0;
@@ -472,14 +493,21 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
                 pb = xyz(pos[jb]);
             }
             const f3 xa = sub(pi, pa), xb = sub(pi, pb);
-            bool acc_a = ina & (dot3(xa, xa) < k.support_sq);
-            bool acc_b = inb & (dot3(xb, xb) < k.support_sq);
+            const bool near_a = dot3(xa, xa) < k.support_sq, near_b = dot3(xb, xb) < k.support_sq;
+            bool acc_a = ina & near_a;
+            bool acc_b = inb & near_b;
+            // Each ballot of a single compare is that compare's lane mask, so
+            // the masks combine in scalar registers.
+            uint64_t ba = __builtin_amdgcn_ballot_w64(ina) & __builtin_amdgcn_ballot_w64(near_a);
+            uint64_t bb = __builtin_amdgcn_ballot_w64(inb) & __builtin_amdgcn_ballot_w64(near_b);
             if (r == 4) {
                 acc_a &= ka != id.p;
                 acc_b &= kb != id.p;
+                ba &= __builtin_amdgcn_ballot_w64(ka != id.p);
+                bb &= __builtin_amdgcn_ballot_w64(kb != id.p);
             }
-            const uint32_t qa = (uint32_t)(__builtin_amdgcn_ballot_w64(acc_a) >> shift) & 0xFu;
-            const uint32_t qb = (uint32_t)(__builtin_amdgcn_ballot_w64(acc_b) >> shift) & 0xFu;
+            const uint32_t qa = (uint32_t)(ba >> shift) & 0xFu;
+            const uint32_t qb = (uint32_t)(bb >> shift) & 0xFu;
             const uint32_t fb = found + __popc(qa);
             append(acc_a, found + __popc(qa & below), ja);
             append(acc_b, fb + __popc(qb & below), jb);
@@ -501,20 +529,36 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
     const uint32_t nn = found < b.cap ? found : b.cap;
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
     const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
-    // The lane's first kHead entries load together (unconditionally: quads
-    // past the count lie inside the list and are not used).
-    uint32_t jh[kHead];
-#pragma unroll
-    for (uint32_t u = 0; u < kHead; ++u) jh[u] = list_at<STAGED>(b, (u < last ? u : last) * step + id.gid);
     auto pair = [&](uint32_t t, uint32_t j) {
         f3 pj, vj;
         load_pv<STAGED>(L, pos, vel, j, pj, vj);
         b.nbrg[t * step + id.gid] = density_pair(k, pi, vi, pj, vj, a);
     };
+    if (STAGED) {
+        // Entry 4t + lane: the first T/4 of the lane's come from the walk's
+        // LDS list and are stored to the list here; any further ones the walk
+        // stored to the list itself. Two loops, so the LDS one waits on no
+        // global load (or, through it, on the stores).
+        const uint32_t T = walk_list(b);
+        const LdsU16* const mine = L.list + (threadIdx.x / kLanes) * T + id.lane;
+        const uint32_t in_lds = cnt < T / kLanes ? cnt : T / kLanes;
+        for (uint32_t t = 0; t < in_lds; ++t) {
+            const uint32_t j = mine[t * kLanes];
+            b.nbr16[t * step + id.gid] = (uint16_t)j;
+            pair(t, j);
+        }
+        for (uint32_t t = in_lds; t < cnt; ++t) pair(t, b.nbr16[t * step + id.gid]);
+    } else {
+        // The lane's first kHead entries load together (unconditionally:
+        // quads past the count lie inside the list and are not used).
+        uint32_t jh[kHead];
 #pragma unroll
-    for (uint32_t u = 0; u < kHead; ++u)
-        if (u < cnt) pair(u, jh[u]);
-    for (uint32_t t = kHead; t < cnt; ++t) pair(t, list_at<STAGED>(b, t * step + id.gid));
+        for (uint32_t u = 0; u < kHead; ++u) jh[u] = b.nbr32[(u < last ? u : last) * step + id.gid];
+#pragma unroll
+        for (uint32_t u = 0; u < kHead; ++u)
+            if (u < cnt) pair(u, jh[u]);
+        for (uint32_t t = kHead; t < cnt; ++t) pair(t, b.nbr32[t * step + id.gid]);
+    }
     if (found > b.cap) {
         uint32_t seen = 0;
         for (uint32_t rr = 0; rr < kRows; ++rr) {
@@ -619,7 +663,8 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
     const StepK& s = sc.step_at(R.scene);
     const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
     uint32_t* counters = b.counters + 2 * sc.index_at(R.scene);
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b)),
+                  (LdsU16*)(reinterpret_cast<char*>(smem) + stage_slots(b) * kPvEntry)};
     if (R.staged)
         density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L);
     else
@@ -989,7 +1034,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
     const SceneK& k = sc.scene_at(R.scene);
     const StepK& s = sc.step_at(R.scene);
     const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
-    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b))};
+    const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b)), nullptr};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
     DenOwn own;
     const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi, own)
@@ -1087,7 +1132,6 @@ __global__ __launch_bounds__(kBlock) void k_reset_passes(uint32_t scenes, uint32
     iterations[i] = 0;
 }
 
-constexpr size_t kPvEntry = sizeof(float4) + sizeof(float2);          // 24
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
 constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
 // A tile within one cell row of its scene stages nine rows of about kTile +
@@ -1138,7 +1182,7 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
     const Span t = span(timer, KERNEL_DENSITY_DIV);
     hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, t.a, nullptr, 0, sc, starts,
                           work.pos, b.regions, tiles, b.stage_cap);
-    hipExtLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), stage_slots(b) * kPvEntry, st, nullptr,
+    hipExtLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), density_lds(b), st, nullptr,
                           t.z, 0, sc, starts, work.pos, work.vel, b);
 }
 

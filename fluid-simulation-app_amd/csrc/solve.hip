@@ -768,13 +768,15 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
     }
 }
 
-// forces_eval's gathers: position, velocity, density, near-density and
-// temperature, 36 bytes a slot as float4 (x, y, z, vx), float4 (vy, vz,
-// rho, rho_near), float (temperature).
+// forces_eval's gathers: position, velocity, density, near-density,
+// temperature and m / rho, 40 bytes a slot as float4 (x, y, z, vx), float4
+// (vy, vz, rho, rho_near), float2 (temperature, m / rho). m / rho is the
+// neighbour's own quotient, so each staged slot divides once instead of each
+// pair.
 struct ForcesLds {
     float4* a;
     float4* c;
-    float* t;
+    float2* t;
 };
 
 struct ForceSums {
@@ -817,7 +819,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
             [&](uint32_t idx, uint32_t it) {
                 L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
                 L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
-                L.t[idx] = tt[it];
+                L.t[idx] = make_float2(tt[it], k.mass / dd[it]);
             },
             early);
         __syncthreads();
@@ -831,20 +833,23 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     const f3 pii = pi;
     for_list<STAGED>(b, id, h, [&](uint32_t j, float gs) {
         f3 pj, vj;
-        float rho_j, near_j, temp_j;
+        float rho_j, near_j, temp_j, mr_j;
         if (STAGED) {
             const float4 a = L.a[j], c = L.c[j];
+            const float2 t = L.t[j];
             pj = mk(a.x, a.y, a.z);
             vj = mk(a.w, c.x, c.y);
             rho_j = c.z;
             near_j = c.w;
-            temp_j = L.t[j];
+            temp_j = t.x;
+            mr_j = t.y;
         } else {
             pj = xyz(pos[j]);
             vj = xyz(vel[j]);
             rho_j = b.density[j];
             near_j = b.rho_near[j];
             temp_j = temperature[j];
+            mr_j = k.mass / rho_j;
         }
         const f3 x = sub(pii, pj);
         const float r2 = dot3(x, x);
@@ -861,7 +866,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         const float sn = (w_i + near_j) * wn * wn;
         F.nearf = add(F.nearf, mk(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm)));
         F.visc = add(F.visc, scl(k.visc2 * pair, dv));
-        F.blend = sub(F.blend, scl(k.mass / rho_j * kernel_q(k, rr * k.inv_h), dv));
+        F.blend = sub(F.blend, scl(mr_j * kernel_q(k, rr * k.inv_h), dv));
         F.heat += k.heat_cond * pair * (temp_i - temp_j);
         F.heat -= k.heat_visc * pair * dot3(dv, dv);
         const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
@@ -881,7 +886,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
     if (!sc.runs_at(R.scene, R.active, R.refine, kEveryPass)) return;
     const SceneK& k = sc.scene_at(R.scene);
     const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
-    const ForcesLds L{smem, smem + stage_slots(b), reinterpret_cast<float*>(smem + 2 * stage_slots(b))};
+    const ForcesLds L{smem, smem + stage_slots(b), reinterpret_cast<float2*>(smem + 2 * stage_slots(b))};
     const f3 z = mk(0.0f, 0.0f, 0.0f);
     ForceSums F{z, z, z, z, 0.0f};
     f3 pi = z;
@@ -1133,7 +1138,7 @@ __global__ __launch_bounds__(kBlock) void k_reset_passes(uint32_t scenes, uint32
 }
 
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
-constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float);    // 36
+constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float2);   // 40
 // A tile within one cell row of its scene stages nine rows of about kTile +
 // 16 slots; 13 tiles' worth leaves room for ragged rows and keeps two 256-slot
 // (four 128-slot) density and den_kappa workgroups' LDS on a CU.
@@ -1149,7 +1154,7 @@ uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 // Sizes the staging capacity from the device's per-workgroup LDS (less 1 KB
 // for static LDS) and raises the sweeps' dynamic-LDS limit to match. The
 // capacity is one for every sweep, since a staged tile's list holds LDS
-// indices: at most kStageMax slots; forces_eval (36 bytes a slot) sizes the
+// indices: at most kStageMax slots; forces_eval (40 bytes a slot) sizes the
 // budget. If the device refuses the larger limit, the capacity fits 64 KB.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;

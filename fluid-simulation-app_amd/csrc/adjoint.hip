@@ -35,29 +35,34 @@ namespace {
 constexpr uint32_t kAdjBlock = 256;
 constexpr uint32_t kQuadBlock = 256 * kListLanes;  // 256 particles a workgroup
 
-// A particle's lane in the four-lane sweeps.
+// A particle's lane in the four-lane sweeps, with the particle's list count
+// and where the lane's entries past the list's fixed rows start.
 struct Quad {
-    uint32_t p, lane, gid;
+    uint32_t p, lane, gid, nn, tail;
     bool live;
 };
-__device__ __forceinline__ Quad quad(uint32_t n) {
+// Every lane of the wave calls it (the tail start is a sum over the wave).
+__device__ __forceinline__ Quad quad(uint32_t n, const SolveBufs& b) {
     Quad q;
     q.gid = blockIdx.x * kQuadBlock + threadIdx.x;
     q.p = q.gid / kListLanes;
     q.lane = q.gid % kListLanes;
     q.live = q.p < n;
+    q.nn = q.live ? b.nbr_n[q.p] : 0u;
+    const uint32_t len = list_tail_len(q.nn);
+    const uint32_t off = wave_inclusive_sum((__lane_id() & (kListLanes - 1)) == 0 ? len : 0u) - len;
+    q.tail = list_tail(b.stride, b.cap, q.p, off) + q.lane;
     return q;
 }
-// Calls f(neighbour slot, dW/r) for the lane's share of particle p's list:
-// entry 4t + lane sits at quad row t, offset gid. The taped forward stages
-// nothing, so its lists hold slots.
+// Calls f(neighbour slot, dW/r) for the lane's share of particle p's list
+// (entries lane, lane + 4, ...: the fixed rows, then the wave's tail block).
+// The taped forward stages nothing, so its lists hold slots.
 template <class F>
 __device__ __forceinline__ void for_lane(const SolveBufs& b, const Quad& q, F&& f) {
-    const uint32_t nn = b.nbr_n[q.p];
     const uint32_t step = b.stride * kListLanes;  // list indices fit 32 bits
     uint32_t t = 0;
-    for (uint32_t m = q.lane; m < nn; m += kListLanes, ++t) {
-        const uint32_t e = t * step + q.gid;
+    for (uint32_t m = q.lane; m < q.nn; m += kListLanes, ++t) {
+        const uint32_t e = t < kListFixed ? t * step + q.gid : q.tail + (t - kListFixed) * kListLanes;
         f(b.nbr32[e], b.nbrg[e]);
     }
 }
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_apply(SceneK k, StepK st, ui
                                                           const float* __restrict__ lam_p, float* __restrict__ lam_D,
                                                           float* __restrict__ lam_rho, float* __restrict__ lam_alpha) {
     const float dt = st.dt;
-    const Quad q = quad(n);
+    const Quad q = quad(n, b);
     float s_lk = 0.0f;
     f3 s_x = mk(0.0f, 0.0f, 0.0f), li = s_x;
     float ki = 0.0f;
@@ -255,7 +260,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_divergence(SceneK k, uint32_
                                                                const float* __restrict__ lam_D,
                                                                float4* __restrict__ lam_v,
                                                                float4* __restrict__ lam_x) {
-    const Quad q = quad(n);
+    const Quad q = quad(n, b);
     f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, vi = s_v;
     float di = 0.0f;
     if (q.live) {
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_viscosity(SceneK k, float mu
                                                               float4* __restrict__ lam_v, float* __restrict__ lam_rho,
                                                               float4* __restrict__ lam_x, float* __restrict__ lam_mu) {
     __shared__ float red[kQuadBlock / 64];
-    const Quad q = quad(n);
+    const Quad q = quad(n, b);
     f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, la = s_v;
     float s_r = 0.0f, ri = 1.0f;
     if (q.live) {
@@ -382,7 +387,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_density_pre(SceneK k, StepK 
                                                                 float* __restrict__ lam_rho,
                                                                 float* __restrict__ lam_D,
                                                                 float4* __restrict__ lam_G) {
-    const Quad q = quad(n);
+    const Quad q = quad(n, b);
     f3 G = mk(0.0f, 0.0f, 0.0f);
     float gsq = 0.0f;
     if (q.live) {
@@ -429,7 +434,7 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_density(SceneK k, uint32_t n
                                                             const float* __restrict__ lam_D,
                                                             const float4* __restrict__ lam_G,
                                                             float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
-    const Quad q = quad(n);
+    const Quad q = quad(n, b);
     f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, vi = s_v, Gi = s_v;
     float di = 0.0f, ri = 0.0f;
     if (q.live) {

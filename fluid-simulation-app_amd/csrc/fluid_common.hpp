@@ -364,6 +364,21 @@ __device__ __forceinline__ float lane_part(const float4* a, uint32_t i, uint32_t
     return reinterpret_cast<const float*>(a)[(size_t)i * 4 + lane];
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave: four row shifts build
+// the 16-lane row scans, then two row broadcasts (lane 15 into rows 1 and 3,
+// lane 31 into rows 2 and 3) carry the row totals across the wave. Every
+// lane of the wave must take part.
+__device__ __forceinline__ uint32_t wave_inclusive_sum(uint32_t x) {
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return (uint32_t)v;
+}
+
 // ---- launch geometry -----------------------------------------------------
 
 constexpr int kBlock = 256;

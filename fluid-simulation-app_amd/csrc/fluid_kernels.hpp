@@ -128,14 +128,18 @@ struct SolveBufs {
     float4* accel;    // xyz; w = dT/dt
     float4* xsph;
     float4* wall_grad;
-    // Per pair, quad-major: entry m of particle p at ((m / 4) * stride + p) *
-    // 4 + m % 4, so the four list lanes of a particle read one quad and a
-    // wave's 64 lanes one contiguous run. The neighbour is its LDS index
-    // (16 bits) when the particle's tile is staged, else its slot (32 bits,
-    // a separate array only unstaged tiles touch); dW/r in its own array.
+    // Per pair, at the place the list layout gives (kListFixed, below): the
+    // neighbour is its LDS index (16 bits) when the particle's tile is
+    // staged, else its slot (32 bits, a separate array only unstaged tiles
+    // touch); dW/r in its own array.
     uint16_t* nbr16;
     uint32_t* nbr32;
     float* nbrg;
+    // density_div's scratch for the entries it cannot place while walking
+    // (those past the fixed rows, whose place depends on the wave's other
+    // counts): list_tail_cap(cap) a slot, slot-major. Shared by every list
+    // of a context (the adjoint's tapes included).
+    uint32_t* spill;
     uint32_t* nbr_n;  // [N]
     uint32_t cap;     // a multiple of 4
     uint32_t stride;  // >= the tiles' slot count
@@ -167,6 +171,33 @@ constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
 uint32_t solve_tiles(uint32_t n);
 // List lanes a particle in the sweeps (entry m belongs to lane m % 4).
 constexpr uint32_t kListLanes = 4;
+
+// Neighbour-list layout (cap x stride entries; stride a multiple of 16).
+// Entries m < 4 kListFixed (quad rows t = m / 4 < kListFixed) sit at fixed
+// places: ((m / 4) * stride + p) * 4 + m % 4, so a sweep loads them before
+// it knows the count and a wave's 64 lanes read one contiguous run a row.
+// The rest are packed: the 16 slots of a wave (16w .. 16w + 15) own a block
+// of 16 * list_tail_cap(cap) entries after the fixed rows, and slot p's
+// entries past the fixed ones lie consecutively from the summed tail
+// lengths of the wave's slots before p. Only the entries that exist take
+// memory and line fetches.
+constexpr uint32_t kListFixed = 6;
+constexpr uint32_t kListWave = 64 / kListLanes;  // slots a wave
+__host__ __device__ __forceinline__ uint32_t list_tail_cap(uint32_t cap) {
+    return cap > kListFixed * kListLanes ? cap - kListFixed * kListLanes : 0u;
+}
+__host__ __device__ __forceinline__ uint32_t list_tail_len(uint32_t nn) {
+    return nn > kListFixed * kListLanes ? nn - kListFixed * kListLanes : 0u;
+}
+// The entry of a fixed-row slot: m < 4 kListFixed.
+__host__ __device__ __forceinline__ uint32_t list_fixed(uint32_t stride, uint32_t p, uint32_t m) {
+    return ((m / kListLanes) * stride + p) * kListLanes + m % kListLanes;
+}
+// The first tail entry of slot p, whose tail starts `off` entries into its
+// wave's block.
+__host__ __device__ __forceinline__ uint32_t list_tail(uint32_t stride, uint32_t cap, uint32_t p, uint32_t off) {
+    return kListFixed * kListLanes * stride + (p / kListWave) * kListWave * list_tail_cap(cap) + off;
+}
 // Slots a solve tile holds (64, 128 or 256, fixed at build time). Smaller
 // tiles stage a little more halo per particle but put more independent
 // workgroups on a CU, which hides the sweeps' memory latency better.

@@ -69,6 +69,9 @@ struct DeviceAllocs {
         int rc;
         if ((rc = alloc(b.nbr16, e, false)) || (rc = alloc(b.nbr32, e, false)) || (rc = alloc(b.nbrg, e, false)))
             return rc;
+        // One spill area serves every list that copies these buffers.
+        const size_t tail = (size_t)list_tail_cap(b.cap) * b.stride;
+        if (!b.spill && tail && (rc = alloc(b.spill, tail, false))) return rc;
         return FLUID_OK;
     }
     void release() {

@@ -358,12 +358,17 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     HIP_TRY(hipMemcpy(raw32.data(), s->solve.nbr32, raw32.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(reg.data(), s->solve.regions, reg.size() * sizeof(TileRegion), hipMemcpyDeviceToHost));
+    uint32_t off = 0;  // the slot's tail offset in its wave's block
     for (size_t i = 0; i < n; ++i) {
+        if (i % kListWave == 0) off = 0;
         const TileRegion& R = reg[i / kSolveTile];
         for (size_t m = 0; m < cap; ++m) {
             uint32_t j = 0;
             if (m < cnt[i]) {
-                const size_t e = ((m / kListLanes) * stride + i) * kListLanes + m % kListLanes;
+                const size_t e = m < kListFixed * kListLanes
+                                     ? list_fixed((uint32_t)stride, (uint32_t)i, (uint32_t)m)
+                                     : list_tail((uint32_t)stride, (uint32_t)cap, (uint32_t)i, off) + m -
+                                           kListFixed * kListLanes;
                 if (R.staged) {
                     // An LDS index: back to the slot through the last stencil
                     // row that starts at or before it.
@@ -377,6 +382,7 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
             }
             nbr[i * cap + m] = j;
         }
+        off += list_tail_len(cnt[i]);
     }
     return FLUID_OK;
 }

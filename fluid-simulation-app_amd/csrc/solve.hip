@@ -252,10 +252,10 @@ __device__ __forceinline__ LaneId lane_id(uint32_t tile, uint32_t slot_end) {
 
 
 // Lane `lane` of particle p walks its share of the list in order (entries
-// lane, lane + 4, ...: quad t of the list holds entry 4t + lane at quad
-// offset gid), calling f(neighbour, dW/r): the neighbour is an LDS index in
-// a staged tile, a slot otherwise. The quad's address is uniform across the
-// wave; kPrefetch quads are loaded before they are used.
+// lane, lane + 4, ...), calling f(neighbour, dW/r): the neighbour is an LDS
+// index in a staged tile, a slot otherwise. The entries of the fixed rows
+// come with the head; the rest, from the wave's tail block, kPrefetch quads
+// at a time.
 template <bool STAGED>
 __device__ __forceinline__ uint32_t list_at(const SolveBufs& b, uint32_t e) {
     return STAGED ? (uint32_t)b.nbr16[e] : b.nbr32[e];
@@ -265,13 +265,22 @@ __device__ __forceinline__ uint32_t list_at(const SolveBufs& b, uint32_t e) {
 // sweeps' own loads need no branch.
 __device__ __forceinline__ uint32_t own_slot(const LaneId& id) { return id.live ? id.p : id.tile * kTile; }
 
-// The particle's entry count and the lane's first kHead quads. A sweep loads
-// them before it stages its tile, so their latency overlaps the staging. The
-// loads are unconditional: quads past the count, and those of a lane past the
-// scene's end, lie inside the list and are never used.
-constexpr uint32_t kHead = 8;
+// Where a particle's tail starts in its wave's block: the summed tail lengths
+// of the wave's earlier particles. Every lane of the wave takes part, the
+// four lanes of a particle with its count, a lane past the scene's end with 0.
+__device__ __forceinline__ uint32_t tail_offset(uint32_t nn) {
+    const uint32_t len = list_tail_len(nn);
+    return wave_inclusive_sum((__lane_id() & (kLanes - 1)) == 0 ? len : 0u) - len;
+}
+
+// The particle's entry count and the lane's entries in the kHead fixed rows.
+// A sweep loads them before it stages its tile, so their latency overlaps
+// the staging. The loads are unconditional: rows past the count, and those
+// of a lane past the scene's end, lie inside the list and are never used.
+constexpr uint32_t kHead = kListFixed;
 struct ListHead {
-    uint32_t nn;  // the particle's entries (its lane's share is taken later)
+    uint32_t nn;    // the particle's entries (its lane's share is taken later)
+    uint32_t tail;  // the list entry of the lane's first entry past the head
     uint32_t j[kHead];
     float g[kHead];
 };
@@ -279,6 +288,7 @@ template <bool STAGED>
 __device__ __forceinline__ ListHead list_head(const SolveBufs& b, const LaneId& id) {
     ListHead h;
     h.nn = b.nbr_n[own_slot(id)];
+    h.tail = 0;
     const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
 #pragma unroll
     for (uint32_t u = 0; u < kHead; ++u) {
@@ -288,21 +298,24 @@ __device__ __forceinline__ ListHead list_head(const SolveBufs& b, const LaneId& 
     }
     return h;
 }
-// Calls f(j, g) over the lane's entries in order: the head, then any further
-// quads kPrefetch at a time.
+// Sets h.tail once the count has arrived; every lane of the wave calls it.
+__device__ __forceinline__ void list_tail_at(const SolveBufs& b, const LaneId& id, ListHead& h) {
+    h.tail = list_tail(b.stride, b.cap, id.p, tail_offset(id.live ? h.nn : 0u)) + id.lane;
+}
+// Calls f(j, g) over the lane's entries in order: the head, then the tail
+// kPrefetch quads at a time.
 template <bool STAGED, class F>
 __device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, const ListHead& h, F&& f) {
     const uint32_t cnt = id.live && h.nn > id.lane ? (h.nn - id.lane + kLanes - 1) / kLanes : 0u;
 #pragma unroll
     for (uint32_t u = 0; u < kHead; ++u)
         if (u < cnt) f(h.j[u], h.g[u]);
-    const uint32_t step = b.stride * kLanes;
     for (uint32_t t = kHead; t < cnt; t += kPrefetch) {
         uint32_t j[kPrefetch];
         float g[kPrefetch];
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch; ++u) {
-            const uint32_t e = (t + u < cnt ? t + u : cnt - 1u) * step + id.gid;
+            const uint32_t e = h.tail + ((t + u < cnt ? t + u : cnt - 1u) - kHead) * kLanes;
             j[u] = list_at<STAGED>(b, e);
             g[u] = b.nbrg[e];
         }
@@ -447,15 +460,20 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
     const uint32_t lbase = id.p * kLanes, lstep = b.stride * kLanes;
     const uint32_t T = walk_list(b);
     LdsU16* const mine = L.list + (threadIdx.x / kLanes) * T;
+    // Staged, entries below T go to LDS; unstaged, those of the fixed rows go
+    // to the list. The rest, up to the cap, wait in the spill area, slot-major
+    // from the first tail entry: their place depends on the wave's counts.
+    const uint32_t fixed = kListFixed * kLanes, low = STAGED ? T : fixed;
+    const uint32_t spill = id.p * list_tail_cap(b.cap) - fixed;
     auto append = [&](bool acc, uint32_t m, uint32_t j) {
         if (!acc) return;
-        if (STAGED) {
-            if (m < T)
+        if (m < low) {
+            if (STAGED)
                 mine[m] = (uint16_t)j;
-            else if (m < b.cap)
-                b.nbr16[(m / kLanes) * lstep + lbase + m % kLanes] = (uint16_t)j;
+            else
+                b.nbr32[(m / kLanes) * lstep + lbase + m % kLanes] = j;
         } else if (m < b.cap) {
-            b.nbr32[(m / kLanes) * lstep + lbase + m % kLanes] = j;
+            b.spill[spill + m] = j;
         }
     };
     uint32_t found = 0;
@@ -517,47 +535,60 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
     return found;
 }
 
-// Phase 2: lane l's share of the pair sums over the compact list (entries l,
-// l + 4, ...), storing each pair's dW/r; past the cap, the neighbours the
-// list dropped are found again by a global walk and summed by the same rule.
+// Phase 2: lane l's share of the pair sums over the list (entries l, l + 4,
+// ...), storing each pair's neighbour and dW/r at its place in the list
+// (`tail`: the lane's first entry past the fixed rows); past the cap, the
+// neighbours the list dropped are found again by a global walk and summed by
+// the same rule.
 template <bool STAGED>
 __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, f3 pi, f3 vi, uint32_t found,
-                                             uint32_t bx, uint32_t by, uint32_t bz,
+                                             uint32_t tail, uint32_t bx, uint32_t by, uint32_t bz,
                                              const uint32_t* __restrict__ starts, const float4* __restrict__ pos,
                                              const float4* __restrict__ vel, const PvLds& L, const SolveBufs& b,
                                              DensitySums& a) {
     const uint32_t nn = found < b.cap ? found : b.cap;
     const uint32_t cnt = nn > id.lane ? (nn - id.lane + kLanes - 1) / kLanes : 0u;
     const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
+    auto at = [&](uint32_t t) { return t < kListFixed ? t * step + id.gid : tail + (t - kListFixed) * kLanes; };
+    // The spill index of the lane's entry 4t + lane is spilled + 4t.
+    const uint32_t spilled = id.p * list_tail_cap(b.cap) - kListFixed * kLanes + id.lane;
     auto pair = [&](uint32_t t, uint32_t j) {
         f3 pj, vj;
         load_pv<STAGED>(L, pos, vel, j, pj, vj);
-        b.nbrg[t * step + id.gid] = density_pair(k, pi, vi, pj, vj, a);
+        b.nbrg[at(t)] = density_pair(k, pi, vi, pj, vj, a);
     };
     if (STAGED) {
-        // Entry 4t + lane: the first T/4 of the lane's come from the walk's
-        // LDS list and are stored to the list here; any further ones the walk
-        // stored to the list itself. Two loops, so the LDS one waits on no
-        // global load (or, through it, on the stores).
+        // The lane's first T/4 entries come from the walk's LDS list, any
+        // further ones from the spill area. Two loops, so the LDS one waits
+        // on no global load (or, through it, on the stores).
         const uint32_t T = walk_list(b);
         const LdsU16* const mine = L.list + (threadIdx.x / kLanes) * T + id.lane;
         const uint32_t in_lds = cnt < T / kLanes ? cnt : T / kLanes;
         for (uint32_t t = 0; t < in_lds; ++t) {
             const uint32_t j = mine[t * kLanes];
-            b.nbr16[t * step + id.gid] = (uint16_t)j;
+            b.nbr16[at(t)] = (uint16_t)j;
             pair(t, j);
         }
-        for (uint32_t t = in_lds; t < cnt; ++t) pair(t, b.nbr16[t * step + id.gid]);
+        for (uint32_t t = in_lds; t < cnt; ++t) {
+            const uint32_t j = b.spill[spilled + t * kLanes];
+            b.nbr16[at(t)] = (uint16_t)j;
+            pair(t, j);
+        }
     } else {
-        // The lane's first kHead entries load together (unconditionally:
-        // quads past the count lie inside the list and are not used).
+        // The fixed rows hold the lane's first kHead entries already; they
+        // load together (unconditionally: rows past the count lie inside the
+        // list and are not used). Further entries come from the spill area.
         uint32_t jh[kHead];
 #pragma unroll
         for (uint32_t u = 0; u < kHead; ++u) jh[u] = b.nbr32[(u < last ? u : last) * step + id.gid];
 #pragma unroll
         for (uint32_t u = 0; u < kHead; ++u)
             if (u < cnt) pair(u, jh[u]);
-        for (uint32_t t = kHead; t < cnt; ++t) pair(t, b.nbr32[t * step + id.gid]);
+        for (uint32_t t = kHead; t < cnt; ++t) {
+            const uint32_t j = b.spill[spilled + t * kLanes];
+            b.nbr32[at(t)] = j;
+            pair(t, j);
+        }
     }
     if (found > b.cap) {
         uint32_t seen = 0;
@@ -636,8 +667,10 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     // Phase 1's list entries, written by any lane of the quad, are read back
     // by lane m % 4 below.
     __syncthreads();
+    const uint32_t nn = id.live ? (found < b.cap ? found : b.cap) : 0u;
+    const uint32_t tail = list_tail(b.stride, b.cap, id.p, tail_offset(nn)) + id.lane;
     DensitySums a{id.lane == 0 ? k.sigma : 0.0f, 0.0f, 0.0f, 0.0f, mk(0.0f, 0.0f, 0.0f)};
-    if (id.live) density_sums<STAGED>(k, id, pi, vi, found, bx, by, bz, starts, pos, vel, L, b, a);
+    if (id.live) density_sums<STAGED>(k, id, pi, vi, found, tail, bx, by, bz, starts, pos, vel, L, b, a);
     const float w_sum = fold4(a.w_sum), rn = fold4(a.rn), gsq_sum = fold4(a.gsq), drho_sum = fold4(a.drho);
     const f3 gsum_sum = fold4(a.gsum);
     if (!id.live || id.lane != 0) return;
@@ -730,6 +763,7 @@ __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, co
     } else {
         early();
     }
+    list_tail_at(b, id, h);
     f3 dv = mk(0.0f, 0.0f, 0.0f);
     if (id.live) {
         pi = xyz(p4);
@@ -826,6 +860,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     } else {
         early();
     }
+    list_tail_at(b, id, h);
     if (!id.live) return;
     pi = xyz(p4);
     const f3 vi = xyz(v4);
@@ -1015,6 +1050,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
     } else {
         early();
     }
+    list_tail_at(b, id, h);
     float drho = 0.0f;
     if (id.live) {
         const f3 pi = xyz(p4);

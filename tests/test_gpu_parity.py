@@ -233,3 +233,34 @@ def test_global_gather_path_matches_staged_path(lds, monkeypatch):
 
 
 STATE = ("pos", "vel", "prev_vel", "prev_pressure", "temperature")
+
+
+@pytest.mark.parametrize("h_factor,cap,lds", [(1.5, 96, None), (1.5, 28, None), (1.35, 64, 8192),
+                                              (1.5, 96, 8192)])
+def test_dense_lists_are_bit_exact_with_the_oracle(h_factor, cap, lds, monkeypatch):
+    """Dense scenes (~60 to ~110 neighbours inside the block): the list
+    entries of the fixed rows, those packed past them, those past the 48 the
+    walk keeps in LDS, and the overflow past the cap, in staged tiles and
+    (8 KB of LDS) unstaged ones."""
+    if lds:
+        monkeypatch.setenv("FLUID_LDS_BYTES", str(lds))
+    sc = fa.scenes.small_block((12, 12, 12), seed=31, h_factor=h_factor)
+    step, refine = _step(sc)
+    with fa.Sim(sc.params, nbr_cap=cap) as s:
+        s.upload_state(sc.pos4)
+        s.substeps(step, 1, refine)
+        st = s.state()
+        nbr, nbr_n = s.neighbours(), s.grid()["nbr_n"]
+        counters = s.counters()
+    o = OracleSim(sc.params, nbr_cap=cap)
+    o.set_state(sc.pos4)
+    o.substep(step, refine)
+    onbr, on = o.nbr(), o.grid()["nbr_n"]
+    assert np.array_equal(nbr_n, on)
+    assert nbr_n.max() > min(48, cap - 1)  # past the LDS list, or up to a smaller cap
+    for i in range(sc.n):
+        assert np.array_equal(nbr[i, : nbr_n[i]], onbr[i, : on[i]]), i
+    ost = o.state()
+    for k in STATE:
+        assert np.array_equal(_bits(st[k]), _bits(ost[k])), k
+    assert counters == o.counters()

@@ -303,22 +303,28 @@ __device__ __forceinline__ void list_tail_at(const SolveBufs& b, const LaneId& i
     h.tail = list_tail(b.stride, b.cap, id.p, tail_offset(id.live ? h.nn : 0u)) + id.lane;
 }
 // Calls f(j, g) over the lane's entries in order: the head, then the tail
-// kPrefetch quads at a time.
-template <bool STAGED, class F>
+// kPrefetch quads at a time. EARLY issues the first tail quads before the
+// head is used, so their latency overlaps the head's pair math (where the
+// registers allow it without costing occupancy).
+template <bool STAGED, bool EARLY = false, class F>
 __device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, const ListHead& h, F&& f) {
     const uint32_t cnt = id.live && h.nn > id.lane ? (h.nn - id.lane + kLanes - 1) / kLanes : 0u;
-#pragma unroll
-    for (uint32_t u = 0; u < kHead; ++u)
-        if (u < cnt) f(h.j[u], h.g[u]);
-    for (uint32_t t = kHead; t < cnt; t += kPrefetch) {
-        uint32_t j[kPrefetch];
-        float g[kPrefetch];
+    uint32_t j[kPrefetch];
+    float g[kPrefetch];
+    auto load = [&](uint32_t t) {
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch; ++u) {
             const uint32_t e = h.tail + ((t + u < cnt ? t + u : cnt - 1u) - kHead) * kLanes;
             j[u] = list_at<STAGED>(b, e);
             g[u] = b.nbrg[e];
         }
+    };
+    if (EARLY && cnt > kHead) load(kHead);
+#pragma unroll
+    for (uint32_t u = 0; u < kHead; ++u)
+        if (u < cnt) f(h.j[u], h.g[u]);
+    for (uint32_t t = kHead; t < cnt; t += kPrefetch) {
+        if (!EARLY || t > kHead) load(t);
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch; ++u)
             if (t + u < cnt) f(j[u], g[u]);
@@ -730,7 +736,7 @@ template <bool STAGED>
 __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, const ListHead& h, f3 pi,
                                             float kdi, const float4* __restrict__ pos, const float4* lds) {
     f3 dv = mk(0.0f, 0.0f, 0.0f);
-    for_list<STAGED>(b, id, h, [&](uint32_t j, float g) {
+    for_list<STAGED, true>(b, id, h, [&](uint32_t j, float g) {
         float4 q;
         if (STAGED) {
             q = lds[j];
@@ -866,7 +872,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     const f3 vi = xyz(v4);
     const float rhoi = rho_i;
     const f3 pii = pi;
-    for_list<STAGED>(b, id, h, [&](uint32_t j, float gs) {
+    for_list<STAGED, true>(b, id, h, [&](uint32_t j, float gs) {
         f3 pj, vj;
         float rho_j, near_j, temp_j, mr_j;
         if (STAGED) {
@@ -1056,7 +1062,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
         const f3 pi = xyz(p4);
         vi = xyz(v4);
         const f3 vii = vi;
-        for_list<STAGED>(b, id, h, [&](uint32_t j, float g) {
+        for_list<STAGED, true>(b, id, h, [&](uint32_t j, float g) {
             f3 pj, vj;
             load_pv<STAGED>(L, pos, vel, j, pj, vj);
             drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));

@@ -176,17 +176,40 @@ __device__ __forceinline__ float kernel_q(const SceneK& k, float q) {
     float t = 2.0f - q;
     return k.sigma * 0.25f * t * t * t;
 }
-// sim_solve.wgsl:184-196: the scalar dW/r that multiplies x.
-__device__ __forceinline__ float grad_scale(const SceneK& k, float r, float q) {
-    float dw;
-    if (q < 1.0f) {
-        dw = k.sigma_h * (-3.0f * q + 2.25f * q * q);
-    } else {
-        float t = 2.0f - q;
-        dw = k.sigma_h * (-0.75f * t * t);
-    }
-    return dw / r;
+// Correctly rounded square root and quotient for the pair loops, in fewer
+// instructions than the general ones. The general sequences are the
+// hardware estimate and a residual correction, wrapped in operand scaling
+// and special-value fixups; here these are identity, so the cores alone give
+// the same bits:
+// - sqrt_rn(x) for x = 0 or x >= 2^-96 (a positive x below that, which only
+//   coincident particles give, returns some value under 2^-47, which every
+//   caller treats as it treats 0);
+// - div_rn(a, b) for a normal b and |a|, |b|, |a / b| well inside 2^+-64
+//   (or a = 0, which returns +0: the callers add such a quotient, times
+//   finite factors, to sums that start at +0).
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __int_as_float(__float_as_int(s) - 1), up = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-dn, s, x) <= 0.0f ? dn : s;
+    r = __builtin_fmaf(-up, s, x) > 0.0f ? up : r;
+    return r;
 }
+__device__ __forceinline__ float div_rn(float a, float b) {
+    const float r0 = __builtin_amdgcn_rcpf(b);
+    const float r1 = __builtin_fmaf(__builtin_fmaf(-b, r0, 1.0f), r0, r0);
+    const float q0 = a * r1;
+    const float q1 = __builtin_fmaf(__builtin_fmaf(-b, q0, a), r1, q0);
+    return __builtin_fmaf(__builtin_fmaf(-b, q1, a), r1, q1);
+}
+
+// sim_solve.wgsl:184-196: dW/dq, the numerator of the scalar dW/r that
+// multiplies x.
+__device__ __forceinline__ float grad_dw(const SceneK& k, float q) {
+    if (q < 1.0f) return k.sigma_h * (-3.0f * q + 2.25f * q * q);
+    float t = 2.0f - q;
+    return k.sigma_h * (-0.75f * t * t);
+}
+__device__ __forceinline__ float grad_scale(const SceneK& k, float r, float q) { return grad_dw(k, q) / r; }
 __device__ __forceinline__ f3 grad_q(const SceneK& k, f3 x, float r, float q) {
     if (r < 1e-9f) return mk(0.0f, 0.0f, 0.0f);
     return scl(grad_scale(k, r, q), x);

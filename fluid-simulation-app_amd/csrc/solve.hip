@@ -341,12 +341,12 @@ struct DensitySums {
 __device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, f3 pj, f3 vj, DensitySums& a) {
     const f3 x = sub(pi, pj);
     const float r2 = dot3(x, x);
-    const float r = sqrtf(r2);
+    const float r = sqrt_rn(r2);
     const float q = r * k.inv_h;
     a.w_sum += kernel_q(k, q);
     const float wn = wmax(1.0f - q, 0.0f);
     a.rn += wn * wn * wn;
-    const float g = r < 1e-9f ? 0.0f : grad_scale(k, r, q);
+    const float g = r < 1e-9f ? 0.0f : div_rn(grad_dw(k, q), r);
     const f3 gw = r < 1e-9f ? mk(0.0f, 0.0f, 0.0f) : scl(g, x);
     a.gsum = add(a.gsum, gw);
     a.gsq += dot3(gw, gw);
@@ -859,7 +859,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
             [&](uint32_t idx, uint32_t it) {
                 L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
                 L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
-                L.t[idx] = make_float2(tt[it], k.mass / dd[it]);
+                L.t[idx] = make_float2(tt[it], div_rn(k.mass, dd[it]));
             },
             early);
         __syncthreads();
@@ -890,19 +890,19 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
             rho_j = b.density[j];
             near_j = b.rho_near[j];
             temp_j = temperature[j];
-            mr_j = k.mass / rho_j;
+            mr_j = div_rn(k.mass, rho_j);
         }
         const f3 x = sub(pii, pj);
         const float r2 = dot3(x, x);
-        const float rr = sqrtf(r2);
+        const float rr = sqrt_rn(r2);
         const f3 g = scl(gs, x);
-        const float f = dot3(x, g) / (r2 + k.eps_h2);
-        const float pair = k.mass / (rhoi * rho_j) * f;
+        const float f = div_rn(dot3(x, g), r2 + k.eps_h2);
+        const float pair = div_rn(k.mass, rhoi * rho_j) * f;
         const f3 dv = sub(vi, vj);
         const float wn = wmax(1.0f - rr * k.inv_h, 0.0f);
         // x / max(r, 1e-5) and x / max(r, 1e-9) (sim_solve.wgsl:559, 567-568)
         // as x times one reciprocal of r, shared when r >= 1e-5.
-        const float inv_rg = 1.0f / wmax(rr, 1e-9f);
+        const float inv_rg = div_rn(1.0f, wmax(rr, 1e-9f));
         const float inv_rm = rr >= 1e-5f ? inv_rg : 1.0f / 1e-5f;
         const float sn = (w_i + near_j) * wn * wn;
         F.nearf = add(F.nearf, mk(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm)));
@@ -910,7 +910,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         F.blend = sub(F.blend, scl(mr_j * kernel_q(k, rr * k.inv_h), dv));
         F.heat += k.heat_cond * pair * (temp_i - temp_j);
         F.heat -= k.heat_visc * pair * dot3(dv, dv);
-        const float corr = 2.0f * k.rho0 / (rhoi + rho_j);
+        const float corr = div_rn(2.0f * k.rho0, rhoi + rho_j);
         const float cm = corr * k.mass * cohesion(rr, k.coh_c, k.coh_k, k.coh_near);
         F.tension = sub(F.tension, mk(cm * x.x * inv_rg, cm * x.y * inv_rg, cm * x.z * inv_rg));
     });

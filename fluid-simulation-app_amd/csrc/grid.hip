@@ -182,26 +182,38 @@ __global__ __launch_bounds__(kBlock) void k_grid_place(uint32_t n, const uint32_
 
 // A slot's particle takes the final slot start + (particles of its cell with
 // a smaller resting index): the stable counting-sort order, whatever order
-// the atomics of grid_count handed out.
+// the atomics of grid_count handed out. The particle's records load first,
+// in flight while the rank's chain of loads (its cell, the cell's slot range,
+// the cell's entries eight at a time) completes; every store follows every
+// load, so no load waits on a store.
 __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32_t* __restrict__ perm_tmp,
                                                         const uint32_t* __restrict__ cellid,
                                                         const uint32_t* __restrict__ starts, Records rest,
                                                         Records work, uint32_t* __restrict__ perm) {
     uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
-    uint32_t i = perm_tmp[s];
-    uint32_t c = cellid[i];
-    uint32_t st = starts[c], en = starts[c + 1];
+    const uint32_t i = perm_tmp[s];
+    const float4 p = rest.pos[i], v = rest.vel[i], pv = rest.prev_vel[i];
+    const float pp = rest.prev_pressure[i], tt = rest.temperature[i];
+    const uint32_t id = rest.ids ? rest.ids[i] : 0u;
+    const uint32_t c = cellid[i];
+    const uint32_t st = starts[c], en = starts[c + 1];
     uint32_t r = 0;
-    for (uint32_t t = st; t < en; ++t) r += perm_tmp[t] < i ? 1u : 0u;
-    uint32_t f = st + r;
+    for (uint32_t t = st; t < en; t += 8) {
+        uint32_t w[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) w[u] = perm_tmp[t + u < en ? t + u : en - 1u];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) r += (t + u < en && w[u] < i) ? 1u : 0u;
+    }
+    const uint32_t f = st + r;
     perm[f] = i;
-    work.pos[f] = rest.pos[i];
-    work.vel[f] = rest.vel[i];
-    work.prev_vel[f] = rest.prev_vel[i];
-    work.prev_pressure[f] = rest.prev_pressure[i];
-    work.temperature[f] = rest.temperature[i];
-    if (rest.ids) work.ids[f] = rest.ids[i];
+    work.pos[f] = p;
+    work.vel[f] = v;
+    work.prev_vel[f] = pv;
+    work.prev_pressure[f] = pp;
+    work.temperature[f] = tt;
+    if (rest.ids) work.ids[f] = id;
 }
 
 }  // namespace

@@ -1111,19 +1111,20 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
     const uint32_t i = tile * kTile + threadIdx.x;
     bool clamped_speed = false;
     if (i < k.slot_end) {
-        const float4 was = work.vel[i];
+        // Every load first: a load issued after a store would wait for it.
+        const float4 was = work.vel[i], pv = work.prev_vel[i], p4 = work.pos[i];
+        const float rho = b.density[i], pr = b.pressure[i];
+        const float pp = work.prev_pressure[i], t = work.temperature[i];
+        const uint32_t id = rest.ids ? work.ids[i] : 0u;
         f3 v = xyz(was);
         const float speed = sqrtf(dot3(v, v));
-        const float rho = b.density[i];
         const float ceiling = rho < 0.5f * k.rho0 ? 3.0f * s.v_clamp : s.v_clamp;
         if (speed > ceiling) {
             v = scl(ceiling / speed, v);
             clamped_speed = true;
         }
-        const float4 pv = work.prev_vel[i];
         const f3 dvv = sub(v, xyz(pv));
         const float acc = sqrtf(dot3(dvv, dvv)) / s.dt;
-        const float4 p4 = work.pos[i];
         const float in[3] = {p4.x + s.dt * v.x, p4.y + s.dt * v.y, p4.z + s.dt * v.z};
         float vv[3] = {v.x, v.y, v.z};
         float out[3];
@@ -1133,15 +1134,13 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
             out[a] = c;
             if (c != in[a]) vv[a] = 0.0f;
         }
-        const float pr = b.pressure[i];
         rest.pos[i] = make_float4(out[0], out[1], out[2], 0.0f);
         rest.vel[i] = make_float4(vv[0], vv[1], vv[2], wmix(was.w, acc, s.chase));
         rest.prev_vel[i] = make_float4(vv[0], vv[1], vv[2], wmix(pv.w, pr, s.chase));
-        const float dp = pr - work.prev_pressure[i];
+        const float dp = pr - pp;
         rest.prev_pressure[i] = pr;
-        const float t = work.temperature[i];
         rest.temperature[i] = t + t * EXPANSION / (rho * HEAT_CAPACITY) * dp;
-        if (rest.ids) rest.ids[i] = work.ids[i];
+        if (rest.ids) rest.ids[i] = id;
     }
     // One atomic per wave for the clamp count, into one of the scene's
     // spread words (by tile).

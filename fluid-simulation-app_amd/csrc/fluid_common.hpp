@@ -347,6 +347,17 @@ __device__ __forceinline__ N wall_fill_t(const SceneK& k, N px, N py, N pz) {
     fill -= wedge_sum_t(lx, ly, lz, hx, hy, hz);
     return fill;
 }
+// Whether a wall term can be nonzero at pos. Every wall term (density,
+// wedges, gradient, adhesion) of a face is 0 once its t = distance x 1/h is
+// at least 2; this is false only when the nearest face's distance times the
+// smaller of the two 1/h scalings the terms use rounds to at least
+// 2 (1 + 2^-22), so every t, under either scaling, rounds to at least 2 and
+// skipping the terms gives the same +0 they would.
+__device__ __forceinline__ bool near_walls(const SceneK& k, f3 p) {
+    const float d = wmin(wmin(wmin(p.x - k.wall_lo[0], k.wall_hi[0] - p.x), wmin(p.y - k.wall_lo[1], k.wall_hi[1] - p.y)),
+                         wmin(p.z - k.wall_lo[2], k.wall_hi[2] - p.z));
+    return d * wmin(k.inv_h, k.inv_h_wall) < 2.000001f;
+}
 // sim_solve.wgsl:349-361
 __device__ __forceinline__ f3 wall_adh_sum(const SceneK& k, f3 pos) {
     float ih = k.inv_h_wall;

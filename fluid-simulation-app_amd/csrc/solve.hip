@@ -631,15 +631,22 @@ __device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, 
     const float gsq = k.mass * k.mass * gsq_sum;
     float drho = k.mass * drho_sum;
     f3 gsum = scl(k.mass, gsum_sum);
-    const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h, (pi.z - k.wall_lo[2]) * k.inv_h);
-    const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h, (k.wall_hi[2] - pi.z) * k.inv_h);
+    // The wall terms, which a wave with no particle near a wall skips (they
+    // would all be +0).
     float fill = 0.0f;
-    fill += wall_density(tl.x) + wall_density(th.x);
-    fill += wall_density(tl.y) + wall_density(th.y);
-    fill += wall_density(tl.z) + wall_density(th.z);
-    fill -= wedge_sum(tl, th);
+    f3 wg = mk(0.0f, 0.0f, 0.0f);
+    if (near_walls(k, pi)) {
+        const f3 tl = mk((pi.x - k.wall_lo[0]) * k.inv_h, (pi.y - k.wall_lo[1]) * k.inv_h,
+                         (pi.z - k.wall_lo[2]) * k.inv_h);
+        const f3 th = mk((k.wall_hi[0] - pi.x) * k.inv_h, (k.wall_hi[1] - pi.y) * k.inv_h,
+                         (k.wall_hi[2] - pi.z) * k.inv_h);
+        fill += wall_density(tl.x) + wall_density(th.x);
+        fill += wall_density(tl.y) + wall_density(th.y);
+        fill += wall_density(tl.z) + wall_density(th.z);
+        fill -= wedge_sum(tl, th);
+        wg = wall_grad_sum(k, pi);
+    }
     rho += k.rho0 * fill;
-    const f3 wg = wall_grad_sum(k, pi);
     b.wall_grad[i] = make_float4(wg.x, wg.y, wg.z, 0.0f);
     gsum = add(gsum, wg);
     b.density[i] = rho;
@@ -940,7 +947,7 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const floa
     const float heat = fold4(F.heat);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
-        const f3 adh = scl(k.beta * k.rho0, wall_adh_sum(k, pi));
+        const f3 adh = scl(k.beta * k.rho0, near_walls(k, pi) ? wall_adh_sum(k, pi) : mk(0.0f, 0.0f, 0.0f));
         const f3 acc = add(add(add(visc, scl(k.k_near, nearf)), scl(k.gamma, tension)), adh);
         b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
         b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);

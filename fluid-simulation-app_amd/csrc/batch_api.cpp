@@ -22,6 +22,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -34,6 +35,9 @@ using namespace fluid;
 namespace {
 
 constexpr int kRing = 2;
+// Slots a scene group of the sweeps holds: a group's neighbour lists and
+// fields (~250 bytes a slot) fit the 256 MB memory-side cache.
+constexpr size_t kGroupSlots = 786432;
 
 struct Ring {
     StepK* host = nullptr;   // pinned
@@ -48,6 +52,8 @@ struct FluidBatch {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, tiles = 0;
+    // Tile boundaries of the scene groups the sweeps run by (empty: one).
+    std::vector<uint32_t> groups;
     std::vector<SceneK> ks;
     std::vector<uint32_t> padded;
     SceneK* d_ks = nullptr;
@@ -149,6 +155,24 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     B->slots = (uint32_t)slots;
     B->cells = (uint32_t)cells;
     B->tiles = (uint32_t)tile_scene.size();
+    // Groups of whole scenes of about kGroupSlots slots each (FLUID_BATCH_GROUP
+    // overrides it; 0 runs the sweeps over the whole batch at once): a group's
+    // list and fields stay in the memory-side cache from one sweep to the
+    // next, where the whole batch's would not.
+    size_t group_slots = kGroupSlots;
+    if (const char* env = getenv("FLUID_BATCH_GROUP")) group_slots = (size_t)strtoull(env, nullptr, 10);
+    if (group_slots && B->slots > group_slots) {
+        B->groups.push_back(0);
+        size_t acc = 0;
+        for (uint32_t s = 0; s < scenes; ++s) {
+            acc += B->padded[s];
+            if (acc >= group_slots && s + 1 < scenes) {
+                B->groups.push_back(B->ks[s + 1].slot_begin / kSolveTile);
+                acc = 0;
+            }
+        }
+        B->groups.push_back(B->tiles);
+    }
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     const size_t n = B->slots, c = B->cells, S = scenes;
@@ -231,7 +255,9 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
         launch_solve_tolerance(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
                                B->tol, B->d_pass_limit, B->t());
     else
-        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t());
+        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(),
+                     B->groups.empty() ? nullptr : B->groups.data(),
+                     B->groups.empty() ? 0u : (uint32_t)B->groups.size() - 1u);
 }
 
 int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) {

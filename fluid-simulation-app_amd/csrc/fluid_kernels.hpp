@@ -26,11 +26,12 @@ struct GridGeom {
 // the scene's pass count (everything but the density passes).
 constexpr uint32_t kEveryPass = 0xffffffffu;
 
-// `flip` reverses a launch's tile order (see tile_index in solve.hip).
+// `flip` reverses a launch's tile order, and a launch covers the tiles from
+// `tile0` on (see tile_index in solve.hip).
 struct OneScene {
     SceneK k;
     StepK s;
-    uint32_t flip = 0;
+    uint32_t flip = 0, tile0 = 0;
     __device__ __forceinline__ uint32_t index(uint32_t) const { return 0; }
     __device__ __forceinline__ const SceneK& scene(uint32_t) const { return k; }
     __device__ __forceinline__ const StepK& step(uint32_t) const { return s; }
@@ -55,7 +56,7 @@ struct SceneTable {
     // [scenes] density passes each scene runs this substep, lowered when it
     // converges (tolerance mode); null: the Step's refine count.
     const uint32_t* pass_limit;
-    uint32_t flip = 0;
+    uint32_t flip = 0, tile0 = 0;
     __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile]; }
     __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[tile_scene[tile]]; }
     __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[tile_scene[tile]]; }
@@ -267,9 +268,14 @@ void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32
 // min(passes, its refine count) density passes.
 void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer);
+// A batch may run the sweeps from density_div to the last den_apply group
+// by group of whole scenes: group g is tiles [groups[g], groups[g + 1]), and
+// ngroups = 0 means one group. Scenes do not interact, so the results are
+// the same; a group's working set fits the memory-side cache, which the
+// whole batch's does not.
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                  KernelTimer* timer);
+                  KernelTimer* timer, const uint32_t* groups = nullptr, uint32_t ngroups = 0);
 // The plain solve writing each stage's velocities and kd to arrays of their
 // own (the adjoint's taped forward): vel[0] enters div_apply (the working
 // velocities the grid build wrote), vel[1] leaves it, vel[2] leaves

@@ -59,7 +59,7 @@ __host__ __device__ __forceinline__ uint32_t stage_slots(const SolveBufs& b) { r
 template <class Sc>
 __device__ __forceinline__ uint32_t tile_index(const Sc& sc) {
     const uint32_t t = xcd_tile(blockIdx.x, gridDim.x);
-    return sc.flip ? gridDim.x - 1u - t : t;
+    return sc.tile0 + (sc.flip ? gridDim.x - 1u - t : t);
 }
 
 __device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
@@ -1247,12 +1247,16 @@ struct NoCheck {
 // updated in place: vel[0] enters div_apply, vel[1] leaves it, vel[2] leaves
 // forces_den_apply, vel[3 + r] leaves den_apply of pass r; kd[0] is the
 // divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
+// With tile groups, the sweeps from density_div to the last den_apply run
+// group by group (tile regions and integrate cover every tile at once).
 template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
-                  const Check& check = Check(), const StageBufs* T = nullptr) {
+                  const Check& check = Check(), const StageBufs* T = nullptr, const uint32_t* groups = nullptr,
+                  uint32_t ngroups = 0) {
     if (n == 0) return;
-    const dim3 tiles(solve_tiles(n)), sweep(kSweepThreads);
+    const uint32_t ntiles = solve_tiles(n);
+    const dim3 sweep(kSweepThreads);
     const size_t lds_kappa = stage_slots(b) * kKappaEntry;
     auto V = [&](uint32_t i) { return T ? T->vel[i] : work.vel; };
     auto KB = [&](uint32_t i) {
@@ -1260,33 +1264,44 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         if (T) c.kd = T->kd[i];
         return c;
     };
-    Records w0 = work;
-    w0.vel = V(0);
-    density_only(st, sc, n, starts, w0, KB(0), timer);
-    Sc rev = sc;  // the reverse-order launches
-    rev.flip = 1;
-    const Span td = span(timer, KERNEL_DIV_APPLY);
-    hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass, work.pos,
-                          V(0), V(1), KB(0));
-    const Span tf = span(timer, KERNEL_FORCES_EVAL);
-    hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z, 0, sc,
-                          work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
-    const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
-    hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev, work.pos, V(1),
-                          V(2), work.temperature, KB(1));
-    for (uint32_t r = 0; r < passes; ++r) {
-        const Span tk = span(timer, KERNEL_DEN_KAPPA);
-        hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, tk.a, tk.z, 0, sc, r,
-                              work.pos, V(2 + r), KB(2 + r));
-        check(st, r);
-        const Span tp = span(timer, KERNEL_DEN_APPLY);
-        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r, work.pos,
-                              V(2 + r), V(3 + r), KB(2 + r));
+    const uint32_t ng = groups && ngroups ? ngroups : 1u;
+    for (uint32_t g = 0; g < ng; ++g) {
+        const uint32_t t0 = ngroups ? groups[g] : 0u, t1 = ngroups ? groups[g + 1] : ntiles;
+        if (t1 <= t0) continue;
+        const dim3 tiles(t1 - t0);
+        Sc fwd = sc;
+        fwd.tile0 = t0;
+        Sc rev = fwd;  // the reverse-order launches
+        rev.flip = 1;
+        const Span tdd = span(timer, KERNEL_DENSITY_DIV);
+        if (g == 0)
+            hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd.a, nullptr, 0,
+                                  sc, starts, work.pos, b.regions, ntiles, b.stage_cap);
+        hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), st, g == 0 ? nullptr : tdd.a,
+                              tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
+        const Span td = span(timer, KERNEL_DIV_APPLY);
+        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
+                              work.pos, V(0), V(1), KB(0));
+        const Span tf = span(timer, KERNEL_FORCES_EVAL);
+        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z, 0, fwd,
+                              work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
+        const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
+        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev, work.pos, V(1),
+                              V(2), work.temperature, KB(1));
+        for (uint32_t r = 0; r < passes; ++r) {
+            const Span tk = span(timer, KERNEL_DEN_KAPPA);
+            hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, tk.a, tk.z, 0, fwd, r,
+                                  work.pos, V(2 + r), KB(2 + r));
+            check(st, r);
+            const Span tp = span(timer, KERNEL_DEN_APPLY);
+            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r, work.pos,
+                                  V(2 + r), V(3 + r), KB(2 + r));
+        }
     }
     Records wl = work;
     wl.vel = V(passes + 2);
     const Span ti = span(timer, KERNEL_INTEGRATE);
-    hipExtLaunchKernelGGL(k_integrate<Sc>, dim3(solve_tiles(n)), dim3(kTile), 0, st, ti.a, ti.z, 0, sc, wl, rest, b);
+    hipExtLaunchKernelGGL(k_integrate<Sc>, dim3(ntiles), dim3(kTile), 0, st, ti.a, ti.z, 0, sc, wl, rest, b);
 }
 
 void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
@@ -1305,12 +1320,12 @@ void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint
 
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                  KernelTimer* timer) {
+                  KernelTimer* timer, const uint32_t* groups, uint32_t ngroups) {
     SolveBufs fixed = b;
     fixed.residual = nullptr;
     SceneTable t = sc;
     t.pass_limit = nullptr;
-    solve(st, t, slots, starts, work, rest, fixed, passes, timer);
+    solve(st, t, slots, starts, work, rest, fixed, passes, timer, NoCheck(), nullptr, groups, ngroups);
 }
 
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,

@@ -81,6 +81,34 @@ def test_batch_matches_lone_sims_bit_for_bit():
             assert np.array_equal(_bits(stats[i]), _bits(ref["stats"])), sc.name
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_scene_groups_give_the_bits_of_one_group(graph, monkeypatch):
+    """FLUID_BATCH_GROUP runs the sweeps group by group of whole scenes
+    (here a group closes after 2,048 slots, so most scenes are a group of
+    their own); scenes do not interact, so every word matches one group."""
+    scenes = _scenes()
+    scale = [1.0, 0.1, 1.0, 0.5, 1.0]
+    plan = [_steps(scenes, k, scale) for k in range(3)]
+
+    def run(group):
+        monkeypatch.setenv("FLUID_BATCH_GROUP", str(group))
+        with fa.Batch([sc.params for sc in scenes]) as b:
+            b.use_graph(graph)
+            for i, sc in enumerate(scenes):
+                b.upload_state(i, sc.pos4)
+            for steps, refine in plan:
+                b.substep(steps, refine)
+            return [(b.state(i), {k: b.field(i, k) for k in ("density", "kd", "pressure", "accel")})
+                    for i in range(len(scenes))]
+
+    one, grouped = run(0), run(2048)
+    for i in range(len(scenes)):
+        for key in STATE:
+            assert np.array_equal(_bits(grouped[i][0][key]), _bits(one[i][0][key])), (i, key)
+        for key in one[i][1]:
+            assert np.array_equal(_bits(grouped[i][1][key]), _bits(one[i][1][key])), (i, key)
+
+
 def test_graph_replays_match_eager_substeps():
     scenes = _scenes()[:4]
     plan = [_steps(scenes, k) for k in range(4)]

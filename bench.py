@@ -176,7 +176,7 @@ def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=64):
                       f"substep, {dt:.1f} s"}
 
 
-def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=4, seed=0, autotune=True):
+def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0, autotune=True):
     """The batched configuration: `scenes` ensemble boxes stepped as one
     batch (one launch per kernel per substep, replayed from a hipGraph),
     against looping the plain step over the same scenes. Each scene takes its

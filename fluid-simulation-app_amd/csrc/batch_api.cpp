@@ -35,9 +35,12 @@ using namespace fluid;
 namespace {
 
 constexpr int kRing = 2;
-// Slots a scene group of the sweeps holds: a group's neighbour lists and
-// fields (~250 bytes a slot) fit the 256 MB memory-side cache.
-constexpr size_t kGroupSlots = 786432;
+// Slots a scene group of the sweeps holds. Two groups run at once (one on
+// each stream), and their neighbour lists and fields (~250 bytes a slot)
+// fit the 256 MB memory-side cache together (measured at 256 scenes: 262K,
+// 328K, 393K and 459K slots a group give 5.17, 5.07, 5.11 and 5.12 ms a
+// substep; one stream at 786K, 5.27 ms).
+constexpr size_t kGroupSlots = 327680;
 
 struct Ring {
     StepK* host = nullptr;   // pinned
@@ -52,8 +55,11 @@ struct FluidBatch {
     int device = 0;
     hipStream_t stream = nullptr;
     uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, tiles = 0;
-    // Tile boundaries of the scene groups the sweeps run by (empty: one).
+    // Tile boundaries of the scene groups the sweeps run by (empty: one), and
+    // the second stream the odd groups run on, with its fork and join events.
     std::vector<uint32_t> groups;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::vector<SceneK> ks;
     std::vector<uint32_t> padded;
     SceneK* d_ks = nullptr;
@@ -92,7 +98,11 @@ struct FluidBatch {
             if (r.host) (void)hipHostFree(r.host);
             if (r.done) (void)hipEventDestroy(r.done);
         }
+        if (stream2) (void)hipStreamSynchronize(stream2);
         mem.release();
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (stream2) (void)hipStreamDestroy(stream2);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -175,6 +185,14 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     }
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    // Two streams for the groups unless FLUID_BATCH_STREAMS=1 (both at the
+    // default priority).
+    const char* streams = getenv("FLUID_BATCH_STREAMS");
+    if (B->groups.size() > 2 && !(streams && strtoul(streams, nullptr, 10) == 1)) {
+        HIP_TRY(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
+    }
     const size_t n = B->slots, c = B->cells, S = scenes;
     int rc = FLUID_OK;
 #define A(ptr, cnt)                            \
@@ -251,13 +269,20 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
     (void)hipMemcpyAsync(r.dev, r.host, (size_t)B->scenes * sizeof(StepK), hipMemcpyHostToDevice, B->stream);
     const SceneTable sc = B->table(r);
     launch_grid_build(B->stream, sc, B->slots, B->cells, B->rest, B->work, B->grid, B->t());
-    if (B->tol.tol > 0.0f)
+    if (B->tol.tol > 0.0f) {
         launch_solve_tolerance(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
                                B->tol, B->d_pass_limit, B->t());
-    else
-        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(),
-                     B->groups.empty() ? nullptr : B->groups.data(),
-                     B->groups.empty() ? 0u : (uint32_t)B->groups.size() - 1u);
+    } else {
+        TileGroups g;
+        if (!B->groups.empty()) {
+            g.first = B->groups.data();
+            g.count = (uint32_t)B->groups.size() - 1u;
+            g.second = B->stream2;
+            g.fork = B->fork;
+            g.join = B->join;
+        }
+        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(), g);
+    }
 }
 
 int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) {

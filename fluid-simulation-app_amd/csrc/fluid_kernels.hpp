@@ -272,10 +272,18 @@ void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint
 // by group of whole scenes: group g is tiles [groups[g], groups[g + 1]), and
 // ngroups = 0 means one group. Scenes do not interact, so the results are
 // the same; a group's working set fits the memory-side cache, which the
-// whole batch's does not.
+// whole batch's does not. With a second stream (and two events to fork and
+// join it), odd groups run on it, so one group's last workgroups overlap
+// the next group's first.
+struct TileGroups {
+    const uint32_t* first = nullptr;
+    uint32_t count = 0;
+    hipStream_t second = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                  KernelTimer* timer, const uint32_t* groups = nullptr, uint32_t ngroups = 0);
+                  KernelTimer* timer, const TileGroups& groups = TileGroups());
 // The plain solve writing each stage's velocities and kd to arrays of their
 // own (the adjoint's taped forward): vel[0] enters div_apply (the working
 // velocities the grid build wrote), vel[1] leaves it, vel[2] leaves

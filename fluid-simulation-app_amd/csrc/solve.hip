@@ -1248,12 +1248,12 @@ struct NoCheck {
 // forces_den_apply, vel[3 + r] leaves den_apply of pass r; kd[0] is the
 // divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
 // With tile groups, the sweeps from density_div to the last den_apply run
-// group by group (tile regions and integrate cover every tile at once).
+// group by group (tile regions and integrate cover every tile at once), odd
+// groups on the second stream when there is one.
 template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
-                  const Check& check = Check(), const StageBufs* T = nullptr, const uint32_t* groups = nullptr,
-                  uint32_t ngroups = 0) {
+                  const Check& check = Check(), const StageBufs* T = nullptr, const TileGroups& G = TileGroups()) {
     if (n == 0) return;
     const uint32_t ntiles = solve_tiles(n);
     const dim3 sweep(kSweepThreads);
@@ -1264,39 +1264,49 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         if (T) c.kd = T->kd[i];
         return c;
     };
-    const uint32_t ng = groups && ngroups ? ngroups : 1u;
+    const uint32_t ng = G.first && G.count ? G.count : 1u;
+    const bool two = ng > 1 && G.second && G.fork && G.join;
+    const Span tdd0 = span(timer, KERNEL_DENSITY_DIV);
+    hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd0.a, nullptr, 0, sc,
+                          starts, work.pos, b.regions, ntiles, b.stage_cap);
+    if (two) {
+        (void)hipEventRecord(G.fork, st);
+        (void)hipStreamWaitEvent(G.second, G.fork, 0);
+    }
     for (uint32_t g = 0; g < ng; ++g) {
-        const uint32_t t0 = ngroups ? groups[g] : 0u, t1 = ngroups ? groups[g + 1] : ntiles;
+        const uint32_t t0 = G.count ? G.first[g] : 0u, t1 = G.count ? G.first[g + 1] : ntiles;
         if (t1 <= t0) continue;
+        hipStream_t s = two && (g & 1u) ? G.second : st;
         const dim3 tiles(t1 - t0);
         Sc fwd = sc;
         fwd.tile0 = t0;
         Sc rev = fwd;  // the reverse-order launches
         rev.flip = 1;
-        const Span tdd = span(timer, KERNEL_DENSITY_DIV);
-        if (g == 0)
-            hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd.a, nullptr, 0,
-                                  sc, starts, work.pos, b.regions, ntiles, b.stage_cap);
-        hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), st, g == 0 ? nullptr : tdd.a,
-                              tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
+        const Span tdd = g == 0 ? tdd0 : span(timer, KERNEL_DENSITY_DIV);
+        hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s, g == 0 ? nullptr : tdd.a, tdd.z, 0,
+                              fwd, starts, work.pos, V(0), KB(0));
         const Span td = span(timer, KERNEL_DIV_APPLY);
-        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
-                              work.pos, V(0), V(1), KB(0));
+        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, td.a, td.z, 0, rev, kEveryPass, work.pos,
+                              V(0), V(1), KB(0));
         const Span tf = span(timer, KERNEL_FORCES_EVAL);
-        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z, 0, fwd,
+        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, s, tf.a, tf.z, 0, fwd,
                               work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
         const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
-        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev, work.pos, V(1),
+        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, s, ta.a, ta.z, 0, rev, work.pos, V(1),
                               V(2), work.temperature, KB(1));
         for (uint32_t r = 0; r < passes; ++r) {
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
-            hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, st, tk.a, tk.z, 0, fwd, r,
+            hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0, fwd, r,
                                   work.pos, V(2 + r), KB(2 + r));
-            check(st, r);
+            check(s, r);
             const Span tp = span(timer, KERNEL_DEN_APPLY);
-            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r, work.pos,
+            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, rev, r, work.pos,
                                   V(2 + r), V(3 + r), KB(2 + r));
         }
+    }
+    if (two) {
+        (void)hipEventRecord(G.join, G.second);
+        (void)hipStreamWaitEvent(st, G.join, 0);
     }
     Records wl = work;
     wl.vel = V(passes + 2);
@@ -1320,12 +1330,12 @@ void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint
 
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                  KernelTimer* timer, const uint32_t* groups, uint32_t ngroups) {
+                  KernelTimer* timer, const TileGroups& groups) {
     SolveBufs fixed = b;
     fixed.residual = nullptr;
     SceneTable t = sc;
     t.pass_limit = nullptr;
-    solve(st, t, slots, starts, work, rest, fixed, passes, timer, NoCheck(), nullptr, groups, ngroups);
+    solve(st, t, slots, starts, work, rest, fixed, passes, timer, NoCheck(), nullptr, groups);
 }
 
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,

@@ -51,7 +51,7 @@ constexpr uint32_t kStagePad = 8;
 __host__ __device__ __forceinline__ uint32_t stage_slots(const SolveBufs& b) { return b.stage_cap + kStagePad; }
 
 // The workgroup's tile. Consecutive list sweeps run in opposite tile orders:
-// one sweep's working set (the neighbour list alone is ~170 bytes a
+// one sweep's working set (the neighbour list alone is ~200 bytes a
 // particle) is about the size of the 256 MB memory-side cache, and a sweep
 // that starts where the previous one ended finds its first tiles' data still
 // there, where the same order every time would evict each line just before

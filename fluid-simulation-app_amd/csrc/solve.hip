@@ -1249,7 +1249,7 @@ struct NoCheck {
 // divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
 // With tile groups, the sweeps from density_div to the last den_apply run
 // group by group (tile regions and integrate cover every tile at once), odd
-// groups on the second stream when there is one.
+// groups on the second stream when there is one and no kernel timer.
 template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
@@ -1265,7 +1265,9 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         return c;
     };
     const uint32_t ng = G.first && G.count ? G.count : 1u;
-    const bool two = ng > 1 && G.second && G.fork && G.join;
+    // A profiled substep keeps every group on one stream, so that each
+    // kernel's time is its own.
+    const bool two = ng > 1 && G.second && G.fork && G.join && !timer;
     const Span tdd0 = span(timer, KERNEL_DENSITY_DIV);
     hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd0.a, nullptr, 0, sc,
                           starts, work.pos, b.regions, ntiles, b.stage_cap);

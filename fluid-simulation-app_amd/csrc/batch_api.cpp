@@ -41,6 +41,7 @@ constexpr int kRing = 2;
 // 328K, 393K and 459K slots a group give 5.17, 5.07, 5.11 and 5.12 ms a
 // substep; one stream at 786K, 5.27 ms).
 constexpr size_t kGroupSlots = 327680;
+constexpr uint32_t kGroupStreams = 2;
 
 struct Ring {
     StepK* host = nullptr;   // pinned
@@ -56,10 +57,11 @@ struct FluidBatch {
     hipStream_t stream = nullptr;
     uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, tiles = 0;
     // Tile boundaries of the scene groups the sweeps run by (empty: one), and
-    // the second stream the odd groups run on, with its fork and join events.
+    // the extra streams the groups take in turn, with the fork and join events.
     std::vector<uint32_t> groups;
-    hipStream_t stream2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    uint32_t nextra = 0;
+    hipStream_t extra[kMaxGroupStreams - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxGroupStreams - 1] = {};
     std::vector<SceneK> ks;
     std::vector<uint32_t> padded;
     SceneK* d_ks = nullptr;
@@ -98,11 +100,13 @@ struct FluidBatch {
             if (r.host) (void)hipHostFree(r.host);
             if (r.done) (void)hipEventDestroy(r.done);
         }
-        if (stream2) (void)hipStreamSynchronize(stream2);
+        for (uint32_t i = 0; i < nextra; ++i) (void)hipStreamSynchronize(extra[i]);
         mem.release();
         if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (stream2) (void)hipStreamDestroy(stream2);
+        for (uint32_t i = 0; i < nextra; ++i) {
+            (void)hipEventDestroy(join[i]);
+            (void)hipStreamDestroy(extra[i]);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -185,13 +189,18 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     }
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    // Two streams for the groups unless FLUID_BATCH_STREAMS=1 (both at the
-    // default priority).
-    const char* streams = getenv("FLUID_BATCH_STREAMS");
-    if (B->groups.size() > 2 && !(streams && strtoul(streams, nullptr, 10) == 1)) {
-        HIP_TRY(hipStreamCreateWithFlags(&B->stream2, hipStreamNonBlocking));
+    // kGroupStreams streams for the groups (FLUID_BATCH_STREAMS overrides it,
+    // 1 to kMaxGroupStreams), all at the default priority.
+    uint32_t nstreams = kGroupStreams;
+    if (const char* env = getenv("FLUID_BATCH_STREAMS")) nstreams = (uint32_t)strtoul(env, nullptr, 10);
+    nstreams = nstreams < 1 ? 1 : (nstreams > kMaxGroupStreams ? kMaxGroupStreams : nstreams);
+    if (B->groups.size() > 2 && nstreams > 1) {
         HIP_TRY(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&B->join, hipEventDisableTiming));
+        for (uint32_t i = 0; i + 1 < nstreams; ++i) {
+            HIP_TRY(hipStreamCreateWithFlags(&B->extra[i], hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming));
+            B->nextra = i + 1;
+        }
     }
     const size_t n = B->slots, c = B->cells, S = scenes;
     int rc = FLUID_OK;
@@ -277,9 +286,12 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
         if (!B->groups.empty()) {
             g.first = B->groups.data();
             g.count = (uint32_t)B->groups.size() - 1u;
-            g.second = B->stream2;
+            g.streams = B->nextra;
             g.fork = B->fork;
-            g.join = B->join;
+            for (uint32_t i = 0; i < B->nextra; ++i) {
+                g.stream[i] = B->extra[i];
+                g.join[i] = B->join[i];
+            }
         }
         launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(), g);
     }

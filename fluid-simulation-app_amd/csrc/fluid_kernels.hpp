@@ -272,14 +272,17 @@ void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint
 // by group of whole scenes: group g is tiles [groups[g], groups[g + 1]), and
 // ngroups = 0 means one group. Scenes do not interact, so the results are
 // the same; a group's working set fits the memory-side cache, which the
-// whole batch's does not. With a second stream (and two events to fork and
-// join it), odd groups run on it, so one group's last workgroups overlap
-// the next group's first.
+// whole batch's does not. With more streams (an event to fork them and one
+// each to join them back), the groups go round them in turn, so one group's
+// last workgroups overlap the next group's first.
+constexpr uint32_t kMaxGroupStreams = 4;
 struct TileGroups {
     const uint32_t* first = nullptr;
     uint32_t count = 0;
-    hipStream_t second = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    uint32_t streams = 0;  // extra streams (0 = the caller's alone)
+    hipStream_t stream[kMaxGroupStreams - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kMaxGroupStreams - 1] = {};
 };
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,

@@ -1248,8 +1248,8 @@ struct NoCheck {
 // forces_den_apply, vel[3 + r] leaves den_apply of pass r; kd[0] is the
 // divergence kappa, kd[1] the warm start, kd[2 + r] den_kappa of pass r.
 // With tile groups, the sweeps from density_div to the last den_apply run
-// group by group (tile regions and integrate cover every tile at once), odd
-// groups on the second stream when there is one and no kernel timer.
+// group by group (tile regions and integrate cover every tile at once), in
+// turn over the extra streams when there are some and no kernel timer.
 template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
@@ -1267,18 +1267,19 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     const uint32_t ng = G.first && G.count ? G.count : 1u;
     // A profiled substep keeps every group on one stream, so that each
     // kernel's time is its own.
-    const bool two = ng > 1 && G.second && G.fork && G.join && !timer;
+    const uint32_t extra = ng > 1 && G.fork && !timer ? G.streams : 0u;
     const Span tdd0 = span(timer, KERNEL_DENSITY_DIV);
     hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd0.a, nullptr, 0, sc,
                           starts, work.pos, b.regions, ntiles, b.stage_cap);
-    if (two) {
+    if (extra) {
         (void)hipEventRecord(G.fork, st);
-        (void)hipStreamWaitEvent(G.second, G.fork, 0);
+        for (uint32_t i = 0; i < extra; ++i) (void)hipStreamWaitEvent(G.stream[i], G.fork, 0);
     }
     for (uint32_t g = 0; g < ng; ++g) {
         const uint32_t t0 = G.count ? G.first[g] : 0u, t1 = G.count ? G.first[g + 1] : ntiles;
         if (t1 <= t0) continue;
-        hipStream_t s = two && (g & 1u) ? G.second : st;
+        const uint32_t lane = g % (extra + 1u);
+        hipStream_t s = lane ? G.stream[lane - 1u] : st;
         const dim3 tiles(t1 - t0);
         Sc fwd = sc;
         fwd.tile0 = t0;
@@ -1306,9 +1307,9 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
                                   V(2 + r), V(3 + r), KB(2 + r));
         }
     }
-    if (two) {
-        (void)hipEventRecord(G.join, G.second);
-        (void)hipStreamWaitEvent(st, G.join, 0);
+    for (uint32_t i = 0; i < extra; ++i) {
+        (void)hipEventRecord(G.join[i], G.stream[i]);
+        (void)hipStreamWaitEvent(st, G.join[i], 0);
     }
     Records wl = work;
     wl.vel = V(passes + 2);

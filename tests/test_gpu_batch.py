@@ -81,14 +81,16 @@ def test_batch_matches_lone_sims_bit_for_bit():
             assert np.array_equal(_bits(stats[i]), _bits(ref["stats"])), sc.name
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_scene_groups_give_the_bits_of_one_group(graph, monkeypatch):
+@pytest.mark.parametrize("graph,streams", [(False, 2), (True, 2), (False, 1), (True, 4)])
+def test_scene_groups_give_the_bits_of_one_group(graph, streams, monkeypatch):
     """FLUID_BATCH_GROUP runs the sweeps group by group of whole scenes
     (here a group closes after 2,048 slots, so most scenes are a group of
-    their own); scenes do not interact, so every word matches one group."""
+    their own), in turn over FLUID_BATCH_STREAMS streams; scenes do not
+    interact, so every word matches one group."""
     scenes = _scenes()
     scale = [1.0, 0.1, 1.0, 0.5, 1.0]
     plan = [_steps(scenes, k, scale) for k in range(3)]
+    monkeypatch.setenv("FLUID_BATCH_STREAMS", str(streams))
 
     def run(group):
         monkeypatch.setenv("FLUID_BATCH_GROUP", str(group))

@@ -84,7 +84,7 @@ __device__ __forceinline__ PairGeom pair_geom(const SceneK& k, f3 xi, f3 xj, flo
     PairGeom P;
     P.r = sub(xi, xj);
     const float r2 = dot3(P.r, P.r);
-    const float rho = sqrtf(r2);
+    const float rho = sqrt_rn(r2);
     const float q = rho * k.inv_h;
     const float dwp = q < 1.0f ? k.sigma_h * (-3.0f + 4.5f * q) : k.sigma_h * (1.5f * (2.0f - q));
     P.g = g;
@@ -339,15 +339,15 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_viscosity(SceneK k, float mu
             const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
             const float r2 = dot3(P.r, P.r);
             const float den = r2 + k.eps_h2;
-            const float f = P.g * r2 / den;
+            const float f = div_rn(P.g * r2, den);
             const float rj = rho[j];
-            const float pair = k.mass / (ri * rj) * f;
+            const float pair = div_rn(k.mass, ri * rj) * f;
             const f3 dl = sub(la, ld3(lam_a, j));
             const float dldv = dot3(dl, sub(vi, ld3(vel, j)));
             s_v = add(s_v, scl(pair, dl));
             s_r += pair * dldv;
-            const float fp_over_r = P.hh * r2 / den + 2.0f * P.g * k.eps_h2 / (den * den);
-            s_x = add(s_x, scl(k.mass / (ri * rj) * fp_over_r * dldv, P.r));
+            const float fp_over_r = div_rn(P.hh * r2, den) + div_rn(2.0f * P.g * k.eps_h2, den * den);
+            s_x = add(s_x, scl(div_rn(k.mass, ri * rj) * fp_over_r * dldv, P.r));
         });
     }
     s_v = fold4(s_v);

@@ -104,7 +104,7 @@ struct FluidBatch {
         mem.release();
         if (fork) (void)hipEventDestroy(fork);
         for (uint32_t i = 0; i < nextra; ++i) {
-            (void)hipEventDestroy(join[i]);
+            if (join[i]) (void)hipEventDestroy(join[i]);
             (void)hipStreamDestroy(extra[i]);
         }
         if (stream) (void)hipStreamDestroy(stream);
@@ -198,8 +198,8 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
         HIP_TRY(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
         for (uint32_t i = 0; i + 1 < nstreams; ++i) {
             HIP_TRY(hipStreamCreateWithFlags(&B->extra[i], hipStreamNonBlocking));
+            B->nextra = i + 1;  // counted as soon as it exists, so a failure below still frees it
             HIP_TRY(hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming));
-            B->nextra = i + 1;
         }
     }
     const size_t n = B->slots, c = B->cells, S = scenes;

@@ -129,6 +129,10 @@ __global__ __launch_bounds__(kBlock) void k_stats_final_many(SceneTable sc, cons
                                                              float* __restrict__ stats) {
     const SceneK& k = sc.ks[blockIdx.x];
     const uint32_t n = k.slot_end - k.slot_begin;
+    if (n == 0) {  // an empty scene reports zeros, as a lone sim without particles does
+        if (threadIdx.x < 16) stats[(size_t)blockIdx.x * 16 + threadIdx.x] = 0.0f;
+        return;
+    }
     stats_final(n, stats_block_count(n), partials + (size_t)blockIdx.x * kStatsMaxBlocks * 16, counters + 2 * blockIdx.x,
                 clamps + (size_t)blockIdx.x * kClampWords, stats + (size_t)blockIdx.x * 16);
 }

@@ -1,7 +1,7 @@
 """The batched step through the C ABI: a batch of uneven scenes gives each
 scene the bits a lone Sim gives it (state, grid, fields, counters, stats),
 in the reference schedule, under hipGraph replay, in tolerance mode and after
-autotuning.
+autotuning. A scene without particles, alone or in a batch, steps as a no-op.
 """
 import numpy as np
 import pytest
@@ -216,3 +216,52 @@ def test_autotune_restores_state_and_keeps_results():
         ref = _lone(sc, [(plan[k][0][i], plan[k][1][i]) for k in range(2)])
         for key in STATE:
             assert np.array_equal(_bits(tuned[i][key]), _bits(ref[key])), (sc.name, key)
+
+
+def _empty_params(sc):
+    """sc's SimParams block with its particle count (bytes 28-31) set to 0."""
+    p = bytearray(sc.params)
+    p[28:32] = (0).to_bytes(4, "little")
+    return bytes(p)
+
+
+def test_an_empty_sim_steps_as_a_no_op():
+    sc = fa.scenes.small_block((4, 4, 4), seed=41)
+    (step,), (refine,) = _steps([sc], 0)
+    with fa.Sim(_empty_params(sc)) as s:
+        assert s.n == 0
+        s.upload_state(np.zeros((0, 4), np.float32))
+        s.substeps(step, 2, refine)
+        s.density_only(step)
+        s.frame(step, 1, refine)
+        st = s.state()
+        assert all(v.shape[0] == 0 for v in st.values())
+        g = s.grid()
+        assert not g["starts"].any()
+        assert s.counters() == (0, 0)
+        assert not s.read_stats().any()
+
+
+def test_an_empty_scene_in_a_batch_leaves_the_others_bit_exact():
+    scenes = [fa.scenes.small_block((5, 6, 7), seed=42), fa.scenes.small_block((4, 4, 4), seed=43),
+              fa.scenes.fill_block("fb_c", 700, 0.3, seed=44)]
+    params = [scenes[0].params, _empty_params(scenes[1]), scenes[2].params]
+    plan = [_steps(scenes, k) for k in range(2)]
+    with fa.Batch(params) as b:
+        b.upload_state(0, scenes[0].pos4)
+        b.upload_state(2, scenes[2].pos4)
+        for steps, refine in plan:
+            b.substep(steps, refine)
+        b.reduce_stats()
+        stats = b.read_stats()
+        counters = b.counters()
+        assert b.state(1)["pos"].shape[0] == 0
+        assert not stats[1].any()
+        assert tuple(int(x) for x in counters[1]) == (0, 0)
+        for i in (0, 2):
+            ref = _lone(scenes[i], [(plan[k][0][i], plan[k][1][i]) for k in range(2)])
+            st = b.state(i)
+            for key in STATE:
+                assert np.array_equal(_bits(st[key]), _bits(ref[key])), (i, key)
+            assert np.array_equal(b.grid(i)["perm"], ref["grid"]["perm"]), i
+            assert np.array_equal(_bits(stats[i]), _bits(ref["stats"])), i

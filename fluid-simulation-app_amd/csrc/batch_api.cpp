@@ -169,6 +169,10 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     B->slots = (uint32_t)slots;
     B->cells = (uint32_t)cells;
     B->tiles = (uint32_t)tile_scene.size();
+    if ((uint64_t)B->cap * std::max<uint64_t>(slots, kSolveTile) >= (1ull << 32)) {
+        set_error("neighbour list of cap %u x %llu slots exceeds 2^32", B->cap, (unsigned long long)slots);
+        return FLUID_ERR_ARG;
+    }
     // Groups of whole scenes of about kGroupSlots slots each (FLUID_BATCH_GROUP
     // overrides it; 0 runs the sweeps over the whole batch at once): a group's
     // list and fields stay in the memory-side cache from one sweep to the

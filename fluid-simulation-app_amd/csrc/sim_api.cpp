@@ -155,6 +155,14 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
         set_error("neighbour cap %u is not a multiple of 4 in [4, 256]", s->cap);
         return FLUID_ERR_ARG;
     }
+    // The list's 32-bit offsets bound cap x slots; refused before anything is
+    // allocated (the allocation repeats the check).
+    const uint64_t list_entries = (uint64_t)s->cap * std::max<uint32_t>(solve_tiles(s->n), 1) * kSolveTile;
+    if (list_entries >= (1ull << 32)) {
+        set_error("neighbour list of %llu entries (cap %u x %u particles) exceeds 2^32",
+                  (unsigned long long)list_entries, s->cap, s->n);
+        return FLUID_ERR_ARG;
+    }
     s->ntr = ntr;
     HIP_TRY(hipGetDevice(&s->device));
     HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

@@ -322,8 +322,8 @@ def main():
         pmc_summary(args.pmc_summary)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the DFSPH step here runs on a single MI355X; "
-              "multi-GPU sharding is design-only (see DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU: the DFSPH step runs on a single MI355X, and no "
+              "multi-GPU path is built.", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

@@ -9,7 +9,7 @@ integrate) over the whole scene, inputs resident in HBM. Per-frame work
 reported on its own.
 
 Prints one JSON line. `roofline` is priced on the canonical algorithmic
-bytes per particle of each kernel over the kernel's average
+bytes per particle of each kernel (BYTES_PER_PARTICLE) over the kernel's average
 launch time measured with HIP events on the sim's own stream during the
 timed region. `cpu_baseline` times the CPU oracle (a C++ restatement of the
 reference's step, OpenMP over the host cores) on a bounded sample of the same
@@ -114,11 +114,11 @@ def pmc_summary(base):
     print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
     print("```")
     for c, d in (("FETCH_SIZE", "pmc_F"), ("WRITE_SIZE", "pmc_W")):
-        print(f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/{d} -o pmc -- {cmd}")
+        print(f"rocprofv3 --pmc {c} --output-format csv -d {base}/{d} -o pmc -- {cmd}")
     for c, d in (("FETCH_SIZE", "pmc_copyF"), ("WRITE_SIZE", "pmc_copyW")):
-        print(f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/{d} -o pmc -- "
+        print(f"rocprofv3 --pmc {c} --output-format csv -d {base}/{d} -o pmc -- "
               f"python3 -u bench.py --copy-calibration {COPY_BYTES}")
-    print("python3 bench.py --pmc-summary gpurun_out > profiles/pmc_block_1m.md")
+    print(f"python3 bench.py --pmc-summary {base} > profiles/pmc_block_1m.md")
     print("```\n")
     print(f"Calibration: a {COPY_BYTES >> 20} MiB device copy (HIP's copy kernel) reads FETCH_SIZE = "
           f"{fcopy:.0f} KiB and WRITE_SIZE = {wcopy:.0f} KiB, so traffic = {ff:.3f} x FETCH_SIZE + "

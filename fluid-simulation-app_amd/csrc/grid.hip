@@ -6,13 +6,13 @@
 // Replaces the reference's count (sim_grid.wgsl:44-50), scan_single
 // (sim_scan.wgsl:97-120) and scatter (sim_grid.wgsl:52-72). The reference's
 // scatter takes an atomic cursor, so its in-cell order is whatever the
-// atomics gave; this build fixes the stable order (within a cell, particles
+// atomics gave; this implementation fixes the stable order (within a cell, particles
 // keep the order of the resting set), one of the orders the reference admits.
 //
 // Launches (all on the caller's stream, no host synchronisation):
 //   grid_count   cell id per particle; histogram via one atomic per run of
 //                equal cells in a wave; a provisional rank; clears the scan's
-//                tile state for this build. In a batch, each scene's cells
+//                tile state for the current grid build. In a batch, each scene's cells
 //                sit at its offset in one shared histogram, so one scan
 //                segments every scene at once.
 //   grid_scan    starts[c] = sum of counts below c, starts[C] = N; clears

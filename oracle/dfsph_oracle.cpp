@@ -1,7 +1,7 @@
 // DFSPH step oracle — TEST INFRASTRUCTURE ONLY.
 //
 // A CPU restatement of the reference's simulation step, used by the tests,
-// __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker.
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg to check the HIP step.
 // The product path (fluid-simulation-app_amd/csrc) never links, loads or
 // calls anything here.
 //

@@ -1,7 +1,7 @@
 """ctypes wrapper over the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
-this module, and only as the checker. The product (the HIP library under
+this module, and only to check the HIP step. The product (the HIP library under
 fluid-simulation-app_amd/) never imports it.
 
 The oracle restates the reference's step (crates/fluid-core/src/sim_grid.wgsl,

@@ -109,7 +109,7 @@ def pmc_summary(base):
             e[key] += float(r["Counter_Value"])
             if primary:
                 e["launches_" + key] += 1
-    cmd = "python3 -u bench.py --no-cpu --no-batched --no-adjoint"
+    cmd = "python3 -u bench.py --no-cpu --no-batched --no-adjoint --no-cfl"
     print("# HBM traffic per launch, block_1m (1,048,576 particles, R = 5)\n")
     print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
     print("```")
@@ -247,6 +247,46 @@ def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0
     }
 
 
+def cfl_leg(fa, sc, develop_s, frames=40):
+    """The same scene on the reference's adaptive schedule: each 1/120 s
+    frame takes substeps_for(dt, v_max of the frame before) substeps
+    (render.rs:62-106), each at R = refine_passes(dt_sub) density passes
+    (render.rs:77-79), from the seeded block run for `develop_s` seconds of
+    simulated time first. Times `frames` whole frames, each with its stats
+    reduction and the v_max readback the next frame's count needs."""
+    sim = fa.Sim(sc.params)
+    sim.upload_state(sc.pos4)
+    state = {"t": 0.0, "v_max": 0.0}
+
+    def frame():
+        n, dt, r, vc = sc.schedule(state["v_max"])
+        for _ in range(n):
+            sim.substeps(sc.step_block(state["t"], dt, vc), 1, r)
+            state["t"] += dt
+        sim.reduce_stats()
+        state["v_max"] = float(sim.read_stats()[6])
+        return n, r
+
+    while state["t"] < develop_s:
+        frame()
+    clamps0, _ = sim.counters()
+    t0 = time.perf_counter()
+    substeps, passes = 0, {}
+    for _ in range(frames):
+        n, r = frame()
+        substeps += n
+        passes[r] = passes.get(r, 0) + n
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    clamps1, _ = sim.counters()
+    sim.close()
+    return {"schedule": "reference adaptive: substeps_for(1/120 s, v_max of the previous frame), R = refine_passes(dt_sub)",
+            "value": sc.n * substeps / elapsed, "unit": "particle-steps/s", "ms_per_step": elapsed * 1e3 / substeps,
+            "frames": frames, "substeps": substeps, "substeps_by_refine": {str(k): v for k, v in sorted(passes.items())},
+            "developed_for_s": develop_s, "speed_clamps": clamps1 - clamps0,
+            "what": "wall time of whole frames (substeps, stats reduction, v_max readback)"}
+
+
 def adjoint_leg(fa, n=65536, steps=64, every=8, seed=0):
     """The adjoint configuration: one 65,536-particle scene, 64 substeps with
     a seeded random-walk tilt of gravity, a checkpoint every 8 substeps; the
@@ -314,6 +354,7 @@ def main():
     ap.add_argument("--batched-only", action="store_true", help="print only the batched-ensemble line")
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-cfl", action="store_true", help="skip the adaptive-schedule leg")
     ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint line")
     ap.add_argument("--pmc-summary", metavar="DIR", default="",
                     help="print the counter summary of rocprofv3 --pmc runs under DIR and exit")
@@ -446,6 +487,8 @@ def main():
         "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
                   "compression_avg": float(stats[0])},
     }
+    if not args.no_cfl:
+        out["cfl_schedule"] = cfl_leg(fa, sc, develop_s=(args.warmup + args.steps) * dt_sub)
     if not args.no_batched:
         out["batched"] = batched_ensemble(fa, scenes=args.scenes, seed=args.seed)
     if not args.no_adjoint:

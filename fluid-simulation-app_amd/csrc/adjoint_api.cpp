@@ -107,6 +107,10 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         set_error("adjoint needs particles, a grid of at least 2 cells a side and at least one substep");
         return FLUID_ERR_ARG;
     }
+    if (!(p.cell > 0.0f) || !(p.h > 0.0f)) {
+        set_error("cell %g and h %g must be positive", p.cell, p.h);
+        return FLUID_ERR_ARG;
+    }
     A->cap = nbr_cap ? nbr_cap : FLUID_SIM_NBR_CAP;
     if (A->cap % kListLanes != 0 || A->cap > 256) {
         set_error("neighbour cap %u is not a multiple of 4 in [4, 256]", A->cap);

@@ -149,6 +149,10 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
                       p.dims[2]);
             return FLUID_ERR_ARG;
         }
+        if (!(p.cell > 0.0f) || !(p.h > 0.0f)) {
+            set_error("scene %u: cell %g and h %g must be positive", s, p.cell, p.h);
+            return FLUID_ERR_ARG;
+        }
         SceneK k = make_scene(p);
         const uint64_t c = (uint64_t)p.dims[0] * p.dims[1] * p.dims[2];
         const uint64_t pad = ((uint64_t)p.count + kSolveTile - 1) / kSolveTile * kSolveTile;
@@ -473,6 +477,10 @@ int fluid_batch_set_tolerance(FluidBatch* b, float tol, uint32_t min_passes, uin
         set_error("tolerance mode needs 0 < min_passes <= max_passes");
         return FLUID_ERR_ARG;
     }
+    // A captured substep holds the tolerance and pass bounds as kernel
+    // arguments, so graphs of the old setting are dropped.
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graphs();
     b->tol = Tolerance{tol > 0.0f ? tol : 0.0f, std::max<uint32_t>(min_passes, 1), max_passes};
     return FLUID_OK;
 }
@@ -550,59 +558,81 @@ int fluid_batch_profile_read(FluidBatch* b, double* ms, uint64_t* launches) {
 
 // Picks the sweeps' LDS staging capacity (slots a workgroup stages) by
 // timing `trials` substeps at each candidate, from a saved copy of the
-// resting set, then restores that set: the results do not depend on the
-// capacity (a tile too big to stage gathers the same values from global
-// memory), only the time does. Writes the chosen capacity.
+// resting set and the cumulative counters, then restores both: the results
+// do not depend on the capacity (a tile too big to stage gathers the same
+// values from global memory), only the time does. Writes the chosen
+// capacity. On failure the capacity and graph setting are left as they were.
 int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t* refine, uint32_t trials,
                          uint32_t* chosen) {
     if (!valid(b) || !steps144) return FLUID_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    const size_t n = b->slots;
+    const size_t n = b->slots, S = b->scenes;
     Records save{};
+    uint32_t *save_counters = nullptr, *save_clamps = nullptr;
     DeviceAllocs tmp;
     int rc = FLUID_OK;
     if ((rc = tmp.alloc(save.pos, n, false)) || (rc = tmp.alloc(save.vel, n, false)) ||
         (rc = tmp.alloc(save.prev_vel, n, false)) || (rc = tmp.alloc(save.prev_pressure, n, false)) ||
-        (rc = tmp.alloc(save.temperature, n, false))) {
+        (rc = tmp.alloc(save.temperature, n, false)) || (rc = tmp.alloc(save_counters, 2 * S, false)) ||
+        (rc = tmp.alloc(save_clamps, kClampWords * S, false))) {
         tmp.release();
         return rc;
     }
-    auto copy = [&](const Records& from, const Records& to) -> int {
-        HIP_TRY(hipMemcpyAsync(to.pos, from.pos, n * 16, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(to.vel, from.vel, n * 16, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(to.prev_vel, from.prev_vel, n * 16, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(to.prev_pressure, from.prev_pressure, n * 4, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync(to.temperature, from.temperature, n * 4, hipMemcpyDeviceToDevice, b->stream));
-        return FLUID_OK;
-    };
+    const uint32_t keep = b->solve.stage_cap;
+    const bool graph = b->use_graph;
+    double best = 1e300;
+    uint32_t best_cap = keep;
     std::vector<uint32_t> caps;
+    // A failed HIP call records its message and leaves through `done`, which
+    // restores the setting and frees the copies.
+#define TUNE_TRY(expr)                                                                                     \
+    do {                                                                                                   \
+        const hipError_t e_ = (expr);                                                                      \
+        if (e_ != hipSuccess) {                                                                            \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);          \
+            rc = FLUID_ERR_HIP;                                                                            \
+            goto done;                                                                                     \
+        }                                                                                                  \
+    } while (0)
+    auto copy = [&](const Records& from, const Records& to, const uint32_t* c_from, uint32_t* c_to,
+                    const uint32_t* k_from, uint32_t* k_to) -> hipError_t {
+        hipError_t e = hipSuccess;
+        const hipMemcpyKind d2d = hipMemcpyDeviceToDevice;
+        if (e == hipSuccess) e = hipMemcpyAsync(to.pos, from.pos, n * 16, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(to.vel, from.vel, n * 16, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(to.prev_vel, from.prev_vel, n * 16, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(to.prev_pressure, from.prev_pressure, n * 4, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(to.temperature, from.temperature, n * 4, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c_to, c_from, 2 * S * 4, d2d, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(k_to, k_from, kClampWords * S * 4, d2d, b->stream);
+        return e;
+    };
     // 0 (no staging) and 4/13 .. 13/13 of the largest capacity, in 64s.
     for (uint32_t f : {0u, 4u, 8u, 10u, 12u, 13u}) {
         const uint32_t c = b->stage_cap_max * f / 13u / 64u * 64u;
         if (caps.empty() || caps.back() != c) caps.push_back(c);
     }
-    const uint32_t keep = b->solve.stage_cap;
-    const bool graph = b->use_graph;
     b->use_graph = false;  // graphs bake the capacity in
     b->drop_graphs();
-    double best = 1e300;
-    uint32_t best_cap = keep;
-    if ((rc = copy(b->rest, save))) goto done;
+    TUNE_TRY(copy(b->rest, save, b->solve.counters, save_counters, b->solve.clamps, save_clamps));
     for (uint32_t c : caps) {
         b->solve.stage_cap = c;
         if ((rc = run_substep(b, steps144, refine))) goto done;  // warm-up
-        HIP_TRY(hipStreamSynchronize(b->stream));
-        const auto t0 = std::chrono::steady_clock::now();
-        for (uint32_t i = 0; i < std::max<uint32_t>(trials, 1); ++i)
-            if ((rc = run_substep(b, steps144, refine))) goto done;
-        HIP_TRY(hipStreamSynchronize(b->stream));
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (dt < best) {
-            best = dt;
-            best_cap = c;
+        TUNE_TRY(hipStreamSynchronize(b->stream));
+        {
+            const auto t0 = std::chrono::steady_clock::now();
+            for (uint32_t i = 0; i < std::max<uint32_t>(trials, 1); ++i)
+                if ((rc = run_substep(b, steps144, refine))) goto done;
+            TUNE_TRY(hipStreamSynchronize(b->stream));
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (dt < best) {
+                best = dt;
+                best_cap = c;
+            }
         }
-        if ((rc = copy(save, b->rest))) goto done;
+        TUNE_TRY(copy(save, b->rest, save_counters, b->solve.counters, save_clamps, b->solve.clamps));
     }
+#undef TUNE_TRY
 done:
     (void)hipStreamSynchronize(b->stream);
     tmp.release();

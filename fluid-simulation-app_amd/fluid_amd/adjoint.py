@@ -30,6 +30,9 @@ class Adjoint:
         self.n = int(np.frombuffer(params64, np.uint32, 1, 28)[0])
         self.steps = int(steps)
         self.mu = None
+        # Counts forward runs: the device tapes belong to the last one, so a
+        # backward can tell whether they are still those of its forward.
+        self.forwards = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -58,6 +61,7 @@ class Adjoint:
         vel_out = np.empty((self.n, 4), np.float32)
         check(self.L.fluid_adjoint_forward(self.h, _f32p(p), _f32p(v), _u8p(blob), _u32p(r), _f32p(pos_out),
                                            _f32p(vel_out)), "adjoint forward")
+        self.forwards += 1
         return pos_out, vel_out
 
     def backward(self, dpos4, dvel4=None):
@@ -100,11 +104,17 @@ def simulate_function():
             steps = steps_for(gravity.detach().cpu().numpy(), dt, v_clamp)
             pos_out, _ = adj.forward(x0, vel4, steps, [refine] * adj.steps)
             ctx.adj = adj
+            ctx.forward_id = adj.forwards
             return torch.from_numpy(pos_out[:, :3].copy()).to(v0.dtype)
 
         @staticmethod
         def backward(ctx, grad_pos):
             adj = ctx.adj
+            if adj.forwards != ctx.forward_id:
+                # Another forward on this Adjoint replaced the tapes this
+                # output was recorded with; its gradient would be that run's.
+                raise RuntimeError("Simulate.backward: the Adjoint ran another forward after this output was "
+                                   "computed; use one Adjoint per differentiable call")
             d4 = np.zeros((adj.n, 4), np.float32)
             d4[:, :3] = grad_pos.detach().cpu().numpy()
             g_v0, _, g_g, g_mu = adj.backward(d4)

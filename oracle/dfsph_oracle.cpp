@@ -98,23 +98,74 @@ inline Dual dexp(const Dual& a) {
     return Dual(e, e * a.d);
 }
 
+// A forward-mode dual number whose value is single precision and whose
+// derivative is double: every value is computed by exactly the float
+// operation the float instantiation performs (so a run takes the FP32 step's
+// path and produces its bits), and the derivative is that of those float
+// operations, carried in double. Its tangent is the exact derivative of the
+// FP32 step itself (the trajectory the GPU runs), which the GPU adjoint, a
+// float reverse sweep over the same trajectory, must reproduce.
+struct DualF {
+    float v = 0.0f;
+    double d = 0.0;
+    DualF() = default;
+    DualF(double x) : v((float)x), d(0.0) {}  // NOLINT: constants and parameters, rounded as float literals are
+    DualF(float x, double dx) : v(x), d(dx) {}
+    explicit operator float() const { return v; }
+    explicit operator double() const { return (double)v; }
+    DualF& operator+=(const DualF& b) { d += b.d; v += b.v; return *this; }
+    DualF& operator-=(const DualF& b) { d -= b.d; v -= b.v; return *this; }
+    DualF& operator*=(const DualF& b) { d = d * (double)b.v + (double)v * b.d; v *= b.v; return *this; }
+    DualF& operator/=(const DualF& b) {
+        const float q = v / b.v;
+        d = (d - (double)q * b.d) / (double)b.v;
+        v = q;
+        return *this;
+    }
+};
+inline DualF operator+(DualF a, const DualF& b) { return a += b; }
+inline DualF operator-(DualF a, const DualF& b) { return a -= b; }
+inline DualF operator*(DualF a, const DualF& b) { return a *= b; }
+inline DualF operator/(DualF a, const DualF& b) { return a /= b; }
+inline DualF operator-(const DualF& a) { return DualF(-a.v, -a.d); }
+inline bool operator<(const DualF& a, const DualF& b) { return a.v < b.v; }
+inline bool operator>(const DualF& a, const DualF& b) { return a.v > b.v; }
+inline bool operator<=(const DualF& a, const DualF& b) { return a.v <= b.v; }
+inline bool operator>=(const DualF& a, const DualF& b) { return a.v >= b.v; }
+inline bool operator==(const DualF& a, const DualF& b) { return a.v == b.v; }
+inline bool operator!=(const DualF& a, const DualF& b) { return a.v != b.v; }
+
 namespace {
+
+// Whether T computes in single precision (float, or DualF's value).
+template <typename T> struct IsF32 : std::is_same<T, float> {};
+template <> struct IsF32<DualF> : std::true_type {};
 
 // Sets the tangent of a dual value (nothing at float or double).
 inline void seed(float&, double) {}
 inline void seed(double&, double) {}
 inline void seed(Dual& x, double d) { x.d = d; }
+inline void seed(DualF& x, double d) { x.d = d; }
 inline double tangent(float) { return 0.0; }
 inline double tangent(double) { return 0.0; }
 inline double tangent(const Dual& x) { return x.d; }
+inline double tangent(const DualF& x) { return x.d; }
 
 // sqrt and exp at precision T (the dual forms carry the derivative).
 inline float tsqrt(float x) { return std::sqrt(x); }
 inline double tsqrt(double x) { return std::sqrt(x); }
 inline Dual tsqrt(const Dual& x) { return dsqrt(x); }
+inline DualF tsqrt(const DualF& x) {
+    const float r = std::sqrt(x.v);
+    return DualF(r, r > 0.0f ? x.d / (2.0 * (double)r) : 0.0);
+}
 inline float texp(float x) { return std::exp(x); }
 inline double texp(double x) { return std::exp(x); }
 inline Dual texp(const Dual& x) { return dexp(x); }
+inline DualF texp(const DualF& x) {
+    const float e = std::exp(x.v);
+    return DualF(e, (double)e * x.d);
+}
 
 struct Params {  // SimParams, sim.rs:325-350 / sim_solve.wgsl:11-23
     float box_min[3];
@@ -246,7 +297,7 @@ inline uint32_t unorm16(float v) {
 
 // A literal at precision T: float instantiations take the f-suffixed literal
 // (rounded once, as a WGSL f32 literal is), double ones the double literal.
-#define K(x) (std::is_same<T, float>::value ? (T)(x##f) : (T)(x))
+#define K(x) (IsF32<T>::value ? (T)(x##f) : (T)(x))
 
 template <typename T>
 struct V3 {
@@ -477,6 +528,10 @@ inline float exp_fixed(float x) {
 }
 inline double exp_fixed(double x) { return std::exp(x); }
 inline Dual exp_fixed(const Dual& x) { return dexp(x); }
+inline DualF exp_fixed(const DualF& x) {
+    const float e = exp_fixed(x.v);
+    return DualF(e, (double)e * x.d);
+}
 
 // sim_solve.wgsl:103-127
 template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v) {
@@ -1051,15 +1106,17 @@ void getd(const std::vector<T>& src, double* dst, size_t count) {
 }
 
 struct Handle {
-    int precision;  // 32, 64, or 128 (the dual-number FP64 run)
+    int precision;  // 32, 64, 128 (the dual-number FP64 run) or 96 (the dual-number FP32 run)
     Sim<float>* f;
     Sim<double>* d;
     Sim<Dual>* a;
+    Sim<DualF>* af;
 };
 
 template <typename F>
 auto dispatch(Handle* h, F&& fn) {
     if (h->precision == 128) return fn(*h->a);
+    if (h->precision == 96) return fn(*h->af);
     if (h->precision == 64) return fn(*h->d);
     return fn(*h->f);
 }
@@ -1068,13 +1125,16 @@ auto dispatch(Handle* h, F&& fn) {
 
 extern "C" {
 
-// Creates an oracle sim from the 64-byte SimParams block; precision 32 or 64.
+// Creates an oracle sim from the 64-byte SimParams block; precision 32, 64,
+// 128 (dual numbers over double) or 96 (dual numbers over float values).
 void* oracle_create(const void* params64, uint32_t nbr_cap, int precision) {
     Params p;
     std::memcpy(&p, params64, 64);
-    Handle* h = new Handle{precision, nullptr, nullptr, nullptr};
+    Handle* h = new Handle{precision, nullptr, nullptr, nullptr, nullptr};
     if (precision == 128)
         h->a = new Sim<Dual>(p, nbr_cap);
+    else if (precision == 96)
+        h->af = new Sim<DualF>(p, nbr_cap);
     else if (precision == 64)
         h->d = new Sim<double>(p, nbr_cap);
     else
@@ -1087,6 +1147,7 @@ void oracle_destroy(void* hv) {
     delete h->f;
     delete h->d;
     delete h->a;
+    delete h->af;
     delete h;
 }
 
@@ -1104,7 +1165,7 @@ void oracle_set_physics(void* hv, const float* phys5) {
     });
 }
 
-// Dual runs (precision 128): tangents of the resting set's positions (0,
+// Dual runs (precision 128 or 96): tangents of the resting set's positions (0,
 // n x 4), velocities (1, n x 4) or previous pressures (2, n).
 void oracle_set_tangent(void* hv, int which, const double* d) {
     dispatch((Handle*)hv, [&](auto& s) {

@@ -149,3 +149,24 @@ def test_adjoint_matches_central_differences():
     assert abs(g_g[1, 0] - fd_g64) <= 0.05 * abs(fd_g64) + 1e-9
     assert abs(g_g[1, 0] - fd_g32) <= 0.05 * abs(fd_g32) + 1e-9
     assert abs(got_v - fd_v32) <= 0.10 * abs(fd_v32) + 1e-9
+
+
+def test_backward_after_a_second_forward_on_the_same_adjoint_raises():
+    """The device tapes belong to an Adjoint's last forward: the output of an
+    earlier forward on it refuses a backward instead of returning the later
+    run's gradient; the later output still differentiates."""
+    import torch
+    steps = 2
+    sc, dt, refine, vc, v0, grav, half, c_t, h_t = _setup(steps, n=600)
+    adj = fad.Adjoint(sc.params, steps, 1)
+    Sim = fad.simulate_function()
+    v = torch.tensor(v0[:, :3], dtype=torch.float32, requires_grad=True)
+    m = torch.tensor(MU, dtype=torch.float64, requires_grad=True)
+    g = torch.tensor(grav[:steps], dtype=torch.float64, requires_grad=True)
+    first = Sim.apply(v, m, g, adj, sc.pos4, dt, vc, refine)
+    second = Sim.apply(v, m, g * 1.01, adj, sc.pos4, dt, vc, refine)
+    with pytest.raises(RuntimeError, match="another forward"):
+        first.sum().backward()
+    second.sum().backward()
+    assert np.isfinite(v.grad.numpy()).all() and np.abs(v.grad.numpy()).sum() > 0
+    adj.close()

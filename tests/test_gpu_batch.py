@@ -200,6 +200,8 @@ def test_tolerance_mode_with_fixed_passes_is_the_reference_schedule():
 
 
 def test_autotune_restores_state_and_keeps_results():
+    """The trial substeps leave no trace: the resting set, and the cumulative
+    clamp and overflow counters, are those of lone sims that never ran them."""
     scenes = _scenes()
     plan = [_steps(scenes, k) for k in range(2)]
     with fa.Batch([sc.params for sc in scenes]) as b:
@@ -209,13 +211,45 @@ def test_autotune_restores_state_and_keeps_results():
         assert cap == b.describe()["stage_cap"]
         for i, sc in enumerate(scenes):
             assert np.array_equal(b.state(i)["pos"], sc.pos4)
+        assert not b.counters().any()
         for steps, refine in plan:
             b.substep(steps, refine)
         tuned = [b.state(i) for i in range(len(scenes))]
+        counters = b.counters()
     for i, sc in enumerate(scenes):
         ref = _lone(sc, [(plan[k][0][i], plan[k][1][i]) for k in range(2)])
         for key in STATE:
             assert np.array_equal(_bits(tuned[i][key]), _bits(ref[key])), (sc.name, key)
+        assert tuple(int(x) for x in counters[i]) == ref["counters"], sc.name
+
+
+def test_tolerance_changes_reach_graph_replays():
+    """A captured substep holds the tolerance and the pass bounds as kernel
+    arguments; changing them between graph substeps gives what an eager run
+    with the same changes gives (passes and bits)."""
+    scenes = _scenes()[:4]
+    plan = [_steps(scenes, k)[0] for k in range(4)]
+    settings = [(1e-2, 1, 12), (1e-6, 2, 12), (1e-2, 3, 12), (1e-6, 1, 5)]
+
+    def run(graph):
+        out = []
+        with fa.Batch([sc.params for sc in scenes]) as b:
+            b.use_graph(graph)
+            for i, sc in enumerate(scenes):
+                b.upload_state(i, sc.pos4)
+            for steps, (tol, lo, hi) in zip(plan, settings):
+                b.set_tolerance(tol, lo, hi)
+                b.substep(steps, [0] * len(scenes))
+                out.append((b.iterations().copy(), [b.state(i) for i in range(len(scenes))]))
+        return out
+
+    eager, graph = run(False), run(True)
+    for k in range(len(plan)):
+        assert np.array_equal(graph[k][0], eager[k][0]), (k, graph[k][0], eager[k][0])
+        assert all(settings[k][1] <= x <= settings[k][2] for x in graph[k][0]), (k, graph[k][0])
+        for i in range(len(scenes)):
+            for key in STATE:
+                assert np.array_equal(_bits(graph[k][1][i][key]), _bits(eager[k][1][i][key])), (k, i, key)
 
 
 def _empty_params(sc):

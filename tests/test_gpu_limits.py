@@ -53,10 +53,17 @@ def test_a_flat_grid_is_refused():
         fa.Batch([sc.params, flat])
 
 
-def test_a_non_positive_smoothing_length_is_refused():
+@pytest.mark.parametrize("h", [0.0, -0.01, float("nan")])
+def test_a_non_positive_smoothing_length_is_refused(h):
+    from fluid_amd import adjoint as fad
     sc = fa.scenes.small_block((4, 4, 4), seed=54)
+    bad = _params(sc, h=h)
     with pytest.raises(RuntimeError, match="must be positive"):
-        fa.Sim(_params(sc, h=0.0))
+        fa.Sim(bad)
+    with pytest.raises(RuntimeError, match="must be positive"):
+        fa.Batch([sc.params, bad])
+    with pytest.raises(RuntimeError, match="must be positive"):
+        fad.Adjoint(bad, 2, 1)
 
 
 def test_the_largest_refusal_leaves_the_device_usable():

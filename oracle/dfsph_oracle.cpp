@@ -36,6 +36,17 @@
 //     exp() (XSPH blend, lens chase, tracer charge decay) are evaluated once
 //     per substep with the host expf/exp.
 //
+// A line-faithful mode (oracle_set_faithful) takes the reference's own
+// arithmetic for the three choices above that differ from it: the density
+// walk of LANES = 8 lanes over strided stencil cells with their lane sums
+// and xor-butterfly fold (sim_solve.wgsl:377, 398-402, 436-471), the list
+// appended in the lanes' lockstep order and split over 8 lanes in every
+// later sweep, x / max(r, eps) as true quotients (sim_solve.wgsl:559,
+// 567-568), exp() in the fingers (sim_solve.wgsl:124), and the 256-thread
+// strided sum of reduce_stats (sim_solve.wgsl:807-815). The HIP step is
+// held to it within the per-substep tolerance, and to the default mode bit
+// for bit.
+//
 // Every floating-point expression keeps the reference's operand order, and
 // the file is compiled with -ffp-contract=off, so float results are those
 // of IEEE single precision evaluated in that order.
@@ -375,8 +386,17 @@ struct Consts {
 // 546, 601, 684): lane l takes entries l, l + kListLanes, ... in order, and
 // the lane partials fold the way lane_fold does (sim_solve.wgsl:398-402),
 // xor 2 then xor 1, so lane 0 holds (p0 + p2) + (p1 + p3).
+//
+// The line-faithful mode (Sim::faithful) splits over the reference's own
+// LANES = 8 and folds xor 4, xor 2, xor 1: lane 0 holds
+// ((p0 + p4) + (p2 + p6)) + ((p1 + p5) + (p3 + p7)).
 constexpr uint32_t kListLanes = 4;
-template <typename A> inline A lane_fold(const A p[kListLanes]) {
+constexpr uint32_t kRefLanes = 8;
+template <typename A> inline A lane_fold(const A* p, uint32_t lanes) {
+    if (lanes == kRefLanes) {
+        const A a0 = p[0] + p[4], a1 = p[1] + p[5], a2 = p[2] + p[6], a3 = p[3] + p[7];
+        return (a0 + a2) + (a1 + a3);
+    }
     const A t0 = p[0] + p[2];
     const A t1 = p[1] + p[3];
     return t0 + t1;
@@ -534,7 +554,7 @@ inline DualF exp_fixed(const DualF& x) {
 }
 
 // sim_solve.wgsl:103-127
-template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v) {
+template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v, bool faithful = false) {
     T pullx = 0, pully = 0, weight = 0;
     for (uint32_t i = 0; i < s.touch_n; ++i) {
         const T* t = s.touches[i];
@@ -548,7 +568,10 @@ template <typename T> inline V3<T> fingers(const StepT<T>& s, V3<T> pos, V3<T> v
         }
     }
     if (weight <= T(0)) return v;
-    T blend = T(1) - exp_fixed(-T(TOUCH_RATE) * weight * s.dt);
+    // The faithful mode calls exp as the shader does; the kernels' fixed
+    // sequence (exp_fixed) is the default.
+    const T arg = -T(TOUCH_RATE) * weight * s.dt;
+    T blend = T(1) - (faithful ? texp(arg) : exp_fixed(arg));
     return v3(mix(v.x, pullx / weight, blend), mix(v.y, pully / weight, blend), v.z);
 }
 
@@ -574,6 +597,12 @@ struct Sim {
     std::vector<int32_t> vel_grid, vel_flat;
 
     Physics phys;
+    // Line-faithful mode: the reference's own arithmetic where the default
+    // follows the HIP kernels' (LANES = 8 and the strided density walk,
+    // true division in forces_eval, exp in the fingers, the 256-thread
+    // strided stats sum).
+    bool faithful = false;
+    uint32_t lanes() const { return faithful ? kRefLanes : kListLanes; }
     void set_physics(const Physics& ph) {
         phys = ph;
         k = Consts<T>(p, ph);
@@ -654,8 +683,13 @@ struct Sim {
     // the list; the pair sums split the neighbours over kListLanes lanes
     // (neighbour m, past the cap included, to lane m % kListLanes; lane 0
     // carries the self term, sim_solve.wgsl:426) and fold with lane_fold
-    // (sim_solve.wgsl:470-471).
+    // (sim_solve.wgsl:470-471). The faithful mode walks as the reference's
+    // lanes do instead (density_walk_faithful).
     void density_div(const StepT<T>& s) {
+        if (faithful) {
+            density_walk_faithful(s);
+            return;
+        }
         uint64_t over = 0;
 #pragma omp parallel for schedule(static) reduction(+ : over)
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
@@ -683,52 +717,125 @@ struct Sim {
                     T r2 = dot(x, x);
                     if (r2 >= k.support_sq) continue;
                     const uint32_t l = found % kListLanes;
-                    T r = tsqrt(r2);
-                    T q = r * k.inv_h;
-                    w_l[l] += kernel_q(k, q);
-                    T wn = tmax(T(1) - q, T(0));
-                    rn_l[l] += wn * wn * wn;
-                    V3<T> gw = grad_q(k, x, r, q);
-                    gsum_l[l] = gsum_l[l] + gw;
-                    gsq_l[l] += dot(gw, gw);
-                    drho_l[l] += dot(vel_i - wv(kk), gw);
+                    density_pair(vel_i, kk, x, r2, w_l[l], rn_l[l], gsq_l[l], drho_l[l], gsum_l[l]);
                     if (found < cap) nbr[(size_t)pidx * cap + found] = kk;
                     found++;
                 }
             }
-            const T w_sum = lane_fold(w_l), rn = lane_fold(rn_l);
-            T grad_sq = lane_fold(gsq_l), drho = lane_fold(drho_l);
-            V3<T> grad_sum = lane_fold(gsum_l);
             nbr_n[pidx] = std::min(found, cap);
             if (found > cap) over += found - cap;
-            T rho = k.mass * w_sum;
-            grad_sq = k.mass * k.mass * grad_sq;
-            drho = k.mass * drho;
-            grad_sum = k.mass * grad_sum;
-            V3<T> lo = v3(k.wall_lo[0], k.wall_lo[1], k.wall_lo[2]);
-            V3<T> hi = v3(k.wall_hi[0], k.wall_hi[1], k.wall_hi[2]);
-            V3<T> tl = k.inv_h * (pos_i - lo);
-            V3<T> th = k.inv_h * (hi - pos_i);
-            T fill = 0;
-            fill += wall_density(tl.x) + wall_density(th.x);
-            fill += wall_density(tl.y) + wall_density(th.y);
-            fill += wall_density(tl.z) + wall_density(th.z);
-            fill -= wedge_sum(tl, th);
-            rho += k.rho0 * fill;
-            V3<T> wg = wall_grad_sum(k, pos_i);
-            wgrad[4 * pidx] = wg.x;
-            wgrad[4 * pidx + 1] = wg.y;
-            wgrad[4 * pidx + 2] = wg.z;
-            wgrad[4 * pidx + 3] = 0;
-            grad_sum = grad_sum + wg;
-            density[pidx] = rho;
-            rho_near[pidx] = rn;
-            T a = rho / tmax(dot(grad_sum, grad_sum) + grad_sq, K(1e-4));
-            alpha[pidx] = a;
-            drho += dot(vel_i, wg);
-            kd[pidx] = tmax(drho, T(0)) * a / tmax(s.dt, K(1e-9)) / rho;
+            density_finish(s, pidx, pos_i, vel_i, lane_fold(w_l, kListLanes), lane_fold(rn_l, kListLanes),
+                           lane_fold(gsq_l, kListLanes), lane_fold(drho_l, kListLanes),
+                           lane_fold(gsum_l, kListLanes));
         }
         nbr_over += over;
+    }
+
+    // One accepted pair's terms of density_div (sim_solve.wgsl:448-460).
+    inline void density_pair(V3<T> vel_i, uint32_t kk, V3<T> x, T r2, T& w, T& rn, T& gsq, T& drho,
+                             V3<T>& gsum) const {
+        T r = tsqrt(r2);
+        T q = r * k.inv_h;
+        w += kernel_q(k, q);
+        T wn = tmax(T(1) - q, T(0));
+        rn += wn * wn * wn;
+        V3<T> gw = grad_q(k, x, r, q);
+        gsum = gsum + gw;
+        gsq += dot(gw, gw);
+        drho += dot(vel_i - wv(kk), gw);
+    }
+
+    // The reference's own walk (sim_solve.wgsl:436-466): lane l of the
+    // particle's LANES = 8 takes the stencil cells l, l + 8, l + 16, l + 24
+    // (cell numbering x fastest, sim_solve.wgsl:437-438) and sums the pairs
+    // it accepts in its own walk order, lane 0 starting from the self term;
+    // the eight partials fold with the xor butterfly. The list append is a
+    // workgroup atomic (sim_solve.wgsl:463); the order taken is that of the
+    // lanes in lockstep: for the lanes' k-th cell, their candidates' i-th
+    // iterations in lane order (the atomics of one instruction resolve in
+    // lane order), then the next i, then the next cell.
+    void density_walk_faithful(const StepT<T>& s) {
+        uint64_t over = 0;
+#pragma omp parallel for schedule(static) reduction(+ : over)
+        for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
+            uint32_t pidx = (uint32_t)pi;
+            V3<T> pos_i = wp(pidx), vel_i = wv(pidx);
+            T w_l[kRefLanes] = {}, rn_l[kRefLanes] = {}, gsq_l[kRefLanes] = {}, drho_l[kRefLanes] = {};
+            V3<T> gsum_l[kRefLanes];
+            for (uint32_t l = 0; l < kRefLanes; ++l) gsum_l[l] = v3(T(0), T(0), T(0));
+            w_l[0] = k.sigma;
+            uint32_t found = 0;
+            int32_t base[3];
+            cell_base(pidx, base);
+            for (uint32_t round = 0; round * kRefLanes < 27; ++round) {
+                uint32_t k0[kRefLanes], len[kRefLanes], longest = 0;
+                for (uint32_t l = 0; l < kRefLanes; ++l) {
+                    const int cell = (int)(round * kRefLanes + l);
+                    k0[l] = len[l] = 0;
+                    if (cell >= 27) continue;
+                    int32_t cx = base[0] + (cell % 3) - 1;
+                    int32_t cy = base[1] + (cell / 3) % 3 - 1;
+                    int32_t cz = base[2] + cell / 9 - 1;
+                    if (cx < 0 || cy < 0 || cz < 0 || cx >= (int32_t)p.dims[0] || cy >= (int32_t)p.dims[1] ||
+                        cz >= (int32_t)p.dims[2])
+                        continue;
+                    uint32_t c = ((uint32_t)cz * p.dims[1] + (uint32_t)cy) * p.dims[0] + (uint32_t)cx;
+                    k0[l] = starts[c];
+                    len[l] = starts[c + 1] - starts[c];
+                    longest = std::max(longest, len[l]);
+                }
+                for (uint32_t it = 0; it < longest; ++it) {
+                    for (uint32_t l = 0; l < kRefLanes; ++l) {
+                        if (it >= len[l]) continue;
+                        const uint32_t kk = k0[l] + it;
+                        if (kk == pidx) continue;
+                        V3<T> x = pos_i - wp(kk);
+                        T r2 = dot(x, x);
+                        if (r2 >= k.support_sq) continue;
+                        density_pair(vel_i, kk, x, r2, w_l[l], rn_l[l], gsq_l[l], drho_l[l], gsum_l[l]);
+                        if (found < cap) nbr[(size_t)pidx * cap + found] = kk;
+                        found++;
+                    }
+                }
+            }
+            nbr_n[pidx] = std::min(found, cap);
+            if (found > cap) over += found - cap;
+            density_finish(s, pidx, pos_i, vel_i, lane_fold(w_l, kRefLanes), lane_fold(rn_l, kRefLanes),
+                           lane_fold(gsq_l, kRefLanes), lane_fold(drho_l, kRefLanes), lane_fold(gsum_l, kRefLanes));
+        }
+        nbr_over += over;
+    }
+
+    // Lane 0's end of density_div (sim_solve.wgsl:478-517): the mass, the
+    // wall fill and gradient, rho, alpha and the divergence kappa / rho.
+    void density_finish(const StepT<T>& s, uint32_t pidx, V3<T> pos_i, V3<T> vel_i, T w_sum, T rn, T grad_sq,
+                        T drho, V3<T> grad_sum) {
+        T rho = k.mass * w_sum;
+        grad_sq = k.mass * k.mass * grad_sq;
+        drho = k.mass * drho;
+        grad_sum = k.mass * grad_sum;
+        V3<T> lo = v3(k.wall_lo[0], k.wall_lo[1], k.wall_lo[2]);
+        V3<T> hi = v3(k.wall_hi[0], k.wall_hi[1], k.wall_hi[2]);
+        V3<T> tl = k.inv_h * (pos_i - lo);
+        V3<T> th = k.inv_h * (hi - pos_i);
+        T fill = 0;
+        fill += wall_density(tl.x) + wall_density(th.x);
+        fill += wall_density(tl.y) + wall_density(th.y);
+        fill += wall_density(tl.z) + wall_density(th.z);
+        fill -= wedge_sum(tl, th);
+        rho += k.rho0 * fill;
+        V3<T> wg = wall_grad_sum(k, pos_i);
+        wgrad[4 * pidx] = wg.x;
+        wgrad[4 * pidx + 1] = wg.y;
+        wgrad[4 * pidx + 2] = wg.z;
+        wgrad[4 * pidx + 3] = 0;
+        grad_sum = grad_sum + wg;
+        density[pidx] = rho;
+        rho_near[pidx] = rn;
+        T a = rho / tmax(dot(grad_sum, grad_sum) + grad_sq, K(1e-4));
+        alpha[pidx] = a;
+        drho += dot(vel_i, wg);
+        kd[pidx] = tmax(drho, T(0)) * a / tmax(s.dt, K(1e-9)) / rho;
     }
 
     // Gradient of the pair (i, j), recomputed exactly as density_div did.
@@ -746,13 +853,14 @@ struct Sim {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i);
             T kd_i = kd[i];
-            V3<T> part[kListLanes];
+            const uint32_t L = lanes();
+            V3<T> part[kRefLanes];
             for (auto& q : part) q = v3(T(0), T(0), T(0));
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                part[m % kListLanes] = part[m % kListLanes] + (kd_i + kd[j]) * pair_grad(pos_i, j);
+                part[m % L] = part[m % L] + (kd_i + kd[j]) * pair_grad(pos_i, j);
             }
-            const V3<T> dv = lane_fold(part);
+            const V3<T> dv = lane_fold(part, L);
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             V3<T> full = k.mass * dv + kd_i * wg;
             V3<T> v = wv(i) - s.dt * full;
@@ -769,15 +877,16 @@ struct Sim {
 #pragma omp parallel for schedule(static)
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t i = (uint32_t)pi;
-            V3<T> visc_l[kListLanes], blend_l[kListLanes], near_l[kListLanes], tension_l[kListLanes];
-            T heat_l[kListLanes] = {};
-            for (uint32_t l = 0; l < kListLanes; ++l)
+            const uint32_t L = lanes();
+            V3<T> visc_l[kRefLanes], blend_l[kRefLanes], near_l[kRefLanes], tension_l[kRefLanes];
+            T heat_l[kRefLanes] = {};
+            for (uint32_t l = 0; l < kRefLanes; ++l)
                 visc_l[l] = blend_l[l] = near_l[l] = tension_l[l] = v3(T(0), T(0), T(0));
             V3<T> pos_i = wp(i), vel_i = wv(i);
             T w_i = rho_near[i], temp_i = wtemp[i], rho_i = density[i];
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                const uint32_t l = m % kListLanes;
+                const uint32_t l = m % L;
                 V3<T>& visc = visc_l[l];
                 V3<T>& blend = blend_l[l];
                 V3<T>& nearf = near_l[l];
@@ -798,18 +907,28 @@ struct Sim {
                 T inv_rg = T(1) / tmax(r, K(1e-9));
                 T inv_rm = r >= K(1e-5) ? inv_rg : T(1) / K(1e-5);
                 T sn = (w_i + rho_near[j]) * wn * wn;
-                nearf = nearf + v3(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm));
+                if (faithful) {  // the shader's quotients, sim_solve.wgsl:559
+                    const T dm = tmax(r, K(1e-5));
+                    nearf = nearf + v3(sn * (x.x / dm), sn * (x.y / dm), sn * (x.z / dm));
+                } else {
+                    nearf = nearf + v3(sn * (x.x * inv_rm), sn * (x.y * inv_rm), sn * (x.z * inv_rm));
+                }
                 visc = visc + (k.visc2 * pair) * dv;
                 blend = blend - (k.mass / rho_j * kernel_q(k, r * k.inv_h)) * dv;
                 heat += k.heat_cond * pair * (temp_i - wtemp[j]);
                 heat -= k.heat_visc * pair * dot(dv, dv);
                 T corr = T(2) * k.rho0 / (rho_i + rho_j);
                 T cm = corr * k.mass * cohesion(r, k.coh_c, k.coh_k, k.coh_near);
-                tension = tension - v3(cm * x.x * inv_rg, cm * x.y * inv_rg, cm * x.z * inv_rg);
+                if (faithful) {  // sim_solve.wgsl:567-568
+                    const T dg = tmax(r, K(1e-9));
+                    tension = tension - v3(cm * x.x / dg, cm * x.y / dg, cm * x.z / dg);
+                } else {
+                    tension = tension - v3(cm * x.x * inv_rg, cm * x.y * inv_rg, cm * x.z * inv_rg);
+                }
             }
-            const V3<T> visc = lane_fold(visc_l), blend = lane_fold(blend_l), nearf = lane_fold(near_l),
-                        tension = lane_fold(tension_l);
-            const T heat = lane_fold(heat_l);
+            const V3<T> visc = lane_fold(visc_l, L), blend = lane_fold(blend_l, L), nearf = lane_fold(near_l, L),
+                        tension = lane_fold(tension_l, L);
+            const T heat = lane_fold(heat_l, L);
             V3<T> adh = (k.beta * k.rho0) * wall_adh_sum(k, pos_i);
             V3<T> acc = visc + k.k_near * nearf + k.gamma * tension + adh;
             accel[4 * i] = acc.x;
@@ -835,18 +954,19 @@ struct Sim {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i);
             T kd_i = kd[i];
-            V3<T> part[kListLanes];
+            const uint32_t L = lanes();
+            V3<T> part[kRefLanes];
             for (auto& q : part) q = v3(T(0), T(0), T(0));
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                part[m % kListLanes] = part[m % kListLanes] + (kd_i + kd[j]) * pair_grad(pos_i, j);
+                part[m % L] = part[m % L] + (kd_i + kd[j]) * pair_grad(pos_i, j);
             }
-            const V3<T> dv = lane_fold(part);
+            const V3<T> dv = lane_fold(part, L);
             V3<T> a = v3(accel[4 * i], accel[4 * i + 1], accel[4 * i + 2]);
             V3<T> r = pos_i, v = wv(i);
             V3<T> fict = -cross(s.domega, r) - cross(s.omega, cross(s.omega, r)) - T(2) * cross(s.omega, v);
             V3<T> xs = v3(xsph[4 * i], xsph[4 * i + 1], xsph[4 * i + 2]);
-            V3<T> forced = fingers(s, r, v + s.dt * (s.force + fict + a) + s.c_xsph * xs);
+            V3<T> forced = fingers(s, r, v + s.dt * (s.force + fict + a) + s.c_xsph * xs, faithful);
             wtemp[i] = wtemp[i] + s.dt * accel[4 * i + 3];
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             V3<T> full = k.mass * dv + kd_i * wg;
@@ -865,12 +985,13 @@ struct Sim {
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t i = (uint32_t)pi;
             V3<T> pos_i = wp(i), vel_i = wv(i);
-            T part[kListLanes] = {};
+            const uint32_t L = lanes();
+            T part[kRefLanes] = {};
             for (uint32_t m = 0; m < nbr_n[i]; ++m) {
                 uint32_t j = nbr[(size_t)i * cap + m];
-                part[m % kListLanes] += dot(vel_i - wv(j), pair_grad(pos_i, j));
+                part[m % L] += dot(vel_i - wv(j), pair_grad(pos_i, j));
             }
-            T drho = lane_fold(part);
+            T drho = lane_fold(part, L);
             T rho = density[i];
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             drho = k.mass * drho + dot(vel_i, wg);
@@ -965,6 +1086,17 @@ struct Sim {
             m = std::max(m, c);
             lo = std::min(lo, d);
             hi = std::max(hi, d);
+        }
+        if (faithful) {
+            // The shader's order (sim_solve.wgsl:807-815, 767-778): thread l
+            // of 256 sums slots l, l + 256, ..., then a tree halves the
+            // partials (l += l + off for off = 128, 64, ..., 1).
+            float part[256] = {};
+            for (uint32_t l = 0; l < 256; ++l)
+                for (uint32_t i = l; i < n; i += 256) part[l] += std::max((float)density[i] / rho0 - 1.0f, 0.0f);
+            for (uint32_t off = 128; off > 0; off >>= 1)
+                for (uint32_t l = 0; l < off; ++l) part[l] += part[l + off];
+            sum = part[0];
         }
         stats[0] = sum / (float)n;
         stats[1] = std::max(m, 0.0f);
@@ -1161,6 +1293,15 @@ void oracle_set_physics(void* hv, const float* phys5) {
     ph.xsph_rate = phys5[4];
     dispatch((Handle*)hv, [&](auto& s) -> int {
         s.set_physics(ph);
+        return 0;
+    });
+}
+
+// The line-faithful mode (1) or the kernels' order (0, the default); see
+// Sim::faithful.
+void oracle_set_faithful(void* hv, int on) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        s.faithful = on != 0;
         return 0;
     });
 }

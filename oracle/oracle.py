@@ -55,6 +55,7 @@ def lib():
         L.oracle_destroy.argtypes = [vp]
         L.oracle_set_threads.argtypes = [i32]
         L.oracle_set_physics.argtypes = [vp, P(f32)]
+        L.oracle_set_faithful.argtypes = [vp, i32]
         L.oracle_set_tangent.argtypes = [vp, i32, P(f64)]
         L.oracle_get_tangent.argtypes = [vp, i32, P(f64)]
         L.oracle_seed_force.argtypes = [vp, P(f64)]
@@ -110,11 +111,16 @@ def _bytes_ptr(b: bytes):
 class OracleSim:
     """One scene on the CPU oracle: the resting set plus the scratch fields."""
 
-    def __init__(self, params64: bytes, nbr_cap: int = 96, precision: int = 32):
+    def __init__(self, params64: bytes, nbr_cap: int = 96, precision: int = 32, faithful: bool = False):
+        """precision 32, 64, 128 (dual numbers over double) or 96 (dual numbers
+        over float values); `faithful` takes the reference's own lane split,
+        walk and quotients instead of the HIP kernels' order."""
         assert len(params64) == 64
         self.L = lib()
         self._params = _bytes_ptr(params64)
         self.h = self.L.oracle_create(self._params, nbr_cap, precision)
+        if faithful:
+            self.L.oracle_set_faithful(self.h, 1)
         self.n = int(np.frombuffer(params64, dtype=np.uint32, count=1, offset=28)[0])
         d = np.frombuffer(params64, dtype=np.uint32, count=3, offset=16)
         self.cells = int(d[0]) * int(d[1]) * int(d[2])

@@ -163,8 +163,8 @@ def _tiny(lattice=(6, 5, 4), seed=0, half=(0.05, 0.05, 0.05), place="floor", h_f
     return fa.scenes.small_block(lattice, seed=seed, spacing=0.01, half=half, place=place, h_factor=h_factor)
 
 
-def _oracle(scene, precision, vel=None, dt=1.0 / 240.0, cap=96):
-    o = OracleSim(scene.params, nbr_cap=cap, precision=precision)
+def _oracle(scene, precision, vel=None, dt=1.0 / 240.0, cap=96, faithful=False):
+    o = OracleSim(scene.params, nbr_cap=cap, precision=precision, faithful=faithful)
     o.set_state(scene.pos4, vel)
     step = fa.pack_step(force=(0.0, -9.81, 0.0), dt=dt, v_clamp=1e30)
     return o, step
@@ -230,14 +230,15 @@ def _brute(scene, o, vel_slot, dt):
     return dict(density=rho, rho_near=rn, alpha=alpha, kd=kd, nbr_count=inside.sum(1))
 
 
+@pytest.mark.parametrize("faithful", [False, True])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-11), (32, 1e-5)])
-def test_density_sweep_matches_brute_force(precision, tol):
+def test_density_sweep_matches_brute_force(precision, tol, faithful):
     sc = _tiny(seed=1)
     rng = np.random.default_rng(5)
     vel = np.zeros((sc.n, 4), np.float32)
     vel[:, :3] = rng.normal(0, 0.1, (sc.n, 3))
     dt = 1.0 / 240.0
-    o, step = _oracle(sc, precision, vel, dt)
+    o, step = _oracle(sc, precision, vel, dt, faithful=faithful)
     o.kernel("grid", step)
     o.kernel("density_div", step)
     perm = o.grid()["perm"]
@@ -315,6 +316,33 @@ def test_neighbour_list_is_the_brute_force_set_in_stencil_order():
         off = coord[js] - coord[i]
         key = (off[:, 2] + 1) * 9 + (off[:, 1] + 1) * 3 + (off[:, 0] + 1)
         expect = [j for _, j in sorted(zip(key, js))]
+        assert list(nbr[i, : nn[i]]) == expect, i
+
+
+def test_faithful_list_is_the_brute_force_set_in_lane_lockstep_order():
+    """The line-faithful walk (sim_solve.wgsl:436-466): lane l of 8 takes the
+    stencil cells l, l + 8, l + 16, l + 24 (x fastest); the appends of one
+    loop iteration land in lane order. So neighbour j of i sorts by (round
+    of its cell = cell // 8, its place in its cell, lane = cell % 8)."""
+    sc = _tiny(lattice=(7, 6, 5), seed=2)
+    o, step = _oracle(sc, 32, faithful=True)
+    o.kernel("grid", step)
+    o.kernel("density_div", step)
+    pos = o.field("work_pos")[:, :3].astype(np.float32)
+    nbr, nn, starts = o.nbr(), o.grid()["nbr_n"], o.grid()["starts"]
+    g = sc.grid
+    mn, cell, dims = np.array(g.min, np.float32), np.float32(g.cell), np.array(g.dims)
+    coord = np.minimum(np.floor(np.maximum((pos - mn) / cell, 0)).astype(np.int64), dims - 1)
+    cid = (coord[:, 2] * dims[1] + coord[:, 1]) * dims[0] + coord[:, 0]
+    h2 = np.frombuffer(sc.params, np.float32, 1, 56)[0]
+    for i in range(sc.n):
+        x = pos[i] - pos
+        r2 = (x[:, 0] * x[:, 0] + x[:, 1] * x[:, 1]) + x[:, 2] * x[:, 2]
+        js = [j for j in np.nonzero(r2 < h2)[0] if j != i]
+        off = coord[js] - coord[i]
+        stencil = (off[:, 2] + 1) * 9 + (off[:, 1] + 1) * 3 + (off[:, 0] + 1)
+        place = np.array(js) - starts[cid[js]]
+        expect = [j for _, _, _, j in sorted(zip(stencil // 8, place, stencil % 8, js))]
         assert list(nbr[i, : nn[i]]) == expect, i
 
 

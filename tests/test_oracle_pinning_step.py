@@ -92,8 +92,8 @@ def _step(**kw):
 
 
 def _run(sc, precision, kernels, step, vel=None, prev_vel=None, prev_pressure=None, temperature=None, phys=None,
-         cap=96):
-    o = OracleSim(sc.params, nbr_cap=cap, precision=precision)
+         cap=96, faithful=False):
+    o = OracleSim(sc.params, nbr_cap=cap, precision=precision, faithful=faithful)
     if phys is not None:
         o.set_physics(*phys)
     o.set_state(sc.pos4, vel, prev_vel, prev_pressure, temperature)
@@ -242,8 +242,9 @@ def test_rotating_frame_forces_are_the_closed_form(precision, tol):
     assert np.abs(got - want).max() <= tol * np.abs(want).max(), (got, want)
 
 
+@pytest.mark.parametrize("faithful", [False, True])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-12), (32, 3e-6)])
-def test_fingers_blend_toward_the_weighted_finger_velocity(precision, tol):
+def test_fingers_blend_toward_the_weighted_finger_velocity(precision, tol, faithful):
     """Fingers within touch_r weigh (1 - q^2)^2 (q = distance / touch_r); the
     particle's screen-plane velocity moves toward their weighted mean velocity
     by 1 - exp(-40 sum(w) dt); z and fingers out of reach do nothing
@@ -260,7 +261,7 @@ def test_fingers_blend_toward_the_weighted_finger_velocity(precision, tol):
     vel = np.zeros((1, 4), np.float32)
     vel[0, :3] = v
     st = fa.pack_step(force=(0.0, 0.0, 0.0), dt=float(dt), v_clamp=1e3, touches=fingers, touch_n=4, touch_r=float(R))
-    o = _run(sc, precision, SUBSTEP[:5], st, vel=vel)
+    o = _run(sc, precision, SUBSTEP[:5], st, vel=vel, faithful=faithful)
     got = o.field("work_vel")[0, :3]
     f = fingers.astype(np.float64)
     q2 = ((f[:, 0] - float(r[0])) ** 2 + (f[:, 1] - float(r[1])) ** 2) / float(R) ** 2
@@ -340,15 +341,17 @@ def _forces_scene(seed=6):
     return sc, vel, temp, pp
 
 
+@pytest.mark.parametrize("faithful", [False, True])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-11), (32, 2e-5)])
-def test_pair_forces_match_the_published_formulas(precision, tol):
+def test_pair_forces_match_the_published_formulas(precision, tol, faithful):
     """A block in a corner (walls on three sides: adhesion acts) with random
     velocities and temperatures: every term of forces_eval, and the warm
-    start kd = p_prev / 2 / rho^2, p = p_prev / 2 (Bender & Koschier 2015)."""
+    start kd = p_prev / 2 / rho^2, p = p_prev / 2 (Bender & Koschier 2015).
+    Both the kernels' order and the line-faithful one."""
     sc, vel, temp, pp = _forces_scene()
     st = _step(force=(0.0, -9.81, 0.0), dt=1.0 / 240.0, v_clamp=1e3)
     o = _run(sc, precision, ("grid", "density_div", "forces_eval"), st, vel=vel, temperature=temp,
-             prev_pressure=pp)
+             prev_pressure=pp, faithful=faithful)
     perm = o.grid()["perm"]
     b = _forces_brute(sc, o, temp[perm].astype(np.float64))
     acc = o.field("accel")
@@ -429,12 +432,13 @@ def _apply_brute(C, pos, vel, kd, wg, dt):
     return vel - dt * (s.sum(1) + kd[:, None] * wg)
 
 
+@pytest.mark.parametrize("faithful", [False, True])
 @pytest.mark.parametrize("precision,tol", [(64, 1e-11), (32, 1e-5)])
-def test_pressure_apply_is_the_dfsph_update(precision, tol):
+def test_pressure_apply_is_the_dfsph_update(precision, tol, faithful):
     sc, vel = _kappa_scene()
     dt = float(np.float32(1.0 / 240.0))
     st = _step(force=(0.0, -9.81, 0.0), dt=dt, v_clamp=1e3)
-    o = _run(sc, precision, ("grid", "density_div"), st, vel=vel)
+    o = _run(sc, precision, ("grid", "density_div"), st, vel=vel, faithful=faithful)
     C = _consts(sc)
     pos = o.field("work_pos")[:, :3]
     before = o.field("work_vel")[:, :3]
@@ -486,10 +490,11 @@ def test_density_pass_is_the_dfsph_prediction_and_its_error_falls(precision, tol
 
 # ---- reduce_stats --------------------------------------------------------------
 
-def test_reduce_stats_is_numpy_over_the_fields():
+@pytest.mark.parametrize("faithful", [False, True])
+def test_reduce_stats_is_numpy_over_the_fields(faithful):
     sc = fa.scenes.small_block((8, 8, 8), seed=10)
     n, dt, refine, vc = sc.schedule()
-    o = OracleSim(sc.params)
+    o = OracleSim(sc.params, faithful=faithful)
     o.set_state(sc.pos4)
     for k in range(3):
         o.substep(sc.step_block(k * dt, dt, vc), refine)
@@ -504,6 +509,17 @@ def test_reduce_stats_is_numpy_over_the_fields():
             s["temperature"].max(), o.counters()[0], o.counters()[1], sp.min(), v[:, 3].min(), v[:, 3].max()]
     assert abs(got[0] - want[0]) <= 1e-5 * want[0]
     assert np.array_equal(got[1:].astype(np.float64), np.array(want[1:], np.float64)), (got, want)
+    if faithful:  # the shader's 256-strided partials, then the halving tree (sim_solve.wgsl:767-778, 807-815)
+        c32 = np.maximum(c, np.float32(0))
+        part = np.zeros(256, np.float32)
+        for l in range(256):
+            for v_ in c32[l::256]:
+                part[l] = np.float32(part[l] + v_)
+        off = 128
+        while off:
+            part[:off] = part[:off] + part[off:2 * off]
+            off //= 2
+        assert got[0] == np.float32(part[0] / np.float32(d.size))
 
 
 # ---- the float-valued dual oracle ---------------------------------------------

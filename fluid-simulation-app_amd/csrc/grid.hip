@@ -20,8 +20,11 @@
 //   grid_place   provisional slot starts[c] + rank <- particle index.
 //   grid_gather  stable rank within the cell; permutation; copies the five
 //                persistent records (plus ids when tracked) into slot order.
+#include <mutex>
+
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
+#include "host_common.hpp"
 
 namespace fluid {
 
@@ -99,10 +102,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __re
 
 // One tile of kScanTile counts per workgroup, tiles taken in ticket order so
 // every predecessor of a tile is already resident: its look-back waits only
-// on workgroups that publish without waiting.
+// on workgroups that publish without waiting. With `clear`, the counts are
+// zeroed once read (the grid build's histogram, ready for the next build).
 __global__ __launch_bounds__(kBlock) void k_grid_scan(uint32_t* __restrict__ counts, uint32_t* __restrict__ starts,
                                                       uint32_t cells, uint64_t* __restrict__ status,
-                                                      uint32_t* __restrict__ ticket) {
+                                                      uint32_t* __restrict__ ticket, bool clear) {
     __shared__ uint32_t s_tile;
     __shared__ uint32_t s_wave[kBlock / 64];
     __shared__ uint32_t s_prefix;
@@ -118,9 +122,11 @@ __global__ __launch_bounds__(kBlock) void k_grid_scan(uint32_t* __restrict__ cou
         v[k] = c < cells ? counts[c] : 0u;
         tsum += v[k];
     }
+    if (clear) {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < cells) counts[base + k] = 0u;  // ready for the next build
+        for (int k = 0; k < kScanItems; ++k)
+            if (base + k < cells) counts[base + k] = 0u;  // ready for the next build
+    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t incl = wave_inclusive_sum(tsum);
     if (lane == 63u) s_wave[wave] = incl;
@@ -230,7 +236,7 @@ static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells,
                           rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
     const Span ts = span(timer, KERNEL_GRID_SCAN);
     hipExtLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, ts.a, ts.z, 0, b.counts, b.starts, cells,
-                          b.status, b.ticket);
+                          b.status, b.ticket, true);
     if (n == 0) return;
     const Span tg = span(timer, KERNEL_GRID_SCATTER);
     hipExtLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, tg.a, nullptr, 0, n, b.cellid,
@@ -255,7 +261,90 @@ void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32
     const uint32_t ntiles = scan_tiles(cells);
     (void)hipMemsetAsync(status, 0, (size_t)ntiles * sizeof(uint64_t), st);
     (void)hipMemsetAsync(ticket, 0, sizeof(uint32_t), st);
-    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, counts, starts, cells, status, ticket);
+    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, counts, starts, cells, status, ticket, true);
 }
 
+// ---- the scan without allocation ----------------------------------------------
+//
+// The look-back words and the ticket of fluid_scan_u32_noalloc live in one
+// file-scope device array (each GPU holds its own copy from the moment the
+// library's code loads on it until the process ends): kNoAllocTiles status
+// words, then one word whose low half is the ticket. Host code reaches it
+// only through the address hipGetSymbolAddress gives for the current device.
+constexpr uint32_t kNoAllocMaxCells = 1u << 24;
+constexpr uint32_t kNoAllocTiles = kNoAllocMaxCells / kScanTile;  // 8192: 64 KB of status words
+static_assert(kNoAllocMaxCells % kScanTile == 0, "whole scan tiles");
+static __device__ __align__(256) uint64_t scratch[kNoAllocTiles + 1];
+
+namespace {
+
+// Zeroes the first `ntiles` look-back words and the ticket (a kernel, not a
+// memset, so the call can sit inside a captured graph).
+__global__ __launch_bounds__(kBlock) void k_clear_scan_state(uint64_t* __restrict__ status, uint32_t ntiles,
+                                                             uint32_t* __restrict__ ticket) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < ntiles) status[i] = 0;
+    if (i == 0) *ticket = 0;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint64_t* g_scratch[kMaxDevices] = {};
+
+// The scratch array's device address on the current device, looked up once
+// per device.
+int scratch_on_device(uint64_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        set_error("hipGetDevice failed: %s", hipGetErrorString(e));
+        return FLUID_ERR_HIP;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device %d is past the %d devices the scan's scratch table holds", dev, kMaxDevices);
+        return FLUID_ERR_DEVICE;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("hipGetSymbolAddress(scan scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+            return FLUID_ERR_HIP;
+        }
+        g_scratch[dev] = static_cast<uint64_t*>(p);
+    }
+    *out = g_scratch[dev];
+    return FLUID_OK;
+}
+
+}  // namespace
+
 }  // namespace fluid
+
+using namespace fluid;
+
+extern "C" int fluid_scan_u32_noalloc(const uint32_t* counts, uint32_t cells, uint32_t* starts, void* stream) {
+    if (!counts || !starts || cells == 0) {
+        set_error("scan needs device counts, device starts and cells > 0");
+        return FLUID_ERR_ARG;
+    }
+    if (cells > kNoAllocMaxCells) {
+        set_error("fluid_scan_u32_noalloc takes at most %u cells (%u given): its scratch holds %u look-back tiles",
+                  kNoAllocMaxCells, cells, kNoAllocTiles);
+        return FLUID_ERR_ARG;
+    }
+    uint64_t* base = nullptr;
+    const int rc = scratch_on_device(&base);
+    if (rc != FLUID_OK) return rc;
+    uint64_t* status = base;
+    uint32_t* ticket = reinterpret_cast<uint32_t*>(base + kNoAllocTiles);
+    const uint32_t ntiles = scan_tiles(cells);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_clear_scan_state, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, status, ntiles, ticket);
+    // The kernel reads the counts and writes only starts (no clear): the
+    // caller's input is left as it was.
+    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, const_cast<uint32_t*>(counts), starts, cells,
+                       status, ticket, false);
+    return check_launch("fluid_scan_u32_noalloc");
+}

@@ -58,6 +58,7 @@ def _declare(L):
         "fluid_sim_kernel_name": ([u32], C.c_char_p),
         "fluid_sim_describe": ([vp, C.c_char_p, C.c_size_t], i32),
         "fluid_scan_u32": ([P(u32), u32, P(u32)], i32),
+        "fluid_scan_u32_noalloc": ([vp, u32, vp, vp], i32),
         "fluid_sim_set_physics": ([vp, P(f32)], i32),
         "fluid_batch_set_physics": ([vp, P(f32)], i32),
         "fluid_adjoint_create": ([u8p, u32, u32, u32], vp),

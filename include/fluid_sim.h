@@ -184,6 +184,21 @@ int fluid_sim_describe(const FluidSim *sim, char *buf, size_t size);
  */
 int fluid_scan_u32(const uint32_t *counts, uint32_t cells, uint32_t *starts);
 
+/*
+ * The same scan without any allocation. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). `counts`
+ * (cells words) and `starts` (cells + 1 words) are DEVICE pointers on the
+ * current device; `stream` is a hipStream_t (NULL: the null stream). The call
+ * is asynchronous on that stream, leaves `counts` unchanged and writes
+ * starts[c] = sum of counts[0..c), starts[cells] = total. At most 2^24 cells
+ * (FLUID_ERR_ARG beyond, nothing launched). Its look-back state lives in a
+ * 64 KB device array that each GPU the library runs on holds until the
+ * process ends, shared by every call on that GPU: calls that overlap on one
+ * GPU race and must be ordered by the caller (one stream, or events).
+ */
+int fluid_scan_u32_noalloc(const uint32_t *counts, uint32_t cells, uint32_t *starts, void *stream);
+
 /* ---- Host helpers: the reference's CPU substrate and schedule ------------
  * Pure host functions (no device needed), C++ restatements of
  * crates/fluid-core/src/sim.rs:1-350, lib.rs:34-36 and render.rs:57-106,

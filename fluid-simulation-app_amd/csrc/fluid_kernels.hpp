@@ -261,6 +261,28 @@ void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uin
                        const Records& work, const GridBufs& b, KernelTimer* timer);
 void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,
                       uint32_t* ticket);
+// The counting sort of the grid build into an index list only (perm), the
+// positions left where they are.
+void launch_index_sort(hipStream_t st, const OneScene& sc, uint32_t n, uint32_t cells, const float4* pos,
+                       const GridBufs& b);
+
+// The stage-0 microbench (render.rs:2429-2729): a seeded slab, its index
+// sort and `sweeps` brute 27-cell density sweeps through the index list,
+// the first frame checked on the host.
+struct Stage0Bench;
+Stage0Bench* stage0_create(const float extent[2], float spacing, uint32_t sweeps);
+void stage0_destroy(Stage0Bench* b);
+// One frame's work (sort + sweeps), synchronised; the first call also
+// validates. Returns FLUID_OK or a negative status.
+int stage0_frame(Stage0Bench* b);
+// The first frame's verdicts (-1 before it): the scan against a CPU scan,
+// every 16th density against a CPU brute-force sum to 1e-3.
+void stage0_validation(const Stage0Bench* b, int* scan_ok, int* density_ok);
+// Particles, cells, and the device's density (seed order) and starts.
+uint32_t stage0_count(const Stage0Bench* b);
+uint32_t stage0_cells(const Stage0Bench* b);
+int stage0_download(Stage0Bench* b, float* density, uint32_t* starts, float* pos4);
+double stage0_last_ms(const Stage0Bench* b);
 
 // One substep's solve on the working set (after the grid build), ending with
 // integrate writing the resting set: density_div, div_apply, forces_eval,

@@ -191,7 +191,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_place(uint32_t n, const uint32_
 // the atomics of grid_count handed out. The particle's records load first,
 // in flight while the rank's chain of loads (its cell, the cell's slot range,
 // the cell's entries eight at a time) completes; every store follows every
-// load, so no load waits on a store.
+// load, so no load waits on a store. Without RECORDS only the permutation
+// is written (an index sort).
+template <bool RECORDS>
 __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32_t* __restrict__ perm_tmp,
                                                         const uint32_t* __restrict__ cellid,
                                                         const uint32_t* __restrict__ starts, Records rest,
@@ -199,9 +201,17 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32
     uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm_tmp[s];
-    const float4 p = rest.pos[i], v = rest.vel[i], pv = rest.prev_vel[i];
-    const float pp = rest.prev_pressure[i], tt = rest.temperature[i];
-    const uint32_t id = rest.ids ? rest.ids[i] : 0u;
+    float4 p{}, v{}, pv{};
+    float pp = 0.0f, tt = 0.0f;
+    uint32_t id = 0u;
+    if (RECORDS) {
+        p = rest.pos[i];
+        v = rest.vel[i];
+        pv = rest.prev_vel[i];
+        pp = rest.prev_pressure[i];
+        tt = rest.temperature[i];
+        id = rest.ids ? rest.ids[i] : 0u;
+    }
     const uint32_t c = cellid[i];
     const uint32_t st = starts[c], en = starts[c + 1];
     uint32_t r = 0;
@@ -214,6 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32
     }
     const uint32_t f = st + r;
     perm[f] = i;
+    if (!RECORDS) return;
     work.pos[f] = p;
     work.vel[f] = v;
     work.prev_vel[f] = pv;
@@ -241,8 +252,8 @@ static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells,
     const Span tg = span(timer, KERNEL_GRID_SCATTER);
     hipExtLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, tg.a, nullptr, 0, n, b.cellid,
                           b.rank, b.starts, b.perm_tmp);
-    hipExtLaunchKernelGGL(k_grid_gather, dim3(blocks_for(n)), dim3(kBlock), 0, st, nullptr, tg.z, 0, n, b.perm_tmp, b.cellid,
-                       b.starts, rest, work, b.perm);
+    hipExtLaunchKernelGGL(k_grid_gather<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st, nullptr, tg.z, 0, n, b.perm_tmp,
+                          b.cellid, b.starts, rest, work, b.perm);
 }
 
 void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,
@@ -252,6 +263,26 @@ void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint3
 void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uint32_t cells, const Records& rest,
                        const Records& work, const GridBufs& b, KernelTimer* timer) {
     grid_build(st, sc, slots, cells, rest, work, b, timer);
+}
+
+// The grid build's counting sort of `n` positions into an index list (the
+// stage-0 microbench's sort): cell ids, the histogram, starts (cells + 1,
+// starts[cells] = n) and perm[slot] = particle, in the stable order. Nothing
+// is copied.
+void launch_index_sort(hipStream_t st, const OneScene& sc, uint32_t n, uint32_t cells, const float4* pos,
+                       const GridBufs& b) {
+    const uint32_t ntiles = scan_tiles(cells);
+    const uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
+    hipLaunchKernelGGL(k_grid_count<OneScene>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, sc, pos, n,
+                       b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
+    hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, b.counts, b.starts, cells, b.status, b.ticket,
+                       true);
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.cellid, b.rank, b.starts,
+                       b.perm_tmp);
+    const Records none{};
+    hipLaunchKernelGGL(k_grid_gather<false>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp, b.cellid,
+                       b.starts, none, none, b.perm);
 }
 
 // A stand-alone exclusive scan of `cells` counts (the grid scan, exposed for

@@ -13,6 +13,7 @@
 
 #include "../../include/fluid_ffi.h"
 #include "../../include/fluid_sim.h"
+#include "fluid_kernels.hpp"
 
 namespace {
 
@@ -173,6 +174,61 @@ struct IdleGate {  // render.rs:1024-1104
     }
 };
 
+// The lens a ramp paints with (render.rs:608-680), by the code
+// fluid_renderer_set_look takes (lib.rs:211-222; 0 and unknown codes are
+// velocity): the two ends the frame's stats block sets, and the narrowest
+// span worth a ramp, in the lens's units.
+constexpr float STANDARD_GRAVITY = 9.80665f;  // lib.rs:5
+void lens_ends(uint32_t lens, const float* stats, float out[2]) {
+    switch (lens) {
+        case 1: out[0] = stats[12]; out[1] = stats[13]; break;  // acceleration
+        case 2: out[0] = stats[4]; out[1] = stats[5]; break;    // pressure
+        case 3: out[0] = stats[2]; out[1] = stats[3]; break;    // proximity (density)
+        default: out[0] = stats[11]; out[1] = stats[6]; break;  // velocity, direction
+    }
+}
+float lens_floor(uint32_t lens, float spacing) {
+    switch (lens) {
+        case 1: return STANDARD_GRAVITY;
+        case 2: return REST_DENSITY * STANDARD_GRAVITY * spacing;
+        case 3: return 0.01f * REST_DENSITY;
+        default: return std::sqrt(2.0f * STANDARD_GRAVITY * spacing);
+    }
+}
+
+// The live ramp's ends, chased frame by frame (render.rs:682-743): opening
+// out with tau 0.15 s, closing in with 0.6 s, never narrower than the
+// lens's floor. None (false) until a stats block with density has landed.
+struct Ramp {
+    static constexpr float OPEN_TAU = 0.15f, CLOSE_TAU = 0.6f;
+    bool has_lens = false;
+    uint32_t lens = 0;
+    float ends[2] = {0.0f, 0.0f};
+    bool follow(uint32_t l, const float* stats, float spacing, float dt, float out[2]) {
+        if (stats[3] <= 0.0f) return false;
+        const float floor = lens_floor(l, spacing);
+        float want[2];
+        lens_ends(l, stats, want);
+        want[1] = std::max(want[1], want[0] + floor);
+        if (!has_lens || lens != l) {
+            has_lens = true;
+            lens = l;
+            ends[0] = want[0];
+            ends[1] = want[1];
+        } else {
+            for (int i = 0; i < 2; ++i) {
+                const bool opening = i == 0 ? want[i] < ends[i] : want[i] > ends[i];
+                const float tau = opening ? OPEN_TAU : CLOSE_TAU;
+                ends[i] += (want[i] - ends[i]) * (1.0f - std::exp(-dt / tau));
+            }
+            ends[1] = std::max(ends[1], ends[0] + floor);
+        }
+        out[0] = ends[0];
+        out[1] = ends[1];
+        return true;
+    }
+};
+
 // Sim::new (render.rs:1468-1612) on the C ABI: the seeded slab, tracers and
 // parameter block at `spacing` in a box of half-extent `extent` and slab depth.
 FluidSim* build_sim(const float extent[2], float spacing, uint32_t tracers, uint32_t* count_out) {
@@ -203,8 +259,19 @@ FluidSim* build_sim(const float extent[2], float spacing, uint32_t tracers, uint
 
 }  // namespace
 
+// The frame driver's four input filters (render.rs:862-1105), one object so
+// that the renderer and the C ABI's fluid_filters_* share them.
+struct FluidFilters {
+    ForceFilter force;
+    RotationTracker rotation;
+    Fingers fingers;
+    IdleGate idle;
+};
+
 struct FluidRenderer {
     FluidSim* sim = nullptr;
+    // The stage-0 microbench mode (bench_sweeps > 0) in place of the sim.
+    fluid::Stage0Bench* bench = nullptr;
     float extent[2] = {0, 0};
     float spacing = 0.0f;
     uint32_t max_substeps = 0, tracers = 0, count = 0, width = 0, height = 0;
@@ -213,13 +280,25 @@ struct FluidRenderer {
     double last_frame_ms = 0.0;
     float stats[FLUID_SIM_STATS] = {};
     bool glass = true;  // the reference's default look, Look::Glass
-    ForceFilter force_filter;
-    RotationTracker rotation;
-    Fingers fingers;
-    IdleGate idle;
+    // A flat or particle look painting a ramp with `lens` (set_look).
+    bool ramp_look = false;
+    uint32_t lens = 0;
+    Ramp ramp;
+    bool ramp_live = false;
+    float ramp_ends[2] = {0.0f, 0.0f};
+    // The last stepped frame's Step blocks and schedule (for callers that
+    // replay or inspect it).
+    uint8_t last_step[144] = {}, last_tracer_step[144] = {};
+    uint32_t last_refine = 0;
+    FluidFilters filters;
+    ForceFilter& force_filter = filters.force;
+    RotationTracker& rotation = filters.rotation;
+    Fingers& fingers = filters.fingers;
+    IdleGate& idle = filters.idle;
     Ring interval_us, gpu_us;
     ~FluidRenderer() {
         if (sim) fluid_sim_destroy(sim);
+        if (bench) fluid::stage0_destroy(bench);
     }
 };
 
@@ -234,8 +313,26 @@ FluidRenderer* fluid_renderer_create(void* metal_layer, uint32_t width, uint32_t
     (void)particle_count;
     (void)sprite_radius;
     if (bench_sweeps > 0) {
-        fprintf(stderr, "fluid_renderer_create: the stage-0 microbench mode is not served by this library\n");
-        return nullptr;
+        // The stage-0 microbench (render.rs:2455-2466, 2861-2862): its own
+        // spacing, 0.8 of the sim's unless given, clamped as the sim's is.
+        if (fluid_sim_device_count() < 1) {
+            fprintf(stderr, "fluid_renderer_create: no HIP device\n");
+            return nullptr;
+        }
+        auto r = std::make_unique<FluidRenderer>();
+        r->width = width;
+        r->height = height;
+        r->extent[0] = (float)width * 0.5f * METRES_PER_PIXEL;
+        r->extent[1] = (float)height * 0.5f * METRES_PER_PIXEL;
+        r->spacing = std::min(std::max(bench_spacing > 0.0f ? bench_spacing : 0.8f * SIM_SPACING, 0.4f * SIM_SPACING),
+                              4.0f * SIM_SPACING);
+        r->bench = fluid::stage0_create(r->extent, r->spacing, bench_sweeps);
+        if (!r->bench) {
+            fprintf(stderr, "fluid_renderer_create: %s\n", fluid_sim_last_error());
+            return nullptr;
+        }
+        r->count = fluid::stage0_count(r->bench);
+        return r.release();
     }
     if (sim_substeps == 0) {
         fprintf(stderr, "fluid_renderer_create: the sprite demo mode is not served by this library\n");
@@ -270,6 +367,15 @@ uint32_t fluid_renderer_frame(FluidRenderer* r, FluidVec3 gravity, FluidVec3 use
     const float dev = r->force_filter.apply(body, force);
     const float rot[3] = {rotation_rate.x, rotation_rate.y, rotation_rate.z};
     r->rotation.apply(rot, interval_us / 1000000.0f, omega, domega);
+    if (r->bench) {
+        // The microbench encodes its sort and sweeps every frame and never
+        // sleeps (render.rs:3042, 3164); the first frame validates.
+        if (r->frames > 0 && interval_us < 500000.0f) r->interval_us.push(interval_us);
+        r->last_frame_ms = now_ms;
+        if (fluid::stage0_frame(r->bench) == FLUID_OK) r->gpu_us.push((float)(fluid::stage0_last_ms(r->bench) * 1e3));
+        r->frames += 1;
+        return 1;
+    }
     if (r->idle.asleep(r->fingers.any_down(), force, dev, omega, r->stats[6])) {
         r->last_frame_ms = now_ms;
         r->idle_frames += 1;
@@ -279,6 +385,9 @@ uint32_t fluid_renderer_frame(FluidRenderer* r, FluidVec3 gravity, FluidVec3 use
     r->last_frame_ms = now_ms;
     const float dt = r->frames == 0 ? 0.0f : std::min(std::max(interval_us / 1000000.0f, 0.0f), MAX_DT);
     const Touches touches = r->fingers.step(r->extent, dt);
+    // The ramp chases the frame's own two ends from the last readback
+    // (render.rs:3069-3077).
+    r->ramp_live = r->ramp_look && r->ramp.follow(r->lens, r->stats, r->spacing, dt, r->ramp_ends);
     // From the one-frame-stale v_max, as the reference's readback gives it.
     r->substeps_used = dt > 0.0f ? fluid_substeps_for(dt, r->stats[6], r->spacing, r->max_substeps) : 0u;
     r->frame_seed += 1;
@@ -294,7 +403,11 @@ uint32_t fluid_renderer_frame(FluidRenderer* r, FluidVec3 gravity, FluidVec3 use
         fluid_pack_step(force, zero, zero, dt, 0.0f, r->frame_seed, nullptr, 0, 0.0f, tracer_step);
         // The glass look alone draws the tracers (render.rs:3041-3043).
         const bool strands = r->glass && r->tracers > 0;
-        fluid_sim_frame(r->sim, step, n, fluid_refine_passes(dt_sub), strands ? tracer_step : nullptr);
+        const uint32_t refine = fluid_refine_passes(dt_sub);
+        std::memcpy(r->last_step, step, sizeof step);
+        std::memcpy(r->last_tracer_step, strands ? tracer_step : step, sizeof step);
+        r->last_refine = refine;
+        fluid_sim_frame(r->sim, step, n, refine, strands ? tracer_step : nullptr);
         fluid_sim_read_stats(r->sim, r->stats);
         const float us = std::chrono::duration<float, std::micro>(std::chrono::steady_clock::now() - started).count();
         r->gpu_us.push(us);
@@ -304,7 +417,7 @@ uint32_t fluid_renderer_frame(FluidRenderer* r, FluidVec3 gravity, FluidVec3 use
 }
 
 void fluid_renderer_set_particles(FluidRenderer* r, float scale) {
-    if (!r) return;
+    if (!r || !r->sim) return;  // render.rs:2943-2946: the sim alone reseeds
     const float spacing = fluid_spacing_for(scale, r->extent);
     if (spacing == r->spacing) return;
     uint32_t count = 0;
@@ -322,7 +435,7 @@ void fluid_renderer_set_particles(FluidRenderer* r, float scale) {
 }
 
 uint32_t fluid_renderer_particles_at(const FluidRenderer* r, float scale) {
-    if (!r) return 0;
+    if (!r || !r->sim) return 0;  // zero outside the sim (render.rs:2969)
     const float half[3] = {r->extent[0], r->extent[1], fluid_slab_depth() * 0.5f};
     uint32_t d[3];
     fluid_lattice_dims(fluid_spacing_for(scale, r->extent), half, 0.5f, d);
@@ -338,13 +451,13 @@ void fluid_renderer_touch(FluidRenderer* r, uint32_t slot, float x, float y, boo
 
 void fluid_renderer_set_look(FluidRenderer* r, bool flat, bool particles, FluidVec3 low, bool gradient,
                              FluidVec3 high, uint32_t lens) {
-    (void)particles;
     (void)low;
-    (void)gradient;
     (void)high;
-    (void)lens;
     if (!r) return;
     r->glass = !flat;
+    (void)particles;  // Flat or Particles: both paint, neither is drawn here
+    r->ramp_look = flat && gradient;
+    r->lens = lens <= 4u ? lens : 0u;  // lib.rs:215-221: unknown codes paint velocity
     r->idle = IdleGate();
 }
 
@@ -382,5 +495,82 @@ FluidRenderStats fluid_renderer_stats(const FluidRenderer* r) {
 }
 
 void fluid_renderer_destroy(FluidRenderer* r) { delete r; }
+
+// ---- additions (include/fluid_sim.h) -------------------------------------------
+
+FluidSim* fluid_renderer_sim(FluidRenderer* r) { return r ? r->sim : nullptr; }
+
+int fluid_renderer_bench(FluidRenderer* r, int* scan_ok, int* density_ok, uint32_t* count, uint32_t* cells,
+                         float* density, uint32_t* starts, float* pos4) {
+    if (!r || !r->bench) return FLUID_ERR_ARG;
+    fluid::stage0_validation(r->bench, scan_ok, density_ok);
+    if (count) *count = fluid::stage0_count(r->bench);
+    if (cells) *cells = fluid::stage0_cells(r->bench);
+    return fluid::stage0_download(r->bench, density, starts, pos4);
+}
+
+int fluid_renderer_last_step(const FluidRenderer* r, uint8_t* step144, uint32_t* substeps, uint32_t* refine,
+                             uint8_t* tracer_step144) {
+    if (!r) return FLUID_ERR_ARG;
+    if (step144) std::memcpy(step144, r->last_step, 144);
+    if (tracer_step144) std::memcpy(tracer_step144, r->last_tracer_step, 144);
+    if (substeps) *substeps = r->substeps_used;
+    if (refine) *refine = r->last_refine;
+    return FLUID_OK;
+}
+
+int fluid_renderer_ramp(const FluidRenderer* r, float* ends2) {
+    if (!r || !r->ramp_live) return 0;
+    if (ends2) {
+        ends2[0] = r->ramp_ends[0];
+        ends2[1] = r->ramp_ends[1];
+    }
+    return 1;
+}
+
+int fluid_lens_ends(uint32_t lens, const float* stats14, float spacing, float* ends2, float* floor) {
+    if (!stats14) return FLUID_ERR_ARG;
+    const uint32_t l = lens <= 4u ? lens : 0u;
+    if (ends2) lens_ends(l, stats14, ends2);
+    if (floor) *floor = lens_floor(l, spacing);
+    return FLUID_OK;
+}
+
+FluidFilters* fluid_filters_create(void) { return new FluidFilters(); }
+void fluid_filters_destroy(FluidFilters* f) { delete f; }
+
+float fluid_filters_force(FluidFilters* f, const float body_force[3], float smooth[3]) {
+    float tmp[3];
+    return f && body_force ? f->force.apply(body_force, smooth ? smooth : tmp) : 0.0f;
+}
+
+void fluid_filters_rotation(FluidFilters* f, const float rate[3], float dt, float omega[3], float domega[3]) {
+    if (!f || !rate) return;
+    float o[3], d[3];
+    f->rotation.apply(rate, dt, omega ? omega : o, domega ? domega : d);
+}
+
+void fluid_filters_touch(FluidFilters* f, uint32_t slot, float x, float y, bool down) {
+    if (!f || slot >= (uint32_t)MAX_TOUCHES) return;
+    f->fingers.down[slot] = down;
+    f->fingers.at[slot][0] = x;
+    f->fingers.at[slot][1] = y;
+}
+
+uint32_t fluid_filters_fingers(FluidFilters* f, const float extent[2], float dt, float* touches20, float* radius) {
+    if (!f || !extent) return 0;
+    const Touches t = f->fingers.step(extent, dt);
+    if (touches20) std::memcpy(touches20, &t.each[0][0], sizeof t.each);
+    if (radius) *radius = t.radius;
+    return t.count;
+}
+
+bool fluid_filters_asleep(FluidFilters* f, bool touched, const float smooth[3], float dev, const float omega[3],
+                          float v_max) {
+    if (!f || !smooth || !omega) return false;
+    return f->idle.asleep(touched, smooth, dev, omega, v_max);
+}
+
+uint32_t fluid_filters_still_frames(void) { return IdleGate::STILL_FRAMES; }
 
 }  // extern "C"

@@ -509,6 +509,29 @@ int fluid_sim_counters(FluidSim* s, uint64_t* clamps, uint64_t* overflow) {
     return FLUID_OK;
 }
 
+int fluid_sim_params(const FluidSim* s, uint8_t* out64) {
+    if (!valid(s) || !out64) return FLUID_ERR_ARG;
+    std::memcpy(out64, &s->params, sizeof s->params);
+    return FLUID_OK;
+}
+
+int fluid_sim_device_records(FluidSim* s, FluidDeviceRecords* out) {
+    if (!valid(s) || !out) return FLUID_ERR_ARG;
+    out->positions = reinterpret_cast<float*>(s->rest.pos);
+    out->velocities = reinterpret_cast<float*>(s->rest.vel);
+    out->prev_vel = reinterpret_cast<float*>(s->rest.prev_vel);
+    out->prev_pressure = s->rest.prev_pressure;
+    out->temperature = s->rest.temperature;
+    // integrate writes the resting set at the working slot, so the last
+    // substep's density is in the resting set's order.
+    out->density = s->solve.density;
+    out->tracers = s->ntr ? s->tracers.tracers : nullptr;
+    out->count = s->n;
+    out->tracer_count = s->ntr;
+    out->stream = s->stream;
+    return FLUID_OK;
+}
+
 int fluid_sim_set_physics(FluidSim* s, const float* phys5) {
     if (!valid(s) || !phys5) return FLUID_ERR_ARG;
     Physics ph;

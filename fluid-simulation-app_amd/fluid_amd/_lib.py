@@ -104,6 +104,21 @@ def _declare(L):
         "fluid_substeps_for": ([f32, f32, f32, u32], u32),
         "fluid_spacing_for": ([f32, P(f32)], f32),
         "fluid_slab_depth": ([], f32),
+        "fluid_renderer_sim": ([vp], vp),
+        "fluid_sim_params": ([vp, u8p], i32),
+        "fluid_sim_device_records": ([vp, vp], i32),
+        "fluid_renderer_bench": ([vp, P(i32), P(i32), P(u32), P(u32), P(f32), P(u32), P(f32)], i32),
+        "fluid_renderer_last_step": ([vp, u8p, P(u32), P(u32), u8p], i32),
+        "fluid_renderer_ramp": ([vp, P(f32)], i32),
+        "fluid_lens_ends": ([u32, P(f32), f32, P(f32), P(f32)], i32),
+        "fluid_filters_create": ([], vp),
+        "fluid_filters_destroy": ([vp], None),
+        "fluid_filters_force": ([vp, P(f32), P(f32)], f32),
+        "fluid_filters_rotation": ([vp, P(f32), f32, P(f32), P(f32)], None),
+        "fluid_filters_touch": ([vp, u32, f32, f32, C.c_bool], None),
+        "fluid_filters_fingers": ([vp, P(f32), f32, P(f32), P(f32)], u32),
+        "fluid_filters_asleep": ([vp, C.c_bool, P(f32), f32, P(f32), f32], C.c_bool),
+        "fluid_filters_still_frames": ([], u32),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

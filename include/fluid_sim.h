@@ -145,6 +145,35 @@ int fluid_sim_download_tracers(FluidSim *sim, uint32_t *packed2);
 int fluid_sim_frame(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32_t refine,
                     const uint8_t *tracer_step144);
 
+/* The sim's 64-byte SimParams block (sim.rs:330-350). */
+int fluid_sim_params(const FluidSim *sim, uint8_t *out64);
+
+/*
+ * The resting set on the device, for consumers that read it in place as the
+ * reference's sprite, body-splat and disc pullers bind it
+ * (render.rs:1818-1828): positions (vec4, w = 0), velocities (vec4, w =
+ * acceleration EMA), prev_vel (vec4, w = pressure EMA), prev_pressure,
+ * temperature, the last substep's density in the same slot order, and the
+ * packed tracers (2 words each; NULL without tracers). The pointers stay
+ * valid for the sim's life (a reseed makes a new sim). `stream` is the
+ * hipStream_t the sim's kernels run on: work queued on it after a frame
+ * reads that frame's state, or a caller records an event on it to wait for.
+ * The sim rewrites these arrays on its next substep.
+ */
+typedef struct FluidDeviceRecords {
+    float *positions;
+    float *velocities;
+    float *prev_vel;
+    float *prev_pressure;
+    float *temperature;
+    float *density;
+    uint32_t *tracers;
+    uint32_t count;
+    uint32_t tracer_count;
+    void *stream;
+} FluidDeviceRecords;
+int fluid_sim_device_records(FluidSim *sim, FluidDeviceRecords *out);
+
 /* Cumulative CFL clamps and neighbour-list overflow (stats[9], stats[10]). */
 int fluid_sim_counters(FluidSim *sim, uint64_t *clamps, uint64_t *overflow);
 
@@ -241,6 +270,61 @@ float fluid_slab_depth(void);                                                   
  * are the reference's; the adjoint's differentiable profile is
  * {mu, 0, 0, 0, 0}. No counterpart in the reference (compile-time there). */
 int fluid_sim_set_physics(FluidSim *sim, const float *phys5);
+
+/* ---- the frame driver behind fluid_renderer_* -------------------------------
+ *
+ * Additions to the reference's renderer ABI (include/fluid_ffi.h), for
+ * callers that drive or inspect what fluid_renderer_frame does. The renderer
+ * handle is fluid_ffi.h's FluidRenderer (named by its struct tag here).
+ */
+
+/* The renderer's sim (NULL in the stage-0 microbench mode): its resting set
+ * is what the reference's sprite and disc pullers bind (render.rs:1818-1828);
+ * see fluid_sim_device_records. Owned by the renderer. */
+FluidSim *fluid_renderer_sim(struct FluidRenderer *r);
+/* The stage-0 microbench mode (fluid_renderer_create with bench_sweeps > 0,
+ * render.rs:2429-2729): the first frame's verdicts (validate_bench: the scan
+ * against a CPU scan, every 16th density against a CPU brute force to 1e-3;
+ * 1 pass, 0 fail, -1 before the first frame), the particle and cell counts,
+ * and, when the pointers are not NULL, the last frame's densities (seed
+ * order), the starts (cells + 1) and the seeded positions (4 floats a
+ * particle). FLUID_ERR_ARG outside the bench mode. */
+int fluid_renderer_bench(struct FluidRenderer *r, int *scan_ok, int *density_ok, uint32_t *count, uint32_t *cells,
+                         float *density, uint32_t *starts, float *pos4);
+/* The last stepped frame's substep Step block, substep count, density passes
+ * and tracer Step block (render.rs:3169-3228): replaying them on another sim
+ * from the same state gives the same state. */
+int fluid_renderer_last_step(const struct FluidRenderer *r, uint8_t *step144, uint32_t *substeps, uint32_t *refine,
+                             uint8_t *tracer_step144);
+/* The live ramp of a flat or particle look painted with a gradient
+ * (Ramp::follow, render.rs:708-743): 1 and its two ends, or 0 when no ramp
+ * is live (a solid or glass look, or no stats yet). */
+int fluid_renderer_ramp(const struct FluidRenderer *r, float *ends2);
+/* Lens::ends and Lens::floor (render.rs:649-680) of lens code `lens` (0
+ * velocity, 1 acceleration, 2 pressure, 3 proximity, 4 direction; other
+ * codes paint velocity, lib.rs:215-221) from a 14-float stats block. */
+int fluid_lens_ends(uint32_t lens, const float *stats14, float spacing, float *ends2, float *floor);
+
+/* The input filters of the frame driver, as the renderer runs them each
+ * frame: ForceFilter (render.rs:862-903), RotationTracker (986-1015),
+ * Fingers (912-984) and IdleGate (1024-1104). Pure host code. */
+typedef struct FluidFilters FluidFilters;
+FluidFilters *fluid_filters_create(void);
+void fluid_filters_destroy(FluidFilters *f);
+/* ForceFilter::apply: writes the smoothed body force, returns the deviation. */
+float fluid_filters_force(FluidFilters *f, const float body_force[3], float smooth[3]);
+/* RotationTracker::apply: the smoothed rate and its derivative. */
+void fluid_filters_rotation(FluidFilters *f, const float rate[3], float dt, float omega[3], float domega[3]);
+/* A finger at drawable point (x, y) in [0, 1]^2 (top left 0, 0), or lifted. */
+void fluid_filters_touch(FluidFilters *f, uint32_t slot, float x, float y, bool down);
+/* Fingers::step: the live fingers packed forward as (x, y, vx, vy) in box
+ * metres, their count returned, their shared radius in *radius. */
+uint32_t fluid_filters_fingers(FluidFilters *f, const float extent[2], float dt, float *touches20, float *radius);
+/* IdleGate::asleep. */
+bool fluid_filters_asleep(FluidFilters *f, bool touched, const float smooth[3], float dev, const float omega[3],
+                          float v_max);
+/* IdleGate::STILL_FRAMES: still frames before the gate sleeps. */
+uint32_t fluid_filters_still_frames(void);
 
 /* ---- batched step ---------------------------------------------------------
  *

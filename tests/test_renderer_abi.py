@@ -5,8 +5,8 @@ reseeding, stats and destroy, headless on the HIP step.
 The frame driver's host pieces (dt clamp, substep count, refine passes,
 body force) are pinned against the reference's rules in
 test_host_helpers.py; here the ABI must step, report sane stats, stay
-deterministic, and refuse the modes it does not serve. The refusals and
-the NULL-handle paths touch no device and run without a GPU.
+deterministic, and refuse the mode it does not serve (the sprite demo). The
+refusal and the NULL-handle paths touch no device and run without a GPU.
 """
 import ctypes as C
 import math
@@ -81,9 +81,10 @@ def _phys(s):
     return {k: getattr(s, k) for k in PHYS}
 
 
-def test_unsupported_modes_return_null():
+def test_the_sprite_demo_mode_returns_null():
+    """The M2 sprite demo (sim_substeps = 0, no bench sweeps) is not served;
+    the stage-0 microbench is (tests/test_gpu_frame_driver.py)."""
     L = _abi()
-    assert not L.fluid_renderer_create(None, 1170, 2532, 0, 0.0, 1, 0.0, 8, 0, 1.0)  # stage-0 microbench
     assert not L.fluid_renderer_create(None, 1170, 2532, 4096, 3.0, 0, 0.0, 0, 0, 1.0)  # sprite demo
 
 

@@ -355,8 +355,10 @@ __device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, f3 
 }
 
 // Distances from the particle to the faces of its own cell: a stencil row or
-// end cell farther than the support holds no neighbour, so the walk skips it
-// (the accepted neighbours and their order are those of the full walk).
+// end cell farther than the support holds no neighbour at a finite
+// position, so the walk skips it (the accepted neighbours and their order
+// are those of the full walk; the guard layer, where non-finite positions
+// sort, is always walked).
 struct Reach {
     float dl[3], dh[3];
     float lim;
@@ -373,7 +375,11 @@ __device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, 
         st.dl[a] = wmax(pp[a] - lo, 0.0f);
         st.dh[a] = wmax(hi - pp[a], 0.0f);
     }
-    st.lim = k.support_sq * 1.001f;  // a margin over the rounding of the faces
+    // A margin over the rounding of the faces; a particle with a non-finite
+    // coordinate keeps every candidate it meets (an unordered distance is
+    // not skipped), so it walks its whole stencil.
+    const bool finite = __builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z);
+    st.lim = finite ? k.support_sq * 1.001f : __builtin_huge_valf();
     return st;
 }
 
@@ -455,9 +461,20 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
         const float ez = dz < 0 ? st.dl[2] : (dz > 0 ? st.dh[2] : 0.0f);
         const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
         const float e2 = ey * ey + ez * ez;
-        const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] && e2 <= st.lim;
-        const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.lim) ? bx - 1 : bx;
-        const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
+        // Rows and end cells in the grid's guard layer (a coordinate of 0 or
+        // dims - 1) are never pruned: a NaN or infinite coordinate sorts
+        // there, and the shader keeps such a candidate whatever the
+        // geometry (its distance is unordered, sim_solve.wgsl:450). Such
+        // cells are empty unless a particle sits exactly on an upper wall.
+        const bool guard_row = cz == 0 || cz == (int)k.dims[2] - 1 || cy == 0 || cy == (int)k.dims[1] - 1;
+        const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] &&
+                              (e2 <= st.lim || guard_row);
+        const uint32_t x0 =
+            (bx > 0 && (guard_row || bx == 1u || st.dl[0] * st.dl[0] + e2 <= st.lim)) ? bx - 1 : bx;
+        const uint32_t x1 =
+            (bx + 1 < k.dims[0] && (guard_row || bx + 2 == k.dims[0] || st.dh[0] * st.dh[0] + e2 <= st.lim))
+                ? bx + 1
+                : bx;
         const uint32_t row = live_row ? k.cell_offset + ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0] : 0u;
         k0s[r] = live_row ? starts[row + x0] : 0u;
         k1s[r] = live_row ? starts[row + x1 + 1] : 0u;
@@ -517,7 +534,9 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
                 pb = xyz(pos[jb]);
             }
             const f3 xa = sub(pi, pa), xb = sub(pi, pb);
-            const bool near_a = dot3(xa, xa) < k.support_sq, near_b = dot3(xb, xb) < k.support_sq;
+            // The shader skips a candidate when r2 >= support_sq
+            // (sim_solve.wgsl:450), so an unordered (NaN) distance is kept.
+            const bool near_a = !(dot3(xa, xa) >= k.support_sq), near_b = !(dot3(xb, xb) >= k.support_sq);
             bool acc_a = ina & near_a;
             bool acc_b = inb & near_b;
             // Each ballot of a single compare is that compare's lane mask, so
@@ -609,7 +628,7 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
             for (uint32_t kk = k0; kk < k1; ++kk) {
                 const f3 pj = xyz(pos[kk]);
                 const f3 x = sub(pi, pj);
-                if (dot3(x, x) < k.support_sq && kk != id.p) {
+                if (!(dot3(x, x) >= k.support_sq) && kk != id.p) {
                     if (seen >= b.cap && seen % kLanes == id.lane) density_pair(k, pi, vi, pj, xyz(vel[kk]), a);
                     seen++;
                 }

@@ -13,8 +13,9 @@
 // Schedule. A substep is the reference's dispatch order (render.rs:3168-3203)
 // over all scenes at once; each scene uses its own Step (dt, gravity, ...) and
 // its own number of density passes. In the reference schedule a scene runs
-// its refine count; in tolerance mode it runs passes until its mean density
-// error is under the tolerance. The whole substep can be captured once in a
+// its refine count; in tolerance mode it runs density passes until its mean
+// density error is under the tolerance, and divergence passes until its mean
+// divergence error is under the divergence tolerance. The whole substep can be captured once in a
 // hipGraph and replayed: the per-scene Steps reach the device through a
 // copy node from pinned host memory (two buffers, alternating, so the host
 // fills one while the GPU reads the other).
@@ -76,7 +77,7 @@ struct FluidBatch {
     size_t lds_bytes = 0;
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
-    Tolerance tol{0.0f, 1, 0};
+    Tolerance tol{0.0f, 1, 0, 0.0f, 1};
     Physics phys{};
     EventTimer timer;
     DeviceAllocs mem;
@@ -249,8 +250,11 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.clamps, (size_t)kClampWords * S);
     A(b.zeros, kZeroRows);
     A(b.residual, S);
+    A(b.div_residual, S);
     A(b.iterations, S);
-    A(B->d_pass_limit, S);
+    A(b.div_iterations, S);
+    A(b.pass_hist, 2 * kHistBins);
+    A(B->d_pass_limit, 2 * S);
     A(B->d_ks, S);
     A(B->d_tile_scene, std::max<uint32_t>(B->tiles, 1));
     int lds = 0;
@@ -286,9 +290,9 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
     (void)hipMemcpyAsync(r.dev, r.host, (size_t)B->scenes * sizeof(StepK), hipMemcpyHostToDevice, B->stream);
     const SceneTable sc = B->table(r);
     launch_grid_build(B->stream, sc, B->slots, B->cells, B->rest, B->work, B->grid, B->t());
-    if (B->tol.tol > 0.0f) {
+    if (B->tol.on()) {
         launch_solve_tolerance(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
-                               B->tol, B->d_pass_limit, B->t());
+                               B->tol, passes, B->d_pass_limit, B->t());
     } else {
         TileGroups g;
         if (!B->groups.empty()) {
@@ -326,7 +330,7 @@ int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) 
     if (active == 0) return FLUID_OK;
     if (B->tol.tol > 0.0f) passes = B->tol.max_passes;
     if (B->use_graph && !B->t()) {
-        const uint64_t key = ((uint64_t)passes << 1) | (B->tol.tol > 0.0f ? 1u : 0u);
+        const uint64_t key = ((uint64_t)passes << 2) | (B->tol.tol > 0.0f ? 2u : 0u) | (B->tol.div_tol > 0.0f ? 1u : 0u);
         auto it = r.graphs.find(key);
         if (it == r.graphs.end()) {
             hipGraph_t g = nullptr;
@@ -473,15 +477,30 @@ int fluid_batch_substep(FluidBatch* b, const uint8_t* steps144, const uint32_t* 
 
 int fluid_batch_set_tolerance(FluidBatch* b, float tol, uint32_t min_passes, uint32_t max_passes) {
     if (!valid(b)) return FLUID_ERR_ARG;
-    if (tol > 0.0f && (max_passes == 0 || min_passes > max_passes)) {
-        set_error("tolerance mode needs 0 < min_passes <= max_passes");
+    if (tol > 0.0f && (max_passes == 0 || min_passes > max_passes || max_passes > kMaxPasses)) {
+        set_error("tolerance mode needs 0 < min_passes <= max_passes <= %u", kMaxPasses);
         return FLUID_ERR_ARG;
     }
     // A captured substep holds the tolerance and pass bounds as kernel
     // arguments, so graphs of the old setting are dropped.
     HIP_TRY(hipStreamSynchronize(b->stream));
     b->drop_graphs();
-    b->tol = Tolerance{tol > 0.0f ? tol : 0.0f, std::max<uint32_t>(min_passes, 1), max_passes};
+    b->tol.tol = tol > 0.0f ? tol : 0.0f;
+    b->tol.min_passes = std::max<uint32_t>(min_passes, 1);
+    b->tol.max_passes = max_passes;
+    return FLUID_OK;
+}
+
+int fluid_batch_set_div_tolerance(FluidBatch* b, float div_tol, uint32_t max_passes) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    if (div_tol > 0.0f && (max_passes == 0 || max_passes > kMaxPasses)) {
+        set_error("divergence tolerance mode needs 0 < max_passes <= %u", kMaxPasses);
+        return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graphs();
+    b->tol.div_tol = div_tol > 0.0f ? div_tol : 0.0f;
+    b->tol.div_max = div_tol > 0.0f ? max_passes : 1u;
     return FLUID_OK;
 }
 
@@ -489,6 +508,21 @@ int fluid_batch_iterations(FluidBatch* b, uint32_t* out) {
     if (!valid(b) || !out) return FLUID_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(b->stream));
     HIP_TRY(hipMemcpy(out, b->solve.iterations, (size_t)b->scenes * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_batch_div_iterations(FluidBatch* b, uint32_t* out) {
+    if (!valid(b) || !out) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    HIP_TRY(hipMemcpy(out, b->solve.div_iterations, (size_t)b->scenes * 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_batch_pass_histogram(FluidBatch* b, uint32_t* out, bool reset) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    if (out) HIP_TRY(hipMemcpy(out, b->solve.pass_hist, 2 * kHistBins * 4, hipMemcpyDeviceToHost));
+    if (reset) HIP_TRY(hipMemset(b->solve.pass_hist, 0, 2 * kHistBins * 4));
     return FLUID_OK;
 }
 
@@ -683,9 +717,11 @@ int fluid_batch_describe(const FluidBatch* b, char* buf, size_t size) {
     if (!valid(b) || !buf || size == 0) return FLUID_ERR_ARG;
     snprintf(buf, size,
              "{\"scenes\": %u, \"slots\": %u, \"cells\": %u, \"tiles\": %u, \"nbr_cap\": %u, \"stage_cap\": %u, "
-             "\"lds_bytes\": %zu, \"graph\": %s, \"tolerance\": %g, \"max_passes\": %u}",
+             "\"lds_bytes\": %zu, \"graph\": %s, \"tolerance\": %g, \"max_passes\": %u, "
+             "\"div_tolerance\": %g, \"div_max_passes\": %u}",
              b->scenes, b->slots, b->cells, b->tiles, b->cap, b->solve.stage_cap, b->lds_bytes,
-             b->use_graph ? "true" : "false", (double)b->tol.tol, b->tol.max_passes);
+             b->use_graph ? "true" : "false", (double)b->tol.tol, b->tol.max_passes, (double)b->tol.div_tol,
+             b->tol.div_max);
     return FLUID_OK;
 }
 

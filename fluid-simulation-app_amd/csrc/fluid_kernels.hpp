@@ -25,6 +25,21 @@ struct GridGeom {
 // The den_kappa / den_apply pass argument of a launch that runs whatever
 // the scene's pass count (everything but the density passes).
 constexpr uint32_t kEveryPass = 0xffffffffu;
+// Tolerance mode's further divergence passes: kDivPass | r is divergence
+// pass r >= 1 (pass 0 is density_div's kappa and div_apply, which always
+// run).
+constexpr uint32_t kDivPass = 0x80000000u;
+
+// Whether a scene runs pass `pass` this substep: density pass r while r is
+// below its pass limit (tolerance mode) or its refine count; divergence pass
+// r while r is below its divergence limit (only tolerance mode launches
+// them).
+__device__ __forceinline__ bool pass_admits(uint32_t pass, uint32_t refine, const uint32_t* den_limit,
+                                            const uint32_t* div_limit, uint32_t scene) {
+    if (pass == kEveryPass) return true;
+    if (pass & kDivPass) return div_limit && (pass & ~kDivPass) < div_limit[scene];
+    return pass < (den_limit ? den_limit[scene] : refine);
+}
 
 // `flip` reverses a launch's tile order, and a launch covers the tiles from
 // `tile0` on (see tile_index in solve.hip).
@@ -32,12 +47,17 @@ struct OneScene {
     SceneK k;
     StepK s;
     uint32_t flip = 0, tile0 = 0;
+    // Tolerance mode: the density and divergence pass limits (device words,
+    // lowered when the scene converges); null: the refine count and one
+    // divergence pass.
+    const uint32_t* pass_limit = nullptr;
+    const uint32_t* div_limit = nullptr;
     __device__ __forceinline__ uint32_t index(uint32_t) const { return 0; }
     __device__ __forceinline__ const SceneK& scene(uint32_t) const { return k; }
     __device__ __forceinline__ const StepK& step(uint32_t) const { return s; }
-    // Whether the tile takes part in a launch of density pass `pass`.
+    // Whether the tile takes part in a launch of pass `pass`.
     __device__ __forceinline__ bool runs(uint32_t, uint32_t pass) const {
-        return s.active != 0u && (pass == kEveryPass || pass < s.refine);
+        return s.active != 0u && pass_admits(pass, s.refine, pass_limit, div_limit, 0);
     }
     // The same from a tile's region record (scene fields of TileRegion);
     // here the kernel arguments already hold them.
@@ -53,27 +73,25 @@ struct SceneTable {
     const SceneK* ks;             // [scenes]
     const StepK* ss;              // [scenes]
     const uint32_t* tile_scene;   // [tiles]
-    // [scenes] density passes each scene runs this substep, lowered when it
-    // converges (tolerance mode); null: the Step's refine count.
-    const uint32_t* pass_limit;
+    // [scenes] density and divergence passes each scene runs this substep,
+    // lowered when it converges (tolerance mode); null: the Step's refine
+    // count and one divergence pass.
+    const uint32_t* pass_limit = nullptr;
+    const uint32_t* div_limit = nullptr;
     uint32_t flip = 0, tile0 = 0;
     __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile]; }
     __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[tile_scene[tile]]; }
     __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[tile_scene[tile]]; }
     __device__ __forceinline__ bool runs(uint32_t tile, uint32_t pass) const {
         const uint32_t i = tile_scene[tile];
-        if (ss[i].active == 0u) return false;
-        if (pass == kEveryPass) return true;
-        return pass < (pass_limit ? pass_limit[i] : ss[i].refine);
+        return ss[i].active != 0u && pass_admits(pass, ss[i].refine, pass_limit, div_limit, i);
     }
     __device__ __forceinline__ uint32_t index_at(uint32_t scene) const { return scene; }
     __device__ __forceinline__ const SceneK& scene_at(uint32_t scene) const { return ks[scene]; }
     __device__ __forceinline__ const StepK& step_at(uint32_t scene) const { return ss[scene]; }
     __device__ __forceinline__ uint32_t slot_end_at(uint32_t slot_end) const { return slot_end; }
     __device__ __forceinline__ bool runs_at(uint32_t scene, uint32_t active, uint32_t refine, uint32_t pass) const {
-        if (active == 0u) return false;
-        if (pass == kEveryPass) return true;
-        return pass < (pass_limit ? pass_limit[scene] : refine);
+        return active != 0u && pass_admits(pass, refine, pass_limit, div_limit, scene);
     }
 };
 
@@ -155,15 +173,25 @@ struct SolveBufs {
     const float4* zeros;
     // Staging capacity (slots) shared by every sweep.
     uint32_t stage_cap;
-    // Tolerance mode (batched): per scene, the fixed-point (2^-32) sum of the
-    // density errors max(rho* - rho0, 0) / rho0 of the current pass, and the
-    // passes run. Null in the reference schedule.
+    // Tolerance mode: per scene, the fixed-point (2^-32) sums of the current
+    // pass's density errors max(rho* - rho0, 0) / rho0 and divergence errors
+    // max(Drho/Dt, 0) dt / rho0, and the passes of each solve this substep.
+    // Null in the reference schedule.
     unsigned long long* residual;
+    unsigned long long* div_residual;
     uint32_t* iterations;
+    uint32_t* div_iterations;
+    // Tolerance mode's pass-count histograms, [2][kHistBins] (density, then
+    // divergence): one count per scene and substep at the passes it ran.
+    uint32_t* pass_hist;
     TileRegion* regions;  // [tiles]
 };
 
 constexpr uint32_t kZeroRows = 4096;
+// Tolerance mode: at most kMaxPasses passes of either solve; the histograms
+// hold pass counts 0 .. kMaxPasses.
+constexpr uint32_t kMaxPasses = 256;
+constexpr uint32_t kHistBins = kMaxPasses + 1;
 constexpr uint32_t kClampSlots = 16;
 constexpr uint32_t kClampStride = 32;
 constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
@@ -322,16 +350,32 @@ struct StageBufs {
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
                          const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
                          const StageBufs& T);
-// Tolerance mode of a batch: up to max_passes density passes, a scene
+// Tolerance mode (DFSPH's two solves iterated to their tolerances): the
+// divergence solve runs density_div's kappa and div_apply, then, while the
+// scene's mean divergence error max(Drho/Dt, 0) dt / rho0 of the last pass
+// is above div_tol, further passes of (divergence kappa, div_apply), at most
+// div_max in all; the density solve runs up to max_passes passes, a scene
 // stopping after the first pass (at least min_passes) whose mean density
-// error is <= tol. sc.pass_limit and b.residual / b.iterations are set.
+// error max(rho* - rho0, 0) / rho0 is <= tol. A pass whose error meets the
+// tolerance still applies its correction, and is the last. tol = 0 keeps
+// the refine count (`passes` for a single scene, each scene's own in a
+// batch); div_tol = 0 keeps one divergence pass. `limits` holds 2 x scenes
+// device words (density limits, then divergence limits); sc's pass_limit
+// and div_limit are set from it, and b's residuals, iterations and pass
+// histograms are used.
 struct Tolerance {
     float tol;
     uint32_t min_passes, max_passes;
+    float div_tol;
+    uint32_t div_max;
+    bool on() const { return tol > 0.0f || div_tol > 0.0f; }
 };
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
                             const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
-                            const Tolerance& tol, uint32_t* pass_limit, KernelTimer* timer);
+                            const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer);
+void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
+                            const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
+                            uint32_t passes, uint32_t* limits, KernelTimer* timer);
 // The density sweep alone (the zero-substep path of the reference's tests).
 void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
                          const Records& work, const SolveBufs& b, KernelTimer* timer);

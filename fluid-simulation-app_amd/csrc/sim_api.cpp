@@ -105,6 +105,9 @@ struct FluidSim {
     TracerBufs tracers{};
     bool track_ids = false;
     size_t lds_bytes = 0;
+    // Tolerance mode (off: the reference schedule) and its two pass limits.
+    Tolerance tol{0.0f, 1, 0, 0.0f, 1};
+    uint32_t* limits = nullptr;
     EventTimer timer;
     DeviceAllocs mem;
 
@@ -204,6 +207,12 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.counters, 4);
     A(b.clamps, kClampWords);
     A(b.zeros, kZeroRows);
+    A(b.residual, 1);
+    A(b.div_residual, 1);
+    A(b.iterations, 1);
+    A(b.div_iterations, 1);
+    A(b.pass_hist, 2 * kHistBins);
+    A(s->limits, 2);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
     s->lds_bytes = (size_t)lds;
@@ -244,7 +253,11 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
     const OneScene sc{s->scene, k};
     for (uint32_t i = 0; i < nsub; ++i) {
         launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
-        launch_solve(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
+        if (s->tol.on())
+            launch_solve_tolerance(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, s->tol, refine,
+                                   s->limits, s->t());
+        else
+            launch_solve(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
     }
     return check_launch("substep");
 }
@@ -392,6 +405,42 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
         }
         off += list_tail_len(cnt[i]);
     }
+    return FLUID_OK;
+}
+
+int fluid_sim_set_tolerance(FluidSim* s, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
+                            uint32_t div_max_passes) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    if (tol > 0.0f && (max_passes == 0 || min_passes > max_passes || max_passes > kMaxPasses)) {
+        set_error("tolerance mode needs 0 < min_passes <= max_passes <= %u", kMaxPasses);
+        return FLUID_ERR_ARG;
+    }
+    if (div_tol > 0.0f && (div_max_passes == 0 || div_max_passes > kMaxPasses)) {
+        set_error("divergence tolerance mode needs 0 < max_passes <= %u", kMaxPasses);
+        return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->tol.tol = tol > 0.0f ? tol : 0.0f;
+    s->tol.min_passes = std::max<uint32_t>(min_passes, 1);
+    s->tol.max_passes = max_passes;
+    s->tol.div_tol = div_tol > 0.0f ? div_tol : 0.0f;
+    s->tol.div_max = div_tol > 0.0f ? div_max_passes : 1u;
+    return FLUID_OK;
+}
+
+int fluid_sim_iterations(FluidSim* s, uint32_t out[2]) {
+    if (!valid(s) || !out) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(out, s->solve.iterations, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(out + 1, s->solve.div_iterations, 4, hipMemcpyDeviceToHost));
+    return FLUID_OK;
+}
+
+int fluid_sim_pass_histogram(FluidSim* s, uint32_t* out, bool reset) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (out) HIP_TRY(hipMemcpy(out, s->solve.pass_hist, 2 * kHistBins * 4, hipMemcpyDeviceToHost));
+    if (reset) HIP_TRY(hipMemset(s->solve.pass_hist, 0, 2 * kHistBins * 4));
     return FLUID_OK;
 }
 

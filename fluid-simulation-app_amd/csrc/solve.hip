@@ -637,11 +637,35 @@ __device__ __forceinline__ void density_sums(const SceneK& k, const LaneId& id, 
     }
 }
 
+// The workgroup's sum of the particles' fixed-point errors: a wave sum, the waves' sums through LDS, one atomic. Integer sums, so the total
+// is the same in any order.
+__device__ __forceinline__ void block_residual(unsigned long long v, unsigned long long* out) {
+    __shared__ unsigned long long s_wave[kSweepThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += (unsigned long long)__shfl_xor((long long)v, o);
+    if (__lane_id() == 0) s_wave[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (uint32_t w = 0; w < kSweepThreads / 64; ++w) t += s_wave[w];
+        if (t) atomicAdd(out, t);
+    }
+}
+
+// A particle's error e as a 2^-32 fixed-point integer (tolerance mode),
+// saturated at 2^40 (e = 256): NaN and non-positive errors count zero.
+__device__ __forceinline__ unsigned long long fixed_err(float e) {
+    e = e * 4294967296.0f;
+    return e > 0.0f ? (unsigned long long)(e < 1099511627776.0f ? e : 1099511627776.0f) : 0ull;
+}
+
 // Lane 0's end of density_div for particle i: the wall terms, ρ, α and the
 // divergence κ/ρ. Returns whether the particle's wall gradient is nonzero.
 __device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                                f3 pi, f3 vi, uint32_t found, float w_sum, float rn, float gsq_sum,
-                                               float drho_sum, f3 gsum_sum, const SolveBufs& b) {
+                                               float drho_sum, f3 gsum_sum, const SolveBufs& b,
+                                               unsigned long long& err) {
     const uint32_t i = id.p;
     const uint32_t nn = found < b.cap ? found : b.cap;
     b.nbr_n[i] = nn;
@@ -674,6 +698,7 @@ __device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, 
     b.alpha[i] = al;
     drho += dot3(vi, wg);
     b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
+    if (b.div_residual) err = fixed_err(wmax(drho, 0.0f) * s.dt / k.rho0);
     return wg.x != 0.0f || wg.y != 0.0f || wg.z != 0.0f;
 }
 
@@ -681,7 +706,7 @@ template <bool STAGED>
 __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                              const RegionS& R, const uint32_t* __restrict__ starts,
                                              const float4* __restrict__ pos, const float4* __restrict__ vel,
-                                             const SolveBufs& b, const PvLds& L) {
+                                             const SolveBufs& b, const PvLds& L, unsigned long long& err) {
     if (STAGED) {
         stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b));
         __syncthreads();
@@ -708,7 +733,7 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     if (!id.live || id.lane != 0) return;
     // A particle near a wall marks its tile (k_tile_regions cleared the mark
     // this substep; any number of lanes may store the same 1).
-    if (density_finish(k, s, counters, id, pi, vi, found, w_sum, rn, gsq_sum, drho_sum, gsum_sum, b))
+    if (density_finish(k, s, counters, id, pi, vi, found, w_sum, rn, gsq_sum, drho_sum, gsum_sum, b, err))
         b.regions[id.tile].walls = 1u;
 }
 
@@ -730,10 +755,13 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
     uint32_t* counters = b.counters + 2 * sc.index_at(R.scene);
     const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b)),
                   (LdsU16*)(reinterpret_cast<char*>(smem) + stage_slots(b) * kPvEntry)};
+    unsigned long long err = 0;
     if (R.staged)
-        density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L);
+        density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L, err);
     else
-        density_tile<false>(k, s, counters, id, R, starts, pos, vel, b, L);
+        density_tile<false>(k, s, counters, id, R, starts, pos, vel, b, L, err);
+    // Tolerance mode: the divergence error of the solve's first pass.
+    if (b.div_residual) block_residual(err, b.div_residual + sc.index_at(R.scene));
 }
 
 // ---- list sweeps -------------------------------------------------------------
@@ -1037,23 +1065,6 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
     }
 }
 
-// The workgroup's sum of the particles' fixed-point density errors: a wave
-// sum, the waves' sums through LDS, one atomic. Integer sums, so the total
-// is the same in any order.
-__device__ __forceinline__ void block_residual(unsigned long long v, unsigned long long* out) {
-    __shared__ unsigned long long s_wave[kSweepThreads / 64];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += (unsigned long long)__shfl_xor((long long)v, o);
-    if (__lane_id() == 0) s_wave[threadIdx.x / 64] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-#pragma unroll
-        for (uint32_t w = 0; w < kSweepThreads / 64; ++w) t += s_wave[w];
-        if (t) atomicAdd(out, t);
-    }
-}
-
 // The particle's own fields den_kappa ends with, loaded with the list head.
 struct DenOwn {
     float rho, alpha, pressure;
@@ -1097,7 +1108,10 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
     return fold4(drho);
 }
 
-template <class Sc>
+// den_kappa (sim_solve.wgsl:674-698), and with DIV the divergence kappa of
+// tolerance mode's further divergence passes: density_div's kappa formula
+// (sim_solve.wgsl:508-516) from the current velocities, no pressure.
+template <class Sc, bool DIV = false>
 __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
@@ -1113,18 +1127,24 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
     const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi, own)
                                  : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi, own);
     unsigned long long err = 0;
+    unsigned long long* const sum = DIV ? b.div_residual : b.residual;
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
         const float rho = own.rho;
         const float d = k.mass * total + dot3(vi, xyz(own.wg));
-        const float rho_star = rho + s.dt * d;
-        const float excess = wmax(rho_star - k.rho0, 0.0f);
-        const float kap = excess * own.alpha / (s.dt * s.dt);
-        b.kd[i] = kap / rho;
-        b.pressure[i] = own.pressure + kap * rho;
-        if (b.residual) err = (unsigned long long)(excess / k.rho0 * 4294967296.0f);
+        if (DIV) {
+            b.kd[i] = wmax(d, 0.0f) * own.alpha / s.dt_guard / rho;
+            err = fixed_err(wmax(d, 0.0f) * s.dt / k.rho0);
+        } else {
+            const float rho_star = rho + s.dt * d;
+            const float excess = wmax(rho_star - k.rho0, 0.0f);
+            const float kap = excess * own.alpha / (s.dt * s.dt);
+            b.kd[i] = kap / rho;
+            b.pressure[i] = own.pressure + kap * rho;
+            if (sum) err = fixed_err(excess / k.rho0);
+        }
     }
-    if (b.residual) block_residual(err, b.residual + sc.index_at(R.scene));
+    if (sum) block_residual(err, sum + sc.index_at(R.scene));
 }
 
 // Reads the working set, writes the resting set at the same slot.
@@ -1176,32 +1196,51 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
                   (uint32_t)__popcll(m));
 }
 
-// Tolerance mode, once per scene after den_kappa of `pass`: the mean
-// density error of the pass; a scene at or below tol (after at least
-// min_passes) runs this pass's den_apply and no further pass.
-__global__ __launch_bounds__(kBlock) void k_converge(SceneTable sc, uint32_t scenes, uint32_t pass, float tol,
+// Tolerance mode, once per scene after the kappa sweep of a pass of one
+// solve (`residual`, `limit`, `iterations` that solve's words): the mean
+// error of the pass; a scene at or below tol (after at least min_passes)
+// runs this pass's apply and no further pass.
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_converge(Sc sc, uint32_t scenes, uint32_t pass, float tol,
                                                      uint32_t min_passes, unsigned long long* __restrict__ residual,
-                                                     uint32_t* __restrict__ pass_limit,
+                                                     uint32_t* __restrict__ limit,
                                                      uint32_t* __restrict__ iterations) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= scenes || sc.ss[i].active == 0u || pass >= pass_limit[i]) return;
-    const SceneK& k = sc.ks[i];
+    if (i >= scenes || sc.step_at(i).active == 0u || pass >= limit[i]) return;
+    const SceneK& k = sc.scene_at(i);
     const uint32_t n = k.slot_end - k.slot_begin;
     const double mean = (double)residual[i] * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
     residual[i] = 0;
     iterations[i] = pass + 1;
-    if (mean <= (double)tol && pass + 1 >= min_passes) pass_limit[i] = pass + 1;
+    if (mean <= (double)tol && pass + 1 >= min_passes) limit[i] = pass + 1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reset_passes(uint32_t scenes, uint32_t max_passes,
-                                                         unsigned long long* __restrict__ residual,
-                                                         uint32_t* __restrict__ pass_limit,
-                                                         uint32_t* __restrict__ iterations) {
+// Tolerance mode, before a substep's solve: clears the residuals and sets
+// each scene's limits to the caps (den_max 0: the density solve keeps its
+// refine count, and its limit is that count).
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_reset_passes(Sc sc, uint32_t scenes, uint32_t den_max, uint32_t div_max,
+                                                         SolveBufs b, uint32_t* __restrict__ limits) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= scenes) return;
-    residual[i] = 0;
-    pass_limit[i] = max_passes;
-    iterations[i] = 0;
+    b.residual[i] = 0;
+    b.div_residual[i] = 0;
+    limits[i] = den_max ? den_max : sc.step_at(i).refine;
+    limits[scenes + i] = div_max;
+    b.iterations[i] = den_max ? 0u : sc.step_at(i).refine;
+    b.div_iterations[i] = div_max > 1u ? 0u : 1u;
+}
+
+// Tolerance mode, after a substep's solve: one histogram count per active
+// scene at the passes each solve ran.
+template <class Sc>
+__global__ __launch_bounds__(kBlock) void k_tally_passes(Sc sc, uint32_t scenes, SolveBufs b) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= scenes || sc.step_at(i).active == 0u) return;
+    const uint32_t den = b.iterations[i] < kMaxPasses ? b.iterations[i] : kMaxPasses;
+    const uint32_t div = b.div_iterations[i] < kMaxPasses ? b.div_iterations[i] : kMaxPasses;
+    atomicAdd(&b.pass_hist[den], 1u);
+    atomicAdd(&b.pass_hist[kHistBins + div], 1u);
 }
 
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
@@ -1230,11 +1269,13 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
                          reinterpret_cast<const void*>(&k_forces_eval<OneScene>),
                          reinterpret_cast<const void*>(&k_forces_den_apply<OneScene>),
                          reinterpret_cast<const void*>(&k_den_kappa<OneScene>),
+                         reinterpret_cast<const void*>(&k_den_kappa<OneScene, true>),
                          reinterpret_cast<const void*>(&k_density_div<SceneTable>),
                          reinterpret_cast<const void*>(&k_kappa_apply<SceneTable>),
                          reinterpret_cast<const void*>(&k_forces_eval<SceneTable>),
                          reinterpret_cast<const void*>(&k_forces_den_apply<SceneTable>),
-                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable>)};
+                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable>),
+                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable, true>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
             avail = 65536 - 1024;
@@ -1259,7 +1300,7 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
 }
 
 struct NoCheck {
-    void operator()(hipStream_t, uint32_t) const {}
+    void operator()(hipStream_t, uint32_t, bool) const {}
 };
 // The substep's launches. With stage buffers (the adjoint's taped forward)
 // each stage's velocities and kd go to arrays of their own instead of being
@@ -1269,10 +1310,14 @@ struct NoCheck {
 // With tile groups, the sweeps from density_div to the last den_apply run
 // group by group (tile regions and integrate cover every tile at once), in
 // turn over the extra streams when there are some and no kernel timer.
+// div_passes > 1 (tolerance mode) adds up to div_passes - 1 further
+// divergence passes (divergence kappa, div_apply) after div_apply; `check`
+// runs after each pass's kappa (density_div's for divergence pass 0).
 template <class Sc, class Check = NoCheck>
 static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer,
-                  const Check& check = Check(), const StageBufs* T = nullptr, const TileGroups& G = TileGroups()) {
+                  const Check& check = Check(), const StageBufs* T = nullptr, const TileGroups& G = TileGroups(),
+                  uint32_t div_passes = 1) {
     if (n == 0) return;
     const uint32_t ntiles = solve_tiles(n);
     const dim3 sweep(kSweepThreads);
@@ -1307,9 +1352,21 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         const Span tdd = g == 0 ? tdd0 : span(timer, KERNEL_DENSITY_DIV);
         hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s, g == 0 ? nullptr : tdd.a, tdd.z, 0,
                               fwd, starts, work.pos, V(0), KB(0));
+        if (div_passes > 1) check(s, 0, true);
         const Span td = span(timer, KERNEL_DIV_APPLY);
         hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, td.a, td.z, 0, rev, kEveryPass, work.pos,
                               V(0), V(1), KB(0));
+        // Tolerance mode's further divergence passes, in place on the
+        // velocities div_apply left (timed with div_apply).
+        for (uint32_t r = 1; r < div_passes; ++r) {
+            const Span tk = span(timer, KERNEL_DIV_APPLY);
+            hipExtLaunchKernelGGL(k_den_kappa<Sc, true>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0,
+                                  fwd, kDivPass | r, work.pos, V(1), KB(0));
+            check(s, r, true);
+            const Span tp = span(timer, KERNEL_DIV_APPLY);
+            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, rev, kDivPass | r,
+                                  work.pos, V(1), V(1), KB(0));
+        }
         const Span tf = span(timer, KERNEL_FORCES_EVAL);
         hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, s, tf.a, tf.z, 0, fwd,
                               work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
@@ -1320,7 +1377,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
             hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0, fwd, r,
                                   work.pos, V(2 + r), KB(2 + r));
-            check(s, r);
+            check(s, r, false);
             const Span tp = span(timer, KERNEL_DEN_APPLY);
             hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, rev, r, work.pos,
                                   V(2 + r), V(3 + r), KB(2 + r));
@@ -1341,23 +1398,28 @@ void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, con
     density_only(st, sc, slots, starts, work, b, timer);
 }
 
+// The reference schedule: no residuals, no pass limits.
+static SolveBufs fixed_bufs(const SolveBufs& b) {
+    SolveBufs f = b;
+    f.residual = nullptr;
+    f.div_residual = nullptr;
+    return f;
+}
+
 void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts, const Records& work,
                   const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer) {
     OneScene one = sc;
     one.s.refine = passes;
-    SolveBufs fixed = b;
-    fixed.residual = nullptr;
-    solve(st, one, slots, starts, work, rest, fixed, passes, timer);
+    one.pass_limit = one.div_limit = nullptr;
+    solve(st, one, slots, starts, work, rest, fixed_bufs(b), passes, timer);
 }
 
 void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
                   const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
                   KernelTimer* timer, const TileGroups& groups) {
-    SolveBufs fixed = b;
-    fixed.residual = nullptr;
     SceneTable t = sc;
-    t.pass_limit = nullptr;
-    solve(st, t, slots, starts, work, rest, fixed, passes, timer, NoCheck(), nullptr, groups);
+    t.pass_limit = t.div_limit = nullptr;
+    solve(st, t, slots, starts, work, rest, fixed_bufs(b), passes, timer, NoCheck(), nullptr, groups);
 }
 
 void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
@@ -1365,24 +1427,51 @@ void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, con
                          const StageBufs& T) {
     OneScene one = sc;
     one.s.refine = passes;
-    SolveBufs fixed = b;
-    fixed.residual = nullptr;
-    solve(st, one, slots, starts, work, rest, fixed, passes, nullptr, NoCheck(), &T);
+    one.pass_limit = one.div_limit = nullptr;
+    solve(st, one, slots, starts, work, rest, fixed_bufs(b), passes, nullptr, NoCheck(), &T);
+}
+
+// Both tolerance forms: reset the limits, run the solve with a convergence
+// check after each kappa sweep of a solve under tolerance, tally the passes.
+// One group of tiles only: a check covers every scene, so each pass must
+// have run on all of them first.
+template <class Sc>
+static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slots, const uint32_t* starts,
+                            const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
+                            uint32_t passes, uint32_t* limits, KernelTimer* timer) {
+    const bool den = tol.tol > 0.0f, div = tol.div_tol > 0.0f && tol.div_max > 1u;
+    sc.pass_limit = limits;
+    sc.div_limit = limits + scenes;
+    SolveBufs t = b;
+    if (!den) t.residual = nullptr;
+    if (!div) t.div_residual = nullptr;
+    const dim3 grid(blocks_for(scenes));
+    hipLaunchKernelGGL(k_reset_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, den ? tol.max_passes : 0u,
+                       div ? tol.div_max : 1u, b, limits);
+    auto check = [&](hipStream_t s2, uint32_t pass, bool divergence) {
+        if (divergence ? !div : !den) return;
+        hipLaunchKernelGGL(k_converge<Sc>, grid, dim3(kBlock), 0, s2, sc, scenes, pass,
+                           divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
+                           divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
+                           divergence ? b.div_iterations : b.iterations);
+    };
+    solve(st, sc, slots, starts, work, rest, t, den ? tol.max_passes : passes, timer, check, nullptr, TileGroups(),
+          div ? tol.div_max : 1u);
+    hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
 }
 
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
                             const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
-                            const Tolerance& tol, uint32_t* pass_limit, KernelTimer* timer) {
-    SceneTable t = sc;
-    t.pass_limit = pass_limit;
-    const dim3 grid(blocks_for(scenes));
-    hipLaunchKernelGGL(k_reset_passes, grid, dim3(kBlock), 0, st, scenes, tol.max_passes, b.residual, pass_limit,
-                       b.iterations);
-    auto check = [&](hipStream_t s2, uint32_t pass) {
-        hipLaunchKernelGGL(k_converge, grid, dim3(kBlock), 0, s2, t, scenes, pass, tol.tol, tol.min_passes,
-                           b.residual, pass_limit, b.iterations);
-    };
-    solve(st, t, slots, starts, work, rest, b, tol.max_passes, timer, check);
+                            const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer) {
+    solve_tolerance(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
+}
+
+void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
+                            const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
+                            uint32_t passes, uint32_t* limits, KernelTimer* timer) {
+    OneScene one = sc;
+    one.s.refine = passes;
+    solve_tolerance(st, one, 1u, slots, starts, work, rest, b, tol, passes, limits, timer);
 }
 
 }  // namespace fluid

@@ -25,6 +25,9 @@ FIELDS = {
     "work_pos": (8, 4),
     "work_vel": (9, 4),
 }
+# Bins of the tolerance-mode pass histograms: pass counts 0 .. 256.
+HIST_BINS = 257
+
 KERNELS = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
            "forces_den_apply", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"]
 NBR_CAP = 96
@@ -244,6 +247,27 @@ class Sim:
     def substeps(self, step144: bytes, n: int, refine: int):
         check(self.L.fluid_sim_substeps(self.h, _u8p(step144), int(n), int(refine)), "substeps")
 
+    def set_tolerance(self, tol: float = 0.0, min_passes: int = 1, max_passes: int = 0, div_tol: float = 0.0,
+                      div_max_passes: int = 0):
+        """Tolerance mode: the density solve iterates until its mean error is
+        at most `tol` (min_passes .. max_passes passes), the divergence solve
+        until its mean error is at most `div_tol` (at most div_max_passes);
+        a tolerance of 0 keeps that solve's reference schedule."""
+        check(self.L.fluid_sim_set_tolerance(self.h, float(tol), int(min_passes), int(max_passes), float(div_tol),
+                                             int(div_max_passes)), "sim set_tolerance")
+
+    def iterations(self) -> tuple:
+        """(density passes, divergence passes) of the last substep."""
+        out = np.zeros(2, np.uint32)
+        check(self.L.fluid_sim_iterations(self.h, _u32p(out)), "sim iterations")
+        return int(out[0]), int(out[1])
+
+    def pass_histogram(self, reset: bool = False) -> np.ndarray:
+        """[2, HIST_BINS] counts of substeps by passes run (density, divergence)."""
+        out = np.zeros(2 * HIST_BINS, np.uint32)
+        check(self.L.fluid_sim_pass_histogram(self.h, _u32p(out), bool(reset)), "sim pass_histogram")
+        return out.reshape(2, HIST_BINS)
+
     def density_only(self, step144: bytes):
         check(self.L.fluid_sim_density_only(self.h, _u8p(step144)), "density_only")
 
@@ -386,10 +410,26 @@ class Batch:
         check(self.L.fluid_batch_set_tolerance(self.h, float(tol), int(min_passes), int(max_passes)),
               "batch set_tolerance")
 
+    def set_div_tolerance(self, div_tol: float, max_passes: int = 0):
+        check(self.L.fluid_batch_set_div_tolerance(self.h, float(div_tol), int(max_passes)),
+              "batch set_div_tolerance")
+
     def iterations(self) -> np.ndarray:
         out = np.empty(self.scenes, np.uint32)
         check(self.L.fluid_batch_iterations(self.h, _u32p(out)), "batch iterations")
         return out
+
+    def div_iterations(self) -> np.ndarray:
+        out = np.empty(self.scenes, np.uint32)
+        check(self.L.fluid_batch_div_iterations(self.h, _u32p(out)), "batch div_iterations")
+        return out
+
+    def pass_histogram(self, reset: bool = False) -> np.ndarray:
+        """[2, HIST_BINS] counts of (scene, substep) pairs by passes run (density,
+        divergence) over the substeps under tolerance mode."""
+        out = np.zeros(2 * HIST_BINS, np.uint32)
+        check(self.L.fluid_batch_pass_histogram(self.h, _u32p(out), bool(reset)), "batch pass_histogram")
+        return out.reshape(2, HIST_BINS)
 
     def use_graph(self, enable=True):
         check(self.L.fluid_batch_use_graph(self.h, bool(enable)), "batch use_graph")

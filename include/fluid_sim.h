@@ -122,6 +122,22 @@ int fluid_sim_download_ids(FluidSim *sim, uint32_t *ids);
 int fluid_sim_substeps(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32_t refine);
 
 /*
+ * Tolerance mode of one sim, the batch's fluid_batch_set_tolerance and
+ * fluid_batch_set_div_tolerance for a single scene: the density solve
+ * iterates to `tol` (min_passes .. max_passes <= 256 passes; the substeps'
+ * `refine` is then ignored) and the divergence solve to `div_tol` (at most
+ * div_max_passes <= 256 passes). A tolerance <= 0 keeps that solve's
+ * reference schedule; both <= 0 is the reference step.
+ */
+int fluid_sim_set_tolerance(FluidSim *sim, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
+                            uint32_t div_max_passes);
+/* Passes of the last substep: out[0] density, out[1] divergence. */
+int fluid_sim_iterations(FluidSim *sim, uint32_t out[2]);
+/* The pass-count histograms of every substep under tolerance mode since the
+ * last reset, as fluid_batch_pass_histogram: 2 x 257 words. */
+int fluid_sim_pass_histogram(FluidSim *sim, uint32_t *out, bool reset);
+
+/*
  * Grid rebuild plus the density sweep alone: the zero-substep path of the
  * reference's test driver (render.rs:3902-3937). Asynchronous.
  */
@@ -360,13 +376,31 @@ int fluid_batch_download_grid(FluidBatch *batch, uint32_t scene, uint32_t *celli
  * `refine` one density-pass count per scene (render.rs:77-79). A scene
  * whose Step has dt <= 0 sits the substep out (render.rs:3165-3167). */
 int fluid_batch_substep(FluidBatch *batch, const uint8_t *steps144, const uint32_t *refine);
-/* Tolerance mode (tol > 0): each scene runs density passes until the mean
- * of max(rho* - rho0, 0) / rho0 over its particles is <= tol, at least
- * min_passes and at most max_passes; `refine` is then ignored. tol <= 0
- * restores the reference schedule. */
+/* Tolerance mode of the density solve (tol > 0): each scene runs density
+ * passes until the mean of max(rho* - rho0, 0) / rho0 over its particles is
+ * <= tol, at least min_passes and at most max_passes (<= 256); `refine` is
+ * then ignored. A pass that meets the tolerance still applies its
+ * correction and is the last. tol <= 0 restores the reference schedule.
+ * The reference fixes the pass count (sim_solve.wgsl:674-703, render.rs:
+ * 77-79); this iterates the same den_kappa / den_apply pair to a tolerance,
+ * as DFSPH's density solver does. */
 int fluid_batch_set_tolerance(FluidBatch *batch, float tol, uint32_t min_passes, uint32_t max_passes);
-/* Density passes each scene ran in the last tolerance-mode substep. */
+/* Tolerance mode of the divergence solve (div_tol > 0): after density_div's
+ * kappa and div_apply (the reference's one pass, sim_solve.wgsl:508-516,
+ * 665-668), each scene runs further passes of (divergence kappa from the
+ * current velocities, div_apply) while the mean of max(Drho/Dt, 0) dt / rho0
+ * of the last pass is above div_tol, at most max_passes (<= 256) in all.
+ * div_tol <= 0 restores the one pass. */
+int fluid_batch_set_div_tolerance(FluidBatch *batch, float div_tol, uint32_t max_passes);
+/* Density passes each scene ran in the last substep under tolerance mode. */
 int fluid_batch_iterations(FluidBatch *batch, uint32_t *out);
+/* Divergence passes each scene ran in the last substep under tolerance mode. */
+int fluid_batch_div_iterations(FluidBatch *batch, uint32_t *out);
+/* The pass-count histograms of every substep under tolerance mode since the
+ * last reset, 2 x 257 words: out[p] counts the (scene, substep) pairs that ran
+ * p density passes, out[257 + p] those that ran p divergence passes. `out`
+ * may be NULL; `reset` clears them afterwards. */
+int fluid_batch_pass_histogram(FluidBatch *batch, uint32_t *out, bool reset);
 /* Capture each substep shape once in a hipGraph and replay it. */
 int fluid_batch_use_graph(FluidBatch *batch, bool enable);
 /* Per scene, as fluid_sim_reduce_stats / fluid_sim_read_stats: out holds

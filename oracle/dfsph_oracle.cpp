@@ -161,6 +161,18 @@ inline double tangent(float) { return 0.0; }
 inline double tangent(double) { return 0.0; }
 inline double tangent(const Dual& x) { return x.d; }
 inline double tangent(const DualF& x) { return x.d; }
+inline float valf(float x) { return x; }
+inline float valf(double x) { return (float)x; }
+inline float valf(const Dual& x) { return (float)x.v; }
+inline float valf(const DualF& x) { return x.v; }
+
+// Tolerance mode's per-particle error e as a 2^-32 fixed-point integer,
+// saturated at 2^40 (e = 256), NaN and non-positive errors zero: the HIP
+// step's fixed_err (solve.hip), whose integer sums are order-free.
+inline uint64_t fixed_err(float e) {
+    e = e * 4294967296.0f;
+    return e > 0.0f ? (uint64_t)(e < 1099511627776.0f ? e : 1099511627776.0f) : 0u;
+}
 
 // sqrt and exp at precision T (the dual forms carry the derivative).
 inline float tsqrt(float x) { return std::sqrt(x); }
@@ -603,6 +615,25 @@ struct Sim {
     // strided stats sum).
     bool faithful = false;
     uint32_t lanes() const { return faithful ? kRefLanes : kListLanes; }
+    // Tolerance mode (no counterpart in the reference, which fixes one
+    // divergence pass and `refine` density passes, sim_solve.wgsl:665-703):
+    // the density solve iterates until the mean of max(rho* - rho0, 0) /
+    // rho0 is <= tol (min_passes .. max_passes passes), the divergence solve
+    // until the mean of max(Drho/Dt, 0) dt / rho0 is <= div_tol (at most
+    // div_max passes); the pass meeting the tolerance applies and is the
+    // last. The means come from order-free fixed-point sums (fixed_err), so
+    // the pass counts are the HIP step's exactly.
+    float tol = 0.0f, div_tol = 0.0f;
+    uint32_t min_passes = 1, max_passes = 0, div_max = 1;
+    bool err_on = false;                 // density_finish / den_kappa fill err
+    std::vector<uint64_t> err;           // per particle, this pass
+    uint32_t last_passes[2] = {0, 0};    // density, divergence
+    std::vector<uint64_t> pass_hist = std::vector<uint64_t>(2 * 257, 0);
+    double mean_err() const {
+        uint64_t sum = 0;
+        for (uint32_t i = 0; i < n; ++i) sum += err[i];
+        return (double)sum * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+    }
     void set_physics(const Physics& ph) {
         phys = ph;
         k = Consts<T>(p, ph);
@@ -836,6 +867,7 @@ struct Sim {
         alpha[pidx] = a;
         drho += dot(vel_i, wg);
         kd[pidx] = tmax(drho, T(0)) * a / tmax(s.dt, K(1e-9)) / rho;
+        if (err_on) err[pidx] = fixed_err(valf(tmax(drho, T(0)) * s.dt / k.rho0));
     }
 
     // Gradient of the pair (i, j), recomputed exactly as density_div did.
@@ -979,8 +1011,10 @@ struct Sim {
             for (int c = 0; c < 3; ++c) wvel[4 * i + c] = out[3 * i + c];
     }
 
-    // sim_solve.wgsl:674-698
-    void den_kappa(const StepT<T>& s) {
+    // sim_solve.wgsl:674-698; with div, tolerance mode's divergence kappa:
+    // density_div's formula (sim_solve.wgsl:508-516) from the current
+    // velocities, no pressure.
+    void den_kappa(const StepT<T>& s, bool div = false) {
 #pragma omp parallel for schedule(static)
         for (int64_t pi = 0; pi < (int64_t)n; ++pi) {
             uint32_t i = (uint32_t)pi;
@@ -995,10 +1029,17 @@ struct Sim {
             T rho = density[i];
             V3<T> wg = v3(wgrad[4 * i], wgrad[4 * i + 1], wgrad[4 * i + 2]);
             drho = k.mass * drho + dot(vel_i, wg);
+            if (div) {
+                kd[i] = tmax(drho, T(0)) * alpha[i] / tmax(s.dt, K(1e-9)) / rho;
+                err[i] = fixed_err(valf(tmax(drho, T(0)) * s.dt / k.rho0));
+                continue;
+            }
             T rho_star = rho + s.dt * drho;
-            T kap = tmax(rho_star - k.rho0, T(0)) * alpha[i] / (s.dt * s.dt);
+            T excess = tmax(rho_star - k.rho0, T(0));
+            T kap = excess * alpha[i] / (s.dt * s.dt);
             kd[i] = kap / rho;
             pressure[i] += kap * rho;
+            if (err_on) err[i] = fixed_err(valf(excess / k.rho0));
         }
     }
 
@@ -1056,13 +1097,36 @@ struct Sim {
         seed(s.force.y, dforce[1]);
         seed(s.force.z, dforce[2]);
         grid_build();
+        const bool div_on = div_tol > 0.0f && div_max > 1u, den_on = tol > 0.0f;
+        err.assign(div_on || den_on ? n : 0u, 0u);
+        err_on = div_on;
         density_div(s);
+        uint32_t div_passes = 1;
+        bool done = !div_on || mean_err() <= (double)div_tol;
         kappa_apply(s);       // div_apply
+        while (!done && div_passes < div_max) {
+            den_kappa(s, true);
+            done = mean_err() <= (double)div_tol;
+            kappa_apply(s);   // div_apply
+            ++div_passes;
+        }
         forces_eval();
         forces_den_apply(s);
-        for (uint32_t r = 0; r < refine; ++r) {
+        err_on = den_on;
+        uint32_t passes = 0;
+        for (const uint32_t cap_r = den_on ? max_passes : refine; passes < cap_r;) {
             den_kappa(s);
+            const bool met = den_on && mean_err() <= (double)tol && passes + 1 >= min_passes;
             kappa_apply(s);   // den_apply
+            ++passes;
+            if (met) break;
+        }
+        err_on = false;
+        last_passes[0] = passes;
+        last_passes[1] = div_passes;
+        if (div_on || den_on) {
+            pass_hist[std::min<uint32_t>(passes, 256)] += 1;
+            pass_hist[257 + std::min<uint32_t>(div_passes, 256)] += 1;
         }
         integrate(s);
     }
@@ -1297,6 +1361,34 @@ void oracle_set_physics(void* hv, const float* phys5) {
     });
 }
 
+// Tolerance mode (see Sim::tol): a tolerance <= 0 keeps that solve's
+// reference schedule.
+void oracle_set_tolerance(void* hv, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
+                          uint32_t div_max) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        s.tol = tol > 0.0f ? tol : 0.0f;
+        s.min_passes = min_passes > 0 ? min_passes : 1u;
+        s.max_passes = max_passes;
+        s.div_tol = div_tol > 0.0f ? div_tol : 0.0f;
+        s.div_max = div_tol > 0.0f ? div_max : 1u;
+        return 0;
+    });
+}
+
+// The last substep's passes {density, divergence} and the pass histograms
+// (2 x 257 counts, as the HIP step's).
+void oracle_passes(void* hv, uint32_t* last2, uint64_t* hist514) {
+    dispatch((Handle*)hv, [&](auto& s) {
+        if (last2) {
+            last2[0] = s.last_passes[0];
+            last2[1] = s.last_passes[1];
+        }
+        if (hist514)
+            for (int i = 0; i < 2 * 257; ++i) hist514[i] = s.pass_hist[i];
+        return 0;
+    });
+}
+
 // The line-faithful mode (1) or the kernels' order (0, the default); see
 // Sim::faithful.
 void oracle_set_faithful(void* hv, int on) {
@@ -1482,6 +1574,10 @@ int oracle_kernel(void* hv, const char* name, const void* step144) {
         else if (k == "forces_eval") s.forces_eval();
         else if (k == "forces_den_apply") s.forces_den_apply(st);
         else if (k == "den_kappa") s.den_kappa(st);
+        else if (k == "div_kappa") {
+            s.err.assign(s.n, 0u);
+            s.den_kappa(st, true);
+        }
         else if (k == "integrate") s.integrate(st);
         else return -1;
         return 0;

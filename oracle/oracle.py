@@ -56,6 +56,8 @@ def lib():
         L.oracle_set_threads.argtypes = [i32]
         L.oracle_set_physics.argtypes = [vp, P(f32)]
         L.oracle_set_faithful.argtypes = [vp, i32]
+        L.oracle_set_tolerance.argtypes = [vp, C.c_float, C.c_uint32, C.c_uint32, C.c_float, C.c_uint32]
+        L.oracle_passes.argtypes = [vp, P(C.c_uint32), P(C.c_uint64)]
         L.oracle_set_tangent.argtypes = [vp, i32, P(f64)]
         L.oracle_get_tangent.argtypes = [vp, i32, P(f64)]
         L.oracle_seed_force.argtypes = [vp, P(f64)]
@@ -227,6 +229,21 @@ class OracleSim:
         out = np.empty(14, np.float32)
         self.L.oracle_reduce_stats(self.h, _fp(out))
         return out
+
+    def set_tolerance(self, tol=0.0, min_passes=1, max_passes=0, div_tol=0.0, div_max_passes=0):
+        """Tolerance mode, as the HIP step's: a tolerance of 0 keeps that
+        solve's reference schedule."""
+        self.L.oracle_set_tolerance(self.h, float(tol), int(min_passes), int(max_passes), float(div_tol),
+                                    int(div_max_passes))
+
+    def passes(self):
+        """((density, divergence) passes of the last substep, [2, 257] pass
+        histograms)."""
+        last = np.zeros(2, np.uint32)
+        hist = np.zeros(2 * 257, np.uint64)
+        self.L.oracle_passes(self.h, last.ctypes.data_as(C.POINTER(C.c_uint32)),
+                             hist.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return (int(last[0]), int(last[1])), hist.reshape(2, 257)
 
     def counters(self):
         a, b = C.c_uint64(), C.c_uint64()

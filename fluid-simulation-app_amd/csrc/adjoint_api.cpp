@@ -159,6 +159,7 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
     A_(b.nbr_n, n);
     A_(b.regions, std::max<uint32_t>(solve_tiles(A->n), 1));
     A_(b.counters, 4);
+    A->grid.nonfinite = b.counters;
     A_(b.clamps, kClampWords);
     A_(b.zeros, kZeroRows);
     int lds = 0;

@@ -56,11 +56,18 @@ struct Ring {
 struct FluidBatch {
     int device = 0;
     hipStream_t stream = nullptr;
-    uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, tiles = 0;
+    // units: 64-slot units (slots / kMinSolveTile); tiles: the variant's tiles.
+    uint32_t scenes = 0, slots = 0, cells = 0, cap = 0, units = 0, tiles = 0;
+    // The solve of the tile size in use, and the launch choices the
+    // autotuner sets: scene-group size (0: one group) and streams.
+    const SolveVariant* V = nullptr;
+    size_t group_slots = 0;
+    uint32_t nstreams = 1;
     // Tile boundaries of the scene groups the sweeps run by (empty: one), and
-    // the extra streams the groups take in turn, with the fork and join events.
+    // the extra streams the groups take in turn, with the fork and join events
+    // (created as a setting first needs them).
     std::vector<uint32_t> groups;
-    uint32_t nextra = 0;
+    uint32_t nextra = 0, created = 0;
     hipStream_t extra[kMaxGroupStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kMaxGroupStreams - 1] = {};
     std::vector<SceneK> ks;
@@ -74,7 +81,7 @@ struct FluidBatch {
     GridBufs grid{};
     SolveBufs solve{};
     StatsBufs stats{};
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0, lds_limit = 0;
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
     Tolerance tol{0.0f, 1, 0, 0.0f, 1};
@@ -87,7 +94,11 @@ struct FluidBatch {
     int alloc(P*& out, size_t count, bool zero = true) {
         return mem.alloc(out, count, zero);
     }
-    SceneTable table(const Ring& r) const { return SceneTable{d_ks, r.dev, d_tile_scene, nullptr}; }
+    SceneTable table(const Ring& r) const {
+        SceneTable t{d_ks, r.dev, d_tile_scene};
+        t.unit_shift = V->tile == 256 ? 2u : (V->tile == 128 ? 1u : 0u);
+        return t;
+    }
     void drop_graphs() {
         for (auto& r : ring) {
             for (auto& g : r.graphs) (void)hipGraphExecDestroy(g.second);
@@ -101,10 +112,10 @@ struct FluidBatch {
             if (r.host) (void)hipHostFree(r.host);
             if (r.done) (void)hipEventDestroy(r.done);
         }
-        for (uint32_t i = 0; i < nextra; ++i) (void)hipStreamSynchronize(extra[i]);
+        for (uint32_t i = 0; i < created; ++i) (void)hipStreamSynchronize(extra[i]);
         mem.release();
         if (fork) (void)hipEventDestroy(fork);
-        for (uint32_t i = 0; i < nextra; ++i) {
+        for (uint32_t i = 0; i < created; ++i) {
             if (join[i]) (void)hipEventDestroy(join[i]);
             (void)hipStreamDestroy(extra[i]);
         }
@@ -129,6 +140,42 @@ bool valid_scene(const FluidBatch* b, uint32_t s) {
         return false;
     }
     return true;
+}
+
+// Sets the solve variant, the scene groups (whole scenes of about
+// group_slots slots, in the variant's tiles; 0: one group) and the streams
+// the groups go round (1 to kMaxGroupStreams), creating extra streams and
+// events as first needed. Captured graphs hold the old launches: the caller
+// drops them.
+int apply_layout(FluidBatch* B, const SolveVariant* V, size_t group_slots, uint32_t nstreams) {
+    B->V = V;
+    B->tiles = B->slots / V->tile;
+    B->group_slots = group_slots;
+    B->groups.clear();
+    if (group_slots && B->slots > group_slots) {
+        B->groups.push_back(0);
+        size_t acc = 0;
+        for (uint32_t s = 0; s < B->scenes; ++s) {
+            acc += B->padded[s];
+            if (acc >= group_slots && s + 1 < B->scenes) {
+                B->groups.push_back(B->ks[s + 1].slot_begin / V->tile);
+                acc = 0;
+            }
+        }
+        B->groups.push_back(B->tiles);
+    }
+    nstreams = nstreams < 1 ? 1 : (nstreams > kMaxGroupStreams ? kMaxGroupStreams : nstreams);
+    B->nstreams = nstreams;
+    const uint32_t want = B->groups.size() > 2 ? nstreams - 1 : 0u;
+    if (want && !B->fork) HIP_TRY(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+    while (B->created < want) {
+        const uint32_t i = B->created;
+        HIP_TRY(hipStreamCreateWithFlags(&B->extra[i], hipStreamNonBlocking));
+        B->created = i + 1;  // counted as soon as it exists, so a failure below still frees it
+        HIP_TRY(hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming));
+    }
+    B->nextra = want;
+    return FLUID_OK;
 }
 
 int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t nbr_cap) {
@@ -156,7 +203,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
         }
         SceneK k = make_scene(p);
         const uint64_t c = (uint64_t)p.dims[0] * p.dims[1] * p.dims[2];
-        const uint64_t pad = ((uint64_t)p.count + kSolveTile - 1) / kSolveTile * kSolveTile;
+        const uint64_t pad = ((uint64_t)p.count + kMaxSolveTile - 1) / kMaxSolveTile * kMaxSolveTile;
         if (slots + pad >= 0x80000000ull || cells + c + 1 >= 0xffffffffull) {
             set_error("batch exceeds 2^31 slots or 2^32 cells at scene %u", s);
             return FLUID_ERR_ARG;
@@ -167,52 +214,32 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
         k.slot_end = (uint32_t)(slots + p.count);
         B->ks[s] = k;
         B->padded[s] = (uint32_t)pad;
-        for (uint64_t t = 0; t < pad / kSolveTile; ++t) tile_scene.push_back(s);
+        for (uint64_t t = 0; t < pad / kMinSolveTile; ++t) tile_scene.push_back(s);
         slots += pad;
         cells += c + 1;
     }
     B->slots = (uint32_t)slots;
     B->cells = (uint32_t)cells;
-    B->tiles = (uint32_t)tile_scene.size();
-    if ((uint64_t)B->cap * std::max<uint64_t>(slots, kSolveTile) >= (1ull << 32)) {
+    B->units = (uint32_t)tile_scene.size();
+    if ((uint64_t)B->cap * std::max<uint64_t>(slots, kMaxSolveTile) >= (1ull << 32)) {
         set_error("neighbour list of cap %u x %llu slots exceeds 2^32", B->cap, (unsigned long long)slots);
         return FLUID_ERR_ARG;
     }
     // Groups of whole scenes of about kGroupSlots slots each (FLUID_BATCH_GROUP
     // overrides it; 0 runs the sweeps over the whole batch at once): a group's
     // list and fields stay in the memory-side cache from one sweep to the
-    // next, where the whole batch's would not.
-    size_t group_slots = kGroupSlots;
-    if (const char* env = getenv("FLUID_BATCH_GROUP")) group_slots = (size_t)strtoull(env, nullptr, 10);
-    if (group_slots && B->slots > group_slots) {
-        B->groups.push_back(0);
-        size_t acc = 0;
-        for (uint32_t s = 0; s < scenes; ++s) {
-            acc += B->padded[s];
-            if (acc >= group_slots && s + 1 < scenes) {
-                B->groups.push_back(B->ks[s + 1].slot_begin / kSolveTile);
-                acc = 0;
-            }
-        }
-        B->groups.push_back(B->tiles);
-    }
+    // next, where the whole batch's would not. kGroupStreams streams for the
+    // groups (FLUID_BATCH_STREAMS overrides it, 1 to kMaxGroupStreams), all at
+    // the default priority.
+    B->group_slots = kGroupSlots;
+    if (const char* env = getenv("FLUID_BATCH_GROUP")) B->group_slots = (size_t)strtoull(env, nullptr, 10);
+    B->nstreams = kGroupStreams;
+    if (const char* env = getenv("FLUID_BATCH_STREAMS")) B->nstreams = (uint32_t)strtoul(env, nullptr, 10);
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    // kGroupStreams streams for the groups (FLUID_BATCH_STREAMS overrides it,
-    // 1 to kMaxGroupStreams), all at the default priority.
-    uint32_t nstreams = kGroupStreams;
-    if (const char* env = getenv("FLUID_BATCH_STREAMS")) nstreams = (uint32_t)strtoul(env, nullptr, 10);
-    nstreams = nstreams < 1 ? 1 : (nstreams > kMaxGroupStreams ? kMaxGroupStreams : nstreams);
-    if (B->groups.size() > 2 && nstreams > 1) {
-        HIP_TRY(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
-        for (uint32_t i = 0; i + 1 < nstreams; ++i) {
-            HIP_TRY(hipStreamCreateWithFlags(&B->extra[i], hipStreamNonBlocking));
-            B->nextra = i + 1;  // counted as soon as it exists, so a failure below still frees it
-            HIP_TRY(hipEventCreateWithFlags(&B->join[i], hipEventDisableTiming));
-        }
-    }
+    int rc = apply_layout(B, solve_variant(kSolveTile), B->group_slots, B->nstreams);
+    if (rc) return rc;
     const size_t n = B->slots, c = B->cells, S = scenes;
-    int rc = FLUID_OK;
 #define A(ptr, cnt)                            \
     if ((rc = B->alloc(ptr, cnt)) != FLUID_OK) \
         return rc;
@@ -242,11 +269,12 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.xsph, n);
     A(b.wall_grad, n);
     b.cap = B->cap;
-    b.stride = std::max<uint32_t>(B->slots, kSolveTile);
+    b.stride = std::max<uint32_t>(B->slots, kMaxSolveTile);
     if ((rc = B->mem.alloc_list(b)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
-    A(b.regions, std::max<uint32_t>(B->tiles, 1));
+    A(b.regions, std::max<uint32_t>(B->units, 1));
     A(b.counters, 2 * S);
+    B->grid.nonfinite = b.counters;
     A(b.clamps, (size_t)kClampWords * S);
     A(b.zeros, kZeroRows);
     A(b.residual, S);
@@ -256,15 +284,15 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.pass_hist, 2 * kHistBins);
     A(B->d_pass_limit, 2 * S);
     A(B->d_ks, S);
-    A(B->d_tile_scene, std::max<uint32_t>(B->tiles, 1));
+    A(B->d_tile_scene, std::max<uint32_t>(B->units, 1));
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, B->device));
-    B->lds_bytes = (size_t)lds;
+    B->lds_limit = (size_t)lds;
     if (const char* env = getenv("FLUID_LDS_BYTES")) {
         const long v = strtol(env, nullptr, 10);
-        if (v > 0 && (size_t)v < B->lds_bytes) B->lds_bytes = (size_t)v;
+        if (v > 0 && (size_t)v < B->lds_limit) B->lds_limit = (size_t)v;
     }
-    B->lds_bytes = configure_solve(b, B->lds_bytes);
+    B->lds_bytes = B->V->configure(b, B->lds_limit);
     B->stage_cap_max = b.stage_cap;
     B->stats.blocks = 1024;
     A(B->stats.partials, S * 1024 * 16);
@@ -277,7 +305,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
         HIP_TRY(hipEventCreateWithFlags(&r.done, hipEventDisableTiming));
     }
     HIP_TRY(hipMemcpy(B->d_ks, B->ks.data(), S * sizeof(SceneK), hipMemcpyHostToDevice));
-    if (B->tiles)
+    if (B->units)
         HIP_TRY(hipMemcpy(B->d_tile_scene, tile_scene.data(), tile_scene.size() * 4, hipMemcpyHostToDevice));
     std::vector<float> warm(std::max<size_t>(n, 1), 293.15f);
     HIP_TRY(hipMemcpy(B->rest.temperature, warm.data(), n * sizeof(float), hipMemcpyHostToDevice));
@@ -291,8 +319,8 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
     const SceneTable sc = B->table(r);
     launch_grid_build(B->stream, sc, B->slots, B->cells, B->rest, B->work, B->grid, B->t());
     if (B->tol.on()) {
-        launch_solve_tolerance(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
-                               B->tol, passes, B->d_pass_limit, B->t());
+        B->V->tolerance_table(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
+                              B->tol, passes, B->d_pass_limit, B->t());
     } else {
         TileGroups g;
         if (!B->groups.empty()) {
@@ -305,7 +333,7 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
                 g.join[i] = B->join[i];
             }
         }
-        launch_solve(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(), g);
+        B->V->solve_table(B->stream, sc, B->slots, B->grid.starts, B->work, B->rest, B->solve, passes, B->t(), g);
     }
 }
 
@@ -590,12 +618,16 @@ int fluid_batch_profile_read(FluidBatch* b, double* ms, uint64_t* launches) {
     return FLUID_OK;
 }
 
-// Picks the sweeps' LDS staging capacity (slots a workgroup stages) by
-// timing `trials` substeps at each candidate, from a saved copy of the
-// resting set and the cumulative counters, then restores both: the results
-// do not depend on the capacity (a tile too big to stage gathers the same
-// values from global memory), only the time does. Writes the chosen
-// capacity. On failure the capacity and graph setting are left as they were.
+// Picks the launch shape by timing `trials` substeps of each candidate, from
+// a saved copy of the resting set and the cumulative counters, then
+// restores both: the results do not depend on the choices (a tile too big to
+// stage gathers the same values from global memory; tiles, groups and
+// streams only reorder independent work), only the time does. In turn, each
+// step keeping the best of the one before: the tile size (64, 128, 256
+// slots, each at its largest staging capacity), the staging capacity (0 and
+// 4/13 .. 13/13 of the largest), then the scene groups and streams (one
+// group; 196K, 328K or 459K-slot groups on 1 or 2 streams). Writes the
+// chosen capacity. On failure the settings are left as they were.
 int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t* refine, uint32_t trials,
                          uint32_t* chosen) {
     if (!valid(b) || !steps144) return FLUID_ERR_ARG;
@@ -612,13 +644,26 @@ int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t*
         tmp.release();
         return rc;
     }
-    const uint32_t keep = b->solve.stage_cap;
+    struct Shape {
+        const SolveVariant* V;
+        uint32_t cap;
+        size_t group_slots;
+        uint32_t streams;
+    };
+    const Shape keep{b->V, b->solve.stage_cap, b->group_slots, b->nstreams};
     const bool graph = b->use_graph;
-    double best = 1e300;
-    uint32_t best_cap = keep;
-    std::vector<uint32_t> caps;
+    Shape best = keep;
+    double best_t = 1e300;
+    // Applies a shape: the variant's LDS attributes and largest capacity,
+    // then the capacity (0: that largest), the groups and the streams.
+    auto apply = [&](const Shape& sh) -> int {
+        b->lds_bytes = sh.V->configure(b->solve, b->lds_limit);
+        b->stage_cap_max = b->solve.stage_cap;
+        if (sh.cap != ~0u) b->solve.stage_cap = std::min(sh.cap, b->stage_cap_max);
+        return apply_layout(b, sh.V, sh.group_slots, sh.streams);
+    };
     // A failed HIP call records its message and leaves through `done`, which
-    // restores the setting and frees the copies.
+    // restores the settings and frees the copies.
 #define TUNE_TRY(expr)                                                                                     \
     do {                                                                                                   \
         const hipError_t e_ = (expr);                                                                      \
@@ -641,39 +686,79 @@ int fluid_batch_autotune(FluidBatch* b, const uint8_t* steps144, const uint32_t*
         if (e == hipSuccess) e = hipMemcpyAsync(k_to, k_from, kClampWords * S * 4, d2d, b->stream);
         return e;
     };
-    // 0 (no staging) and 4/13 .. 13/13 of the largest capacity, in 64s.
-    for (uint32_t f : {0u, 4u, 8u, 10u, 12u, 13u}) {
-        const uint32_t c = b->stage_cap_max * f / 13u / 64u * 64u;
-        if (caps.empty() || caps.back() != c) caps.push_back(c);
-    }
-    b->use_graph = false;  // graphs bake the capacity in
+    b->use_graph = false;  // graphs bake the launch shape in
     b->drop_graphs();
     TUNE_TRY(copy(b->rest, save, b->solve.counters, save_counters, b->solve.clamps, save_clamps));
-    for (uint32_t c : caps) {
-        b->solve.stage_cap = c;
-        if ((rc = run_substep(b, steps144, refine))) goto done;  // warm-up
-        TUNE_TRY(hipStreamSynchronize(b->stream));
-        {
-            const auto t0 = std::chrono::steady_clock::now();
-            for (uint32_t i = 0; i < std::max<uint32_t>(trials, 1); ++i)
-                if ((rc = run_substep(b, steps144, refine))) goto done;
-            TUNE_TRY(hipStreamSynchronize(b->stream));
-            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            if (dt < best) {
-                best = dt;
-                best_cap = c;
+    for (int stage = 0; stage < 3; ++stage) {
+        std::vector<Shape> cand;
+        if (stage == 0) {
+            for (uint32_t t : {64u, 128u, 256u}) cand.push_back({solve_variant(t), ~0u, best.group_slots, best.streams});
+        } else if (stage == 1) {
+            if ((rc = apply({best.V, ~0u, best.group_slots, best.streams}))) goto done;
+            const uint32_t top = b->stage_cap_max;
+            for (uint32_t f : {0u, 4u, 8u, 10u, 12u, 13u}) {
+                const uint32_t c = top * f / 13u / 64u * 64u;
+                if (cand.empty() || cand.back().cap != c) cand.push_back({best.V, c, best.group_slots, best.streams});
             }
+        } else {
+            cand.push_back({best.V, best.cap, 0, 1});
+            for (size_t g : {196608ul, 327680ul, 458752ul})
+                for (uint32_t st : {1u, 2u}) cand.push_back({best.V, best.cap, g, st});
         }
-        TUNE_TRY(copy(save, b->rest, save_counters, b->solve.counters, save_clamps, b->solve.clamps));
+        for (const Shape& sh : cand) {
+            if ((rc = apply(sh))) goto done;
+            Shape got = sh;
+            got.cap = b->solve.stage_cap;
+            if ((rc = run_substep(b, steps144, refine))) goto done;  // warm-up
+            TUNE_TRY(hipStreamSynchronize(b->stream));
+            {
+                const auto t0 = std::chrono::steady_clock::now();
+                for (uint32_t i = 0; i < std::max<uint32_t>(trials, 1); ++i)
+                    if ((rc = run_substep(b, steps144, refine))) goto done;
+                TUNE_TRY(hipStreamSynchronize(b->stream));
+                const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (dt < best_t) {
+                    best_t = dt;
+                    best = got;
+                }
+            }
+            TUNE_TRY(copy(save, b->rest, save_counters, b->solve.counters, save_clamps, b->solve.clamps));
+        }
+        best_t = 1e300;  // each stage picks among its own candidates, timed alike
+        if (stage == 0) best.cap = ~0u;
     }
 #undef TUNE_TRY
 done:
     (void)hipStreamSynchronize(b->stream);
     tmp.release();
-    b->solve.stage_cap = rc == FLUID_OK ? best_cap : keep;
+    if (apply(rc == FLUID_OK ? best : keep) != FLUID_OK && rc == FLUID_OK) rc = FLUID_ERR_HIP;
     b->use_graph = graph;
     if (chosen) *chosen = b->solve.stage_cap;
     return rc;
+}
+
+int fluid_batch_get_tuning(const FluidBatch* b, FluidBatchTuning* out) {
+    if (!valid(b) || !out) return FLUID_ERR_ARG;
+    out->tile = b->V->tile;
+    out->stage_cap = b->solve.stage_cap;
+    out->group_slots = (uint32_t)b->group_slots;
+    out->streams = b->nstreams;
+    return FLUID_OK;
+}
+
+int fluid_batch_set_tuning(FluidBatch* b, const FluidBatchTuning* t) {
+    if (!valid(b) || !t) return FLUID_ERR_ARG;
+    const SolveVariant* v = solve_variant(t->tile);
+    if (!v || t->streams < 1 || t->streams > kMaxGroupStreams) {
+        set_error("tuning needs a tile of 64, 128 or 256 slots and 1 to %u streams", kMaxGroupStreams);
+        return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graphs();
+    b->lds_bytes = v->configure(b->solve, b->lds_limit);
+    b->stage_cap_max = b->solve.stage_cap;
+    b->solve.stage_cap = std::min(t->stage_cap / 64u * 64u, b->stage_cap_max);
+    return apply_layout(b, v, t->group_slots, t->streams);
 }
 
 int fluid_batch_set_physics(FluidBatch* b, const float* phys5) {
@@ -718,10 +803,12 @@ int fluid_batch_describe(const FluidBatch* b, char* buf, size_t size) {
     snprintf(buf, size,
              "{\"scenes\": %u, \"slots\": %u, \"cells\": %u, \"tiles\": %u, \"nbr_cap\": %u, \"stage_cap\": %u, "
              "\"lds_bytes\": %zu, \"graph\": %s, \"tolerance\": %g, \"max_passes\": %u, "
-             "\"div_tolerance\": %g, \"div_max_passes\": %u}",
+             "\"div_tolerance\": %g, \"div_max_passes\": %u, \"tile\": %u, \"group_slots\": %zu, "
+             "\"groups\": %zu, \"streams\": %u}",
              b->scenes, b->slots, b->cells, b->tiles, b->cap, b->solve.stage_cap, b->lds_bytes,
              b->use_graph ? "true" : "false", (double)b->tol.tol, b->tol.max_passes, (double)b->tol.div_tol,
-             b->tol.div_max);
+             b->tol.div_max, b->V->tile, b->group_slots, b->groups.empty() ? (size_t)1 : b->groups.size() - 1,
+             b->groups.size() > 2 ? b->nextra + 1 : 1u);
     return FLUID_OK;
 }
 

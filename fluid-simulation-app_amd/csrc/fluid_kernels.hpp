@@ -17,6 +17,31 @@ struct GridGeom {
     uint32_t dims[3];
 };
 
+// Slots a solve tile holds: 64, 128 or 256. The solve is built once for
+// each (solve.hip compiled three times, into namespaces t64, t128 and t256),
+// and a sim or batch picks one at run time (SolveVariant, below). Smaller
+// tiles stage a little more halo per particle but put more independent
+// workgroups on a CU. FLUID_SOLVE_TILE is the size a translation unit's
+// unqualified solve functions and kSolveTile refer to (128 unless set).
+#ifndef FLUID_SOLVE_TILE
+#define FLUID_SOLVE_TILE 128
+#endif
+#if FLUID_SOLVE_TILE == 64
+#define FLUID_TILE_NS t64
+#elif FLUID_SOLVE_TILE == 128
+#define FLUID_TILE_NS t128
+#elif FLUID_SOLVE_TILE == 256
+#define FLUID_TILE_NS t256
+#else
+#error "solve tile of 64, 128 or 256 slots"
+#endif
+constexpr uint32_t kSolveTile = FLUID_SOLVE_TILE;
+// The smallest and largest tile: a batch pads each scene to whole
+// kMaxSolveTile slots (so no tile of any size spans two scenes) and indexes
+// its tile-to-scene table by kMinSolveTile-slot units.
+constexpr uint32_t kMinSolveTile = 64;
+constexpr uint32_t kMaxSolveTile = 256;
+
 // Where a kernel takes its scene and substep constants from, per solve tile
 // (kSolveTile consecutive slots, never shared by two scenes):
 //   OneScene    the plain step: one scene, by value in the kernel arguments;
@@ -53,6 +78,8 @@ struct OneScene {
     const uint32_t* pass_limit = nullptr;
     const uint32_t* div_limit = nullptr;
     __device__ __forceinline__ uint32_t index(uint32_t) const { return 0; }
+    __device__ __forceinline__ const SceneK& scene_of_slot(uint32_t) const { return k; }
+    __device__ __forceinline__ uint32_t index_of_slot(uint32_t) const { return 0; }
     __device__ __forceinline__ const SceneK& scene(uint32_t) const { return k; }
     __device__ __forceinline__ const StepK& step(uint32_t) const { return s; }
     // Whether the tile takes part in a launch of pass `pass`.
@@ -72,18 +99,23 @@ struct OneScene {
 struct SceneTable {
     const SceneK* ks;             // [scenes]
     const StepK* ss;              // [scenes]
-    const uint32_t* tile_scene;   // [tiles]
+    const uint32_t* tile_scene;   // [slots / kMinSolveTile]: the scene of each 64-slot unit
+    // log2(tile / kMinSolveTile) of the solve variant launched: tile t
+    // starts at unit t << unit_shift.
+    uint32_t unit_shift = 0;
     // [scenes] density and divergence passes each scene runs this substep,
     // lowered when it converges (tolerance mode); null: the Step's refine
     // count and one divergence pass.
     const uint32_t* pass_limit = nullptr;
     const uint32_t* div_limit = nullptr;
     uint32_t flip = 0, tile0 = 0;
-    __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile]; }
-    __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[tile_scene[tile]]; }
-    __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[tile_scene[tile]]; }
+    __device__ __forceinline__ uint32_t index(uint32_t tile) const { return tile_scene[tile << unit_shift]; }
+    __device__ __forceinline__ uint32_t index_of_slot(uint32_t slot) const { return tile_scene[slot / kMinSolveTile]; }
+    __device__ __forceinline__ const SceneK& scene_of_slot(uint32_t slot) const { return ks[index_of_slot(slot)]; }
+    __device__ __forceinline__ const SceneK& scene(uint32_t tile) const { return ks[index(tile)]; }
+    __device__ __forceinline__ const StepK& step(uint32_t tile) const { return ss[index(tile)]; }
     __device__ __forceinline__ bool runs(uint32_t tile, uint32_t pass) const {
-        const uint32_t i = tile_scene[tile];
+        const uint32_t i = index(tile);
         return ss[i].active != 0u && pass_admits(pass, ss[i].refine, pass_limit, div_limit, i);
     }
     __device__ __forceinline__ uint32_t index_at(uint32_t scene) const { return scene; }
@@ -115,6 +147,11 @@ struct GridBufs {
     uint32_t* perm;      // [N] stable slot -> resting index
     uint64_t* status;    // [tiles] look-back words
     uint32_t* ticket;    // [1] tile ticket
+    // Per scene, word 2 s of the solve's counters (null: not kept): set by
+    // the grid build when a resting position has a non-finite coordinate
+    // (such a particle sorts into the grid's guard layer, whose cells the
+    // density walk then walks whole), cleared by integrate.
+    uint32_t* nonfinite;
 };
 
 // The nine stencil-row slot ranges holding every neighbour of one tile of
@@ -162,7 +199,9 @@ struct SolveBufs {
     uint32_t* nbr_n;  // [N]
     uint32_t cap;     // a multiple of 4
     uint32_t stride;  // >= the tiles' slot count
-    uint32_t* counters;  // per scene: [1] neighbour overflow pairs (cumulative); [0] unused
+    // Per scene: [1] neighbour overflow pairs (cumulative); [0] the grid
+    // build's non-finite flag (GridBufs::nonfinite).
+    uint32_t* counters;
     // Per scene, the speed-clamp count (cumulative) as kClampSlots words
     // kClampStride apart, one per cache line, whose sum is the count: many
     // workgroups add to it each substep, and one word would serialise them.
@@ -196,8 +235,6 @@ constexpr uint32_t kClampSlots = 16;
 constexpr uint32_t kClampStride = 32;
 constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
 
-// Tiles of the solve (consecutive slots a workgroup owns).
-uint32_t solve_tiles(uint32_t n);
 // List lanes a particle in the sweeps (entry m belongs to lane m % 4).
 constexpr uint32_t kListLanes = 4;
 
@@ -227,18 +264,6 @@ __host__ __device__ __forceinline__ uint32_t list_fixed(uint32_t stride, uint32_
 __host__ __device__ __forceinline__ uint32_t list_tail(uint32_t stride, uint32_t cap, uint32_t p, uint32_t off) {
     return kListFixed * kListLanes * stride + (p / kListWave) * kListWave * list_tail_cap(cap) + off;
 }
-// Slots a solve tile holds (64, 128 or 256, fixed at build time). Smaller
-// tiles stage a little more halo per particle but put more independent
-// workgroups on a CU, which hides the sweeps' memory latency better.
-#ifndef FLUID_SOLVE_TILE
-#define FLUID_SOLVE_TILE 128
-#endif
-constexpr uint32_t kSolveTile = FLUID_SOLVE_TILE;
-static_assert(kSolveTile == 64 || kSolveTile == 128 || kSolveTile == 256, "solve tile of 64, 128 or 256 slots");
-
-// Sets the sweeps' staging capacity for the device's per-workgroup LDS (and
-// raises their dynamic-LDS limit); returns the dynamic LDS budget used.
-size_t configure_solve(SolveBufs& b, size_t lds_bytes);
 
 enum KernelId {
     KERNEL_GRID_COUNT = 0,
@@ -312,12 +337,6 @@ uint32_t stage0_cells(const Stage0Bench* b);
 int stage0_download(Stage0Bench* b, float* density, uint32_t* starts, float* pos4);
 double stage0_last_ms(const Stage0Bench* b);
 
-// One substep's solve on the working set (after the grid build), ending with
-// integrate writing the resting set: density_div, div_apply, forces_eval,
-// forces_den_apply, passes x (den_kappa, den_apply), integrate. A scene runs
-// min(passes, its refine count) density passes.
-void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts, const Records& work,
-                  const Records& rest, const SolveBufs& b, uint32_t passes, KernelTimer* timer);
 // A batch may run the sweeps from density_div to the last den_apply group
 // by group of whole scenes: group g is tiles [groups[g], groups[g + 1]), and
 // ngroups = 0 means one group. Scenes do not interact, so the results are
@@ -334,9 +353,6 @@ struct TileGroups {
     hipEvent_t fork = nullptr;
     hipEvent_t join[kMaxGroupStreams - 1] = {};
 };
-void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,
-                  const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                  KernelTimer* timer, const TileGroups& groups = TileGroups());
 // The plain solve writing each stage's velocities and kd to arrays of their
 // own (the adjoint's taped forward): vel[0] enters div_apply (the working
 // velocities the grid build wrote), vel[1] leaves it, vel[2] leaves
@@ -347,9 +363,6 @@ struct StageBufs {
     float4* vel[kMaxStagePasses + 3];
     float* kd[kMaxStagePasses + 2];
 };
-void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
-                         const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,
-                         const StageBufs& T);
 // Tolerance mode (DFSPH's two solves iterated to their tolerances): the
 // divergence solve runs density_div's kappa and div_apply, then, while the
 // scene's mean divergence error max(Drho/Dt, 0) dt / rho0 of the last pass
@@ -370,15 +383,65 @@ struct Tolerance {
     uint32_t div_max;
     bool on() const { return tol > 0.0f || div_tol > 0.0f; }
 };
-void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
-                            const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
-                            const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer);
-void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
-                            const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
-                            uint32_t passes, uint32_t* limits, KernelTimer* timer);
-// The density sweep alone (the zero-substep path of the reference's tests).
-void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,
-                         const Records& work, const SolveBufs& b, KernelTimer* timer);
+
+// One tile size's solve, for a choice at run time.
+struct SolveVariant {
+    uint32_t tile;
+    uint32_t (*tiles)(uint32_t n);
+    size_t (*configure)(SolveBufs& b, size_t lds_bytes);
+    void (*solve_one)(hipStream_t, const OneScene&, uint32_t, const uint32_t*, const Records&, const Records&,
+                      const SolveBufs&, uint32_t, KernelTimer*);
+    void (*solve_table)(hipStream_t, const SceneTable&, uint32_t, const uint32_t*, const Records&,
+                        const Records&, const SolveBufs&, uint32_t, KernelTimer*, const TileGroups&);
+    void (*tolerance_one)(hipStream_t, const OneScene&, uint32_t, const uint32_t*, const Records&,
+                          const Records&, const SolveBufs&, const Tolerance&, uint32_t, uint32_t*, KernelTimer*);
+    void (*tolerance_table)(hipStream_t, const SceneTable&, uint32_t, uint32_t, const uint32_t*, const Records&,
+                            const Records&, const SolveBufs&, const Tolerance&, uint32_t, uint32_t*,
+                            KernelTimer*);
+    void (*density_only)(hipStream_t, const OneScene&, uint32_t, const uint32_t*, const Records&,
+                         const SolveBufs&, KernelTimer*);
+};
+// The solve of one tile size (namespaces t64, t128, t256; see kSolveTile).
+#define FLUID_SOLVE_API(NS)                                                                                      \
+    namespace NS {                                                                                               \
+    /* Tiles of the solve (consecutive slots a workgroup owns). */                                               \
+    uint32_t solve_tiles(uint32_t n);                                                                            \
+    /* Sets the sweeps' staging capacity for the device's per-workgroup LDS (and raises their dynamic-LDS     \
+       limit); returns the dynamic LDS budget used. */                                                           \
+    size_t configure_solve(SolveBufs& b, size_t lds_bytes);                                                      \
+    /* One substep's solve on the working set (after the grid build), ending with integrate writing the       \
+       resting set: density_div, div_apply, forces_eval, forces_den_apply, passes x (den_kappa, den_apply),   \
+       integrate. A scene runs min(passes, its refine count) density passes. */                               \
+    void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,               \
+                      const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,             \
+                      KernelTimer* timer);                                                                       \
+    void launch_solve(hipStream_t st, const SceneTable& sc, uint32_t slots, const uint32_t* starts,             \
+                      const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,             \
+                      KernelTimer* timer, const TileGroups& groups = TileGroups());                              \
+    void launch_solve_staged(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,        \
+                             const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,      \
+                             const StageBufs& T);                                                                \
+    void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,          \
+                                const uint32_t* starts, const Records& work, const Records& rest,                \
+                                const SolveBufs& b, const Tolerance& tol, uint32_t passes, uint32_t* limits,     \
+                                KernelTimer* timer);                                                             \
+    void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,     \
+                                const Records& work, const Records& rest, const SolveBufs& b,                    \
+                                const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer);    \
+    /* The density sweep alone (the zero-substep path of the reference's tests). */                            \
+    void launch_density_only(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,        \
+                             const Records& work, const SolveBufs& b, KernelTimer* timer);                       \
+    const SolveVariant& variant();                                                                               \
+    }
+FLUID_SOLVE_API(t64)
+FLUID_SOLVE_API(t128)
+FLUID_SOLVE_API(t256)
+#undef FLUID_SOLVE_API
+// The translation unit's own tile size unqualified (the adjoint's, 128).
+using namespace FLUID_TILE_NS;
+
+// The variant of `tile` slots (64, 128 or 256), or null.
+const SolveVariant* solve_variant(uint32_t tile);
 
 struct StatsBufs {
     float* partials;   // [blocks][16]

@@ -55,19 +55,22 @@ template <class Sc>
 __global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __restrict__ pos, uint32_t n,
                                                        uint32_t* __restrict__ cellid, uint32_t* __restrict__ rank,
                                                        uint32_t* __restrict__ counts, uint64_t* __restrict__ status,
-                                                       uint32_t ntiles, uint32_t* __restrict__ ticket) {
+                                                       uint32_t ntiles, uint32_t* __restrict__ ticket,
+                                                       uint32_t* __restrict__ nonfinite) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < ntiles) status[i] = 0;
     if (i == 0) *ticket = 0;
     bool live = i < n;
     uint32_t c = 0xffffffffu;
-    // A wave's 64 slots lie in one solve tile, so in one scene.
+    // A wave's 64 slots lie in one 64-slot unit, so in one scene.
     const uint32_t first = blockIdx.x * kBlock + (threadIdx.x & ~63u);
     if (first < n) {
-        const SceneK& g = sc.scene(first / kSolveTile);
+        const SceneK& g = sc.scene_of_slot(first);
         if (live) {
             if (i < g.slot_end) {
                 float4 p = pos[i];
+                if (nonfinite && !(__builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z)))
+                    atomicOr(&nonfinite[2 * sc.index_of_slot(first)], 1u);
                 uint32_t cx = cell_coord(p.x, g.box_min[0], g.cell, g.dims[0]);
                 uint32_t cy = cell_coord(p.y, g.box_min[1], g.cell, g.dims[1]);
                 uint32_t cz = cell_coord(p.z, g.box_min[2], g.cell, g.dims[2]);
@@ -244,7 +247,7 @@ static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells,
     uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
     const Span tc = span(timer, KERNEL_GRID_COUNT);
     hipExtLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, tc.a, tc.z, 0, sc,
-                          rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
+                          rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket, b.nonfinite);
     const Span ts = span(timer, KERNEL_GRID_SCAN);
     hipExtLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, ts.a, ts.z, 0, b.counts, b.starts, cells,
                           b.status, b.ticket, true);
@@ -274,7 +277,7 @@ void launch_index_sort(hipStream_t st, const OneScene& sc, uint32_t n, uint32_t 
     const uint32_t ntiles = scan_tiles(cells);
     const uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
     hipLaunchKernelGGL(k_grid_count<OneScene>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, sc, pos, n,
-                       b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket);
+                       b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket, nullptr);
     hipLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, b.counts, b.starts, cells, b.status, b.ticket,
                        true);
     if (n == 0) return;

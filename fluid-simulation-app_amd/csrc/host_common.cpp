@@ -32,4 +32,13 @@ int check_launch(const char* what) {
     return FLUID_OK;
 }
 
+const SolveVariant* solve_variant(uint32_t tile) {
+    switch (tile) {
+        case 64: return &t64::variant();
+        case 128: return &t128::variant();
+        case 256: return &t256::variant();
+        default: return nullptr;
+    }
+}
+
 }  // namespace fluid

@@ -104,7 +104,10 @@ struct FluidSim {
     StatsBufs stats{};
     TracerBufs tracers{};
     bool track_ids = false;
-    size_t lds_bytes = 0;
+    size_t lds_bytes = 0;   // the dynamic LDS budget of the sweeps
+    size_t lds_limit = 0;   // the device's per-workgroup LDS (FLUID_LDS_BYTES caps it)
+    // The solve of the tile size in use (128 slots unless set).
+    const SolveVariant* V = nullptr;
     // Tolerance mode (off: the reference schedule) and its two pass limits.
     Tolerance tol{0.0f, 1, 0, 0.0f, 1};
     uint32_t* limits = nullptr;
@@ -126,6 +129,12 @@ struct FluidSim {
 };
 
 namespace {
+
+// The list's stride: the slots rounded up to the largest tile, so one list
+// layout serves every tile size.
+uint64_t list_stride(uint32_t n) {
+    return std::max<uint64_t>(((uint64_t)n + kMaxSolveTile - 1) / kMaxSolveTile * kMaxSolveTile, kMaxSolveTile);
+}
 
 int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_cap) {
     std::memcpy(&s->params, params64, sizeof s->params);
@@ -160,7 +169,9 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     }
     // The list's 32-bit offsets bound cap x slots; refused before anything is
     // allocated (the allocation repeats the check).
-    const uint64_t list_entries = (uint64_t)s->cap * std::max<uint32_t>(solve_tiles(s->n), 1) * kSolveTile;
+    // Sized for every tile size: slots rounded up to the largest tile, one
+    // region per smallest tile.
+    const uint64_t list_entries = (uint64_t)s->cap * list_stride(s->n);
     if (list_entries >= (1ull << 32)) {
         set_error("neighbour list of %llu entries (cap %u x %u particles) exceeds 2^32",
                   (unsigned long long)list_entries, s->cap, s->n);
@@ -200,11 +211,12 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.xsph, n);
     A(b.wall_grad, n);
     b.cap = s->cap;
-    b.stride = std::max<uint32_t>(solve_tiles(s->n), 1) * kSolveTile;
+    b.stride = (uint32_t)list_stride(s->n);
     if ((rc = s->mem.alloc_list(b)) != FLUID_OK) return rc;
     A(b.nbr_n, n);
-    A(b.regions, std::max<uint32_t>(solve_tiles(s->n), 1));
+    A(b.regions, std::max<uint32_t>((s->n + kMinSolveTile - 1) / kMinSolveTile, 1));
     A(b.counters, 4);
+    s->grid.nonfinite = b.counters;
     A(b.clamps, kClampWords);
     A(b.zeros, kZeroRows);
     A(b.residual, 1);
@@ -215,13 +227,14 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(s->limits, 2);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));
-    s->lds_bytes = (size_t)lds;
+    s->lds_limit = (size_t)lds;
     // FLUID_LDS_BYTES caps the LDS a sweep workgroup may take (a tuning knob).
     if (const char* env = getenv("FLUID_LDS_BYTES")) {
         const long v = strtol(env, nullptr, 10);
-        if (v > 0 && (size_t)v < s->lds_bytes) s->lds_bytes = (size_t)v;
+        if (v > 0 && (size_t)v < s->lds_limit) s->lds_limit = (size_t)v;
     }
-    s->lds_bytes = configure_solve(b, s->lds_bytes);
+    s->V = solve_variant(kSolveTile);
+    s->lds_bytes = s->V->configure(b, s->lds_limit);
     s->stats.blocks = stats_blocks(s->n);
     A(s->stats.partials, (size_t)s->stats.blocks * 16);
     A(s->stats.stats, 16);
@@ -254,10 +267,10 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
     for (uint32_t i = 0; i < nsub; ++i) {
         launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
         if (s->tol.on())
-            launch_solve_tolerance(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, s->tol, refine,
-                                   s->limits, s->t());
+            s->V->tolerance_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, s->tol, refine,
+                                s->limits, s->t());
         else
-            launch_solve(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
+            s->V->solve_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
     }
     return check_launch("substep");
 }
@@ -374,7 +387,7 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     std::vector<uint16_t> raw16(cap * stride);
     std::vector<uint32_t> raw32(cap * stride);
     std::vector<uint32_t> cnt(n);
-    std::vector<TileRegion> reg(solve_tiles(s->n));
+    std::vector<TileRegion> reg(s->V->tiles(s->n));
     HIP_TRY(hipMemcpy(raw16.data(), s->solve.nbr16, raw16.size() * 2, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(raw32.data(), s->solve.nbr32, raw32.size() * 4, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(cnt.data(), s->solve.nbr_n, n * 4, hipMemcpyDeviceToHost));
@@ -382,7 +395,7 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     uint32_t off = 0;  // the slot's tail offset in its wave's block
     for (size_t i = 0; i < n; ++i) {
         if (i % kListWave == 0) off = 0;
-        const TileRegion& R = reg[i / kSolveTile];
+        const TileRegion& R = reg[i / s->V->tile];
         for (size_t m = 0; m < cap; ++m) {
             uint32_t j = 0;
             if (m < cnt[i]) {
@@ -407,6 +420,21 @@ int fluid_sim_download_neighbours(FluidSim* s, uint32_t* nbr) {
     }
     return FLUID_OK;
 }
+
+int fluid_sim_set_tile(FluidSim* s, uint32_t tile) {
+    if (!valid(s)) return FLUID_ERR_ARG;
+    const SolveVariant* v = solve_variant(tile);
+    if (!v) {
+        set_error("solve tile of %u slots: 64, 128 or 256", tile);
+        return FLUID_ERR_ARG;
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->V = v;
+    s->lds_bytes = v->configure(s->solve, s->lds_limit);
+    return FLUID_OK;
+}
+
+uint32_t fluid_sim_tile(const FluidSim* s) { return s && s->V ? s->V->tile : 0; }
 
 int fluid_sim_set_tolerance(FluidSim* s, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
                             uint32_t div_max_passes) {
@@ -490,7 +518,7 @@ int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
     std::memcpy(&sb, step144, sizeof sb);
     const OneScene sc{s->scene, make_step(sb, s->phys)};
     launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
-    launch_density_only(s->stream, sc, s->n, s->grid.starts, s->work, s->solve, s->t());
+    s->V->density_only(s->stream, sc, s->n, s->grid.starts, s->work, s->solve, s->t());
     return check_launch("density_only");
 }
 
@@ -627,7 +655,7 @@ int fluid_sim_describe(const FluidSim* s, char* buf, size_t size) {
     snprintf(buf, size,
              "{\"arch\": \"%s\", \"lds_bytes\": %zu, \"nbr_cap\": %u, \"stage_cap\": %u, "
              "\"list_lanes\": %u, \"tile\": %u}",
-             name, s->lds_bytes, s->cap, s->solve.stage_cap, kListLanes, kSolveTile);
+             name, s->lds_bytes, s->cap, s->solve.stage_cap, kListLanes, s->V->tile);
     return FLUID_OK;
 }
 

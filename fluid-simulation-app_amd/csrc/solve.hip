@@ -37,6 +37,7 @@
 #include "fluid_kernels.hpp"
 
 namespace fluid {
+namespace FLUID_TILE_NS {
 
 namespace {
 
@@ -357,14 +358,18 @@ __device__ __forceinline__ float density_pair(const SceneK& k, f3 pi, f3 vi, f3 
 // Distances from the particle to the faces of its own cell: a stencil row or
 // end cell farther than the support holds no neighbour at a finite
 // position, so the walk skips it (the accepted neighbours and their order
-// are those of the full walk; the guard layer, where non-finite positions
-// sort, is always walked).
+// are those of the full walk: a skipped candidate would fail the distance
+// test). A non-finite position has an unordered distance to every
+// candidate, which the shader keeps (sim_solve.wgsl:450), so a particle
+// with one, and every particle of a scene that holds one (`guard`), walks
+// its whole stencil.
 struct Reach {
     float dl[3], dh[3];
     float lim;
 };
 
-__device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, uint32_t by, uint32_t bz) {
+__device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, uint32_t by, uint32_t bz,
+                                            bool guard) {
     Reach st;
     const float pp[3] = {p.x, p.y, p.z};
     const uint32_t b[3] = {bx, by, bz};
@@ -375,11 +380,9 @@ __device__ __forceinline__ Reach make_reach(const SceneK& k, f3 p, uint32_t bx, 
         st.dl[a] = wmax(pp[a] - lo, 0.0f);
         st.dh[a] = wmax(hi - pp[a], 0.0f);
     }
-    // A margin over the rounding of the faces; a particle with a non-finite
-    // coordinate keeps every candidate it meets (an unordered distance is
-    // not skipped), so it walks its whole stencil.
+    // A margin over the rounding of the faces.
     const bool finite = __builtin_isfinite(p.x) && __builtin_isfinite(p.y) && __builtin_isfinite(p.z);
-    st.lim = finite ? k.support_sq * 1.001f : __builtin_huge_valf();
+    st.lim = finite && !guard ? k.support_sq * 1.001f : __builtin_huge_valf();
     return st;
 }
 
@@ -448,8 +451,8 @@ template <bool STAGED>
 __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& id, f3 pi, uint32_t bx, uint32_t by,
                                                  uint32_t bz, const uint32_t* __restrict__ starts,
                                                  const float4* __restrict__ pos, const PvLds& L,
-                                                 const RegionS& R, const SolveBufs& b) {
-    const Reach st = make_reach(k, pi, bx, by, bz);
+                                                 const RegionS& R, const SolveBufs& b, bool guard) {
+    const Reach st = make_reach(k, pi, bx, by, bz, guard);
     const uint32_t shift = __lane_id() & ~(kLanes - 1);
     const uint32_t below = (1u << id.lane) - 1u;
     // Every row's candidate range first, so the 18 bound loads overlap.
@@ -461,20 +464,9 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
         const float ez = dz < 0 ? st.dl[2] : (dz > 0 ? st.dh[2] : 0.0f);
         const float ey = dy < 0 ? st.dl[1] : (dy > 0 ? st.dh[1] : 0.0f);
         const float e2 = ey * ey + ez * ez;
-        // Rows and end cells in the grid's guard layer (a coordinate of 0 or
-        // dims - 1) are never pruned: a NaN or infinite coordinate sorts
-        // there, and the shader keeps such a candidate whatever the
-        // geometry (its distance is unordered, sim_solve.wgsl:450). Such
-        // cells are empty unless a particle sits exactly on an upper wall.
-        const bool guard_row = cz == 0 || cz == (int)k.dims[2] - 1 || cy == 0 || cy == (int)k.dims[1] - 1;
-        const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] &&
-                              (e2 <= st.lim || guard_row);
-        const uint32_t x0 =
-            (bx > 0 && (guard_row || bx == 1u || st.dl[0] * st.dl[0] + e2 <= st.lim)) ? bx - 1 : bx;
-        const uint32_t x1 =
-            (bx + 1 < k.dims[0] && (guard_row || bx + 2 == k.dims[0] || st.dh[0] * st.dh[0] + e2 <= st.lim))
-                ? bx + 1
-                : bx;
+        const bool live_row = cz >= 0 && cz < (int)k.dims[2] && cy >= 0 && cy < (int)k.dims[1] && e2 <= st.lim;
+        const uint32_t x0 = (bx > 0 && st.dl[0] * st.dl[0] + e2 <= st.lim) ? bx - 1 : bx;
+        const uint32_t x1 = (bx + 1 < k.dims[0] && st.dh[0] * st.dh[0] + e2 <= st.lim) ? bx + 1 : bx;
         const uint32_t row = live_row ? k.cell_offset + ((uint32_t)cz * k.dims[1] + (uint32_t)cy) * k.dims[0] : 0u;
         k0s[r] = live_row ? starts[row + x0] : 0u;
         k1s[r] = live_row ? starts[row + x1 + 1] : 0u;
@@ -661,7 +653,9 @@ __device__ __forceinline__ unsigned long long fixed_err(float e) {
 }
 
 // Lane 0's end of density_div for particle i: the wall terms, ρ, α and the
-// divergence κ/ρ. Returns whether the particle's wall gradient is nonzero.
+// divergence κ/ρ (with TOL, its fixed-point divergence error too). Returns
+// whether the particle's wall gradient is nonzero.
+template <bool TOL>
 __device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                                f3 pi, f3 vi, uint32_t found, float w_sum, float rn, float gsq_sum,
                                                float drho_sum, f3 gsum_sum, const SolveBufs& b,
@@ -698,11 +692,11 @@ __device__ __forceinline__ bool density_finish(const SceneK& k, const StepK& s, 
     b.alpha[i] = al;
     drho += dot3(vi, wg);
     b.kd[i] = wmax(drho, 0.0f) * al / s.dt_guard / rho;
-    if (b.div_residual) err = fixed_err(wmax(drho, 0.0f) * s.dt / k.rho0);
+    if (TOL) err = fixed_err(wmax(drho, 0.0f) * s.dt / k.rho0);
     return wg.x != 0.0f || wg.y != 0.0f || wg.z != 0.0f;
 }
 
-template <bool STAGED>
+template <bool STAGED, bool TOL>
 __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, uint32_t* counters, const LaneId& id,
                                              const RegionS& R, const uint32_t* __restrict__ starts,
                                              const float4* __restrict__ pos, const float4* __restrict__ vel,
@@ -719,7 +713,7 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
         bx = cell_coord(pi.x, k.box_min[0], k.cell, k.dims[0]);
         by = cell_coord(pi.y, k.box_min[1], k.cell, k.dims[1]);
         bz = cell_coord(pi.z, k.box_min[2], k.cell, k.dims[2]);
-        found = density_walk<STAGED>(k, id, pi, bx, by, bz, starts, pos, L, R, b);
+        found = density_walk<STAGED>(k, id, pi, bx, by, bz, starts, pos, L, R, b, counters[0] != 0u);
     }
     // Phase 1's list entries, written by any lane of the quad, are read back
     // by lane m % 4 below.
@@ -733,15 +727,16 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
     if (!id.live || id.lane != 0) return;
     // A particle near a wall marks its tile (k_tile_regions cleared the mark
     // this substep; any number of lanes may store the same 1).
-    if (density_finish(k, s, counters, id, pi, vi, found, w_sum, rn, gsq_sum, drho_sum, gsum_sum, b, err))
+    if (density_finish<TOL>(k, s, counters, id, pi, vi, found, w_sum, rn, gsq_sum, drho_sum, gsum_sum, b, err))
         b.regions[id.tile].walls = 1u;
 }
 
 
 // At most 80 VGPRs (six waves a SIMD: three 512-thread workgroups a CU); the
 // scene-table form would take a few more and drop to two, which costs it
-// 16% against a 16-byte spill.
-template <class Sc>
+// 16% against a 16-byte spill. TOL (tolerance mode) adds the divergence
+// error of the solve's first pass.
+template <class Sc, bool TOL = false>
 __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6))) void k_density_div(Sc sc, const uint32_t* __restrict__ starts,
                                                                const float4* __restrict__ pos,
                                                                const float4* __restrict__ vel, SolveBufs b) {
@@ -757,11 +752,10 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
                   (LdsU16*)(reinterpret_cast<char*>(smem) + stage_slots(b) * kPvEntry)};
     unsigned long long err = 0;
     if (R.staged)
-        density_tile<true>(k, s, counters, id, R, starts, pos, vel, b, L, err);
+        density_tile<true, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
     else
-        density_tile<false>(k, s, counters, id, R, starts, pos, vel, b, L, err);
-    // Tolerance mode: the divergence error of the solve's first pass.
-    if (b.div_residual) block_residual(err, b.div_residual + sc.index_at(R.scene));
+        density_tile<false, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
+    if (TOL) block_residual(err, b.div_residual + sc.index_at(R.scene));
 }
 
 // ---- list sweeps -------------------------------------------------------------
@@ -1155,6 +1149,10 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
     const StepK& s = sc.step(tile);
     if (!sc.runs(tile, kEveryPass)) return;
     const uint32_t i = tile * kTile + threadIdx.x;
+    // The scene's first tile clears the grid build's non-finite flag: the
+    // walk of this substep has read it, and the resting set written here is
+    // the next build's to judge.
+    if (threadIdx.x == 0 && i == k.slot_begin) b.counters[2 * sc.index(tile)] = 0u;
     bool clamped_speed = false;
     if (i < k.slot_end) {
         // Every load first: a load issued after a store would wait for it.
@@ -1265,6 +1263,8 @@ uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
     const void* fns[] = {reinterpret_cast<const void*>(&k_density_div<OneScene>),
+                         reinterpret_cast<const void*>(&k_density_div<OneScene, true>),
+                         reinterpret_cast<const void*>(&k_density_div<SceneTable, true>),
                          reinterpret_cast<const void*>(&k_kappa_apply<OneScene>),
                          reinterpret_cast<const void*>(&k_forces_eval<OneScene>),
                          reinterpret_cast<const void*>(&k_forces_den_apply<OneScene>),
@@ -1350,8 +1350,12 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         Sc rev = fwd;  // the reverse-order launches
         rev.flip = 1;
         const Span tdd = g == 0 ? tdd0 : span(timer, KERNEL_DENSITY_DIV);
-        hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s, g == 0 ? nullptr : tdd.a, tdd.z, 0,
-                              fwd, starts, work.pos, V(0), KB(0));
+        if (b.div_residual)
+            hipExtLaunchKernelGGL(k_density_div<Sc, true>, tiles, sweep, density_lds(b), s,
+                                  g == 0 ? nullptr : tdd.a, tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
+        else
+            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s, g == 0 ? nullptr : tdd.a,
+                                  tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
         if (div_passes > 1) check(s, 0, true);
         const Span td = span(timer, KERNEL_DIV_APPLY);
         hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, td.a, td.z, 0, rev, kEveryPass, work.pos,
@@ -1474,4 +1478,12 @@ void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, 
     solve_tolerance(st, one, 1u, slots, starts, work, rest, b, tol, passes, limits, timer);
 }
 
+const SolveVariant& variant() {
+    static const SolveVariant v = {kSolveTile,    &solve_tiles,  &configure_solve,        &launch_solve,
+                                   &launch_solve, &launch_solve_tolerance, &launch_solve_tolerance,
+                                   &launch_density_only};
+    return v;
+}
+
+}  // namespace FLUID_TILE_NS
 }  // namespace fluid

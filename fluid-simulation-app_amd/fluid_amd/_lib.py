@@ -45,6 +45,8 @@ def _declare(L):
         "fluid_sim_download_ids": ([vp, P(u32)], i32),
         "fluid_sim_substeps": ([vp, u8p, u32, u32], i32),
         "fluid_sim_set_tolerance": ([vp, f32, u32, u32, f32, u32], i32),
+        "fluid_sim_set_tile": ([vp, u32], i32),
+        "fluid_sim_tile": ([vp], u32),
         "fluid_sim_iterations": ([vp, P(u32)], i32),
         "fluid_sim_pass_histogram": ([vp, P(u32), C.c_bool], i32),
         "fluid_sim_density_only": ([vp, u8p], i32),
@@ -83,6 +85,8 @@ def _declare(L):
         "fluid_batch_set_tolerance": ([vp, f32, u32, u32], i32),
         "fluid_batch_iterations": ([vp, P(u32)], i32),
         "fluid_batch_set_div_tolerance": ([vp, f32, u32], i32),
+        "fluid_batch_get_tuning": ([vp, P(u32)], i32),
+        "fluid_batch_set_tuning": ([vp, P(u32)], i32),
         "fluid_batch_div_iterations": ([vp, P(u32)], i32),
         "fluid_batch_pass_histogram": ([vp, P(u32), C.c_bool], i32),
         "fluid_batch_use_graph": ([vp, C.c_bool], i32),

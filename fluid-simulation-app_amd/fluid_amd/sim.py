@@ -256,6 +256,14 @@ class Sim:
         check(self.L.fluid_sim_set_tolerance(self.h, float(tol), int(min_passes), int(max_passes), float(div_tol),
                                              int(div_max_passes)), "sim set_tolerance")
 
+    def set_tile(self, tile: int):
+        """The solve's tile size: 64, 128 or 256 slots a workgroup."""
+        check(self.L.fluid_sim_set_tile(self.h, int(tile)), "sim set_tile")
+
+    @property
+    def tile(self) -> int:
+        return int(self.L.fluid_sim_tile(self.h))
+
     def iterations(self) -> tuple:
         """(density passes, divergence passes) of the last substep."""
         out = np.zeros(2, np.uint32)
@@ -418,6 +426,16 @@ class Batch:
         out = np.empty(self.scenes, np.uint32)
         check(self.L.fluid_batch_iterations(self.h, _u32p(out)), "batch iterations")
         return out
+
+    def tuning(self) -> dict:
+        """The launch shape: tile, stage_cap, group_slots, streams."""
+        out = np.zeros(4, np.uint32)
+        check(self.L.fluid_batch_get_tuning(self.h, _u32p(out)), "batch get_tuning")
+        return dict(zip(("tile", "stage_cap", "group_slots", "streams"), map(int, out)))
+
+    def set_tuning(self, tile: int, stage_cap: int, group_slots: int, streams: int):
+        arr = np.array([tile, stage_cap, group_slots, streams], np.uint32)
+        check(self.L.fluid_batch_set_tuning(self.h, _u32p(arr)), "batch set_tuning")
 
     def div_iterations(self) -> np.ndarray:
         out = np.empty(self.scenes, np.uint32)

@@ -131,6 +131,10 @@ int fluid_sim_substeps(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32
  */
 int fluid_sim_set_tolerance(FluidSim *sim, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
                             uint32_t div_max_passes);
+/* The solve's tile size (64, 128 or 256 slots a workgroup; 128 at
+ * creation). Results do not depend on it. */
+int fluid_sim_set_tile(FluidSim *sim, uint32_t tile);
+uint32_t fluid_sim_tile(const FluidSim *sim);
 /* Passes of the last substep: out[0] density, out[1] divergence. */
 int fluid_sim_iterations(FluidSim *sim, uint32_t out[2]);
 /* The pass-count histograms of every substep under tolerance mode since the
@@ -412,9 +416,26 @@ int fluid_batch_counters(FluidBatch *batch, uint64_t *out);
 int fluid_batch_synchronize(FluidBatch *batch);
 int fluid_batch_profile(FluidBatch *batch, uint32_t kernel_mask);
 int fluid_batch_profile_read(FluidBatch *batch, double *ms, uint64_t *launches);
-/* Times `trials` substeps at each candidate LDS staging capacity from a
- * saved copy of the state (restored afterwards; results do not depend on
- * the capacity) and keeps the fastest; writes it to *chosen. */
+/* The batch's launch shape: the solve's tile size (64, 128 or 256 slots a
+ * workgroup), the LDS staging capacity (slots; a tile whose stencil rows
+ * need more gathers from global memory), the scene-group size (slots; 0 runs
+ * the sweeps over the whole batch at once) and the streams the groups go
+ * round (1 to 4). None of them changes a result bit. */
+typedef struct FluidBatchTuning {
+    uint32_t tile;
+    uint32_t stage_cap;
+    uint32_t group_slots;
+    uint32_t streams;
+} FluidBatchTuning;
+int fluid_batch_get_tuning(const FluidBatch *batch, FluidBatchTuning *out);
+/* Sets a launch shape (stage_cap is rounded down to 64s and capped at the
+ * tile size's largest). */
+int fluid_batch_set_tuning(FluidBatch *batch, const FluidBatchTuning *tuning);
+/* The autotuner: times `trials` substeps of each candidate launch shape from
+ * a saved copy of the state (restored afterwards, the cumulative counters
+ * included) and keeps the fastest, choosing in turn the tile size, the
+ * staging capacity, then the scene groups and streams. Writes the chosen
+ * capacity to *chosen; fluid_batch_get_tuning reads the whole shape. */
 int fluid_batch_autotune(FluidBatch *batch, const uint8_t *steps144, const uint32_t *refine, uint32_t trials,
                          uint32_t *chosen);
 int fluid_batch_describe(const FluidBatch *batch, char *buf, size_t size);

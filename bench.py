@@ -82,15 +82,19 @@ def _kernel_classes(names):
     return out
 
 
-def pmc_summary(base):
+def pmc_summary(base, workload="block_1m", steps=0, warmup=0):
     """Prints the markdown summary of four rocprofv3 --pmc runs under `base`:
-    pmc_F / pmc_W (FETCH_SIZE, WRITE_SIZE of a short 1M bench run) and
-    pmc_copyF / pmc_copyW (the same counters over --copy-calibration). Each
-    counter is scaled by what it reads for the plain copy of known size."""
+    pmc_F / pmc_W (FETCH_SIZE, WRITE_SIZE of a short bench run of
+    `workload`) and pmc_copyF / pmc_copyW (the same counters over
+    --copy-calibration). Each counter is scaled by what it reads for the
+    plain copy of known size. With a kernel trace of the same workload under
+    base/trace (rocprofv3 --kernel-trace), adds each kernel class's average
+    launch time and its achieved fraction of the HBM roofline."""
     import csv
+    import glob
 
-    def rows(d):
-        f = os.path.join(base, d, "pmc_counter_collection.csv")
+    def rows(d, name="pmc_counter_collection.csv"):
+        f = os.path.join(base, d, name)
         return sorted(csv.DictReader(open(f)), key=lambda r: int(r["Dispatch_Id"]))
 
     def copy_factor(d):
@@ -109,8 +113,15 @@ def pmc_summary(base):
             e[key] += float(r["Counter_Value"])
             if primary:
                 e["launches_" + key] += 1
-    cmd = "python3 -u bench.py --no-cpu --no-batched --no-adjoint --no-cfl"
-    print("# HBM traffic per launch, block_1m (1,048,576 particles, R = 5)\n")
+    import fluid_amd as fa
+    sc = {"block_1m": fa.scenes.block_1m, "block_8m": fa.scenes.block_8m,
+          "dam_break_16k": fa.scenes.dam_break}[workload](seed=0)
+    n, cells = sc.n, sc.grid.cell_count()
+    refine = sc.schedule()[2]
+    window = (f" --steps {steps} --warmup {warmup}" if steps else "")
+    cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + window + \
+        " --no-cpu --no-batched --no-adjoint --no-cfl"
+    print(f"# HBM traffic per launch, {workload} ({n:,} particles, {cells:,} cells, R = {refine})\n")
     print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
     print("```")
     for c, d in (("FETCH_SIZE", "pmc_F"), ("WRITE_SIZE", "pmc_W")):
@@ -118,36 +129,154 @@ def pmc_summary(base):
     for c, d in (("FETCH_SIZE", "pmc_copyF"), ("WRITE_SIZE", "pmc_copyW")):
         print(f"rocprofv3 --pmc {c} --output-format csv -d {base}/{d} -o pmc -- "
               f"python3 -u bench.py --copy-calibration {COPY_BYTES}")
-    print(f"python3 bench.py --pmc-summary {base} > profiles/pmc_block_1m.md")
+    traces = glob.glob(os.path.join(base, "trace", "*kernel_trace.csv"))
+    if traces:
+        print(f"rocprofv3 --kernel-trace --stats --output-format csv -d {base}/trace -o trace -- {cmd}")
+    out = "profiles/pmc_block_1m.md" if workload == "block_1m" else f"profiles/roofline_{workload}.md"
+    print(f"python3 bench.py --pmc-summary {base}" + ("" if workload == "block_1m" else f" --workload {workload}") +
+          (f" --steps {steps} --warmup {warmup}" if steps else "") + f" > {out}")
     print("```\n")
     print(f"Calibration: a {COPY_BYTES >> 20} MiB device copy (HIP's copy kernel) reads FETCH_SIZE = "
           f"{fcopy:.0f} KiB and WRITE_SIZE = {wcopy:.0f} KiB, so traffic = {ff:.3f} x FETCH_SIZE + "
           f"{fw:.3f} x WRITE_SIZE (KiB x 1024 bytes).\n")
     print("| kernel | launches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic MB | algorithmic MB |")
     print("|---|---|---|---|---|---|")
-    n, cells = 1048576, 357911
-    for cls in ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
-                "forces_den_apply", "den_kappa", "den_apply", "integrate"]:
+    classes = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
+               "forces_den_apply", "den_kappa", "den_apply", "integrate"]
+    traffic = {}
+    for cls in classes:
         e = per.get(cls)
         if not e or not e["launches_fetch"] or not e["launches_write"]:
             continue
         fk = e["fetch"] / e["launches_fetch"]
         wk = e["write"] / e["launches_write"]
-        traffic = (ff * fk + fw * wk) * 1024.0
+        traffic[cls] = (ff * fk + fw * wk) * 1024.0
         alg = kernel_bytes(cls, n, cells)
-        print(f"| {cls} | {e['launches_fetch']} | {fk:.0f} | {wk:.0f} | {traffic / 1e6:.1f} | "
+        print(f"| {cls} | {e['launches_fetch']} | {fk:.0f} | {wk:.0f} | {traffic[cls] / 1e6:.1f} | "
               f"{alg / 1e6:.1f} |")
+    if not traces:
+        return
+    rs = sorted(csv.DictReader(open(traces[0])), key=lambda r: int(r["Start_Timestamp"]))
+    rs = [r for r in rs if "k_" in r["Kernel_Name"]]
+    tot, occ = {}, {}
+    for r, (cls, primary) in zip(rs, _kernel_classes([r["Kernel_Name"] for r in rs])):
+        if cls is None:
+            continue
+        tot[cls] = tot.get(cls, 0.0) + (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
+        occ[cls] = occ.get(cls, 0) + (1 if primary else 0)
+    print("\nPer-kernel roofline, from the kernel trace of the same command: average launch time (a class's"
+          " launches summed, place + gather as grid_scatter, tile_regions with density_div), algorithmic"
+          f" bytes over that time, and its fraction of MI355X's {HBM_PEAK_GBS / 1000:.0f} TB/s.\n")
+    print("| kernel | launches | avg µs | algorithmic GB/s | frac of HBM peak | measured traffic GB/s |")
+    print("|---|---|---|---|---|---|")
+    sub = 0.0
+    for cls in classes:
+        if not occ.get(cls):
+            continue
+        us = tot[cls] / occ[cls]
+        alg = kernel_bytes(cls, n, cells) / (us * 1e-6) / 1e9
+        meas = traffic[cls] / (us * 1e-6) / 1e9 if cls in traffic else float("nan")
+        print(f"| {cls} | {occ[cls]} | {us:.1f} | {alg:.0f} | {alg / HBM_PEAK_GBS:.3f} | {meas:.0f} |")
+        sub += us * occ[cls]
+    substeps = occ.get("integrate", 1)
+    canon = canonical_bytes(n, cells, refine)
+    print(f"\nA substep's kernels: {sub / substeps:.1f} µs of kernel time; canonical bytes "
+          f"{canon / 1e6:.1f} MB, {canon / (sub / substeps * 1e-6) / 1e9:.0f} GB/s "
+          f"({canon / (sub / substeps * 1e-6) / 1e9 / HBM_PEAK_GBS:.3f} of peak).")
+
+
+def _short_name(name):
+    import re
+    m = re.search(r"\b(k_\w+?)(<[^()]*>)?\(", name)
+    if not m:
+        return name.split("(")[0]
+    args = m.group(2) or ""
+    return m.group(1) + args.replace("fluid::", "")
+
+
+def trace_summary(base, workload="block_1m", steps=0, warmup=0):
+    """Prints the markdown table of a rocprofv3 --kernel-trace --stats run
+    under base/trace (trace_kernel_stats.csv): calls, average, min and max
+    duration of each kernel."""
+    import csv
+    f = os.path.join(base, "trace", "trace_kernel_stats.csv")
+    rs = list(csv.DictReader(open(f)))
+    window = (f" --steps {steps} --warmup {warmup}" if steps else "")
+    cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + window + \
+        " --no-cpu --no-batched --no-adjoint --no-cfl"
+    print(f"# DFSPH substep kernels, {workload}, kernel trace\n")
+    print("Commands, from the repository root on the MI355X box:\n")
+    print("```")
+    print(f"rocprofv3 --kernel-trace --stats --output-format csv -d {base}/trace -o trace -- {cmd}")
+    print(f"python3 bench.py --trace-summary {base}" + ("" if workload == "block_1m" else f" --workload {workload}") +
+          (f" --steps {steps} --warmup {warmup}" if steps else "") + f" > profiles/kernels_{workload}.md")
+    print("```\n")
+    print("`k_kappa_apply` serves div_apply (1 a substep) and den_apply (R a substep); the solve's kernels are the"
+          " 128-slot tile build (namespace t128).\n")
+    print("| kernel | calls | avg µs | min µs | max µs | total ms | % |")
+    print("|---|---|---|---|---|---|---|")
+    for r in sorted(rs, key=lambda r: -float(r["TotalDurationNs"])):
+        print(f"| {_short_name(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+              f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+              f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
+
+
+def sq_summary(base, workload="block_1m"):
+    """Prints per-wave shader counters of two rocprofv3 --pmc runs under
+    base/sq_A and base/sq_B, averaged over every launch of each kernel."""
+    import csv
+    acc = {}
+    for d in ("sq_A", "sq_B"):
+        f = os.path.join(base, d, "pmc_counter_collection.csv")
+        for r in csv.DictReader(open(f)):
+            if "k_" not in r["Kernel_Name"]:
+                continue
+            k = _short_name(r["Kernel_Name"])
+            disp = acc.setdefault(k, {}).setdefault((d, r["Dispatch_Id"]), {})
+            disp[r["Counter_Name"]] = float(r["Counter_Value"])
+    cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + \
+        " --no-cpu --no-batched --no-adjoint --no-cfl"
+    print(f"# Shader counters per wave, {workload}\n")
+    print("Commands, from the repository root on the MI355X box (two passes within the per-block counter limits):\n")
+    print("```")
+    print("rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_LDS SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR "
+          f"SQ_WAVE_CYCLES SQ_BUSY_CYCLES --output-format csv -d {base}/sq_A -o pmc -- {cmd}")
+    print("rocprofv3 --pmc SQ_WAIT_INST_ANY SQ_ACTIVE_INST_VALU SQ_ACTIVE_INST_LDS SQ_ACTIVE_INST_ANY "
+          f"SQ_LDS_BANK_CONFLICT SQ_WAIT_ANY GRBM_GUI_ACTIVE --output-format csv -d {base}/sq_B -o pmc -- {cmd}")
+    print(f"python3 bench.py --sq-summary {base} > profiles/sq_{workload}.md")
+    print("```\n")
+    print("Per wave, averaged over every launch of the run (the counters' own cycle units).\n")
+    cols = [("SQ_INSTS_VALU", "VALU"), ("SQ_INSTS_SALU", "SALU"), ("SQ_INSTS_LDS", "LDS"),
+            ("SQ_INSTS_VMEM_RD", "VMEM rd"), ("SQ_INSTS_VMEM_WR", "VMEM wr"), ("SQ_WAVE_CYCLES", "wave cycles"),
+            ("SQ_WAIT_ANY", "wait cycles"), ("SQ_LDS_BANK_CONFLICT", "LDS bank conflicts")]
+    print("| kernel | waves a launch | " + " | ".join(c[1] for c in cols) + " |")
+    print("|---|---|" + "---|" * len(cols))
+    for k, disps in sorted(acc.items()):
+        waves = [v["SQ_WAVES"] for (d, _), v in disps.items() if d == "sq_A" and v.get("SQ_WAVES")]
+        if not waves:
+            continue
+        wave_n = sum(waves) / len(waves)
+        row = []
+        for c, _ in cols:
+            vals = [v[c] for v in disps.values() if c in v]
+            row.append(f"{sum(vals) / len(vals) / wave_n:.1f}" if vals else "-")
+        print(f"| {k} | {wave_n:.0f} | " + " | ".join(row) + " |")
 
 
 def load_traffic(path=PMC_SUMMARY):
     """kernel class -> measured HBM bytes per launch from the committed
-    counter summary, or {} when there is none."""
+    counter summary's traffic table, or {} when there is none."""
     out = {}
     if not os.path.exists(path):
         return out
+    in_table = False
     for line in open(path):
+        if line.startswith("|"):
+            in_table = in_table or "traffic MB" in line
+        else:
+            in_table = False
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and (cells[0] in BYTES_PER_PARTICLE or cells[0] in BYTES_PER_CELL):
+        if in_table and len(cells) == 6 and (cells[0] in BYTES_PER_PARTICLE or cells[0] in BYTES_PER_CELL):
             try:
                 out[cells[0]] = float(cells[4]) * 1e6
             except ValueError:
@@ -181,6 +310,71 @@ def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=64):
 TOL_DIVERGENCE = 1e-3
 TOL_DENSITY = 1e-4
 MAX_PASSES = 256
+
+
+def cpu_configs(fa, runs=5):
+    """The CPU baseline plan's runs of the oracle's FP32 step on this host's
+    cores: the 16,384-particle dam break for 200 frames in full (median of
+    `runs` runs, after 1 warm-up frame) with the time of each kernel class,
+    10 frames of block_1m and 1 frame of block_8m, each reported per
+    particle-substep."""
+    import oracle
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "1024")))
+    oracle.set_threads(cores)
+    out = {"cores": cores, "kind": "port", "timer": "time.perf_counter around whole substeps"}
+
+    def frames_of(sc, frames, first=0):
+        n, dt, refine, vc = sc.schedule()
+        return [(sc.step_block((first * n + k) * dt, dt, vc, seed=first * n + k), refine)
+                for k in range(frames * n)], n, refine
+
+    # config 1 in full
+    sc = fa.scenes.dam_break(seed=0)
+    warm, n, refine = frames_of(sc, 1)
+    plan, _, _ = frames_of(sc, 200, first=1)
+    times = []
+    for _ in range(runs):
+        o = oracle.OracleSim(sc.params, precision=32)
+        o.set_state(sc.pos4)
+        for st, r in warm:
+            o.substep(st, r)
+        t0 = time.perf_counter()
+        for st, r in plan:
+            o.substep(st, r)
+        times.append(time.perf_counter() - t0)
+        o.close()
+    med = sorted(times)[len(times) // 2]
+    # the kernel classes of one frame, launch by launch
+    o = oracle.OracleSim(sc.params, precision=32)
+    o.set_state(sc.pos4)
+    per = {}
+    for st, r in plan[:n]:
+        for name in ("grid", "density_div", "div_apply", "forces_eval", "forces_den_apply") + \
+                ("den_kappa", "den_apply") * r + ("integrate",):
+            t0 = time.perf_counter()
+            o.kernel(name, st)
+            per[name] = per.get(name, 0.0) + (time.perf_counter() - t0) * 1e3 / n
+    o.close()
+    out["dam_break_16k"] = {"particles": sc.n, "frames": 200, "substeps": len(plan), "refine": refine,
+                            "runs": runs, "median_s": med, "value": sc.n * len(plan) / med,
+                            "unit": "particle-substeps/s", "ms_per_substep": med * 1e3 / len(plan),
+                            "per_kernel_ms_per_substep": {k: round(v, 4) for k, v in per.items()}}
+    # configs 2 and 3, subsets
+    for name, maker, frames in (("block_1m", fa.scenes.block_1m, 10), ("block_8m", fa.scenes.block_8m, 1)):
+        sc = maker(seed=0)
+        plan, n, refine = frames_of(sc, frames)
+        o = oracle.OracleSim(sc.params, precision=32)
+        o.set_state(sc.pos4)
+        t0 = time.perf_counter()
+        for st, r in plan:
+            o.substep(st, r)
+        dt_ = time.perf_counter() - t0
+        o.close()
+        out[name] = {"particles": sc.n, "frames": frames, "substeps": len(plan), "refine": refine, "seconds": dt_,
+                     "value": sc.n * len(plan) / dt_, "unit": "particle-substeps/s",
+                     "ms_per_substep": dt_ * 1e3 / len(plan), "subset": f"the first {frames} frame(s)"}
+        print(f"cpu {name}: {out[name]['ms_per_substep']:.1f} ms a substep", file=sys.stderr, flush=True)
+    return out
 
 
 def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0, autotune=True,
@@ -414,11 +608,25 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-cfl", action="store_true", help="skip the adaptive-schedule leg")
     ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint line")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="print only the oracle's CPU timings of configs 1-3 (the CPU baseline plan)")
+    ap.add_argument("--trace-summary", metavar="DIR", default="",
+                    help="print the kernel table of a rocprofv3 --kernel-trace --stats run under DIR/trace and exit")
+    ap.add_argument("--sq-summary", metavar="DIR", default="",
+                    help="print the per-wave shader counters of the runs under DIR/sq_A, DIR/sq_B and exit")
     ap.add_argument("--pmc-summary", metavar="DIR", default="",
                     help="print the counter summary of rocprofv3 --pmc runs under DIR and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.workload, args.steps if args.steps != 300 else 0,
+                      args.warmup if args.steps != 300 else 0)
+        return
+    if args.sq_summary:
+        sq_summary(args.sq_summary, args.workload)
+        return
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary)
+        pmc_summary(args.pmc_summary, args.workload, args.steps if args.steps != 300 else 0,
+                    args.warmup if args.steps != 300 else 0)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the DFSPH step runs on a single MI355X, and no "
@@ -429,6 +637,9 @@ def main():
         return
 
     import fluid_amd as fa
+    if args.cpu_configs:
+        print(json.dumps({"cpu_configs": cpu_configs(fa)}), flush=True)
+        return
     if fa.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)

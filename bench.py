@@ -284,7 +284,7 @@ def load_traffic(path=PMC_SUMMARY):
     return out
 
 
-def cpu_baseline(scene, step, refine, budget_s=15.0, max_substeps=64):
+def cpu_baseline(scene, step, refine, budget_s=10.0, max_substeps=64):
     """Times the oracle's FP32 step on this host's cores on a bounded sample."""
     import oracle
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "1024")))

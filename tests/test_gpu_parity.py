@@ -84,19 +84,18 @@ def test_one_substep_matches_oracle(maker):
     assert g["counters"] == o["counters"]
 
 
-def test_drift_over_substeps_is_reported_and_small():
+def test_ten_substeps_keep_every_particle_and_word():
     sc = fa.scenes.dam_break(seed=5)
     step, refine = _step(sc)
     steps = [step] * 10
     g = run_gpu(sc, steps, refine, ids=True)
     o = run_oracle(sc, steps, refine)
-    ok = np.array_equal(g["ids"], o["ids"])
-    print("same particle order after 10 substeps:", ok)
-    if ok:
-        for k in ("pos", "vel"):
-            print(k, "drift after 10 substeps vs fp32 oracle", rel_err(g[k], o[k]))
-        assert rel_err(g["pos"], o["pos"]) <= 1e-5
-        assert rel_err(g["vel"], o["vel"]) <= 1e-3
+    # the same particle in the same slot, then the words themselves
+    assert np.array_equal(g["ids"], o["ids"])
+    for k in ("pos", "vel"):
+        err = rel_err(g[k], o[k])
+        print(k, "drift after 10 substeps vs fp32 oracle", err)
+        assert err == 0.0, k
 
 
 def test_200_substeps_stay_bit_exact_with_the_fp32_oracle():

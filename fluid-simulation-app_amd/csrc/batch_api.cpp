@@ -84,7 +84,7 @@ struct FluidBatch {
     size_t lds_bytes = 0, lds_limit = 0;
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
-    Tolerance tol{0.0f, 1, 0, 0.0f, 1};
+    Tolerance tol{0.0f, 1, 0, 0.0f, 1, true};
     Physics phys{};
     EventTimer timer;
     DeviceAllocs mem;
@@ -235,6 +235,9 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     if (const char* env = getenv("FLUID_BATCH_GROUP")) B->group_slots = (size_t)strtoull(env, nullptr, 10);
     B->nstreams = kGroupStreams;
     if (const char* env = getenv("FLUID_BATCH_STREAMS")) B->nstreams = (uint32_t)strtoul(env, nullptr, 10);
+    // Tolerance mode's passes on work lists (FLUID_TOL_PERSISTENT=0: every
+    // tile dispatched each pass).
+    if (const char* env = getenv("FLUID_TOL_PERSISTENT")) B->tol.persistent = strtol(env, nullptr, 10) != 0;
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
     int rc = apply_layout(B, solve_variant(kSolveTile), B->group_slots, B->nstreams);
@@ -282,6 +285,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.iterations, S);
     A(b.div_iterations, S);
     A(b.pass_hist, 2 * kHistBins);
+    A(b.tile_lists, 2 * (2 + (S + 1) + S));
     A(B->d_pass_limit, 2 * S);
     A(B->d_ks, S);
     A(B->d_tile_scene, std::max<uint32_t>(B->units, 1));
@@ -529,6 +533,14 @@ int fluid_batch_set_div_tolerance(FluidBatch* b, float div_tol, uint32_t max_pas
     b->drop_graphs();
     b->tol.div_tol = div_tol > 0.0f ? div_tol : 0.0f;
     b->tol.div_max = div_tol > 0.0f ? max_passes : 1u;
+    return FLUID_OK;
+}
+
+int fluid_batch_persistent_passes(FluidBatch* b, bool enable) {
+    if (!valid(b)) return FLUID_ERR_ARG;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    b->drop_graphs();
+    b->tol.persistent = enable;
     return FLUID_OK;
 }
 

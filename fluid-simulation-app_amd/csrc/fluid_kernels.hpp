@@ -223,6 +223,9 @@ struct SolveBufs {
     // Tolerance mode's pass-count histograms, [2][kHistBins] (density, then
     // divergence): one count per scene and substep at the passes it ran.
     uint32_t* pass_hist;
+    // Tolerance mode of a batch: two work lists of the tiles still
+    // iterating, 2 + (scenes + 1) + scenes words each (solve.hip).
+    uint32_t* tile_lists;
     TileRegion* regions;  // [tiles]
 };
 
@@ -376,11 +379,16 @@ struct StageBufs {
 // device words (density limits, then divergence limits); sc's pass_limit
 // and div_limit are set from it, and b's residuals, iterations and pass
 // histograms are used.
+// A batch may run the passes of each solve under tolerance on work lists
+// (`persistent`): a fixed grid of workgroups loops over the tiles of the
+// scenes still iterating, so a pass no scene needs costs a few microseconds
+// instead of a dispatch of every tile.
 struct Tolerance {
     float tol;
     uint32_t min_passes, max_passes;
     float div_tol;
     uint32_t div_max;
+    bool persistent;
     bool on() const { return tol > 0.0f || div_tol > 0.0f; }
 };
 

@@ -109,7 +109,7 @@ struct FluidSim {
     // The solve of the tile size in use (128 slots unless set).
     const SolveVariant* V = nullptr;
     // Tolerance mode (off: the reference schedule) and its two pass limits.
-    Tolerance tol{0.0f, 1, 0, 0.0f, 1};
+    Tolerance tol{0.0f, 1, 0, 0.0f, 1, false};
     uint32_t* limits = nullptr;
     EventTimer timer;
     DeviceAllocs mem;

@@ -831,10 +831,9 @@ __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, co
 // Reads only kd of the neighbours, so the velocity update is in place
 // (vel_out = vel_in) unless the caller keeps both (the taped forward).
 template <class Sc>
-__global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t pass, const float4* __restrict__ pos,
-                                                               const float4* vel_in, float4* vel_out, SolveBufs b) {
-    extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index(sc);
+__device__ __forceinline__ void kappa_apply_on(const Sc& sc, uint32_t tile, uint32_t pass,
+                                               const float4* __restrict__ pos, const float4* vel_in,
+                                               float4* vel_out, const SolveBufs& b, float4* smem) {
     const RegionS R = load_region(b.regions, tile);
     if (!sc.runs_at(R.scene, R.active, R.refine, pass)) return;
     const SceneK& k = sc.scene_at(R.scene);
@@ -854,6 +853,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
         vel_out[id.p] = make_float4(w.x - s.dt * full.x, w.y - s.dt * full.y, w.z - s.dt * full.z, w.w);
     }
+}
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t pass, const float4* __restrict__ pos,
+                                                               const float4* vel_in, float4* vel_out, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    kappa_apply_on<Sc>(sc, tile_index(sc), pass, pos, vel_in, vel_out, b, smem);
 }
 
 // forces_eval's gathers: position, velocity, density, near-density,
@@ -1105,11 +1110,10 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
 // den_kappa (sim_solve.wgsl:674-698), and with DIV the divergence kappa of
 // tolerance mode's further divergence passes: density_div's kappa formula
 // (sim_solve.wgsl:508-516) from the current velocities, no pressure.
-template <class Sc, bool DIV = false>
-__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
-                                                             const float4* __restrict__ vel, SolveBufs b) {
-    extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index(sc);
+template <class Sc, bool DIV>
+__device__ __forceinline__ void den_kappa_on(const Sc& sc, uint32_t tile, uint32_t pass,
+                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                             const SolveBufs& b, float4* smem) {
     const RegionS R = load_region(b.regions, tile);
     if (!sc.runs_at(R.scene, R.active, R.refine, pass)) return;
     const SceneK& k = sc.scene_at(R.scene);
@@ -1139,6 +1143,12 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pas
         }
     }
     if (sum) block_residual(err, sum + sc.index_at(R.scene));
+}
+template <class Sc, bool DIV = false>
+__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
+                                                             const float4* __restrict__ vel, SolveBufs b) {
+    extern __shared__ float4 smem[];
+    den_kappa_on<Sc, DIV>(sc, tile_index(sc), pass, pos, vel, b, smem);
 }
 
 // Reads the working set, writes the resting set at the same slot.
@@ -1241,6 +1251,134 @@ __global__ __launch_bounds__(kBlock) void k_tally_passes(Sc sc, uint32_t scenes,
     atomicAdd(&b.pass_hist[kHistBins + div], 1u);
 }
 
+// Tolerance mode of a batch, work lists: each pass's sweeps run on a fixed
+// grid of workgroups that loop over the tiles of the scenes still iterating,
+// listed by k_converge_list, so that a pass costs the tiles it has and a
+// pass no scene needs costs a few microseconds. A list (list_words(scenes)
+// words) holds the tile count, the listed scenes, the prefix of their tile
+// counts (one more entry) and each one's first tile.
+__host__ __device__ __forceinline__ uint32_t list_words(uint32_t scenes) { return 2u + (scenes + 1u) + scenes; }
+
+// After the kappa sweep of `pass` (when `check`), k_converge's rule for every
+// scene; then the list of the scenes that run pass `next`. One workgroup.
+template <class Sc>
+__global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, uint32_t pass, uint32_t check,
+                                                        float tol, uint32_t min_passes,
+                                                        unsigned long long* __restrict__ residual,
+                                                        uint32_t* __restrict__ limit, uint32_t* __restrict__ iterations,
+                                                        uint32_t next, uint32_t* __restrict__ list) {
+    __shared__ uint32_t s_tiles[16], s_scenes[16];
+    uint32_t* const first = list + 2;
+    uint32_t* const tile0 = list + 3 + scenes;
+    const uint32_t w = threadIdx.x / 64u;
+    uint32_t tiles_carry = 0, scenes_carry = 0;
+    for (uint32_t base = 0; base < scenes; base += 1024u) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t cnt = 0, t0 = 0;
+        if (i < scenes && sc.step_at(i).active != 0u) {
+            const SceneK& k = sc.scene_at(i);
+            uint32_t lim = limit[i];
+            if (check && pass < lim) {
+                const uint32_t n = k.slot_end - k.slot_begin;
+                const double mean = (double)residual[i] * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+                residual[i] = 0;
+                iterations[i] = pass + 1;
+                if (mean <= (double)tol && pass + 1 >= min_passes) {
+                    lim = pass + 1;
+                    limit[i] = lim;
+                }
+            }
+            if (next < lim && k.slot_end > k.slot_begin) {
+                t0 = k.slot_begin / kTile;
+                cnt = (k.slot_end + kTile - 1u) / kTile - t0;
+            }
+        }
+        const uint32_t has = cnt ? 1u : 0u;
+        const uint32_t ct = wave_inclusive_sum(cnt), hs = wave_inclusive_sum(has);
+        if (threadIdx.x % 64u == 63u) {
+            s_tiles[w] = ct;
+            s_scenes[w] = hs;
+        }
+        __syncthreads();
+        uint32_t tb = tiles_carry, sb = scenes_carry, tt = tiles_carry, st = scenes_carry;
+        for (uint32_t j = 0; j < 16u; ++j) {
+            if (j < w) {
+                tb += s_tiles[j];
+                sb += s_scenes[j];
+            }
+            tt += s_tiles[j];
+            st += s_scenes[j];
+        }
+        if (has) {
+            first[sb + hs - 1u] = tb + ct - cnt;
+            tile0[sb + hs - 1u] = t0;
+        }
+        tiles_carry = tt;
+        scenes_carry = st;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        list[0] = tiles_carry;
+        list[1] = scenes_carry;
+        first[scenes_carry] = tiles_carry;
+    }
+}
+
+// Entry k of workgroup b's share of a list of `total` tiles on a grid of g
+// workgroups (a multiple of the XCD count): each XCD takes a contiguous run
+// of the list, as xcd_tile does for a launch of one workgroup a tile, and
+// its workgroups interleave over the run; `flip` walks the list backwards.
+__device__ __forceinline__ bool list_entry(uint32_t b, uint32_t g, uint32_t k, uint32_t total, bool flip,
+                                           uint32_t& e) {
+    const uint32_t x = b % kXcds, w = b / kXcds, g8 = g / kXcds;
+    const uint32_t hi = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
+    const uint32_t idx = (uint32_t)((uint64_t)total * x / kXcds) + w + k * g8;
+    if (idx >= hi) return false;
+    e = flip ? total - 1u - idx : idx;
+    return true;
+}
+
+// The tile of list entry e: the listed scene whose prefix holds e.
+__device__ __forceinline__ uint32_t list_tile(const uint32_t* __restrict__ list, uint32_t scenes, uint32_t e) {
+    const uint32_t* first = list + 2;
+    uint32_t lo = 0, hi = list[1];
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) / 2u;
+        if (first[mid] <= e)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return list[3 + scenes + lo] + (e - first[lo]);
+}
+
+template <bool DIV>
+__global__ __launch_bounds__(kSweepThreads) void k_den_kappa_list(SceneTable sc, uint32_t pass,
+                                                                  const float4* __restrict__ pos,
+                                                                  const float4* __restrict__ vel, SolveBufs b,
+                                                                  const uint32_t* __restrict__ list, uint32_t scenes) {
+    extern __shared__ float4 smem[];
+    const uint32_t total = list[0];
+    uint32_t e = 0;
+    for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, false, e); ++k) {
+        den_kappa_on<SceneTable, DIV>(sc, list_tile(list, scenes, e), pass, pos, vel, b, smem);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_kappa_apply_list(SceneTable sc, uint32_t pass,
+                                                                    const float4* __restrict__ pos, float4* vel,
+                                                                    SolveBufs b, const uint32_t* __restrict__ list,
+                                                                    uint32_t scenes) {
+    extern __shared__ float4 smem[];
+    const uint32_t total = list[0];
+    uint32_t e = 0;
+    for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, true, e); ++k) {
+        kappa_apply_on<SceneTable>(sc, list_tile(list, scenes, e), pass, pos, vel, vel, b, smem);
+        __syncthreads();
+    }
+}
+
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
 constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float2);   // 40
 // A tile within one cell row of its scene stages nine rows of about kTile +
@@ -1275,7 +1413,10 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
                          reinterpret_cast<const void*>(&k_forces_eval<SceneTable>),
                          reinterpret_cast<const void*>(&k_forces_den_apply<SceneTable>),
                          reinterpret_cast<const void*>(&k_den_kappa<SceneTable>),
-                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable, true>)};
+                         reinterpret_cast<const void*>(&k_den_kappa<SceneTable, true>),
+                         reinterpret_cast<const void*>(&k_den_kappa_list<false>),
+                         reinterpret_cast<const void*>(&k_den_kappa_list<true>),
+                         reinterpret_cast<const void*>(&k_kappa_apply_list)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
             avail = 65536 - 1024;
@@ -1464,10 +1605,100 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
     hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
 }
 
+// Workgroups of the work-list sweeps: enough to fill the device a few times
+// over, a multiple of the XCD count.
+constexpr uint32_t kListGrid = 2048;
+
+// A batch's substep solve in tolerance mode with each solve under tolerance
+// run on work lists (k_converge_list, k_den_kappa_list, k_kappa_apply_list):
+// a pass costs the tiles of the scenes still iterating. The launches around
+// them are solve()'s; the results are its bits.
+static void solve_table_listed(hipStream_t st, SceneTable sc, uint32_t scenes, uint32_t n, const uint32_t* starts,
+                               const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
+                               uint32_t passes, uint32_t* limits, KernelTimer* timer) {
+    const bool den = tol.tol > 0.0f, div = tol.div_tol > 0.0f && tol.div_max > 1u;
+    sc.pass_limit = limits;
+    sc.div_limit = limits + scenes;
+    SolveBufs t = b;
+    if (!den) t.residual = nullptr;
+    if (!div) t.div_residual = nullptr;
+    const dim3 grid(blocks_for(scenes));
+    hipLaunchKernelGGL(k_reset_passes<SceneTable>, grid, dim3(kBlock), 0, st, sc, scenes, den ? tol.max_passes : 0u,
+                       div ? tol.div_max : 1u, b, limits);
+    if (n > 0) {
+        const uint32_t ntiles = solve_tiles(n);
+        const dim3 tiles(ntiles), sweep(kSweepThreads);
+        const uint32_t lg = ntiles < kListGrid ? (ntiles + kXcds - 1) / kXcds * kXcds : kListGrid;
+        const size_t lds_kappa = stage_slots(b) * kKappaEntry, lds_pv = stage_slots(b) * kPvEntry;
+        const uint32_t words = list_words(scenes);
+        uint32_t* const lists[2] = {b.tile_lists, b.tile_lists + words};
+        SceneTable rev = sc;
+        rev.flip = 1;
+        // converge pass `pass` of one solve (check) and list the scenes of pass `next`
+        auto converge = [&](uint32_t pass, bool check, bool divergence, uint32_t next) {
+            hipLaunchKernelGGL(k_converge_list<SceneTable>, dim3(1), dim3(1024), 0, st, sc, scenes, pass,
+                               check ? 1u : 0u, divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
+                               divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
+                               divergence ? b.div_iterations : b.iterations, next, lists[next % 2u]);
+        };
+        const Span tr = span(timer, KERNEL_DENSITY_DIV);
+        hipExtLaunchKernelGGL(k_tile_regions<SceneTable>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tr.a,
+                              nullptr, 0, sc, starts, work.pos, b.regions, ntiles, b.stage_cap);
+        if (div)
+            hipExtLaunchKernelGGL(k_density_div<SceneTable, true>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0,
+                                  sc, starts, work.pos, work.vel, t);
+        else
+            hipExtLaunchKernelGGL(k_density_div<SceneTable>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0, sc,
+                                  starts, work.pos, work.vel, t);
+        if (div) converge(0, true, true, 1);
+        const Span td = span(timer, KERNEL_DIV_APPLY);
+        hipExtLaunchKernelGGL(k_kappa_apply<SceneTable>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
+                              work.pos, work.vel, work.vel, t);
+        for (uint32_t r = 1; div && r < tol.div_max; ++r) {
+            const Span tk = span(timer, KERNEL_DIV_APPLY);
+            hipExtLaunchKernelGGL(k_den_kappa_list<true>, dim3(lg), sweep, lds_pv, st, tk.a, nullptr, 0, sc,
+                                  kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
+            converge(r, true, true, r + 1);
+            hipExtLaunchKernelGGL(k_kappa_apply_list, dim3(lg), sweep, lds_kappa, st, nullptr, tk.z, 0, sc,
+                                  kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
+        }
+        const Span tf = span(timer, KERNEL_FORCES_EVAL);
+        hipExtLaunchKernelGGL(k_forces_eval<SceneTable>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z,
+                              0, sc, work.pos, work.vel, work.temperature, work.prev_pressure, t);
+        const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
+        hipExtLaunchKernelGGL(k_forces_den_apply<SceneTable>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev,
+                              work.pos, work.vel, work.vel, work.temperature, t);
+        if (den) converge(0, false, false, 0);
+        const uint32_t count = den ? tol.max_passes : passes;
+        for (uint32_t r = 0; r < count; ++r) {
+            const Span tk = span(timer, KERNEL_DEN_KAPPA);
+            const Span tp = span(timer, KERNEL_DEN_APPLY);
+            if (den) {
+                hipExtLaunchKernelGGL(k_den_kappa_list<false>, dim3(lg), sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
+                                      work.pos, work.vel, t, lists[r % 2u], scenes);
+                converge(r, true, false, r + 1);
+                hipExtLaunchKernelGGL(k_kappa_apply_list, dim3(lg), sweep, lds_kappa, st, tp.a, tp.z, 0, sc, r,
+                                      work.pos, work.vel, t, lists[r % 2u], scenes);
+            } else {
+                hipExtLaunchKernelGGL(k_den_kappa<SceneTable>, tiles, sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
+                                      work.pos, work.vel, t);
+                hipExtLaunchKernelGGL(k_kappa_apply<SceneTable>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r,
+                                      work.pos, work.vel, work.vel, t);
+            }
+        }
+        const Span ti = span(timer, KERNEL_INTEGRATE);
+        hipExtLaunchKernelGGL(k_integrate<SceneTable>, tiles, dim3(kTile), 0, st, ti.a, ti.z, 0, sc, work, rest, t);
+    }
+    hipLaunchKernelGGL(k_tally_passes<SceneTable>, grid, dim3(kBlock), 0, st, sc, scenes, b);
+}
+
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
                             const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
                             const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer) {
-    solve_tolerance(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
+    if (tol.persistent && b.tile_lists)
+        solve_table_listed(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
+    else
+        solve_tolerance(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
 }
 
 void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,

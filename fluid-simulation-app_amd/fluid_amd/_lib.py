@@ -88,6 +88,7 @@ def _declare(L):
         "fluid_batch_get_tuning": ([vp, P(u32)], i32),
         "fluid_batch_set_tuning": ([vp, P(u32)], i32),
         "fluid_batch_div_iterations": ([vp, P(u32)], i32),
+        "fluid_batch_persistent_passes": ([vp, C.c_bool], i32),
         "fluid_batch_pass_histogram": ([vp, P(u32), C.c_bool], i32),
         "fluid_batch_use_graph": ([vp, C.c_bool], i32),
         "fluid_batch_reduce_stats": ([vp], i32),

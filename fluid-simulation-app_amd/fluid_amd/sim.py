@@ -437,6 +437,11 @@ class Batch:
         arr = np.array([tile, stage_cap, group_slots, streams], np.uint32)
         check(self.L.fluid_batch_set_tuning(self.h, _u32p(arr)), "batch set_tuning")
 
+    def persistent_passes(self, enable=True):
+        """Tolerance mode's passes on work lists of the tiles still iterating
+        (default) or over every tile; the same bits either way."""
+        check(self.L.fluid_batch_persistent_passes(self.h, bool(enable)), "batch persistent_passes")
+
     def div_iterations(self) -> np.ndarray:
         out = np.empty(self.scenes, np.uint32)
         check(self.L.fluid_batch_div_iterations(self.h, _u32p(out)), "batch div_iterations")

@@ -396,6 +396,12 @@ int fluid_batch_set_tolerance(FluidBatch *batch, float tol, uint32_t min_passes,
  * of the last pass is above div_tol, at most max_passes (<= 256) in all.
  * div_tol <= 0 restores the one pass. */
 int fluid_batch_set_div_tolerance(FluidBatch *batch, float div_tol, uint32_t max_passes);
+/* Whether tolerance mode runs the passes of each solve under tolerance on
+ * work lists (the default: a fixed grid of workgroups loops over the tiles
+ * of the scenes still iterating, so a pass no scene needs costs a few
+ * microseconds) or dispatches every tile each pass. The results are the
+ * same bits. */
+int fluid_batch_persistent_passes(FluidBatch *batch, bool enable);
 /* Density passes each scene ran in the last substep under tolerance mode. */
 int fluid_batch_iterations(FluidBatch *batch, uint32_t *out);
 /* Divergence passes each scene ran in the last substep under tolerance mode. */

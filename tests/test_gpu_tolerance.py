@@ -151,6 +151,35 @@ def test_batch_tolerance_matches_lone_sims(graph):
     print("divergence passes by substep and scene:", [list(map(int, x[1])) for x in its])
 
 
+def test_persistent_passes_give_the_words_of_a_launch_a_pass():
+    """A batch's tolerance mode runs each solve as one persistent cooperative
+    launch by default; a launch a pass gives the same passes and words."""
+    scenes = [fa.scenes.fill_block(f"fb{i}", n, f, seed=60 + i)
+              for i, (n, f) in enumerate([(2100, 0.3), (9000, 0.45), (30000, 0.25), (4096, 0.5)])]
+    plans = [[_plan(sc, k) for sc in scenes] for k in range(3)]
+    out = []
+    for persistent in (True, False):
+        with fa.Batch([sc.params for sc in scenes]) as b:
+            b.persistent_passes(persistent)
+            b.set_tolerance(1e-4, 1, 64)
+            b.set_div_tolerance(1e-3, 32)
+            for i, sc in enumerate(scenes):
+                b.upload_state(i, sc.pos4)
+            its = []
+            for k in range(3):
+                b.substep([p[0] for p in plans[k]], [p[1] for p in plans[k]])
+                its.append((b.iterations().copy(), b.div_iterations().copy()))
+            out.append((its, [b.state(i) for i in range(len(scenes))], b.pass_histogram()))
+    (its_a, st_a, h_a), (its_b, st_b, h_b) = out
+    print("persistent passes:", [list(map(int, x[0])) for x in its_a], [list(map(int, x[1])) for x in its_a])
+    for a, b_ in zip(its_a, its_b):
+        assert np.array_equal(a[0], b_[0]) and np.array_equal(a[1], b_[1])
+    assert np.array_equal(h_a, h_b)
+    for i in range(len(scenes)):
+        for key in STATE:
+            assert np.array_equal(_bits(st_a[i][key]), _bits(st_b[i][key])), (i, key)
+
+
 def test_configs_tolerances_converge_below_the_cap_in_a_developed_ensemble():
     """The configs' tolerances, 0.1% divergence and 0.01% density, on 32
     ensemble scenes after 30 reference substeps: every scene and substep

@@ -310,6 +310,8 @@ def cpu_baseline(scene, step, refine, budget_s=10.0, max_substeps=64):
 TOL_DIVERGENCE = 1e-3
 TOL_DENSITY = 1e-4
 MAX_PASSES = 256
+# Substeps of the plain scene's tolerance leg (the first untimed).
+TOL_SUBSTEPS = 11
 
 
 def cpu_configs(fa, runs=5):
@@ -375,6 +377,14 @@ def cpu_configs(fa, runs=5):
                      "ms_per_substep": dt_ * 1e3 / len(plan), "subset": f"the first {frames} frame(s)"}
         print(f"cpu {name}: {out[name]['ms_per_substep']:.1f} ms a substep", file=sys.stderr, flush=True)
     return out
+
+
+def hist_summary(h):
+    """A pass-count histogram as {passes: count}, with its mean and max."""
+    cnt = {int(p): int(c) for p, c in enumerate(h) if c}
+    n_ = sum(cnt.values())
+    return {"histogram": cnt, "mean": sum(p * c for p, c in cnt.items()) / max(n_, 1),
+            "max": max(cnt) if cnt else 0, "at_cap": cnt.get(MAX_PASSES, 0)}
 
 
 def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0, autotune=True,
@@ -468,11 +478,6 @@ def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0
     tol_batched = total * steps / t_tol
     tol_plain = total * tol_plain_rounds / t_tol_plain
 
-    def hist_summary(h):
-        cnt = {int(p): int(c) for p, c in enumerate(h) if c}
-        n_ = sum(cnt.values())
-        return {"histogram": cnt, "mean": sum(p * c for p, c in cnt.items()) / max(n_, 1),
-                "max": max(cnt) if cnt else 0, "at_cap": cnt.get(MAX_PASSES, 0)}
     canon = sum(canonical_bytes(sc.n, sc.grid.cell_count(), sd[2]) for sc, sd in zip(ens, sched))
     return {
         "metric": "DFSPH particle-steps/s, batched ensemble",
@@ -607,6 +612,7 @@ def main():
     ap.add_argument("--scenes", type=int, default=256)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-cfl", action="store_true", help="skip the adaptive-schedule leg")
+    ap.add_argument("--no-tolerance", action="store_true", help="skip the configs' tolerance leg of the scene")
     ap.add_argument("--adjoint-only", action="store_true", help="print only the adjoint line")
     ap.add_argument("--cpu-configs", action="store_true",
                     help="print only the oracle's CPU timings of configs 1-3 (the CPU baseline plan)")
@@ -661,7 +667,7 @@ def main():
 
     # A bench step is one substep of the reference schedule from the seeded
     # block: `warmup` untimed substeps, `steps` timed ones, then 3 profiled.
-    total = args.warmup + args.steps + 3
+    total = args.warmup + args.steps + 3 + TOL_SUBSTEPS
     steps = [sc.step_block(k * dt_sub, dt_sub, v_clamp) for k in range(total)]
     frame_blocks = [fa.pack_step(force=sc.gravity(k * dt_sub), dt=dt_sub * n_sub, seed=k) for k in range(total)]
 
@@ -712,6 +718,28 @@ def main():
     frame_extra_ms = (time.perf_counter() - t1) * 1e3 / frames
     stats = sim.read_stats()
 
+    # the same scene from where the timed run left it, both solves iterated
+    # to the configs' tolerances (the configs' own reading of the step)
+    tol_leg = None
+    if not args.no_tolerance:
+        sim.set_tolerance(TOL_DENSITY, 1, MAX_PASSES, TOL_DIVERGENCE, MAX_PASSES)
+        sim.pass_histogram(reset=True)
+        first = args.warmup + args.steps + 3
+        sim.substeps(steps[first], 1, refine)  # untimed
+        sim.synchronize()
+        t1 = time.perf_counter()
+        for k in range(1, TOL_SUBSTEPS):
+            sim.substeps(steps[first + k], 1, refine)
+        sim.synchronize()
+        dt_tol = (time.perf_counter() - t1) / (TOL_SUBSTEPS - 1)
+        h = sim.pass_histogram()
+        sim.set_tolerance(0.0)
+        tol_leg = {"density_tol": TOL_DENSITY, "divergence_tol": TOL_DIVERGENCE, "max_passes": MAX_PASSES,
+                   "substeps": TOL_SUBSTEPS - 1, "ms_per_step": dt_tol * 1e3, "value": sc.n / dt_tol,
+                   "density_passes": hist_summary(h[0]), "divergence_passes": hist_summary(h[1]),
+                   "what": "the timed scene continued with both solves iterated until the mean error is at "
+                           "most the tolerance (the histograms include the untimed first substep)"}
+
     ms_per_step = elapsed * 1e3 / args.steps
     value = sc.n * args.steps / elapsed
     b_dom = kernel_bytes(dominant, sc.n, sim.cells)
@@ -756,6 +784,8 @@ def main():
         "stats": {"v_max": float(stats[6]), "rho_min": float(stats[2]), "rho_max": float(stats[3]),
                   "compression_avg": float(stats[0])},
     }
+    if tol_leg:
+        out["tolerance_mode"] = tol_leg
     if not args.no_cfl:
         out["cfl_schedule"] = cfl_leg(fa, sc, develop_s=(args.warmup + args.steps) * dt_sub)
     if not args.no_batched:

@@ -109,7 +109,7 @@ struct FluidSim {
     // The solve of the tile size in use (128 slots unless set).
     const SolveVariant* V = nullptr;
     // Tolerance mode (off: the reference schedule) and its two pass limits.
-    Tolerance tol{0.0f, 1, 0, 0.0f, 1, false};
+    Tolerance tol{0.0f, 1, 0, 0.0f, 1, true};
     uint32_t* limits = nullptr;
     EventTimer timer;
     DeviceAllocs mem;
@@ -224,6 +224,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.iterations, 1);
     A(b.div_iterations, 1);
     A(b.pass_hist, 2 * kHistBins);
+    A(b.tile_lists, 2 * (2 + 2 + 1));
     A(s->limits, 2);
     int lds = 0;
     HIP_TRY(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device));

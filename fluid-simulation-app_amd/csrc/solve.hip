@@ -1352,8 +1352,8 @@ __device__ __forceinline__ uint32_t list_tile(const uint32_t* __restrict__ list,
     return list[3 + scenes + lo] + (e - first[lo]);
 }
 
-template <bool DIV>
-__global__ __launch_bounds__(kSweepThreads) void k_den_kappa_list(SceneTable sc, uint32_t pass,
+template <class Sc, bool DIV>
+__global__ __launch_bounds__(kSweepThreads) void k_den_kappa_list(Sc sc, uint32_t pass,
                                                                   const float4* __restrict__ pos,
                                                                   const float4* __restrict__ vel, SolveBufs b,
                                                                   const uint32_t* __restrict__ list, uint32_t scenes) {
@@ -1361,12 +1361,13 @@ __global__ __launch_bounds__(kSweepThreads) void k_den_kappa_list(SceneTable sc,
     const uint32_t total = list[0];
     uint32_t e = 0;
     for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, false, e); ++k) {
-        den_kappa_on<SceneTable, DIV>(sc, list_tile(list, scenes, e), pass, pos, vel, b, smem);
+        den_kappa_on<Sc, DIV>(sc, list_tile(list, scenes, e), pass, pos, vel, b, smem);
         __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_kappa_apply_list(SceneTable sc, uint32_t pass,
+template <class Sc>
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_kappa_apply_list(Sc sc, uint32_t pass,
                                                                     const float4* __restrict__ pos, float4* vel,
                                                                     SolveBufs b, const uint32_t* __restrict__ list,
                                                                     uint32_t scenes) {
@@ -1374,7 +1375,7 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8
     const uint32_t total = list[0];
     uint32_t e = 0;
     for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, true, e); ++k) {
-        kappa_apply_on<SceneTable>(sc, list_tile(list, scenes, e), pass, pos, vel, vel, b, smem);
+        kappa_apply_on<Sc>(sc, list_tile(list, scenes, e), pass, pos, vel, vel, b, smem);
         __syncthreads();
     }
 }
@@ -1414,9 +1415,12 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
                          reinterpret_cast<const void*>(&k_forces_den_apply<SceneTable>),
                          reinterpret_cast<const void*>(&k_den_kappa<SceneTable>),
                          reinterpret_cast<const void*>(&k_den_kappa<SceneTable, true>),
-                         reinterpret_cast<const void*>(&k_den_kappa_list<false>),
-                         reinterpret_cast<const void*>(&k_den_kappa_list<true>),
-                         reinterpret_cast<const void*>(&k_kappa_apply_list)};
+                         reinterpret_cast<const void*>(&k_den_kappa_list<SceneTable, false>),
+                         reinterpret_cast<const void*>(&k_den_kappa_list<SceneTable, true>),
+                         reinterpret_cast<const void*>(&k_kappa_apply_list<SceneTable>),
+                         reinterpret_cast<const void*>(&k_den_kappa_list<OneScene, false>),
+                         reinterpret_cast<const void*>(&k_den_kappa_list<OneScene, true>),
+                         reinterpret_cast<const void*>(&k_kappa_apply_list<OneScene>)};
     for (const void* f : fns)
         if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)avail) != hipSuccess)
             avail = 65536 - 1024;
@@ -1609,11 +1613,12 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
 // over, a multiple of the XCD count.
 constexpr uint32_t kListGrid = 2048;
 
-// A batch's substep solve in tolerance mode with each solve under tolerance
-// run on work lists (k_converge_list, k_den_kappa_list, k_kappa_apply_list):
-// a pass costs the tiles of the scenes still iterating. The launches around
-// them are solve()'s; the results are its bits.
-static void solve_table_listed(hipStream_t st, SceneTable sc, uint32_t scenes, uint32_t n, const uint32_t* starts,
+// A substep solve in tolerance mode with each solve under tolerance run on
+// work lists (k_converge_list, k_den_kappa_list, k_kappa_apply_list): a pass
+// costs the tiles of the scenes still iterating. The launches around them
+// are solve()'s; the results are its bits.
+template <class Sc>
+static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, const uint32_t* starts,
                                const Records& work, const Records& rest, const SolveBufs& b, const Tolerance& tol,
                                uint32_t passes, uint32_t* limits, KernelTimer* timer) {
     const bool den = tol.tol > 0.0f, div = tol.div_tol > 0.0f && tol.div_max > 1u;
@@ -1623,7 +1628,7 @@ static void solve_table_listed(hipStream_t st, SceneTable sc, uint32_t scenes, u
     if (!den) t.residual = nullptr;
     if (!div) t.div_residual = nullptr;
     const dim3 grid(blocks_for(scenes));
-    hipLaunchKernelGGL(k_reset_passes<SceneTable>, grid, dim3(kBlock), 0, st, sc, scenes, den ? tol.max_passes : 0u,
+    hipLaunchKernelGGL(k_reset_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, den ? tol.max_passes : 0u,
                        div ? tol.div_max : 1u, b, limits);
     if (n > 0) {
         const uint32_t ntiles = solve_tiles(n);
@@ -1632,41 +1637,41 @@ static void solve_table_listed(hipStream_t st, SceneTable sc, uint32_t scenes, u
         const size_t lds_kappa = stage_slots(b) * kKappaEntry, lds_pv = stage_slots(b) * kPvEntry;
         const uint32_t words = list_words(scenes);
         uint32_t* const lists[2] = {b.tile_lists, b.tile_lists + words};
-        SceneTable rev = sc;
+        Sc rev = sc;
         rev.flip = 1;
         // converge pass `pass` of one solve (check) and list the scenes of pass `next`
         auto converge = [&](uint32_t pass, bool check, bool divergence, uint32_t next) {
-            hipLaunchKernelGGL(k_converge_list<SceneTable>, dim3(1), dim3(1024), 0, st, sc, scenes, pass,
+            hipLaunchKernelGGL(k_converge_list<Sc>, dim3(1), dim3(1024), 0, st, sc, scenes, pass,
                                check ? 1u : 0u, divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
                                divergence ? b.div_iterations : b.iterations, next, lists[next % 2u]);
         };
         const Span tr = span(timer, KERNEL_DENSITY_DIV);
-        hipExtLaunchKernelGGL(k_tile_regions<SceneTable>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tr.a,
+        hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tr.a,
                               nullptr, 0, sc, starts, work.pos, b.regions, ntiles, b.stage_cap);
         if (div)
-            hipExtLaunchKernelGGL(k_density_div<SceneTable, true>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0,
+            hipExtLaunchKernelGGL(k_density_div<Sc, true>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0,
                                   sc, starts, work.pos, work.vel, t);
         else
-            hipExtLaunchKernelGGL(k_density_div<SceneTable>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0, sc,
+            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0, sc,
                                   starts, work.pos, work.vel, t);
         if (div) converge(0, true, true, 1);
         const Span td = span(timer, KERNEL_DIV_APPLY);
-        hipExtLaunchKernelGGL(k_kappa_apply<SceneTable>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
+        hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
                               work.pos, work.vel, work.vel, t);
         for (uint32_t r = 1; div && r < tol.div_max; ++r) {
             const Span tk = span(timer, KERNEL_DIV_APPLY);
-            hipExtLaunchKernelGGL(k_den_kappa_list<true>, dim3(lg), sweep, lds_pv, st, tk.a, nullptr, 0, sc,
+            hipExtLaunchKernelGGL(k_den_kappa_list<Sc, true>, dim3(lg), sweep, lds_pv, st, tk.a, nullptr, 0, sc,
                                   kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
             converge(r, true, true, r + 1);
-            hipExtLaunchKernelGGL(k_kappa_apply_list, dim3(lg), sweep, lds_kappa, st, nullptr, tk.z, 0, sc,
+            hipExtLaunchKernelGGL(k_kappa_apply_list<Sc>, dim3(lg), sweep, lds_kappa, st, nullptr, tk.z, 0, sc,
                                   kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
         }
         const Span tf = span(timer, KERNEL_FORCES_EVAL);
-        hipExtLaunchKernelGGL(k_forces_eval<SceneTable>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z,
+        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z,
                               0, sc, work.pos, work.vel, work.temperature, work.prev_pressure, t);
         const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
-        hipExtLaunchKernelGGL(k_forces_den_apply<SceneTable>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev,
+        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev,
                               work.pos, work.vel, work.vel, work.temperature, t);
         if (den) converge(0, false, false, 0);
         const uint32_t count = den ? tol.max_passes : passes;
@@ -1674,29 +1679,29 @@ static void solve_table_listed(hipStream_t st, SceneTable sc, uint32_t scenes, u
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
             const Span tp = span(timer, KERNEL_DEN_APPLY);
             if (den) {
-                hipExtLaunchKernelGGL(k_den_kappa_list<false>, dim3(lg), sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
+                hipExtLaunchKernelGGL(k_den_kappa_list<Sc, false>, dim3(lg), sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
                                       work.pos, work.vel, t, lists[r % 2u], scenes);
                 converge(r, true, false, r + 1);
-                hipExtLaunchKernelGGL(k_kappa_apply_list, dim3(lg), sweep, lds_kappa, st, tp.a, tp.z, 0, sc, r,
+                hipExtLaunchKernelGGL(k_kappa_apply_list<Sc>, dim3(lg), sweep, lds_kappa, st, tp.a, tp.z, 0, sc, r,
                                       work.pos, work.vel, t, lists[r % 2u], scenes);
             } else {
-                hipExtLaunchKernelGGL(k_den_kappa<SceneTable>, tiles, sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
+                hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
                                       work.pos, work.vel, t);
-                hipExtLaunchKernelGGL(k_kappa_apply<SceneTable>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r,
+                hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r,
                                       work.pos, work.vel, work.vel, t);
             }
         }
         const Span ti = span(timer, KERNEL_INTEGRATE);
-        hipExtLaunchKernelGGL(k_integrate<SceneTable>, tiles, dim3(kTile), 0, st, ti.a, ti.z, 0, sc, work, rest, t);
+        hipExtLaunchKernelGGL(k_integrate<Sc>, tiles, dim3(kTile), 0, st, ti.a, ti.z, 0, sc, work, rest, t);
     }
-    hipLaunchKernelGGL(k_tally_passes<SceneTable>, grid, dim3(kBlock), 0, st, sc, scenes, b);
+    hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
 }
 
 void launch_solve_tolerance(hipStream_t st, const SceneTable& sc, uint32_t scenes, uint32_t slots,
                             const uint32_t* starts, const Records& work, const Records& rest, const SolveBufs& b,
                             const Tolerance& tol, uint32_t passes, uint32_t* limits, KernelTimer* timer) {
     if (tol.persistent && b.tile_lists)
-        solve_table_listed(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
+        solve_listed(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
     else
         solve_tolerance(st, sc, scenes, slots, starts, work, rest, b, tol, passes, limits, timer);
 }
@@ -1706,7 +1711,10 @@ void launch_solve_tolerance(hipStream_t st, const OneScene& sc, uint32_t slots, 
                             uint32_t passes, uint32_t* limits, KernelTimer* timer) {
     OneScene one = sc;
     one.s.refine = passes;
-    solve_tolerance(st, one, 1u, slots, starts, work, rest, b, tol, passes, limits, timer);
+    if (tol.persistent && b.tile_lists)
+        solve_listed(st, one, 1u, slots, starts, work, rest, b, tol, passes, limits, timer);
+    else
+        solve_tolerance(st, one, 1u, slots, starts, work, rest, b, tol, passes, limits, timer);
 }
 
 const SolveVariant& variant() {

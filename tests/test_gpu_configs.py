@@ -6,6 +6,8 @@
   the FP32 oracle's; then the bench's developed window (330 substeps), the
   state finite and inside the box with no list overflow, and one more
   substep from that state equal to the oracle's, staged and unstaged.
+- block_1m at the configs' tolerances: two substeps with both solves
+  iterated, the same passes and words as the oracle's tolerance mode.
 - block_8m (8,388,608 particles, random-walk tilt): one substep from the
   seeded block and one from 40 substeps on, each equal to the oracle's.
 - ensemble_256 (256 scenes, 2,048 to 65,536 particles each) under hipGraph
@@ -176,6 +178,40 @@ def test_block_1m_developed_flow_stays_bounded_and_bit_exact(monkeypatch):
     assert np.array_equal(staged_nn, ref["grid"]["nbr_n"])
     assert (c1[0] - c0[0], c1[1] - c0[1]) == ref["counters"]
     _say(f"developed state: neighbours per particle mean {staged_nn.mean():.1f}, max {staged_nn.max()}")
+
+
+@pytest.mark.timeout(600)
+def test_block_1m_at_the_configs_tolerances_is_bit_exact_with_the_oracle():
+    """Config 2's tolerances (divergence 0.1%, density 0.01%; caps of 64
+    passes each) on block_1m after 30 substeps of the reference schedule:
+    two substeps on the GPU and on the oracle's tolerance mode, the same
+    pass counts and every state word equal."""
+    sc = fa.scenes.block_1m(seed=0)
+    _, dt, refine, vc = sc.schedule()
+    _say(f"block_1m: 30 substeps at R = {refine}, then 2 in tolerance mode")
+    setting = (1e-4, 1, 64, 1e-3, 64)
+    with fa.Sim(sc.params) as s:
+        s.upload_state(sc.pos4)
+        for k in range(30):
+            s.substeps(sc.step_block(k * dt, dt, vc), 1, refine)
+        dev = s.state()
+        s.set_tolerance(*setting)
+        got = []
+        for k in range(30, 32):
+            s.substeps(sc.step_block(k * dt, dt, vc), 1, refine)
+            got.append(s.iterations())
+        g = s.state()
+    o = OracleSim(sc.params)
+    o.set_state(dev["pos"], dev["vel"], dev["prev_vel"], dev["prev_pressure"], dev["temperature"])
+    o.set_tolerance(*setting)
+    want = []
+    for k in range(30, 32):
+        o.substep(sc.step_block(k * dt, dt, vc), refine)
+        want.append(o.passes()[0])
+    _say(f"(density, divergence) passes: GPU {got}, oracle {want}")
+    assert got == want
+    _assert_state_bits(g, o.state(), "block_1m in tolerance mode")
+    o.close()
 
 
 @pytest.mark.timeout(900)

@@ -204,6 +204,8 @@ def trace_summary(base, workload="block_1m", steps=0, warmup=0):
     window = (f" --steps {steps} --warmup {warmup}" if steps else "")
     cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + window + \
         " --no-cpu --no-batched --no-adjoint --no-cfl"
+    if workload.startswith("ensemble"):
+        cmd = "python3 -u bench.py --batched-only"
     print(f"# DFSPH substep kernels, {workload}, kernel trace\n")
     print("Commands, from the repository root on the MI355X box:\n")
     print("```")
@@ -211,8 +213,13 @@ def trace_summary(base, workload="block_1m", steps=0, warmup=0):
     print(f"python3 bench.py --trace-summary {base}" + ("" if workload == "block_1m" else f" --workload {workload}") +
           (f" --steps {steps} --warmup {warmup}" if steps else "") + f" > profiles/kernels_{workload}.md")
     print("```\n")
-    print("`k_kappa_apply` serves div_apply (1 a substep) and den_apply (R a substep); the solve's kernels are the"
-          " 128-slot tile build (namespace t128).\n")
+    if workload.startswith("ensemble"):
+        print("The run holds the autotuner's trials, the batched substeps under graph replay, the same batch at the"
+              " configs' tolerances (work-list passes: `_list` kernels), a profiled substep, and the plain loop"
+              " over the same scenes, fixed and at the tolerances (`OneScene` kernels).\n")
+    else:
+        print("`k_kappa_apply` serves div_apply (1 a substep) and den_apply (R a substep); the solve's kernels are"
+              " the 128-slot tile build (namespace t128).\n")
     print("| kernel | calls | avg µs | min µs | max µs | total ms | % |")
     print("|---|---|---|---|---|---|---|")
     for r in sorted(rs, key=lambda r: -float(r["TotalDurationNs"])):
@@ -604,7 +611,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the cpu_baseline leg")
-    ap.add_argument("--workload", default="block_1m", choices=["block_1m", "block_8m", "dam_break_16k"])
+    ap.add_argument("--workload", default="block_1m",
+                    choices=["block_1m", "block_8m", "dam_break_16k", "ensemble_256"],
+                    help="the plain step's scene (ensemble_256 only names the batched leg in summaries)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device copies of BYTES (counter calibration) and exit")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-ensemble leg")
@@ -652,7 +661,7 @@ def main():
     if args.adjoint_only:
         print(json.dumps(adjoint_leg(fa, seed=args.seed)), flush=True)
         return
-    if args.batched_only:
+    if args.batched_only or args.workload == "ensemble_256":
         print(json.dumps(batched_ensemble(fa, scenes=args.scenes, seed=args.seed)), flush=True)
         return
     maker = {"block_1m": fa.scenes.block_1m, "block_8m": fa.scenes.block_8m,

@@ -1,21 +1,26 @@
 // The adjoint of the DFSPH substep for gfx950: reverse sweeps that carry the
 // gradient of a loss on the final state back through one substep.
 //
-// What is differentiated (the adjoint's differentiable profile: near
-// pressure, cohesion, adhesion and XSPH switched off, no touches, no
-// rotation; the grid permutation and neighbour lists held fixed per
-// substep, as recorded by the taped forward):
-//   density_div   rho, alpha and the divergence kappa (sim_solve.wgsl:411-518)
-//                 including the wall fill and wall gradient (:212-328)
+// What is differentiated (the whole step at any physics setting, with no
+// touches; the grid permutation and neighbour lists held fixed per substep,
+// as recorded by the taped forward):
+//   density_div   rho, rho_near, alpha and the divergence kappa
+//                 (sim_solve.wgsl:411-518), including the wall fill and wall
+//                 gradient (:212-328)
 //   div_apply     the divergence-free correction (:594-621, 665-668)
-//   forces_eval   Morris viscosity (:554-560) and the warm start (:586-590)
-//   forces_den_apply  gravity + viscosity, then the warm-start correction
+//   forces_eval   Morris viscosity, near pressure, cohesion with its
+//                 density correction, wall adhesion and the XSPH blend
+//                 (:524-583), and the warm start (:586-590)
+//   forces_den_apply  gravity, the rotating frame's fictitious forces, the
+//                 accelerations and the XSPH blend, then the warm-start
+//                 correction (:630-663)
 //   den_kappa / den_apply   the constant-density Jacobi passes (:670-703),
 //                 reversed pass by pass: each transposed Jacobi pass is a
 //                 neighbour sweep of the same shape as the forward one
 //   integrate     the speed clamp and the wall clamp (:705-756)
 // Temperature, the lens EMAs and the stats do not feed back into the motion
-// and carry no adjoint.
+// and carry no adjoint; the finger pull (touches) has no reverse, and the
+// forward refuses substeps that have touches.
 //
 // Pair geometry: r = x_i - x_j, rho = |r|, grad W_ij = g r with g the list's
 // dW/r, and the Hessian H_ij = g I + hh r r^T with hh = (dw'(q)/h - g) / rho^2.
@@ -75,7 +80,7 @@ __device__ __forceinline__ void add3(float4* a, uint32_t i, f3 v) {
 
 struct PairGeom {
     f3 r;
-    float g, hh;
+    float r2, d, g, hh;  // |r|^2, |r|
 };
 
 // dw'(q): the q-derivative of the kernel-gradient magnitude of grad_scale
@@ -87,6 +92,8 @@ __device__ __forceinline__ PairGeom pair_geom(const SceneK& k, f3 xi, f3 xj, flo
     const float rho = sqrt_rn(r2);
     const float q = rho * k.inv_h;
     const float dwp = q < 1.0f ? k.sigma_h * (-3.0f + 4.5f * q) : k.sigma_h * (1.5f * (2.0f - q));
+    P.r2 = r2;
+    P.d = rho;
     P.g = g;
     P.hh = rho > 1e-9f ? (dwp * k.inv_h - g) / r2 : 0.0f;
     return P;
@@ -283,30 +290,40 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_divergence(SceneK k, uint32_
     add3(lam_x, i, add(scl(k.mass, s_x), scl(di, jt(J, i, vi))));
 }
 
-// forces_den_apply's first half in reverse: v2 = v1 + dt (g + a). lam_v (the
-// adjoint of v2) stays as v1's; lam_a = dt lam_v; the gravity adjoint of the
-// substep is dt sum lam_v (one atomic per block). The warm start kd_w =
-// 0.5 pp / rho^2 and p = 0.5 pp: lam_pp and lam_rho.
+// forces_den_apply's first half in reverse: v2 = v1 + dt (g + f(x, v1) + a)
+// + c_xsph xs with the rotating frame's fictitious force f = -dw x x -
+// w x (w x x) - 2 w x v1. lam_f, the adjoint of v2, arrives in lam_v and
+// leaves as v1's: lam_f + 2 dt w x lam_f; lam_x gains dt (dw x lam_f -
+// w x (w x lam_f)); lam_a = dt lam_f, lam_xs = c_xsph lam_f; the gravity
+// adjoint of the substep is dt sum lam_f (one atomic per block). The warm
+// start kd_w = 0.5 pp / rho^2 and p = 0.5 pp: lam_pp and lam_rho.
 __global__ __launch_bounds__(kAdjBlock) void k_rev_forces_den(StepK s, uint32_t n, const float* __restrict__ rho,
                                                               const float* __restrict__ pp,
-                                                              const float4* __restrict__ lam_v,
+                                                              float4* __restrict__ lam_v,
                                                               const float* __restrict__ lam_kdw,
                                                               const float* __restrict__ lam_p,
-                                                              float4* __restrict__ lam_a, float* __restrict__ lam_pp,
+                                                              float4* __restrict__ lam_a, float4* __restrict__ lam_xs,
+                                                              float4* __restrict__ lam_x, float* __restrict__ lam_pp,
                                                               float* __restrict__ lam_rho, float* __restrict__ lam_g) {
     __shared__ float red[3][kAdjBlock];
     const uint32_t i = blockIdx.x * kAdjBlock + threadIdx.x;
-    f3 lv = mk(0.0f, 0.0f, 0.0f);
+    f3 lf = mk(0.0f, 0.0f, 0.0f);
     if (i < n) {
-        lv = ld3(lam_v, i);
-        lam_a[i] = make_float4(s.dt * lv.x, s.dt * lv.y, s.dt * lv.z, 0.0f);
+        lf = ld3(lam_v, i);
+        const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
+        const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
+        const f3 lv = add(lf, scl(2.0f * s.dt, cross3(om, lf)));
+        lam_v[i] = make_float4(lv.x, lv.y, lv.z, 0.0f);
+        add3(lam_x, i, scl(s.dt, sub(cross3(dom, lf), cross3(om, cross3(om, lf)))));
+        lam_a[i] = make_float4(s.dt * lf.x, s.dt * lf.y, s.dt * lf.z, 0.0f);
+        lam_xs[i] = make_float4(s.c_xsph * lf.x, s.c_xsph * lf.y, s.c_xsph * lf.z, 0.0f);
         const float r = rho[i], lk = lam_kdw[i];
         lam_pp[i] = 0.5f * lk / (r * r) + 0.5f * lam_p[i];
         lam_rho[i] += -lk * pp[i] / (r * r * r);
     }
-    red[0][threadIdx.x] = lv.x;
-    red[1][threadIdx.x] = lv.y;
-    red[2][threadIdx.x] = lv.z;
+    red[0][threadIdx.x] = lf.x;
+    red[1][threadIdx.x] = lf.y;
+    red[2][threadIdx.x] = lf.z;
     __syncthreads();
     for (uint32_t o = kAdjBlock / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o)
@@ -316,50 +333,138 @@ __global__ __launch_bounds__(kAdjBlock) void k_rev_forces_den(StepK s, uint32_t 
     if (threadIdx.x < 3) atomicAdd(&lam_g[threadIdx.x], s.dt * red[threadIdx.x][0]);
 }
 
-// forces_eval's viscosity in reverse: a_i = visc2 sum P_ij (v_i - v_j), P_ij
-// = m f(|r|) / (rho_i rho_j), f = g r^2 / (r^2 + eps). lam_v, lam_rho, lam_x,
-// and d loss / d mu = sum lam_a . a / mu (one atomic per block).
-__global__ __launch_bounds__(kQuadBlock) void k_rev_viscosity(SceneK k, float mu, uint32_t n, SolveBufs b,
-                                                              const float4* __restrict__ pos,
-                                                              const float4* __restrict__ vel,
-                                                              const float* __restrict__ rho,
-                                                              const float4* __restrict__ accel,
-                                                              const float4* __restrict__ lam_a,
-                                                              float4* __restrict__ lam_v, float* __restrict__ lam_rho,
-                                                              float4* __restrict__ lam_x, float* __restrict__ lam_mu) {
+// d cohesion / dr (sim_solve.wgsl:201-210): a = (c - r)^3 r^3, a' = 3 (c -
+// r)^2 r^2 (c - 2 r), doubled on the near branch.
+__device__ __forceinline__ float cohesion_dr(const SceneK& k, float r) {
+    const float c = k.coh_c;
+    if (r >= c || r < 1e-9f) return 0.0f;
+    const float cr = c - r;
+    const float da = 3.0f * cr * cr * r * r * (c - 2.0f * r);
+    return 2.0f * r > c ? k.coh_k * da : 2.0f * k.coh_k * da;
+}
+
+// d wall_adhesion / dt (sim_solve.wgsl:334-340): u = 2 - t where it lies in
+// (0, 2), else the clamp holds u and the derivative is 0.
+__device__ __forceinline__ float wall_adhesion_dt(float t) {
+    const float u = 2.0f - t;
+    if (!(u > 0.0f && u < 2.0f)) return 0.0f;
+    const float P = 1.7847015e-3f + u * (3.3392196e-3f + u * (-4.3394677e-3f + u * (1.6711868e-3f + u * -2.1788750e-4f)));
+    const float dP = 3.3392196e-3f + u * (2.0f * -4.3394677e-3f + u * (3.0f * 1.6711868e-3f + u * (4.0f * -2.1788750e-4f)));
+    return -(2.0f * u * P + u * u * dP);
+}
+
+// forces_eval in reverse (sim_solve.wgsl:544-583). With la = lam_a and lxs =
+// lam_xs, the pair terms of particle i and their adjoints, gathered by i
+// over its (symmetric) list:
+//   viscosity  a_i += visc2 P_ij (v_i - v_j), P_ij = m f(|r|) / (rho_i rho_j),
+//              f = g r^2 / (r^2 + eps);
+//   XSPH       xs_i -= (m / rho_j) W_ij (v_i - v_j);
+//   near       a_i += k_near (w_i + w_j) s(|r|) r, s = wn^2 / max(|r|, 1e-5),
+//              wn = max(1 - |r| / h, 0), w the rho_near of density_div;
+//   cohesion   a_i -= gamma corr_ij c(|r|) r, c = m C(|r|) / max(|r|, 1e-9),
+//              corr_ij = 2 rho0 / (rho_i + rho_j);
+// and the per-particle wall adhesion a_i += beta rho0 A(x_i) (a diagonal
+// Jacobian). Writes lam_w (the adjoint of rho_near), adds to lam_v, lam_rho
+// and lam_x, and d loss / d mu = 2 sum P_ij la_i . (v_i - v_j) (one atomic
+// per block).
+__global__ __launch_bounds__(kQuadBlock) void k_rev_forces(SceneK k, uint32_t n, SolveBufs b,
+                                                           const float4* __restrict__ pos,
+                                                           const float4* __restrict__ vel,
+                                                           const float* __restrict__ rho,
+                                                           const float* __restrict__ w_near,
+                                                           const float4* __restrict__ lam_a,
+                                                           const float4* __restrict__ lam_xs,
+                                                           float4* __restrict__ lam_v, float* __restrict__ lam_rho,
+                                                           float4* __restrict__ lam_x, float* __restrict__ lam_w,
+                                                           float* __restrict__ lam_mu) {
     __shared__ float red[kQuadBlock / 64];
     const Quad q = quad(n, b);
-    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, la = s_v;
-    float s_r = 0.0f, ri = 1.0f;
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, la = s_v, xi = s_v;
+    float s_r = 0.0f, s_w = 0.0f, s_mu = 0.0f, ri = 1.0f;
     if (q.live) {
-        const f3 xi = ld3(pos, q.p), vi = ld3(vel, q.p);
+        xi = ld3(pos, q.p);
+        const f3 vi = ld3(vel, q.p), lxi = ld3(lam_xs, q.p);
         la = ld3(lam_a, q.p);
         ri = rho[q.p];
+        const float wi = w_near[q.p];
+        const float mri = div_rn(k.mass, ri);
         for_lane(b, q, [&](uint32_t j, float g) {
             const PairGeom P = pair_geom(k, xi, ld3(pos, j), g);
-            const float r2 = dot3(P.r, P.r);
-            const float den = r2 + k.eps_h2;
-            const float f = div_rn(P.g * r2, den);
-            const float rj = rho[j];
-            const float pair = div_rn(k.mass, ri * rj) * f;
+            const float r2 = P.r2, d = P.d;
+            const float rj = rho[j], mrj = div_rn(k.mass, rj);
+            const f3 dv = sub(vi, ld3(vel, j)), lxj = ld3(lam_xs, j);
             const f3 dl = sub(la, ld3(lam_a, j));
-            const float dldv = dot3(dl, sub(vi, ld3(vel, j)));
-            s_v = add(s_v, scl(pair, dl));
-            s_r += pair * dldv;
+            // viscosity
+            const float den = r2 + k.eps_h2;
+            const float mrr = div_rn(k.mass, ri * rj);
+            const float pair = mrr * div_rn(P.g * r2, den);
+            const float dldv = dot3(dl, dv);
+            s_v = add(s_v, scl(k.visc2 * pair, dl));
+            s_r -= k.visc2 * pair * dldv / ri;
             const float fp_over_r = div_rn(P.hh * r2, den) + div_rn(2.0f * P.g * k.eps_h2, den * den);
-            s_x = add(s_x, scl(div_rn(k.mass, ri * rj) * fp_over_r * dldv, P.r));
+            float sx = k.visc2 * mrr * fp_over_r * dldv;
+            s_mu += 2.0f * pair * dot3(la, dv);
+            // XSPH
+            const float W = kernel_q(k, d * k.inv_h);
+            s_v = add(s_v, scl(W, sub(scl(mri, lxj), scl(mrj, lxi))));
+            const float lxjdv = dot3(lxj, dv);
+            s_r -= mri / ri * W * lxjdv;
+            sx += P.g * (mri * lxjdv - mrj * dot3(lxi, dv));
+            // near pressure
+            const float wn = wmax(1.0f - d * k.inv_h, 0.0f);
+            if (wn > 0.0f) {
+                const bool far = d >= 1e-5f;
+                const float inv_rm = far ? 1.0f / d : 1.0f / 1e-5f;
+                const float sn = wn * wn * inv_rm;
+                const float snp = -2.0f * wn * k.inv_h * inv_rm - (far ? sn / d : 0.0f);  // ds / d|r|
+                const f3 dln = scl(k.k_near, dl);
+                const float xd = dot3(P.r, dln);
+                s_w += sn * xd;
+                const float wsum = wi + w_near[j];
+                s_x = add(s_x, scl(wsum * sn, dln));
+                if (d > 1e-9f) sx += wsum * snp / d * xd;
+            }
+            // cohesion
+            const float C = cohesion(d, k.coh_c, k.coh_k, k.coh_near), Cp = cohesion_dr(k, d);
+            if (C != 0.0f || Cp != 0.0f) {
+                const float inv_rg = 1.0f / wmax(d, 1e-9f);
+                const float c = k.mass * C * inv_rg;
+                const float cp = k.mass * (Cp * inv_rg - C * inv_rg * inv_rg);  // dc / d|r|
+                const float corr = div_rn(2.0f * k.rho0, ri + rj);
+                const f3 dlt = scl(k.gamma, dl);
+                const float xt = dot3(P.r, dlt);
+                s_r += corr / (ri + rj) * c * xt;
+                s_x = sub(s_x, scl(corr * c, dlt));
+                sx -= corr * cp * inv_rg * xt;
+            }
+            s_x = add(s_x, scl(sx, P.r));
         });
     }
     s_v = fold4(s_v);
     s_x = fold4(s_x);
     s_r = fold4(s_r);
+    s_w = fold4(s_w);
+    s_mu = fold4(s_mu);
     float dmu = 0.0f;
     if (q.live && q.lane == 0) {
         const uint32_t i = q.p;
-        add3(lam_v, i, scl(k.visc2, s_v));
-        lam_rho[i] += -k.visc2 / ri * s_r;
-        add3(lam_x, i, scl(k.visc2, s_x));
-        dmu = dot3(la, ld3(accel, i)) / mu;
+        // wall adhesion: A_a = -wa((x_a - lo_a) / h) + wa((hi_a - x_a) / h)
+        if (near_walls(k, xi)) {
+            const float ih = k.inv_h_wall, s_adh = k.beta * k.rho0;
+            const float xa[3] = {xi.x, xi.y, xi.z}, lav[3] = {la.x, la.y, la.z};
+            float ad[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float tl = (xa[a] - k.wall_lo[a]) * ih, th = (k.wall_hi[a] - xa[a]) * ih;
+                ad[a] = -s_adh * ih * (wall_adhesion_dt(tl) + wall_adhesion_dt(th)) * lav[a];
+            }
+            s_x = add(s_x, mk(ad[0], ad[1], ad[2]));
+        }
+        add3(lam_v, i, s_v);
+        lam_rho[i] += s_r;
+        add3(lam_x, i, s_x);
+        lam_w[i] = s_w;
+        dmu = s_mu;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) dmu += __shfl_xor(dmu, o);
@@ -423,7 +528,9 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_density_pre(SceneK k, StepK 
 }
 
 // density_div in reverse, part 2 (the sweep): lam_v from D, lam_x from D, G,
-// gsq, rho and the walls. rho = m (sigma + sum W) + rho0 fill(x).
+// gsq, rho, rho_near and the walls. rho = m (sigma + sum W) + rho0 fill(x);
+// rho_near w_i = sum wn_ij^3, wn = max(1 - |r| / h, 0), whose adjoint lam_w
+// forces_eval's reverse wrote.
 __global__ __launch_bounds__(kQuadBlock) void k_rev_density(SceneK k, uint32_t n, SolveBufs b,
                                                             const float4* __restrict__ pos,
                                                             const float4* __restrict__ vel,
@@ -433,15 +540,17 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_density(SceneK k, uint32_t n
                                                             const float* __restrict__ lam_rho,
                                                             const float* __restrict__ lam_D,
                                                             const float4* __restrict__ lam_G,
+                                                            const float* __restrict__ lam_w,
                                                             float4* __restrict__ lam_v, float4* __restrict__ lam_x) {
     const Quad q = quad(n, b);
-    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, vi = s_v, Gi = s_v;
+    f3 s_v = mk(0.0f, 0.0f, 0.0f), s_x = s_v, s_n = s_v, vi = s_v, Gi = s_v;
     float di = 0.0f, ri = 0.0f;
     if (q.live) {
         const f3 xi = ld3(pos, q.p);
         vi = ld3(vel, q.p);
         di = lam_D[q.p];
         ri = lam_rho[q.p];
+        const float wi = lam_w[q.p];
         const float4 Gi4 = lam_G[q.p];
         Gi = xyz(Gi4);
         const float si = Gi4.w;
@@ -456,15 +565,19 @@ __global__ __launch_bounds__(kQuadBlock) void k_rev_density(SceneK k, uint32_t n
             t = add(t, scl(2.0f * k.mass * (si + Gj4.w), hess(P, gw)));
             t = add(t, scl(ri + lam_rho[j], gw));
             s_x = add(s_x, t);
+            const float wn = wmax(1.0f - P.d * k.inv_h, 0.0f);
+            if (wn > 0.0f && P.d > 1e-9f)
+                s_n = add(s_n, scl((wi + lam_w[j]) * (-3.0f * wn * wn * k.inv_h / P.d), P.r));
         });
     }
     s_v = fold4(s_v);
     s_x = fold4(s_x);
+    s_n = fold4(s_n);
     if (!q.live || q.lane != 0) return;
     const uint32_t i = q.p;
     add3(lam_v, i, add(scl(k.mass, s_v), scl(di, ld3(wg, i))));
     f3 w = add(scl(di, jt(J, i, vi)), jt(J, i, Gi));
-    w = add(w, scl(k.rho0 * ri, ld3(dfill, i)));
+    w = add(w, add(scl(k.rho0 * ri, ld3(dfill, i)), s_n));
     add3(lam_x, i, add(scl(k.mass, s_x), w));
 }
 
@@ -502,7 +615,7 @@ void adj_zero(hipStream_t st, float* a, size_t count) {
 // pressure it carried over; on exit L.x, L.v, L.pp hold the adjoints of the
 // previous resting set (its order), and L.g / L.mu have this substep's
 // gravity and viscosity terms added.
-void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, float mu, uint32_t n, uint32_t passes,
+void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, uint32_t passes,
                             const Tape& T, const SolveBufs& b, const AdjointBufs& L) {
     if (n == 0) return;
     const dim3 g = adj_grid(n), blk(kAdjBlock), gq = quad_grid(n), bq(kQuadBlock);
@@ -520,11 +633,11 @@ void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, flo
     // forces_den_apply: the warm-start correction, then v2 = v1 + dt (g + a)
     hipLaunchKernelGGL(k_rev_apply<false>, gq, bq, 0, st, k, s, n, b, T.pos, T.kd[1], T.wall_grad, L.J, L.v, L.kd,
                        L.x, T.density, T.alpha, L.p, L.D, L.rho, L.alpha);
-    hipLaunchKernelGGL(k_rev_forces_den, g, blk, 0, st, s, n, T.density, T.pp, L.v, L.kd, L.p, L.a, L.pp, L.rho,
-                       L.g);
-    // forces_eval: viscosity at the velocities after div_apply
-    hipLaunchKernelGGL(k_rev_viscosity, gq, bq, 0, st, k, mu, n, b, T.pos, T.vel[1], T.density, T.accel, L.a, L.v,
-                       L.rho, L.x, L.mu);
+    hipLaunchKernelGGL(k_rev_forces_den, g, blk, 0, st, s, n, T.density, T.pp, L.v, L.kd, L.p, L.a, L.xs, L.x,
+                       L.pp, L.rho, L.g);
+    // forces_eval at the velocities after div_apply
+    hipLaunchKernelGGL(k_rev_forces, gq, bq, 0, st, k, n, b, T.pos, T.vel[1], T.density, T.rho_near, L.a, L.xs, L.v,
+                       L.rho, L.x, L.w, L.mu);
     // div_apply with the divergence kappa
     hipLaunchKernelGGL(k_rev_apply<false>, gq, bq, 0, st, k, s, n, b, T.pos, T.kd[0], T.wall_grad, L.J, L.v, L.kd,
                        L.x, T.density, T.alpha, L.p, L.D, L.rho, L.alpha);
@@ -532,7 +645,7 @@ void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, flo
     hipLaunchKernelGGL(k_rev_density_pre, gq, bq, 0, st, k, s, n, b, T.pos, T.wall_grad, T.density, T.alpha,
                        T.kd[0], L.kd, L.alpha, L.rho, L.D, L.G);
     hipLaunchKernelGGL(k_rev_density, gq, bq, 0, st, k, n, b, T.pos, T.vel[0], T.wall_grad, L.J, L.dfill, L.rho,
-                       L.D, L.G, L.v, L.x);
+                       L.D, L.G, L.w, L.v, L.x);
     hipLaunchKernelGGL(k_rev_gather, g, blk, 0, st, n, T.perm, L.x, L.v, L.pp, L.x_out, L.v_out, L.pp_out);
 }
 

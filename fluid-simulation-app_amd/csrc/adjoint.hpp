@@ -21,6 +21,7 @@ struct Tape {
     float4* vel[kMaxTapePasses + 3];
     float* kd[kMaxTapePasses + 2];
     float* density;
+    float* rho_near;
     float* alpha;
     float4* wall_grad;
     float4* accel;
@@ -35,13 +36,15 @@ struct AdjointBufs {
     float* pp_out;
     float *kd, *D, *rho, *alpha, *pp;
     float4 *a, *G;            // G.w: the adjoint of gsq
+    float4* xs;               // of the XSPH blend forces_eval leaves
+    float* w;                 // of rho_near (the near-pressure density)
     float4 *dfill, *J;        // grad fill; d wall_grad / d x_a (3 a particle)
     float* g;                 // [3] this substep's gravity adjoint (added to)
     float* mu;                // [1] the viscosity adjoint (added to)
 };
 
 void adj_zero(hipStream_t st, float* a, size_t count);
-void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, float mu, uint32_t n, uint32_t passes,
+void launch_reverse_substep(hipStream_t st, const SceneK& k, const StepK& s, uint32_t n, uint32_t passes,
                             const Tape& T, const SolveBufs& b, const AdjointBufs& L);
 
 }  // namespace fluid

@@ -181,12 +181,14 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
         for (auto& v : T.vel) A_(v, n);
         for (auto& k : T.kd) A_(k, n);
         A_(T.density, n);
+        A_(T.rho_near, n);
         A_(T.alpha, n);
         A_(T.wall_grad, n);
         A_(T.accel, n);
         A_(T.perm, n);
         SolveBufs tb = b;
         tb.density = T.density;
+        tb.rho_near = T.rho_near;
         tb.alpha = T.alpha;
         tb.wall_grad = T.wall_grad;
         tb.accel = T.accel;
@@ -215,6 +217,8 @@ int init_adjoint(FluidAdjoint* A, const uint8_t* params64, uint32_t steps, uint3
     A_(L.pp, n);
     A_(L.a, n);
     A_(L.G, n);
+    A_(L.xs, n);
+    A_(L.w, n);
     A_(L.dfill, n);
     A_(L.J, 3 * n);
     A_(A->grad_g, 3 * (size_t)steps);
@@ -318,6 +322,11 @@ int fluid_adjoint_forward(FluidAdjoint* A, const float* pos4, const float* vel4,
             set_error("substep %u: the adjoint takes dt > 0 and at most %u density passes", s, kMaxTapePasses);
             return FLUID_ERR_ARG;
         }
+        if (A->blocks[s].touch_n != 0) {
+            set_error("substep %u has %u touches: the adjoint has no reverse of the finger pull", s,
+                      A->blocks[s].touch_n);
+            return FLUID_ERR_ARG;
+        }
     }
     const size_t n = A->n;
     HIP_TRY(hipStreamSynchronize(A->stream));
@@ -385,7 +394,7 @@ int fluid_adjoint_backward(FluidAdjoint* A, const float* dpos4, const float* dve
         for (uint32_t s = s1; s-- > s0;) {
             const uint32_t t = s - s0;
             L.g = A->grad_g + 3 * (size_t)s;
-            launch_reverse_substep(A->stream, A->scene, make_step(A->blocks[s], A->phys), A->phys.mu, A->n,
+            launch_reverse_substep(A->stream, A->scene, make_step(A->blocks[s], A->phys), A->n,
                                    A->refine[s], A->tapes[set][t], A->tape_solve[set][t], L);
             std::swap(L.x, L.x_out);
             std::swap(L.v, L.v_out);

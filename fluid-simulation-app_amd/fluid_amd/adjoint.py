@@ -16,13 +16,20 @@ import numpy as np
 from ._lib import check, lib
 from .sim import _f32p, _u32p, _u8p, pack_step
 
-DIFFERENTIABLE = dict(k_near=0.0, tension_scale=0.0, adhesion_scale=0.0, xsph_rate=0.0)
+# Physics settings past the viscosity (fluid_adjoint_set_physics's last four
+# values). FULL is the reference's step; VISCOUS switches off near pressure,
+# cohesion, adhesion and the XSPH blend.
+FULL = dict(k_near=3000.0, tension_scale=1.0, adhesion_scale=1.0, xsph_rate=6.0)
+VISCOUS = dict(k_near=0.0, tension_scale=0.0, adhesion_scale=0.0, xsph_rate=0.0)
 
 
 class Adjoint:
     """One scene, `steps` substeps, a checkpoint every `every` substeps."""
 
-    def __init__(self, params64: bytes, steps: int, every: int = 8, nbr_cap: int = 0):
+    def __init__(self, params64: bytes, steps: int, every: int = 8, nbr_cap: int = 0, physics=None):
+        """`physics`: a dict of k_near, tension_scale, adhesion_scale and
+        xsph_rate (FULL, VISCOUS or any mix); VISCOUS when None."""
+        self.physics = dict(VISCOUS if physics is None else physics)
         self.L = lib()
         self.h = self.L.fluid_adjoint_create(_u8p(params64), int(steps), int(every), int(nbr_cap))
         if not self.h:
@@ -46,8 +53,9 @@ class Adjoint:
             pass
 
     def set_mu(self, mu: float):
-        """The differentiable profile at viscosity mu."""
-        arr = (C.c_float * 5)(float(mu), 0.0, 0.0, 0.0, 0.0)
+        """Viscosity mu with this Adjoint's other physics settings."""
+        p = self.physics
+        arr = (C.c_float * 5)(float(mu), p["k_near"], p["tension_scale"], p["adhesion_scale"], p["xsph_rate"])
         check(self.L.fluid_adjoint_set_physics(self.h, arr), "adjoint set_physics")
         self.mu = float(mu)
 
@@ -81,9 +89,10 @@ class Adjoint:
         return float(a.value), float(b.value)
 
 
-def steps_for(gravity, dt: float, v_clamp: float):
-    """One Step block per substep from a (K, 3) gravity array."""
-    return [pack_step(force=tuple(float(x) for x in g), dt=dt, v_clamp=v_clamp, seed=k)
+def steps_for(gravity, dt: float, v_clamp: float, omega=(0.0, 0.0, 0.0), domega=(0.0, 0.0, 0.0)):
+    """One Step block per substep from a (K, 3) gravity array, in a frame
+    turning at omega (rad/s) with angular acceleration domega."""
+    return [pack_step(force=tuple(float(x) for x in g), omega=omega, domega=domega, dt=dt, v_clamp=v_clamp, seed=k)
             for k, g in enumerate(np.asarray(gravity, np.float64))]
 
 
@@ -94,14 +103,15 @@ def simulate_function():
 
     class Simulate(torch.autograd.Function):
         """final positions (N, 3) = K substeps of the step from (x0, v0) with
-        viscosity mu and per-substep gravity g (K, 3)."""
+        viscosity mu and per-substep gravity g (K, 3), in a frame turning at
+        frame = (omega, domega) (not differentiated)."""
 
         @staticmethod
-        def forward(ctx, v0, mu, gravity, adj, x0, dt, v_clamp, refine):
+        def forward(ctx, v0, mu, gravity, adj, x0, dt, v_clamp, refine, frame=((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))):
             adj.set_mu(float(mu))
             vel4 = np.zeros((adj.n, 4), np.float32)
             vel4[:, :3] = v0.detach().cpu().numpy()
-            steps = steps_for(gravity.detach().cpu().numpy(), dt, v_clamp)
+            steps = steps_for(gravity.detach().cpu().numpy(), dt, v_clamp, frame[0], frame[1])
             pos_out, _ = adj.forward(x0, vel4, steps, [refine] * adj.steps)
             ctx.adj = adj
             ctx.forward_id = adj.forwards
@@ -119,7 +129,7 @@ def simulate_function():
             d4[:, :3] = grad_pos.detach().cpu().numpy()
             g_v0, _, g_g, g_mu = adj.backward(d4)
             return (torch.from_numpy(g_v0[:, :3].copy()), torch.tensor(g_mu, dtype=torch.float64),
-                    torch.from_numpy(g_g.astype(np.float64)), None, None, None, None, None)
+                    torch.from_numpy(g_g.astype(np.float64)), None, None, None, None, None, None)
 
     return Simulate
 

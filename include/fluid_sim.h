@@ -453,10 +453,13 @@ int fluid_batch_set_physics(FluidBatch *batch, const float *phys5);
  * The gradient of a loss on the final state of `steps` substeps with respect
  * to the initial velocities and positions, the dynamic viscosity and each
  * substep's gravity. The reference has no adjoint; this is the hand-written
- * reverse of the step above in its differentiable profile (set it with
- * fluid_adjoint_set_physics: {mu, 0, 0, 0, 0} switches off near pressure,
- * cohesion, adhesion and XSPH; Steps must have no touches and no rotation).
- * The grid permutation and neighbour lists are held fixed per substep. The
+ * reverse of the whole step above: viscosity, near pressure, cohesion, wall
+ * adhesion, the XSPH blend, the rotating frame and both pressure solves, at
+ * the settings of fluid_adjoint_set_physics ({mu, k_near, tension_scale,
+ * adhesion_scale, xsph_rate}; the default is the reference's step). Steps
+ * must have no touches (the forward refuses them: the finger pull has no
+ * reverse). The grid permutation and neighbour lists are held fixed per
+ * substep. The
  * forward keeps a checkpoint of the resting set every `checkpoint_every`
  * substeps; the backward replays each segment with a tape and runs the
  * reverse sweeps (the density solve's passes as transposed Jacobi sweeps).

@@ -170,3 +170,99 @@ def test_backward_after_a_second_forward_on_the_same_adjoint_raises():
     second.sum().backward()
     assert np.isfinite(v.grad.numpy()).all() and np.abs(v.grad.numpy()).sum() > 0
     adj.close()
+
+
+# The full step's terms one at a time over the viscous profile, then all of
+# them in a turning frame: near pressure, cohesion, wall adhesion, the XSPH
+# blend, the rotating frame's fictitious forces.
+FRAME = ((0.0, 0.0, 0.8), (0.0, 0.0, 0.3))
+NO_FRAME = ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+PROFILES = {
+    "near": (dict(fad.VISCOUS, k_near=3000.0), NO_FRAME),
+    "cohesion": (dict(fad.VISCOUS, tension_scale=1.0), NO_FRAME),
+    "adhesion": (dict(fad.VISCOUS, adhesion_scale=1.0), NO_FRAME),
+    "xsph": (dict(fad.VISCOUS, xsph_rate=6.0), NO_FRAME),
+    "rotation": (fad.VISCOUS, FRAME),
+    "full": (fad.FULL, FRAME),
+}
+
+
+def _tangent(sc, dt, refine, vc, v0, grav, half, c_t, h_t, physics, frame, precision=96, dv=None, dmu=0.0,
+             dg=None, mu=MU):
+    """d loss along a seeded direction through the oracle's dual numbers:
+    precision 96 is the FP32 step's own derivative (float values, double
+    tangents: the trajectory the GPU runs, word for word), 128 the FP64
+    step's."""
+    import torch
+    o = OracleSim(sc.params, precision=precision)
+    o.set_physics(mu, physics["k_near"], physics["tension_scale"], physics["adhesion_scale"], physics["xsph_rate"])
+    o.set_state(sc.pos4, v0)
+    if dv is not None:
+        t = np.zeros((sc.n, 4))
+        t[:, :3] = dv
+        o.set_tangent("vel", t)
+    if dmu:
+        o.seed_mu(dmu)
+    blocks = fad.steps_for(grav, dt, vc, *frame)
+    for k in range(len(grav)):
+        o.seed_force(np.zeros(3) if dg is None else dg[k])
+        o.substep(blocks[k], refine)
+    x = torch.tensor(o.state_f64()["pos"][:, :3], requires_grad=True)
+    L = fad.loss(x, half, c_t, torch.tensor(h_t), xp=torch)
+    L.backward()
+    d = float((x.grad.numpy() * o.tangent("pos")[:, :3]).sum())
+    o.close()
+    return d
+
+
+@pytest.mark.parametrize("profile", list(PROFILES))
+def test_full_step_adjoint_is_the_fp32_steps_derivative(profile):
+    """The adjoint with each of the full step's terms switched on (and all of
+    them in a turning frame) against the forward-mode derivative of the same
+    FP32 trajectory, to 1e-3, along seeded directions of the initial
+    velocities, the viscosity and the gravity vectors; the FP64 step's
+    derivative to 2%. Each term moves the derivative along the initial
+    velocities by far more than the bar (against the viscous profile's
+    derivative), so a dropped or mis-signed term fails."""
+    import torch
+    physics, frame = PROFILES[profile]
+    steps, every = 4, 2
+    sc, dt, refine, vc, v0, grav, half, c_t, h_t = _setup(steps)
+    grav = grav[:steps]
+    adj = fad.Adjoint(sc.params, steps, every, physics=physics)
+    Sim = fad.simulate_function()
+    v = torch.tensor(v0[:, :3], dtype=torch.float32, requires_grad=True)
+    m = torch.tensor(MU, dtype=torch.float64, requires_grad=True)
+    g = torch.tensor(grav, dtype=torch.float64, requires_grad=True)
+    x = Sim.apply(v, m, g, adj, sc.pos4, dt, vc, refine, frame).double()
+    L = fad.loss(x, half, c_t, torch.tensor(h_t), xp=torch)
+    L.backward()
+    adj.close()
+    rng = np.random.default_rng(7)
+    u = rng.normal(0.0, 1.0, (sc.n, 3))
+    ug = rng.normal(0.0, 1.0, (steps, 3))
+    case = (sc, dt, refine, vc, v0, grav, half, c_t, h_t, physics, frame)
+    got = {"v0": float((v.grad.numpy().astype(np.float64) * u).sum()), "mu": float(m.grad),
+           "g": float((g.grad.numpy() * ug).sum())}
+    want = {"v0": _tangent(*case, dv=u), "mu": _tangent(*case, dmu=1.0), "g": _tangent(*case, dg=ug)}
+    fp64 = {"v0": _tangent(*case, precision=128, dv=u)}
+    base = _tangent(sc, dt, refine, vc, v0, grav, half, c_t, h_t, fad.VISCOUS, NO_FRAME, dv=u)
+    print(f"{profile}: loss {float(L):.6e}; v0: adjoint {got['v0']:.6e} fp32 {want['v0']:.6e} "
+          f"fp64 {fp64['v0']:.6e} viscous {base:.6e}; mu: adjoint {got['mu']:.6e} fp32 {want['mu']:.6e}; "
+          f"g: adjoint {got['g']:.6e} fp32 {want['g']:.6e}")
+    for key in got:
+        assert abs(got[key] - want[key]) <= 1e-3 * abs(want[key]), key
+    assert abs(got["v0"] - fp64["v0"]) <= 0.02 * abs(fp64["v0"])
+    assert abs(want["v0"] - base) > 10 * 1e-3 * abs(want["v0"]), "the term does not move the derivative"
+
+
+def test_adjoint_refuses_touches():
+    sc = fa.scenes.fill_block("adj", 600, 0.3, seed=3)
+    nsub, dt, refine, vc = sc.schedule()
+    adj = fad.Adjoint(sc.params, 1, 1)
+    adj.set_mu(MU)
+    touched = fa.pack_step(force=(0.0, -9.81, 0.0), dt=dt, v_clamp=vc, touches=[(0.0, 0.0, 0.1, 0.0)], touch_n=1,
+                           touch_r=0.2)
+    with pytest.raises(RuntimeError, match="touches"):
+        adj.forward(sc.pos4, None, [touched], [refine])
+    adj.close()

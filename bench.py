@@ -555,8 +555,10 @@ def adjoint_leg(fa, n=65536, steps=64, every=8, seed=0):
     """The adjoint configuration: one 65,536-particle scene, 64 substeps with
     a seeded random-walk tilt of gravity, a checkpoint every 8 substeps; the
     forward run and the backward run (segment replays + reverse sweeps),
-    each timed with HIP events on the adjoint's stream. The loss seed is the
-    centroid term's gradient (the cost does not depend on the seed)."""
+    each timed with HIP events on the adjoint's stream, for the reference's
+    whole step and for the viscous profile (near pressure, cohesion,
+    adhesion and the XSPH blend off). The loss seed is the centroid term's
+    gradient (the cost does not depend on the seed)."""
     from fluid_amd import adjoint as fad
     sc = fa.scenes.fill_block("adjoint_64k", n, 0.3, seed=seed)
     _, dt, refine, vc = sc.schedule()
@@ -565,23 +567,30 @@ def adjoint_leg(fa, n=65536, steps=64, every=8, seed=0):
     grav = np.stack([9.81 * np.sin(th), -9.81 * np.cos(th), np.zeros(steps)], 1)
     v0 = np.zeros((sc.n, 4), np.float32)
     v0[:, :3] = rng.normal(0.0, 0.05, (sc.n, 3))
-    adj = fad.Adjoint(sc.params, steps, every)
-    adj.set_mu(1.002e-3)
     blocks = fad.steps_for(grav, dt, vc)
-    pos, _ = adj.forward(sc.pos4, v0, blocks, [refine] * steps)  # warm-up
-    c_t = pos[:, :3].mean(0) + 0.01
-    seed4 = np.zeros((sc.n, 4), np.float32)
-    seed4[:, :3] = 2.0 * (pos[:, :3].mean(0) - c_t) / sc.n
-    adj.backward(seed4)
-    pos, _ = adj.forward(sc.pos4, v0, blocks, [refine] * steps)
-    adj.backward(seed4)
-    fwd_ms, bwd_ms = adj.timing()
-    adj.close()
+
+    def timed(physics):
+        adj = fad.Adjoint(sc.params, steps, every, physics=physics)
+        adj.set_mu(1.002e-3)
+        pos, _ = adj.forward(sc.pos4, v0, blocks, [refine] * steps)  # warm-up
+        c_t = pos[:, :3].mean(0) + 0.01
+        seed4 = np.zeros((sc.n, 4), np.float32)
+        seed4[:, :3] = 2.0 * (pos[:, :3].mean(0) - c_t) / sc.n
+        adj.backward(seed4)
+        adj.forward(sc.pos4, v0, blocks, [refine] * steps)
+        adj.backward(seed4)
+        fwd_ms, bwd_ms = adj.timing()
+        adj.close()
+        return fwd_ms, bwd_ms
+
+    fwd_ms, bwd_ms = timed(fad.FULL)
+    vf, vb = timed(fad.VISCOUS)
     return {"config": {"workload": "adjoint_64k", "particles": sc.n, "substeps": steps, "checkpoint_every": every,
-                       "refine": refine, "profile": "differentiable (viscosity, gravity; near/cohesion/adhesion/"
-                                                    "XSPH off)"},
+                       "refine": refine, "profile": "the whole step (viscosity, near pressure, cohesion, adhesion, "
+                                                    "XSPH); gradients of v0, mu and the gravity vectors"},
             "forward_ms": fwd_ms, "backward_ms": bwd_ms, "adjoint_over_forward": bwd_ms / fwd_ms,
-            "forward_particle_substeps_per_s": sc.n * steps / (fwd_ms * 1e-3)}
+            "forward_particle_substeps_per_s": sc.n * steps / (fwd_ms * 1e-3),
+            "viscous": {"forward_ms": vf, "backward_ms": vb, "adjoint_over_forward": vb / vf}}
 
 
 def copy_calibration(nbytes: int, reps: int = 4) -> None:

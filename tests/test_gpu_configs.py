@@ -303,14 +303,16 @@ def test_ensemble_256_under_graph_replay_matches_lone_sims_and_the_oracle():
 MU = 1.002e-3
 
 
-def _adjoint_config(frame_substeps=None, seed=0, n=65536, steps=64):
+def _adjoint_config(frame_substeps=None, seed=0, n=65536, steps=64, physics="full"):
     """65,536 particles filling 30% of the unit box on the floor, seeded
     initial velocities (N(0, 0.05) m/s a component), a seeded random-walk
     tilt of gravity (sigma 0.05 rad a substep), and seeded targets: the
     centroid moved by (0.02, 0.01, -0.01) m and the 32-bin density histogram
     of the initial block moved 0.03 m along x. The substep is the reference
     schedule's at this spacing (2 a 1/120 s frame, R = 5), or 1/120 s split
-    into `frame_substeps` with R = refine_passes(dt_sub) (render.rs:77-79)."""
+    into `frame_substeps` with R = refine_passes(dt_sub) (render.rs:77-79).
+    `physics`: "full", the reference's step, or "viscous" (near pressure,
+    cohesion, adhesion and the XSPH blend off)."""
     sc = fa.scenes.fill_block("adjoint_64k", n, 0.3, seed=seed)
     _, dt, refine, vc = sc.schedule()
     if frame_substeps:
@@ -324,7 +326,8 @@ def _adjoint_config(frame_substeps=None, seed=0, n=65536, steps=64):
     grav = np.stack([9.81 * np.sin(th), -9.81 * np.cos(th), np.zeros(steps)], 1)
     c_t = sc.pos4[:, :3].astype(np.float64).mean(0) + np.array([0.02, 0.01, -0.01])
     h_t = fad.soft_histogram(sc.pos4[:, :3].astype(np.float64) + np.array([0.03, 0.0, 0.0]), sc.half)
-    return dict(sc=sc, dt=dt, refine=refine, vc=vc, v0=v0, grav=grav, c_t=c_t, h_t=h_t)
+    return dict(sc=sc, dt=dt, refine=refine, vc=vc, v0=v0, grav=grav, c_t=c_t, h_t=h_t,
+                physics=fad.FULL if physics == "full" else fad.VISCOUS)
 
 
 def _loss_of(pos, C):
@@ -333,7 +336,7 @@ def _loss_of(pos, C):
 
 def _gpu_loss(C, v0=None, grav=None, mu=MU):
     grav = C["grav"] if grav is None else grav
-    adj = fad.Adjoint(C["sc"].params, len(grav), len(grav))
+    adj = fad.Adjoint(C["sc"].params, len(grav), len(grav), physics=C["physics"])
     adj.set_mu(mu)
     pos, _ = adj.forward(C["sc"].pos4, C["v0"] if v0 is None else v0, fad.steps_for(grav, C["dt"], C["vc"]),
                          [C["refine"]] * len(grav))
@@ -346,7 +349,7 @@ def _gpu_gradient(C, every):
     wrapper) with respect to v0, mu and the gravity vectors."""
     import torch
     sc, steps = C["sc"], len(C["grav"])
-    adj = fad.Adjoint(sc.params, steps, every)
+    adj = fad.Adjoint(sc.params, steps, every, physics=C["physics"])
     Sim = fad.simulate_function()
     v = torch.tensor(C["v0"][:, :3], dtype=torch.float32, requires_grad=True)
     m = torch.tensor(MU, dtype=torch.float64, requires_grad=True)
@@ -373,7 +376,8 @@ def _oracle_run(C, precision, v0=None, grav=None, mu=MU, dv=None, dmu=0.0, dg=No
     sc = C["sc"]
     grav = C["grav"] if grav is None else grav
     o = OracleSim(sc.params, precision=precision)
-    o.set_physics(mu, 0.0, 0.0, 0.0, 0.0)
+    ph = C["physics"]
+    o.set_physics(mu, ph["k_near"], ph["tension_scale"], ph["adhesion_scale"], ph["xsph_rate"])
     o.set_state(sc.pos4, C["v0"] if v0 is None else v0)
     dual = precision in (96, 128)
     if dv is not None:
@@ -405,7 +409,8 @@ def _close(got, want, rel, what):
 
 
 @pytest.mark.timeout(900)
-def test_adjoint_64k_is_the_exact_derivative_of_the_fp32_step():
+@pytest.mark.parametrize("physics", ["full", "viscous"])
+def test_adjoint_64k_is_the_exact_derivative_of_the_fp32_step(physics):
     """The bench's adjoint workload: 64 substeps of the reference schedule
     at this spacing (4.17 ms each, R = 5), a checkpoint every 8 (eight taped
     segment replays). The gradient's directional derivatives along seeded
@@ -419,10 +424,10 @@ def test_adjoint_64k_is_the_exact_derivative_of_the_fp32_step():
     and are printed, not held to the adjoint (the next test holds them at a
     substep where the linearisation is stable)."""
     steps, every = 64, 8
-    C = _adjoint_config(steps=steps)
+    C = _adjoint_config(steps=steps, physics=physics)
     sc = C["sc"]
-    _say(f"adjoint_64k: {sc.n} particles, {steps} substeps of {C['dt'] * 1e3:.2f} ms at R = {C['refine']}, "
-         f"checkpoint every {every}")
+    _say(f"adjoint_64k ({physics}): {sc.n} particles, {steps} substeps of {C['dt'] * 1e3:.2f} ms at R = "
+         f"{C['refine']}, checkpoint every {every}")
     G = _gpu_gradient(C, every)
     rng = np.random.default_rng(7)
     u = rng.normal(0.0, 1.0, (sc.n, 3))
@@ -434,7 +439,8 @@ def test_adjoint_64k_is_the_exact_derivative_of_the_fp32_step():
 
 
 @pytest.mark.timeout(900)
-def test_adjoint_64k_matches_fp64_and_central_differences_at_a_stable_substep():
+@pytest.mark.parametrize("physics", ["full", "viscous"])
+def test_adjoint_64k_matches_fp64_and_central_differences_at_a_stable_substep(physics):
     """The same scene and loss over 64 substeps of 1.04 ms (1/120 s in 8, R =
     2, the reference's short-substep regime), a checkpoint every 8, where the
     step's linearisation stays stable: the adjoint equals the FP32 tangent to
@@ -448,10 +454,10 @@ def test_adjoint_64k_matches_fp64_and_central_differences_at_a_stable_substep():
     clamp and of the histogram, so there the adjoint is held to the FP32
     tangent alone."""
     steps, every = 64, 8
-    C = _adjoint_config(frame_substeps=8, steps=steps)
+    C = _adjoint_config(frame_substeps=8, steps=steps, physics=physics)
     sc = C["sc"]
-    _say(f"adjoint_64k: {sc.n} particles, {steps} substeps of {C['dt'] * 1e3:.2f} ms at R = {C['refine']}, "
-         f"checkpoint every {every}")
+    _say(f"adjoint_64k ({physics}): {sc.n} particles, {steps} substeps of {C['dt'] * 1e3:.2f} ms at R = "
+         f"{C['refine']}, checkpoint every {every}")
     G = _gpu_gradient(C, every)
     eps = 1e-3
     p = sc.pos4[:, :3].astype(np.float64)

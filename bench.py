@@ -33,14 +33,14 @@ import numpy as np  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 # Canonical algorithmic bytes per particle of each kernel class (per launch).
 BYTES_PER_PARTICLE = {
-    "grid_count": 24, "grid_scatter": 124, "density_div": 64, "div_apply": 68, "forces_eval": 84,
-    "forces_den_apply": 108, "den_kappa": 68, "den_apply": 68, "integrate": 120,
+    "grid_count": 24, "grid_scatter": 124, "density_div": 64, "div_apply": 68, "forces": 124,
+    "den_kappa": 68, "den_apply": 68, "integrate": 120,
 }
 BYTES_PER_CELL = {"grid_scan": 16}
 
 
 def canonical_bytes(n, cells, refine):
-    return n * (592 + 136 * refine) + 16 * cells
+    return n * (524 + 136 * refine) + 16 * cells
 
 
 def kernel_bytes(name, n, cells):
@@ -71,7 +71,7 @@ def _kernel_classes(names):
         k = m.group(1) if m else name
         cls = {"k_grid_count": "grid_count", "k_grid_scan": "grid_scan", "k_grid_place": "grid_scatter",
                "k_grid_gather": "grid_scatter", "k_tile_regions": "density_div", "k_density_div": "density_div",
-               "k_forces_eval": "forces_eval", "k_forces_den_apply": "forces_den_apply",
+               "k_forces": "forces",
                "k_den_kappa": "den_kappa", "k_integrate": "integrate"}.get(k)
         if k == "k_kappa_apply":
             cls = "div_apply" if after == "k_density_div" else "den_apply"
@@ -120,7 +120,7 @@ def pmc_summary(base, workload="block_1m", steps=0, warmup=0):
     refine = sc.schedule()[2]
     window = (f" --steps {steps} --warmup {warmup}" if steps else "")
     cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + window + \
-        " --no-cpu --no-batched --no-adjoint --no-cfl"
+        " --no-cpu --no-batched --no-adjoint --no-cfl --no-tolerance"
     print(f"# HBM traffic per launch, {workload} ({n:,} particles, {cells:,} cells, R = {refine})\n")
     print("Commands, from the repository root on the MI355X box (each counter in a run of its own):\n")
     print("```")
@@ -141,8 +141,8 @@ def pmc_summary(base, workload="block_1m", steps=0, warmup=0):
           f"{fw:.3f} x WRITE_SIZE (KiB x 1024 bytes).\n")
     print("| kernel | launches | FETCH_SIZE KiB | WRITE_SIZE KiB | traffic MB | algorithmic MB |")
     print("|---|---|---|---|---|---|")
-    classes = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
-               "forces_den_apply", "den_kappa", "den_apply", "integrate"]
+    classes = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces", "den_kappa",
+               "den_apply", "integrate"]
     traffic = {}
     for cls in classes:
         e = per.get(cls)
@@ -203,7 +203,7 @@ def trace_summary(base, workload="block_1m", steps=0, warmup=0):
     rs = list(csv.DictReader(open(f)))
     window = (f" --steps {steps} --warmup {warmup}" if steps else "")
     cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + window + \
-        " --no-cpu --no-batched --no-adjoint --no-cfl"
+        " --no-cpu --no-batched --no-adjoint --no-cfl --no-tolerance"
     if workload.startswith("ensemble"):
         cmd = "python3 -u bench.py --batched-only"
     print(f"# DFSPH substep kernels, {workload}, kernel trace\n")
@@ -242,7 +242,7 @@ def sq_summary(base, workload="block_1m"):
             disp = acc.setdefault(k, {}).setdefault((d, r["Dispatch_Id"]), {})
             disp[r["Counter_Name"]] = float(r["Counter_Value"])
     cmd = "python3 -u bench.py" + ("" if workload == "block_1m" else f" --workload {workload}") + \
-        " --no-cpu --no-batched --no-adjoint --no-cfl"
+        " --no-cpu --no-batched --no-adjoint --no-cfl --no-tolerance"
     print(f"# Shader counters per wave, {workload}\n")
     print("Commands, from the repository root on the MI355X box (two passes within the per-block counter limits):\n")
     print("```")

@@ -274,8 +274,8 @@ enum KernelId {
     KERNEL_GRID_SCATTER,
     KERNEL_DENSITY_DIV,
     KERNEL_DIV_APPLY,
-    KERNEL_FORCES_EVAL,
-    KERNEL_FORCES_DEN_APPLY,
+    KERNEL_FORCES,        // forces_eval and forces_den_apply, one sweep
+    KERNEL_FORCES_UNUSED,  // no launches (the place forces_den_apply had)
     KERNEL_DEN_KAPPA,
     KERNEL_DEN_APPLY,
     KERNEL_INTEGRATE,
@@ -418,8 +418,8 @@ struct SolveVariant {
        limit); returns the dynamic LDS budget used. */                                                           \
     size_t configure_solve(SolveBufs& b, size_t lds_bytes);                                                      \
     /* One substep's solve on the working set (after the grid build), ending with integrate writing the       \
-       resting set: density_div, div_apply, forces_eval, forces_den_apply, passes x (den_kappa, den_apply),   \
-       integrate. A scene runs min(passes, its refine count) density passes. */                               \
+       resting set: density_div, div_apply, forces (forces_eval + forces_den_apply), passes x (den_kappa,   \
+       den_apply), integrate. A scene runs min(passes, its refine count) density passes. */                               \
     void launch_solve(hipStream_t st, const OneScene& sc, uint32_t slots, const uint32_t* starts,               \
                       const Records& work, const Records& rest, const SolveBufs& b, uint32_t passes,             \
                       KernelTimer* timer);                                                                       \

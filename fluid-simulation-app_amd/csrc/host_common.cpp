@@ -20,8 +20,8 @@ void set_error(const char* fmt, ...) {
 const char* last_error() { return g_error.c_str(); }
 
 const char* kKernelNames[KERNEL_COUNT_] = {
-    "grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
-    "forces_den_apply", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"};
+    "grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces",
+    "unused", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"};
 
 int check_launch(const char* what) {
     hipError_t e = hipGetLastError();

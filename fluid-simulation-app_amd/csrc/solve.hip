@@ -5,8 +5,8 @@
 //   density_div        411-518   27-cell walk; density, near-density, alpha,
 //                                divergence kappa, wall gradient, neighbour list
 //   div_apply          665-668   (kappa_partial + apply_kappa_wide, 594-621)
-//   forces_eval        524-592
-//   forces_den_apply   630-663
+//   forces_eval        524-592   } one sweep (k_forces): both read the same list,
+//   forces_den_apply   630-663   } and the warm start's kd needs no neighbour
 //   den_kappa          674-698
 //   den_apply          700-703
 //   integrate          709-756
@@ -760,8 +760,8 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
 
 // ---- list sweeps -------------------------------------------------------------
 
-// Stages position and kd (the gathers of div_apply, den_apply and
-// forces_den_apply) as one float4 per slot.
+// Stages position and kd (the gathers of div_apply and den_apply) as one
+// float4 per slot.
 template <class Early>
 __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __restrict__ pos,
                                              const float* __restrict__ kd, float4* lds, uint32_t slots,
@@ -861,31 +861,45 @@ __global__ __launch_bounds__(kSweepThreads) void k_kappa_apply(Sc sc, uint32_t p
     kappa_apply_on<Sc>(sc, tile_index(sc), pass, pos, vel_in, vel_out, b, smem);
 }
 
-// forces_eval's gathers: position, velocity, density, near-density,
-// temperature and m / rho, 40 bytes a slot as float4 (x, y, z, vx), float4
-// (vy, vz, rho, rho_near), float2 (temperature, m / rho). m / rho is the
-// neighbour's own quotient, so each staged slot divides once instead of each
-// pair.
+// The forces sweep's gathers: position, velocity, density, near-density,
+// temperature, m / rho and the warm start's kd = 0.5 pp / rho^2, 44 bytes a
+// slot as float4 (x, y, z, vx), float4 (vy, vz, rho, rho_near), float2
+// (temperature, m / rho) and float (kd). m / rho and kd are the neighbour's
+// own quotients, so each staged slot divides once instead of each pair.
 struct ForcesLds {
     float4* a;
     float4* c;
     float2* t;
+    float* w;
 };
 
 struct ForceSums {
-    f3 visc, blend, nearf, tension;
+    f3 visc, blend, nearf, tension, warm;
     float heat;
 };
 
+// The particle's own fields the forces sweep ends with.
+struct ForcesOwn {
+    f3 pi;
+    float4 v;     // vel_in, w kept
+    float rho, prev_p, temp;
+    float gv;     // component `lane` of wall_grad
+};
+
+// forces_eval's pair terms (sim_solve.wgsl:544-583) and, over the same list
+// entries in the same order, the warm start's sum (kd_i + kd_j) grad W_ij of
+// forces_den_apply (kappa_partial, sim_solve.wgsl:596-606, 642): the two
+// kernels read the same list, and the warm start's kd is each particle's own
+// 0.5 pp / rho^2, known before the sweep.
 template <bool STAGED>
 __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, const RegionS& R,
                                             const float4* __restrict__ pos, const float4* __restrict__ vel,
                                             const float* __restrict__ temperature,
                                             const float* __restrict__ prev_pressure, const SolveBufs& b,
-                                            const ForcesLds& L, ForceSums& F, f3& pi, float& rho_i, float& prev_p) {
+                                            const ForcesLds& L, ForceSums& F, ForcesOwn& own) {
     ListHead h;
     float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v4 = p4;
-    float w_i = 0.0f, temp_i = 0.0f;
+    float w_i = 0.0f, temp_i = 0.0f, rho_i = 0.0f, prev_p = 0.0f, gv = 0.0f;
     auto early = [&] {
         const uint32_t o = own_slot(id);
         h = list_head<STAGED>(b, id);
@@ -895,11 +909,12 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         temp_i = temperature[o];
         rho_i = b.density[o];
         prev_p = prev_pressure[o];
+        gv = wall_part(b, R, o, id.lane);
     };
     if (STAGED) {
         constexpr uint32_t IT = 3;
         float4 pp[IT], vv[IT];
-        float dd[IT], nr[IT], tt[IT];
+        float dd[IT], nr[IT], tt[IT], pq[IT];
         stage_flat<kSweepThreads, IT>(
             R, stage_slots(b),
             [&](uint32_t slot, uint32_t it) {
@@ -908,11 +923,13 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
                 dd[it] = b.density[slot];
                 nr[it] = b.rho_near[slot];
                 tt[it] = temperature[slot];
+                pq[it] = prev_pressure[slot];
             },
             [&](uint32_t idx, uint32_t it) {
                 L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
                 L.c[idx] = make_float4(vv[it].y, vv[it].z, dd[it], nr[it]);
                 L.t[idx] = make_float2(tt[it], div_rn(k.mass, dd[it]));
+                L.w[idx] = (0.5f * pq[it]) / (dd[it] * dd[it]);
             },
             early);
         __syncthreads();
@@ -920,14 +937,20 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         early();
     }
     list_tail_at(b, id, h);
+    own.pi = xyz(p4);
+    own.v = v4;
+    own.rho = rho_i;
+    own.prev_p = prev_p;
+    own.temp = temp_i;
+    own.gv = gv;
     if (!id.live) return;
-    pi = xyz(p4);
     const f3 vi = xyz(v4);
     const float rhoi = rho_i;
-    const f3 pii = pi;
+    const f3 pii = own.pi;
+    const float kd_i = (0.5f * prev_p) / (rho_i * rho_i);
     for_list<STAGED, true>(b, id, h, [&](uint32_t j, float gs) {
         f3 pj, vj;
-        float rho_j, near_j, temp_j, mr_j;
+        float rho_j, near_j, temp_j, mr_j, kd_j;
         if (STAGED) {
             const float4 a = L.a[j], c = L.c[j];
             const float2 t = L.t[j];
@@ -937,6 +960,7 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
             near_j = c.w;
             temp_j = t.x;
             mr_j = t.y;
+            kd_j = L.w[j];
         } else {
             pj = xyz(pos[j]);
             vj = xyz(vel[j]);
@@ -944,11 +968,13 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
             near_j = b.rho_near[j];
             temp_j = temperature[j];
             mr_j = div_rn(k.mass, rho_j);
+            kd_j = (0.5f * prev_pressure[j]) / (rho_j * rho_j);
         }
         const f3 x = sub(pii, pj);
         const float r2 = dot3(x, x);
         const float rr = sqrt_rn(r2);
         const f3 g = scl(gs, x);
+        F.warm = add(F.warm, scl(kd_i + kd_j, g));
         const float f = div_rn(dot3(x, g), r2 + k.eps_h2);
         const float pair = div_rn(k.mass, rhoi * rho_j) * f;
         const f3 dv = sub(vi, vj);
@@ -969,40 +995,6 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
     });
 }
 
-template <class Sc>
-__global__ __launch_bounds__(kSweepThreads) void k_forces_eval(Sc sc, const float4* __restrict__ pos,
-                                                               const float4* __restrict__ vel,
-                                                               const float* __restrict__ temperature,
-                                                               const float* __restrict__ prev_pressure, SolveBufs b) {
-    extern __shared__ float4 smem[];
-    const uint32_t tile = tile_index(sc);
-    const RegionS R = load_region(b.regions, tile);
-    if (!sc.runs_at(R.scene, R.active, R.refine, kEveryPass)) return;
-    const SceneK& k = sc.scene_at(R.scene);
-    const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
-    const ForcesLds L{smem, smem + stage_slots(b), reinterpret_cast<float2*>(smem + 2 * stage_slots(b))};
-    const f3 z = mk(0.0f, 0.0f, 0.0f);
-    ForceSums F{z, z, z, z, 0.0f};
-    f3 pi = z;
-    float rho_i = 0.0f, prev_p = 0.0f;
-    if (R.staged)
-        forces_tile<true>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, pi, rho_i, prev_p);
-    else
-        forces_tile<false>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, pi, rho_i, prev_p);
-    const f3 visc = fold4(F.visc), blend = fold4(F.blend), nearf = fold4(F.nearf), tension = fold4(F.tension);
-    const float heat = fold4(F.heat);
-    if (id.live && id.lane == 0) {
-        const uint32_t i = id.p;
-        const f3 adh = scl(k.beta * k.rho0, near_walls(k, pi) ? wall_adh_sum(k, pi) : mk(0.0f, 0.0f, 0.0f));
-        const f3 acc = add(add(add(visc, scl(k.k_near, nearf)), scl(k.gamma, tension)), adh);
-        b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
-        b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
-        const float half = 0.5f * prev_p;
-        b.kd[i] = half / (rho_i * rho_i);
-        b.pressure[i] = half;
-    }
-}
-
 // sim_solve.wgsl:103-127
 __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
     float pullx = 0.0f, pully = 0.0f, weight = 0.0f;
@@ -1021,10 +1013,19 @@ __device__ __forceinline__ f3 fingers(const StepK& s, f3 pos, f3 v) {
     return mk(wmix(v.x, pullx / weight, blend), wmix(v.y, pully / weight, blend), v.z);
 }
 
+// forces_eval + forces_den_apply (sim_solve.wgsl:524-592, 630-663) in one
+// sweep: the accelerations, the XSPH blend and the warm start's kd and
+// pressure, then at once the particle's forced velocity less the warm-start
+// correction. It reads the neighbours' velocities and temperatures, so the
+// new velocity and temperature go to arrays of their own (vel_out,
+// temp_out), not in place.
 template <class Sc>
-__global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const float4* __restrict__ pos,
-                                                                    const float4* vel_in, float4* vel_out,
-                                                                    float* __restrict__ temperature, SolveBufs b) {
+__global__ __launch_bounds__(kSweepThreads) void k_forces(Sc sc, const float4* __restrict__ pos,
+                                                          const float4* __restrict__ vel,
+                                                          const float* __restrict__ temperature,
+                                                          const float* __restrict__ prev_pressure,
+                                                          float4* __restrict__ vel_out,
+                                                          float* __restrict__ temp_out, SolveBufs b) {
     extern __shared__ float4 smem[];
     const uint32_t tile = tile_index(sc);
     const RegionS R = load_region(b.regions, tile);
@@ -1032,35 +1033,44 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces_den_apply(Sc sc, const
     const SceneK& k = sc.scene_at(R.scene);
     const StepK& s = sc.step_at(R.scene);
     const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
-    f3 pi = mk(0.0f, 0.0f, 0.0f);
-    float kdi = 0.0f;
-    // Component `lane` of accel, vel_in, xsph and wall_grad, and temperature.
-    float av = 0.0f, wv = 0.0f, xv = 0.0f, gv = 0.0f, t0 = 0.0f;
-    auto own = [&](uint32_t o) {
-        av = lane_part(b.accel, o, id.lane);
-        wv = lane_part(vel_in, o, id.lane);
-        xv = lane_part(b.xsph, o, id.lane);
-        gv = wall_part(b, R, o, id.lane);
-        t0 = temperature[o];
-    };
-    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
-                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi, own);
-    const float4 a = quad_float4(av), was = quad_float4(wv), xs = quad_float4(xv), wg = quad_float4(gv);
+    const uint32_t slots = stage_slots(b);
+    float2* const t2 = reinterpret_cast<float2*>(smem + 2 * slots);
+    const ForcesLds L{smem, smem + slots, t2, reinterpret_cast<float*>(t2 + slots)};
+    const f3 z = mk(0.0f, 0.0f, 0.0f);
+    ForceSums F{z, z, z, z, z, 0.0f};
+    ForcesOwn own{};
+    if (R.staged)
+        forces_tile<true>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, own);
+    else
+        forces_tile<false>(k, id, R, pos, vel, temperature, prev_pressure, b, L, F, own);
+    const f3 visc = fold4(F.visc), blend = fold4(F.blend), nearf = fold4(F.nearf), tension = fold4(F.tension);
+    const f3 warm = fold4(F.warm);
+    const float heat = fold4(F.heat);
+    const float4 wg = quad_float4(own.gv);
     if (id.live && id.lane == 0) {
         const uint32_t i = id.p;
-        const f3 v = xyz(was);
+        const f3 pi = own.pi;
+        const f3 adh = scl(k.beta * k.rho0, near_walls(k, pi) ? wall_adh_sum(k, pi) : mk(0.0f, 0.0f, 0.0f));
+        const f3 acc = add(add(add(visc, scl(k.k_near, nearf)), scl(k.gamma, tension)), adh);
+        b.accel[i] = make_float4(acc.x, acc.y, acc.z, heat);
+        b.xsph[i] = make_float4(blend.x, blend.y, blend.z, 0.0f);
+        const float half = 0.5f * own.prev_p;
+        const float kdi = half / (own.rho * own.rho);
+        b.kd[i] = kdi;
+        b.pressure[i] = half;
+        const f3 v = xyz(own.v);
         const f3 om = mk(s.omega[0], s.omega[1], s.omega[2]);
         const f3 dom = mk(s.domega[0], s.domega[1], s.domega[2]);
         const f3 c1 = cross3(dom, pi);
         const f3 c2 = cross3(om, cross3(om, pi));
         const f3 c3 = scl(2.0f, cross3(om, v));
         const f3 fict = sub(sub(mk(-c1.x, -c1.y, -c1.z), c2), c3);
-        const f3 fsum = add(add(mk(s.force[0], s.force[1], s.force[2]), fict), xyz(a));
-        const f3 forced = fingers(s, pi, add(add(v, scl(s.dt, fsum)), scl(s.c_xsph, xyz(xs))));
-        temperature[i] = t0 + s.dt * a.w;
-        const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
+        const f3 fsum = add(add(mk(s.force[0], s.force[1], s.force[2]), fict), acc);
+        const f3 forced = fingers(s, pi, add(add(v, scl(s.dt, fsum)), scl(s.c_xsph, blend)));
+        temp_out[i] = own.temp + s.dt * heat;
+        const f3 full = add(scl(k.mass, warm), scl(kdi, xyz(wg)));
         vel_out[i] = make_float4(forced.x - s.dt * full.x, forced.y - s.dt * full.y, forced.z - s.dt * full.z,
-                                 was.w);
+                                 own.v.w);
     }
 }
 
@@ -1381,7 +1391,7 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8
 }
 
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
-constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float2);   // 40
+constexpr size_t kForcesEntry = 2 * sizeof(float4) + sizeof(float2) + sizeof(float);  // 44
 // A tile within one cell row of its scene stages nine rows of about kTile +
 // 16 slots; 13 tiles' worth leaves room for ragged rows and keeps two 256-slot
 // (four 128-slot) density and den_kappa workgroups' LDS on a CU.
@@ -1397,7 +1407,7 @@ uint32_t solve_tiles(uint32_t n) { return (n + kTile - 1) / kTile; }
 // Sizes the staging capacity from the device's per-workgroup LDS (less 1 KB
 // for static LDS) and raises the sweeps' dynamic-LDS limit to match. The
 // capacity is one for every sweep, since a staged tile's list holds LDS
-// indices: at most kStageMax slots; forces_eval (40 bytes a slot) sizes the
+// indices: at most kStageMax slots; the forces sweep (44 bytes a slot) sizes the
 // budget. If the device refuses the larger limit, the capacity fits 64 KB.
 size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     size_t avail = lds_bytes > 1024 ? lds_bytes - 1024 : 0;
@@ -1405,14 +1415,12 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
                          reinterpret_cast<const void*>(&k_density_div<OneScene, true>),
                          reinterpret_cast<const void*>(&k_density_div<SceneTable, true>),
                          reinterpret_cast<const void*>(&k_kappa_apply<OneScene>),
-                         reinterpret_cast<const void*>(&k_forces_eval<OneScene>),
-                         reinterpret_cast<const void*>(&k_forces_den_apply<OneScene>),
+                         reinterpret_cast<const void*>(&k_forces<OneScene>),
                          reinterpret_cast<const void*>(&k_den_kappa<OneScene>),
                          reinterpret_cast<const void*>(&k_den_kappa<OneScene, true>),
                          reinterpret_cast<const void*>(&k_density_div<SceneTable>),
                          reinterpret_cast<const void*>(&k_kappa_apply<SceneTable>),
-                         reinterpret_cast<const void*>(&k_forces_eval<SceneTable>),
-                         reinterpret_cast<const void*>(&k_forces_den_apply<SceneTable>),
+                         reinterpret_cast<const void*>(&k_forces<SceneTable>),
                          reinterpret_cast<const void*>(&k_den_kappa<SceneTable>),
                          reinterpret_cast<const void*>(&k_den_kappa<SceneTable, true>),
                          reinterpret_cast<const void*>(&k_den_kappa_list<SceneTable, false>),
@@ -1467,7 +1475,11 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     const uint32_t ntiles = solve_tiles(n);
     const dim3 sweep(kSweepThreads);
     const size_t lds_kappa = stage_slots(b) * kKappaEntry;
-    auto V = [&](uint32_t i) { return T ? T->vel[i] : work.vel; };
+    // Velocities from the forces sweep on go to the resting record's array
+    // (dead between the grid build's gather and integrate, which finishes
+    // each slot there): the forces sweep reads its input's neighbours, so
+    // its output cannot be in place.
+    auto V = [&](uint32_t i) { return T ? T->vel[i] : (i >= 2 ? rest.vel : work.vel); };
     auto KB = [&](uint32_t i) {
         SolveBufs c = b;
         if (T) c.kd = T->kd[i];
@@ -1516,19 +1528,17 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
             hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, rev, kDivPass | r,
                                   work.pos, V(1), V(1), KB(0));
         }
-        const Span tf = span(timer, KERNEL_FORCES_EVAL);
-        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, s, tf.a, tf.z, 0, fwd,
-                              work.pos, V(1), work.temperature, work.prev_pressure, KB(1));
-        const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
-        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, s, ta.a, ta.z, 0, rev, work.pos, V(1),
-                              V(2), work.temperature, KB(1));
+        const Span tf = span(timer, KERNEL_FORCES);
+        hipExtLaunchKernelGGL(k_forces<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, s, tf.a, tf.z, 0, fwd,
+                              work.pos, V(1), work.temperature, work.prev_pressure, V(2), rest.temperature, KB(1));
+        // the forces sweep ran forward, so the passes start in reverse
         for (uint32_t r = 0; r < passes; ++r) {
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
-            hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0, fwd, r,
+            hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0, rev, r,
                                   work.pos, V(2 + r), KB(2 + r));
             check(s, r, false);
             const Span tp = span(timer, KERNEL_DEN_APPLY);
-            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, rev, r, work.pos,
+            hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, s, tp.a, tp.z, 0, fwd, r, work.pos,
                                   V(2 + r), V(3 + r), KB(2 + r));
         }
     }
@@ -1538,6 +1548,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     }
     Records wl = work;
     wl.vel = V(passes + 2);
+    wl.temperature = rest.temperature;
     const Span ti = span(timer, KERNEL_INTEGRATE);
     hipExtLaunchKernelGGL(k_integrate<Sc>, dim3(ntiles), dim3(kTile), 0, st, ti.a, ti.z, 0, sc, wl, rest, b);
 }
@@ -1667,12 +1678,11 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
             hipExtLaunchKernelGGL(k_kappa_apply_list<Sc>, dim3(lg), sweep, lds_kappa, st, nullptr, tk.z, 0, sc,
                                   kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
         }
-        const Span tf = span(timer, KERNEL_FORCES_EVAL);
-        hipExtLaunchKernelGGL(k_forces_eval<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z,
-                              0, sc, work.pos, work.vel, work.temperature, work.prev_pressure, t);
-        const Span ta = span(timer, KERNEL_FORCES_DEN_APPLY);
-        hipExtLaunchKernelGGL(k_forces_den_apply<Sc>, tiles, sweep, lds_kappa, st, ta.a, ta.z, 0, rev,
-                              work.pos, work.vel, work.vel, work.temperature, t);
+        // the forces sweep writes the velocities (and temperatures) the
+        // density passes and integrate take, at the resting records' arrays
+        const Span tf = span(timer, KERNEL_FORCES);
+        hipExtLaunchKernelGGL(k_forces<Sc>, tiles, sweep, stage_slots(b) * kForcesEntry, st, tf.a, tf.z, 0, sc,
+                              work.pos, work.vel, work.temperature, work.prev_pressure, rest.vel, rest.temperature, t);
         if (den) converge(0, false, false, 0);
         const uint32_t count = den ? tol.max_passes : passes;
         for (uint32_t r = 0; r < count; ++r) {
@@ -1680,19 +1690,22 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
             const Span tp = span(timer, KERNEL_DEN_APPLY);
             if (den) {
                 hipExtLaunchKernelGGL(k_den_kappa_list<Sc, false>, dim3(lg), sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
-                                      work.pos, work.vel, t, lists[r % 2u], scenes);
+                                      work.pos, rest.vel, t, lists[r % 2u], scenes);
                 converge(r, true, false, r + 1);
                 hipExtLaunchKernelGGL(k_kappa_apply_list<Sc>, dim3(lg), sweep, lds_kappa, st, tp.a, tp.z, 0, sc, r,
-                                      work.pos, work.vel, t, lists[r % 2u], scenes);
+                                      work.pos, rest.vel, t, lists[r % 2u], scenes);
             } else {
-                hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, lds_pv, st, tk.a, tk.z, 0, sc, r,
-                                      work.pos, work.vel, t);
-                hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, rev, r,
-                                      work.pos, work.vel, work.vel, t);
+                hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, lds_pv, st, tk.a, tk.z, 0, rev, r,
+                                      work.pos, rest.vel, t);
+                hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, tp.a, tp.z, 0, sc, r,
+                                      work.pos, rest.vel, rest.vel, t);
             }
         }
+        Records wl = work;
+        wl.vel = rest.vel;
+        wl.temperature = rest.temperature;
         const Span ti = span(timer, KERNEL_INTEGRATE);
-        hipExtLaunchKernelGGL(k_integrate<Sc>, tiles, dim3(kTile), 0, st, ti.a, ti.z, 0, sc, work, rest, t);
+        hipExtLaunchKernelGGL(k_integrate<Sc>, tiles, dim3(kTile), 0, st, ti.a, ti.z, 0, sc, wl, rest, t);
     }
     hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
 }

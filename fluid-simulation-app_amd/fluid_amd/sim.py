@@ -28,8 +28,10 @@ FIELDS = {
 # Bins of the tolerance-mode pass histograms: pass counts 0 .. 256.
 HIST_BINS = 257
 
-KERNELS = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces_eval",
-           "forces_den_apply", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"]
+# Kernel classes of the timer, by FLUID_KERNEL_* index ("forces" is
+# forces_eval and forces_den_apply in one sweep; "unused" has no launches).
+KERNELS = ["grid_count", "grid_scan", "grid_scatter", "density_div", "div_apply", "forces",
+           "unused", "den_kappa", "den_apply", "integrate", "reduce_stats", "tracers"]
 NBR_CAP = 96
 STATS = 14
 

@@ -210,8 +210,10 @@ int fluid_sim_synchronize(FluidSim *sim);
 #define FLUID_KERNEL_GRID_SCATTER 2u
 #define FLUID_KERNEL_DENSITY_DIV 3u
 #define FLUID_KERNEL_DIV_APPLY 4u
-#define FLUID_KERNEL_FORCES_EVAL 5u
-#define FLUID_KERNEL_FORCES_DEN_APPLY 6u
+/* forces_eval and forces_den_apply (sim_solve.wgsl:524-663) run as one
+ * sweep, timed as FLUID_KERNEL_FORCES; class 6 has no launches. */
+#define FLUID_KERNEL_FORCES 5u
+#define FLUID_KERNEL_UNUSED 6u
 #define FLUID_KERNEL_DEN_KAPPA 7u
 #define FLUID_KERNEL_DEN_APPLY 8u
 #define FLUID_KERNEL_INTEGRATE 9u

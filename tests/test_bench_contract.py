@@ -1,5 +1,5 @@
 """bench.py's accounting, without a GPU: the per-kernel canonical bytes add
-up to the substep's B(N, C, R) = N (592 + 136 R) + 16 C; the committed
+up to the substep's B(N, C, R) = N (524 + 136 R) + 16 C; the committed
 counter summary names every kernel class; more than one GPU is refused."""
 import os
 import subprocess
@@ -11,8 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
-SUBSTEP_ONCE = ("grid_count", "grid_scatter", "density_div", "div_apply", "forces_eval", "forces_den_apply",
-                "integrate")
+SUBSTEP_ONCE = ("grid_count", "grid_scatter", "density_div", "div_apply", "forces", "integrate")
 PER_PASS = ("den_kappa", "den_apply")
 
 

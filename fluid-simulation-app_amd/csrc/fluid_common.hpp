@@ -397,6 +397,18 @@ __device__ __forceinline__ float4 quad_float4(float v) {
 __device__ __forceinline__ float lane_part(const float4* a, uint32_t i, uint32_t lane) {
     return reinterpret_cast<const float*>(a)[(size_t)i * 4 + lane];
 }
+// base[i] through a 32-bit byte offset (i * sizeof(T) < 2^32, which the
+// allocations guarantee): the load form that keeps a uniform base in scalar
+// registers and takes the offset from one vector register, with no 64-bit
+// address arithmetic a lane.
+template <class T>
+__device__ __forceinline__ T at32(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T* at32p(T* base, uint32_t i) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+}
 
 // Inclusive prefix sum over the 64 lanes of a wave: four row shifts build
 // the 16-lane row scans, then two row broadcasts (lane 15 into rows 1 and 3,

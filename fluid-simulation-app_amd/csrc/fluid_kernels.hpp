@@ -250,7 +250,10 @@ constexpr uint32_t kListLanes = 4;
 // entries past the fixed ones lie consecutively from the summed tail
 // lengths of the wave's slots before p. Only the entries that exist take
 // memory and line fetches.
-constexpr uint32_t kListFixed = 6;
+#ifndef FLUID_LIST_FIXED
+#define FLUID_LIST_FIXED 6
+#endif
+constexpr uint32_t kListFixed = FLUID_LIST_FIXED;
 constexpr uint32_t kListWave = 64 / kListLanes;  // slots a wave
 __host__ __device__ __forceinline__ uint32_t list_tail_cap(uint32_t cap) {
     return cap > kListFixed * kListLanes ? cap - kListFixed * kListLanes : 0u;
@@ -262,6 +265,10 @@ __host__ __device__ __forceinline__ uint32_t list_tail_len(uint32_t nn) {
 __host__ __device__ __forceinline__ uint32_t list_fixed(uint32_t stride, uint32_t p, uint32_t m) {
     return ((m / kListLanes) * stride + p) * kListLanes + m % kListLanes;
 }
+// Entries each list array holds past cap x stride: a sweep's tail loads run
+// up to three quads past a lane's last entry, which for the last wave's
+// block is past the list.
+constexpr uint32_t kListPad = 64;
 // The first tail entry of slot p, whose tail starts `off` entries into its
 // wave's block.
 __host__ __device__ __forceinline__ uint32_t list_tail(uint32_t stride, uint32_t cap, uint32_t p, uint32_t off) {

@@ -62,12 +62,18 @@ struct DeviceAllocs {
     int alloc_list(SolveBufs& b) {
         const size_t e = (size_t)b.cap * b.stride;
         // The kernels index the list with 32-bit offsets.
-        if (e >= ((size_t)1 << 32)) {
+        if (e + kListPad >= ((size_t)1 << 32)) {
             set_error("neighbour list of %zu entries (cap %u x %u slots) exceeds 2^32", e, b.cap, b.stride);
             return FLUID_ERR_ARG;
         }
+        // ... and the per-slot float4 arrays with 32-bit byte offsets.
+        if ((size_t)b.stride * sizeof(float4) >= ((size_t)1 << 32)) {
+            set_error("%u slots exceed the 2^28 a sweep addresses", b.stride);
+            return FLUID_ERR_ARG;
+        }
         int rc;
-        if ((rc = alloc(b.nbr16, e, false)) || (rc = alloc(b.nbr32, e, false)) || (rc = alloc(b.nbrg, e, false)))
+        const size_t ep = e + kListPad;
+        if ((rc = alloc(b.nbr16, ep, false)) || (rc = alloc(b.nbr32, ep, false)) || (rc = alloc(b.nbrg, ep, false)))
             return rc;
         // One spill area serves every list that copies these buffers.
         const size_t tail = (size_t)list_tail_cap(b.cap) * b.stride;

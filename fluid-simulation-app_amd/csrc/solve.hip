@@ -156,23 +156,6 @@ __device__ __forceinline__ RegionS load_region(const TileRegion* __restrict__ re
     return s;
 }
 
-// The delta of the region row holding flat LDS index idx: that of the last
-// row whose base <= idx (empty rows share the next row's base, so the last one
-// is the non-empty one). Accumulated as d0 plus the step at each row boundary
-// passed, a chain of adds that stays in registers.
-__device__ __forceinline__ int32_t region_delta(const RegionS& R, uint32_t idx) {
-    int32_t d = R.d0;
-    d += (int32_t)(idx >= R.b1) * (R.d1 - R.d0);
-    d += (int32_t)(idx >= R.b2) * (R.d2 - R.d1);
-    d += (int32_t)(idx >= R.b3) * (R.d3 - R.d2);
-    d += (int32_t)(idx >= R.b4) * (R.d4 - R.d3);
-    d += (int32_t)(idx >= R.b5) * (R.d5 - R.d4);
-    d += (int32_t)(idx >= R.b6) * (R.d6 - R.d5);
-    d += (int32_t)(idx >= R.b7) * (R.d7 - R.d6);
-    d += (int32_t)(idx >= R.b8) * (R.d8 - R.d7);
-    return d;
-}
-
 // Slot o's wall gradient (component `lane`, or whole): wall_grad for a tile
 // with a particle near a wall, else the zero rows. A pointer select, not a
 // branch, so the load is issued like the others.
@@ -185,11 +168,45 @@ __device__ __forceinline__ float4 wall_grad_of(const SolveBufs& b, const RegionS
     return src[R.walls ? o : (o & (kZeroRows - 1u))];
 }
 
-// Copies `total` staged slots into LDS: NT threads, ITERS slots each, all
-// loads issued before the stores. `load(slot, it)` fetches into registers,
-// `store(idx, it)` writes them to LDS. `early()` runs once, after the first
-// chunk's loads are issued: the sweep's own loads placed there are in flight
-// while the stores and the barrier complete.
+// The nine rows of a tile's region as lanes 0..8 of a wave, taken from the
+// region's scalar registers: lane r holds row r's first LDS index and its
+// delta (a slot j of row r sits at LDS index j + delta). Lanes past 8 are
+// never read.
+struct RowLanes {
+    uint32_t base;
+    int32_t delta;
+};
+__device__ __forceinline__ RowLanes row_lanes(const RegionS& R) {
+    const uint32_t l = __lane_id();
+    // Independent selects (each a compare and a conditional move), no
+    // branches.
+    const uint32_t bs[9] = {0u, R.b1, R.b2, R.b3, R.b4, R.b5, R.b6, R.b7, R.b8};
+    const int32_t ds[9] = {R.d0, R.d1, R.d2, R.d3, R.d4, R.d5, R.d6, R.d7, R.d8};
+    RowLanes T{0u, R.d0};
+#pragma unroll
+    for (uint32_t r = 1; r < kRows; ++r) {
+        const bool here = l == r;
+        T.base = here ? bs[r] : T.base;
+        T.delta = here ? ds[r] : T.delta;
+    }
+    return T;
+}
+// The row holding flat LDS index i (uniform): the last row whose base <= i
+// (empty rows share the next row's base, so that is the non-empty one).
+__device__ __forceinline__ uint32_t row_of(const RowLanes& T, uint32_t i) {
+    return (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(T.base <= i) & 0x1FEull);
+}
+
+// Copies the region's `total` slots into LDS: NT threads, ITERS slots each,
+// all loads issued before the stores. `load(slot, it)` fetches into
+// registers, `store(idx, it)` writes them to LDS. `early()` runs once, after
+// the first chunk's loads are issued: the sweep's own loads placed there are
+// in flight while the stores and the barrier complete.
+//
+// Each wave copies 64 consecutive LDS indices at a time. Its first and last
+// index name the rows the 64 span (uniform), usually one or two: the wave
+// takes the first row's delta and, per row boundary inside the span, one
+// compare and select a lane, instead of each thread searching the nine rows.
 //
 // Every load and store is issued unconditionally, so the compiler knows how
 // many loads are in flight and waits for the staging loads alone before the
@@ -198,18 +215,28 @@ __device__ __forceinline__ float4 wall_grad_of(const SolveBufs& b, const RegionS
 // addresses, so no bank serialisation); an index past the array's `slots`
 // stores into its last slot.
 template <uint32_t NT, uint32_t ITERS, class Load, class Store, class Early>
-__device__ __forceinline__ void stage_flat(const RegionS& R, uint32_t slots, Load&& load, Store&& store,
+__device__ __forceinline__ void stage_rows(const RegionS& R, uint32_t slots, Load&& load, Store&& store,
                                            Early&& early) {
     if (R.total == 0) {
         early();
         return;
     }
+    const RowLanes T = row_lanes(R);
+    const uint32_t lane = __lane_id(), wave0 = uni(threadIdx.x & ~63u), last = R.total - 1u;
     auto chunk_loads = [&](uint32_t chunk) {
 #pragma unroll
         for (uint32_t it = 0; it < ITERS; ++it) {
-            const uint32_t idx = chunk + it * NT + threadIdx.x;
-            const uint32_t ic = idx < R.total ? idx : R.total - 1u;
-            load((uint32_t)((int32_t)ic - region_delta(R, ic)), it);
+            const uint32_t i0 = chunk + it * NT + wave0;  // the wave's first index (uniform)
+            const uint32_t f = i0 < last ? i0 : last, l = i0 + 63u < last ? i0 + 63u : last;
+            const uint32_t r0 = row_of(T, f), r1 = row_of(T, l);
+            const uint32_t ic = i0 + lane < last ? i0 + lane : last;
+            int32_t d = __builtin_amdgcn_readlane(T.delta, (int)r0);
+            for (uint32_t r = r0 + 1u; r <= r1; ++r) {
+                const uint32_t br = (uint32_t)__builtin_amdgcn_readlane((int)T.base, (int)r);
+                const int32_t dr = __builtin_amdgcn_readlane(T.delta, (int)r);
+                d = ic >= br ? dr : d;
+            }
+            load((uint32_t)((int32_t)ic - d), it);
         }
     };
     auto chunk_stores = [&](uint32_t chunk) {
@@ -233,6 +260,19 @@ struct NoEarly {
 
 
 __device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
+#ifndef FLUID_RECOMPUTE_G
+#define FLUID_RECOMPUTE_G 0
+#endif
+// dW/r of a pair at x = x_i - x_j, as density_div computes and lists it.
+__device__ __forceinline__ float pair_g(const SceneK& k, f3 x) {
+    const float r = sqrt_rn(dot3(x, x));
+    const float q = r * k.inv_h;
+    return r < 1e-9f ? 0.0f : div_rn(grad_dw(k, q), r);
+}
+__device__ __forceinline__ f3 sweep_grad(const SceneK& k, f3 pi, f3 pj, float g) {
+    const f3 x = sub(pi, pj);
+    return scl(FLUID_RECOMPUTE_G ? pair_g(k, x) : g, x);
+}
 
 struct LaneId {
     uint32_t tile, p, lane, gid;
@@ -257,9 +297,25 @@ __device__ __forceinline__ LaneId lane_id(uint32_t tile, uint32_t slot_end) {
 // index in a staged tile, a slot otherwise. The entries of the fixed rows
 // come with the head; the rest, from the wave's tail block, kPrefetch quads
 // at a time.
+//
+// Every list load takes a base that is uniform over the wave (a fixed row's
+// start, or the wave's tail block advanced by the pass through it) plus a
+// 32-bit byte offset the lane keeps, so it needs no address arithmetic of its
+// own. A tail load may run up to kPrefetch - 1 quads past the lane's last
+// entry: into later entries of the wave's block, or past the last block into
+// the list's padding (kListPad), and never used.
 template <bool STAGED>
-__device__ __forceinline__ uint32_t list_at(const SolveBufs& b, uint32_t e) {
-    return STAGED ? (uint32_t)b.nbr16[e] : b.nbr32[e];
+using ListIdx = typename std::conditional<STAGED, uint16_t, uint32_t>::type;
+template <bool STAGED>
+__device__ __forceinline__ const char* list_ids(const SolveBufs& b, size_t e) {
+    return STAGED ? reinterpret_cast<const char*>(b.nbr16 + e) : reinterpret_cast<const char*>(b.nbr32 + e);
+}
+template <bool STAGED>
+__device__ __forceinline__ uint32_t list_id_at(const char* base, uint32_t byte_off) {
+    return *reinterpret_cast<const ListIdx<STAGED>*>(base + byte_off);
+}
+__device__ __forceinline__ float list_g_at(const char* base, uint32_t byte_off) {
+    return *reinterpret_cast<const float*>(base + byte_off);
 }
 // The slot whose own fields a thread reads: its particle, or for a lane past
 // the scene's end the tile's first slot (a tile that runs holds one), so the
@@ -273,6 +329,10 @@ __device__ __forceinline__ uint32_t tail_offset(uint32_t nn) {
     const uint32_t len = list_tail_len(nn);
     return wave_inclusive_sum((__lane_id() & (kLanes - 1)) == 0 ? len : 0u) - len;
 }
+// The first entry of the wave's tail block (uniform).
+__device__ __forceinline__ uint32_t wave_tail_block(const SolveBufs& b, const LaneId& id) {
+    return uni(list_tail(b.stride, b.cap, id.p & ~(kListWave - 1u), 0u));
+}
 
 // The particle's entry count and the lane's entries in the kHead fixed rows.
 // A sweep loads them before it stages its tile, so their latency overlaps
@@ -280,28 +340,33 @@ __device__ __forceinline__ uint32_t tail_offset(uint32_t nn) {
 // of a lane past the scene's end, lie inside the list and are never used.
 constexpr uint32_t kHead = kListFixed;
 struct ListHead {
-    uint32_t nn;    // the particle's entries (its lane's share is taken later)
-    uint32_t tail;  // the list entry of the lane's first entry past the head
+    uint32_t nn;     // the particle's entries (its lane's share is taken later)
+    uint32_t wbase;  // the wave's tail block (uniform)
+    uint32_t lane_off;  // the lane's first entry past the head, from wbase
     uint32_t j[kHead];
     float g[kHead];
 };
 template <bool STAGED>
 __device__ __forceinline__ ListHead list_head(const SolveBufs& b, const LaneId& id) {
     ListHead h;
-    h.nn = b.nbr_n[own_slot(id)];
-    h.tail = 0;
-    const uint32_t step = b.stride * kLanes, last = b.cap / kLanes - 1u;
+    h.nn = at32(b.nbr_n, own_slot(id));
+    h.wbase = 0;
+    h.lane_off = 0;
+    const uint32_t last = b.cap / kLanes - 1u;
+    const uint32_t vj = id.gid * (uint32_t)sizeof(ListIdx<STAGED>), vg = id.gid * 4u;
 #pragma unroll
     for (uint32_t u = 0; u < kHead; ++u) {
-        const uint32_t e = (u < last ? u : last) * step + id.gid;
-        h.j[u] = list_at<STAGED>(b, e);
-        h.g[u] = b.nbrg[e];
+        const size_t row = (size_t)(u < last ? u : last) * b.stride * kLanes;
+        h.j[u] = list_id_at<STAGED>(list_ids<STAGED>(b, row), vj);
+        h.g[u] = list_g_at(reinterpret_cast<const char*>(b.nbrg + row), vg);
     }
     return h;
 }
-// Sets h.tail once the count has arrived; every lane of the wave calls it.
+// Sets the lane's tail place once the count has arrived; every lane of the
+// wave calls it.
 __device__ __forceinline__ void list_tail_at(const SolveBufs& b, const LaneId& id, ListHead& h) {
-    h.tail = list_tail(b.stride, b.cap, id.p, tail_offset(id.live ? h.nn : 0u)) + id.lane;
+    h.wbase = wave_tail_block(b, id);
+    h.lane_off = tail_offset(id.live ? h.nn : 0u) + id.lane;
 }
 // Calls f(j, g) over the lane's entries in order: the head, then the tail
 // kPrefetch quads at a time. EARLY issues the first tail quads before the
@@ -312,12 +377,17 @@ __device__ __forceinline__ void for_list(const SolveBufs& b, const LaneId& id, c
     const uint32_t cnt = id.live && h.nn > id.lane ? (h.nn - id.lane + kLanes - 1) / kLanes : 0u;
     uint32_t j[kPrefetch];
     float g[kPrefetch];
+    const uint32_t vj = h.lane_off * (uint32_t)sizeof(ListIdx<STAGED>), vg = h.lane_off * 4u;
     auto load = [&](uint32_t t) {
+        // t is uniform: the quad's entries start kLanes (t - kHead) past the
+        // block's base.
+        const size_t e = (size_t)h.wbase + (size_t)(t - kHead) * kLanes;
+        const char* jb = list_ids<STAGED>(b, e);
+        const char* gb = reinterpret_cast<const char*>(b.nbrg + e);
 #pragma unroll
         for (uint32_t u = 0; u < kPrefetch; ++u) {
-            const uint32_t e = h.tail + ((t + u < cnt ? t + u : cnt - 1u) - kHead) * kLanes;
-            j[u] = list_at<STAGED>(b, e);
-            g[u] = b.nbrg[e];
+            j[u] = list_id_at<STAGED>(jb + u * kLanes * sizeof(ListIdx<STAGED>), vj);
+            g[u] = list_g_at(gb + u * kLanes * sizeof(float), vg);
         }
     };
     if (EARLY && cnt > kHead) load(kHead);
@@ -424,17 +494,16 @@ __device__ __forceinline__ void load_pv(const PvLds& L, const float4* __restrict
     }
 }
 
-template <uint32_t NT, class Early = NoEarly>
+template <uint32_t NT, uint32_t IT, class Early = NoEarly>
 __device__ __forceinline__ void stage_pv(const RegionS& R, const float4* __restrict__ pos,
                                          const float4* __restrict__ vel, const PvLds& L, uint32_t slots,
                                          Early&& early = Early()) {
-    constexpr uint32_t IT = 3;
     float4 pp[IT], vv[IT];
-    stage_flat<NT, IT>(
+    stage_rows<NT, IT>(
         R, slots,
         [&](uint32_t slot, uint32_t it) {
-            pp[it] = pos[slot];
-            vv[it] = vel[slot];
+            pp[it] = at32(pos, slot);
+            vv[it] = at32(vel, slot);
         },
         [&](uint32_t idx, uint32_t it) {
             L.pv[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
@@ -478,7 +547,8 @@ __device__ __forceinline__ uint32_t density_walk(const SceneK& k, const LaneId& 
     // Staged, entries below T go to LDS; unstaged, those of the fixed rows go
     // to the list. The rest, up to the cap, wait in the spill area, slot-major
     // from the first tail entry: their place depends on the wave's counts.
-    const uint32_t fixed = kListFixed * kLanes, low = STAGED ? T : fixed;
+    // (A cap below the fixed rows leaves only cap / 4 of them in the list.)
+    const uint32_t fixed = kListFixed * kLanes, low = STAGED ? T : (fixed < b.cap ? fixed : b.cap);
     const uint32_t spill = id.p * list_tail_cap(b.cap) - fixed;
     auto append = [&](bool acc, uint32_t m, uint32_t j) {
         if (!acc) return;
@@ -702,7 +772,7 @@ __device__ __forceinline__ void density_tile(const SceneK& k, const StepK& s, ui
                                              const float4* __restrict__ pos, const float4* __restrict__ vel,
                                              const SolveBufs& b, const PvLds& L, unsigned long long& err) {
     if (STAGED) {
-        stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b));
+        stage_pv<kSweepThreads, 3>(R, pos, vel, L, stage_slots(b));
         __syncthreads();
     }
     f3 pi = mk(0.0f, 0.0f, 0.0f), vi = pi;
@@ -769,11 +839,11 @@ __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __r
     constexpr uint32_t IT = 3;
     float4 pp[IT];
     float kk[IT];
-    stage_flat<kSweepThreads, IT>(
+    stage_rows<kSweepThreads, IT>(
         R, slots,
         [&](uint32_t slot, uint32_t it) {
-            pp[it] = pos[slot];
-            kk[it] = kd[slot];
+            pp[it] = at32(pos, slot);
+            kk[it] = at32(kd, slot);
         },
         [&](uint32_t idx, uint32_t it) { lds[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, kk[it]); }, early);
 }
@@ -781,8 +851,8 @@ __device__ __forceinline__ void stage_pos_kd(const RegionS& R, const float4* __r
 // Lane `lane` of particle p: the partial sum of (kd_i + kd_j) grad W_ij over
 // its share of the list (kappa_partial, sim_solve.wgsl:596-606).
 template <bool STAGED>
-__device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id, const ListHead& h, f3 pi,
-                                            float kdi, const float4* __restrict__ pos, const float4* lds) {
+__device__ __forceinline__ f3 kappa_partial(const SceneK& k, const SolveBufs& b, const LaneId& id, const ListHead& h,
+                                            f3 pi, float kdi, const float4* __restrict__ pos, const float4* lds) {
     f3 dv = mk(0.0f, 0.0f, 0.0f);
     for_list<STAGED, true>(b, id, h, [&](uint32_t j, float g) {
         float4 q;
@@ -792,14 +862,14 @@ __device__ __forceinline__ f3 kappa_partial(const SolveBufs& b, const LaneId& id
             const float4 pj = pos[j];
             q = make_float4(pj.x, pj.y, pj.z, b.kd[j]);
         }
-        dv = add(dv, scl(kdi + q.w, listed_grad(pi, xyz(q), g)));
+        dv = add(dv, scl(kdi + q.w, sweep_grad(k, pi, xyz(q), g)));
     });
     return dv;
 }
 
 // `own(slot)` loads the caller's own fields with the list head.
 template <bool STAGED, class Own>
-__device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
+__device__ __forceinline__ f3 kappa_sweep(const SceneK& k, const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                           const SolveBufs& b, float4* smem, f3& pi, float& kdi, Own&& own) {
     ListHead h;
     float4 p4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -822,7 +892,7 @@ __device__ __forceinline__ f3 kappa_sweep(const LaneId& id, const RegionS& R, co
     if (id.live) {
         pi = xyz(p4);
         kdi = kd0;
-        dv = kappa_partial<STAGED>(b, id, h, pi, kdi, pos, smem);
+        dv = kappa_partial<STAGED>(k, b, id, h, pi, kdi, pos, smem);
     }
     return fold4(dv);
 }
@@ -846,8 +916,8 @@ __device__ __forceinline__ void kappa_apply_on(const Sc& sc, uint32_t tile, uint
         wv = lane_part(vel_in, o, id.lane);
         gv = wall_part(b, R, o, id.lane);
     };
-    const f3 total = R.staged ? kappa_sweep<true>(id, R, pos, b, smem, pi, kdi, own)
-                              : kappa_sweep<false>(id, R, pos, b, smem, pi, kdi, own);
+    const f3 total = R.staged ? kappa_sweep<true>(k, id, R, pos, b, smem, pi, kdi, own)
+                              : kappa_sweep<false>(k, id, R, pos, b, smem, pi, kdi, own);
     const float4 w = quad_float4(wv), wg = quad_float4(gv);
     if (id.live && id.lane == 0) {
         const f3 full = add(scl(k.mass, total), scl(kdi, xyz(wg)));
@@ -915,15 +985,15 @@ __device__ __forceinline__ void forces_tile(const SceneK& k, const LaneId& id, c
         constexpr uint32_t IT = 3;
         float4 pp[IT], vv[IT];
         float dd[IT], nr[IT], tt[IT], pq[IT];
-        stage_flat<kSweepThreads, IT>(
+        stage_rows<kSweepThreads, IT>(
             R, stage_slots(b),
             [&](uint32_t slot, uint32_t it) {
-                pp[it] = pos[slot];
-                vv[it] = vel[slot];
-                dd[it] = b.density[slot];
-                nr[it] = b.rho_near[slot];
-                tt[it] = temperature[slot];
-                pq[it] = prev_pressure[slot];
+                pp[it] = at32(pos, slot);
+                vv[it] = at32(vel, slot);
+                dd[it] = at32(b.density, slot);
+                nr[it] = at32(b.rho_near, slot);
+                tt[it] = at32(temperature, slot);
+                pq[it] = at32(prev_pressure, slot);
             },
             [&](uint32_t idx, uint32_t it) {
                 L.a[idx] = make_float4(pp[it].x, pp[it].y, pp[it].z, vv[it].x);
@@ -1081,7 +1151,7 @@ struct DenOwn {
 };
 
 template <bool STAGED>
-__device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
+__device__ __forceinline__ float den_kappa_sweep(const SceneK& k, const LaneId& id, const RegionS& R, const float4* __restrict__ pos,
                                                  const float4* __restrict__ vel, const SolveBufs& b, const PvLds& L,
                                                  f3& vi, DenOwn& own) {
     ListHead h;
@@ -1097,7 +1167,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
         own.wg = wall_grad_of(b, R, o);
     };
     if (STAGED) {
-        stage_pv<kSweepThreads>(R, pos, vel, L, stage_slots(b), early);
+        stage_pv<kSweepThreads, 3>(R, pos, vel, L, stage_slots(b), early);
         __syncthreads();
     } else {
         early();
@@ -1111,7 +1181,7 @@ __device__ __forceinline__ float den_kappa_sweep(const LaneId& id, const RegionS
         for_list<STAGED, true>(b, id, h, [&](uint32_t j, float g) {
             f3 pj, vj;
             load_pv<STAGED>(L, pos, vel, j, pj, vj);
-            drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
+            drho += dot3(sub(vii, vj), sweep_grad(k, pi, pj, g));
         });
     }
     return fold4(drho);
@@ -1132,8 +1202,8 @@ __device__ __forceinline__ void den_kappa_on(const Sc& sc, uint32_t tile, uint32
     const PvLds L{smem, reinterpret_cast<float2*>(smem + stage_slots(b)), nullptr};
     f3 vi = mk(0.0f, 0.0f, 0.0f);
     DenOwn own;
-    const float total = R.staged ? den_kappa_sweep<true>(id, R, pos, vel, b, L, vi, own)
-                                 : den_kappa_sweep<false>(id, R, pos, vel, b, L, vi, own);
+    const float total = R.staged ? den_kappa_sweep<true>(k, id, R, pos, vel, b, L, vi, own)
+                                 : den_kappa_sweep<false>(k, id, R, pos, vel, b, L, vi, own);
     unsigned long long err = 0;
     unsigned long long* const sum = DIV ? b.div_residual : b.residual;
     if (id.live && id.lane == 0) {
@@ -1155,7 +1225,7 @@ __device__ __forceinline__ void den_kappa_on(const Sc& sc, uint32_t tile, uint32
     if (sum) block_residual(err, sum + sc.index_at(R.scene));
 }
 template <class Sc, bool DIV = false>
-__global__ __launch_bounds__(kSweepThreads) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
                                                              const float4* __restrict__ vel, SolveBufs b) {
     extern __shared__ float4 smem[];
     den_kappa_on<Sc, DIV>(sc, tile_index(sc), pass, pos, vel, b, smem);

@@ -187,7 +187,12 @@ def test_single_particle_and_dt_zero():
         assert np.array_equal(s.state()["pos"], sc.pos4)
 
 
-def test_neighbour_cap_overflow_is_counted_like_the_oracle():
+@pytest.mark.parametrize("lds", [None, 8192])
+def test_neighbour_cap_overflow_is_counted_like_the_oracle(lds, monkeypatch):
+    """A cap of 8, below the list's fixed rows, in staged tiles and (8 KB of
+    LDS) unstaged ones, whose walk writes the list directly."""
+    if lds:
+        monkeypatch.setenv("FLUID_LDS_BYTES", str(lds))
     sc = _scene_small(seed=12)
     step, refine = _step(sc)
     g = run_gpu(sc, [step], refine, nbr_cap=8)

@@ -174,6 +174,102 @@ struct TileRegion {
     uint32_t walls;
 };
 
+// The next substep's grid count, taken by integrate as it writes the
+// resting positions (the same cells, histogram, provisional ranks and
+// non-finite flag grid_count would compute from them, and the scan's
+// look-back words cleared), so that build can skip grid_count. Null cellid:
+// off (grid_count runs).
+struct NextCount {
+    uint32_t* cellid = nullptr;
+    uint32_t* rank = nullptr;
+    uint32_t* counts = nullptr;
+    uint64_t* status = nullptr;
+    uint32_t* ticket = nullptr;
+    uint32_t* nonfinite = nullptr;  // word 0 of the scene's counters
+    uint32_t scan_tiles = 0;
+};
+
+// The histogram add of a wave's 64 cells, live lanes a prefix of the wave:
+// consecutive lanes that share a cell (a cell-sorted set gives runs of them)
+// take one atomic between them, all runs of the wave at once. Returns the
+// lane's provisional rank in its cell (the old count plus its offset in its
+// run). Every lane of the wave takes part.
+__device__ __forceinline__ uint32_t count_runs(uint32_t c, bool live, uint32_t* __restrict__ counts) {
+    const uint32_t lane = __lane_id();
+    const uint32_t prev = (uint32_t)__shfl((int)c, lane > 0 ? (int)lane - 1 : 0);
+    const bool leader = live && (lane == 0 || prev != c);
+    const uint64_t lead = __ballot(leader);
+    const uint32_t nlive = (uint32_t)__popcll(__ballot(live));
+    uint32_t base = 0;
+    if (leader) {
+        const uint64_t above = lane == 63 ? 0ull : lead & ~((2ull << lane) - 1ull);
+        uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
+        end = end < nlive ? end : nlive;
+        base = atomicAdd(&counts[c], end - lane);
+    }
+    const uint64_t upto = lane == 63 ? lead : lead & ((2ull << lane) - 1ull);
+    const uint32_t my_leader = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
+    base = (uint32_t)__shfl((int)base, (int)my_leader);
+    return base + (lane - my_leader);
+}
+
+// The scene-local cell of a position.
+__device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
+    const uint32_t cx = cell_coord(p.x, k.box_min[0], k.cell, k.dims[0]);
+    const uint32_t cy = cell_coord(p.y, k.box_min[1], k.cell, k.dims[1]);
+    const uint32_t cz = cell_coord(p.z, k.box_min[2], k.cell, k.dims[2]);
+    return (cz * k.dims[1] + cy) * k.dims[0] + cx;
+}
+
+// Solve tile t's region record from the scene-local cells of its first and
+// last slot: each stencil row (dy, dz) of every cell
+// in [c_first, c_last] lies in cells [c_first + off - 1, c_last + off + 1]
+// of the row's offset off; starts (indexed at the scene's cell offset) give
+// their slot ranges.
+template <class Sc>
+__device__ __forceinline__ void write_region(const Sc& sc, uint32_t t, int64_t c_first, int64_t c_last,
+                                             const uint32_t* __restrict__ starts, TileRegion* __restrict__ out,
+                                             uint32_t stage_cap) {
+    const SceneK& k = sc.scene(t);
+    const int64_t cells = k.cells;
+    const uint32_t* __restrict__ sst = starts + k.cell_offset;
+    const int64_t plane = (int64_t)k.dims[0] * k.dims[1];
+    TileRegion R;
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < 9; ++r) {
+        const int64_t dz = (int64_t)(r / 3) - 1, dy = (int64_t)(r % 3) - 1;
+        const int64_t off = dz * plane + dy * (int64_t)k.dims[0];
+        int64_t lo = c_first + off - 1, hi = c_last + off + 1;
+        if (lo < 0) lo = 0;
+        if (hi > cells - 1) hi = cells - 1;
+        uint32_t a = 0, e = 0;
+        if (lo <= hi) {
+            a = sst[lo];
+            e = sst[hi + 1];
+        }
+        R.delta[r] = (int32_t)total - (int32_t)a;
+        R.base[r] = total;
+        total += e - a;
+    }
+    R.total = total;
+    R.staged = total <= stage_cap ? 1u : 0u;
+    const StepK& st = sc.step(t);
+    R.scene = sc.index(t);
+    R.slot_end = k.slot_end;
+    R.active = st.active;
+    R.refine = st.refine;
+    R.walls = 0;
+    out[t] = R;
+}
+
+// The region records the grid build's gather writes too (the solve's tile
+// size and staging capacity), when it takes k_tile_regions' work.
+struct RegionJob {
+    TileRegion* regions = nullptr;
+    uint32_t tiles = 0, tile = 0, stage_cap = 0;
+};
+
 // Per-substep scratch of the solver, all in slot (cell) order.
 struct SolveBufs {
     float* density;
@@ -227,6 +323,10 @@ struct SolveBufs {
     // iterating, 2 + (scenes + 1) + scenes words each (solve.hip).
     uint32_t* tile_lists;
     TileRegion* regions;  // [tiles]
+    NextCount next;
+    // The grid build's gather wrote this substep's region records (the
+    // solve skips k_tile_regions).
+    bool regions_ready = false;
 };
 
 constexpr uint32_t kZeroRows = 4096;
@@ -318,8 +418,11 @@ inline Span span(KernelTimer* t, int kernel) {
 uint32_t scan_tiles(uint32_t cells);
 // The grid build over `slots` resting slots and `cells` histogram cells
 // (a batch's cells include one padding cell per scene).
+// `counted`: the last substep's integrate already took this build's count
+// (NextCount), so grid_count is skipped.
 void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,
-                       const Records& work, const GridBufs& b, KernelTimer* timer);
+                       const Records& work, const GridBufs& b, KernelTimer* timer, bool counted = false,
+                       const RegionJob& regions = RegionJob());
 void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uint32_t cells, const Records& rest,
                        const Records& work, const GridBufs& b, KernelTimer* timer);
 void launch_scan_only(hipStream_t st, uint32_t* counts, uint32_t* starts, uint32_t cells, uint64_t* status,

@@ -81,26 +81,9 @@ __global__ __launch_bounds__(kBlock) void k_grid_count(Sc sc, const float4* __re
             cellid[i] = c;
         }
     }
-    // Consecutive lanes that share a cell (a cell-sorted resting set gives
-    // runs of them) take one atomic between them, all runs of the wave at
-    // once; each lane gets the old count plus its offset in its run. Live
-    // lanes are a prefix of the wave.
-    const uint32_t lane = lane_id();
-    const uint32_t prev = (uint32_t)__shfl((int)c, lane > 0 ? (int)lane - 1 : 0);
-    const bool leader = live && (lane == 0 || prev != c);
-    const uint64_t lead = __ballot(leader);
-    const uint32_t nlive = (uint32_t)__popcll(__ballot(live));
-    uint32_t base = 0;
-    if (leader) {
-        const uint64_t above = lane == 63 ? 0ull : lead & ~((2ull << lane) - 1ull);
-        uint32_t end = above ? (uint32_t)(__ffsll((unsigned long long)above) - 1) : 64u;
-        end = end < nlive ? end : nlive;
-        base = atomicAdd(&counts[c], end - lane);
-    }
-    const uint64_t upto = lane == 63 ? lead : lead & ((2ull << lane) - 1ull);
-    const uint32_t my_leader = upto ? 63u - (uint32_t)__clzll((long long)upto) : 0u;
-    base = (uint32_t)__shfl((int)base, (int)my_leader);
-    if (live) rank[i] = base + (lane - my_leader);
+    // Live lanes are a prefix of the wave.
+    const uint32_t r = count_runs(c, live, counts);
+    if (live) rank[i] = r;
 }
 
 // One tile of kScanTile counts per workgroup, tiles taken in ticket order so
@@ -186,7 +169,10 @@ __global__ __launch_bounds__(kBlock) void k_grid_place(uint32_t n, const uint32_
                                                        uint32_t* __restrict__ perm_tmp) {
     uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    perm_tmp[starts[cellid[i]] + rank[i]] = i;
+    // A histogram that agrees with the cells puts every slot below n; the
+    // guard keeps one that does not inside the array.
+    const uint32_t f = starts[cellid[i]] + rank[i];
+    if (f < n) perm_tmp[f] = i;
 }
 
 // A slot's particle takes the final slot start + (particles of its cell with
@@ -196,12 +182,30 @@ __global__ __launch_bounds__(kBlock) void k_grid_place(uint32_t n, const uint32_
 // the cell's entries eight at a time) completes; every store follows every
 // load, so no load waits on a store. Without RECORDS only the permutation
 // is written (an index sort).
-template <bool RECORDS>
+//
+// With a RegionJob (the plain step), the launch's first region_blocks
+// workgroups write the solve's tile region records instead (k_tile_regions'
+// work): a tile's first and last slot hold, after place, particles of the
+// cells those slots end up with, so their cells come from perm_tmp and
+// cellid, with no wait for the gathered positions.
+template <bool RECORDS, class Sc>
 __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32_t* __restrict__ perm_tmp,
                                                         const uint32_t* __restrict__ cellid,
                                                         const uint32_t* __restrict__ starts, Records rest,
-                                                        Records work, uint32_t* __restrict__ perm) {
-    uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+                                                        Records work, uint32_t* __restrict__ perm, Sc sc,
+                                                        RegionJob rj, uint32_t region_blocks) {
+    if (blockIdx.x < region_blocks) {
+        const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+        if (t >= rj.tiles) return;
+        const SceneK& k = sc.scene(t);
+        const uint32_t s0 = t * rj.tile;
+        const uint32_t s1 = s0 + rj.tile < k.slot_end ? s0 + rj.tile : k.slot_end;
+        const int64_t c_first = (int64_t)cellid[perm_tmp[s0]] - k.cell_offset;
+        const int64_t c_last = (int64_t)cellid[perm_tmp[s1 - 1]] - k.cell_offset;
+        write_region(sc, t, c_first, c_last, starts, rj.regions, rj.stage_cap);
+        return;
+    }
+    uint32_t s = (blockIdx.x - region_blocks) * kBlock + threadIdx.x;
     if (s >= n) return;
     const uint32_t i = perm_tmp[s];
     float4 p{}, v{}, pv{};
@@ -216,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32
         id = rest.ids ? rest.ids[i] : 0u;
     }
     const uint32_t c = cellid[i];
-    const uint32_t st = starts[c], en = starts[c + 1];
+    const uint32_t st = starts[c], en = starts[c + 1] < n ? starts[c + 1] : n;
     uint32_t r = 0;
     for (uint32_t t = st; t < en; t += 8) {
         uint32_t w[8];
@@ -226,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_grid_gather(uint32_t n, const uint32
         for (uint32_t u = 0; u < 8; ++u) r += (t + u < en && w[u] < i) ? 1u : 0u;
     }
     const uint32_t f = st + r;
+    if (f >= n) return;  // (the same guard as place's)
     perm[f] = i;
     if (!RECORDS) return;
     work.pos[f] = p;
@@ -242,12 +247,15 @@ uint32_t scan_tiles(uint32_t cells) { return (cells + kScanTile - 1) / kScanTile
 
 template <class Sc>
 static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells, const Records& rest,
-                       const Records& work, const GridBufs& b, KernelTimer* timer) {
+                       const Records& work, const GridBufs& b, KernelTimer* timer, bool counted,
+                       const RegionJob& rj) {
     const uint32_t ntiles = scan_tiles(cells);
     uint32_t count_threads = n > ntiles + 1 ? n : ntiles + 1;
-    const Span tc = span(timer, KERNEL_GRID_COUNT);
-    hipExtLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, tc.a, tc.z, 0, sc,
-                          rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket, b.nonfinite);
+    if (!counted) {
+        const Span tc = span(timer, KERNEL_GRID_COUNT);
+        hipExtLaunchKernelGGL(k_grid_count<Sc>, dim3(blocks_for(count_threads)), dim3(kBlock), 0, st, tc.a, tc.z, 0,
+                              sc, rest.pos, n, b.cellid, b.rank, b.counts, b.status, ntiles, b.ticket, b.nonfinite);
+    }
     const Span ts = span(timer, KERNEL_GRID_SCAN);
     hipExtLaunchKernelGGL(k_grid_scan, dim3(ntiles), dim3(kBlock), 0, st, ts.a, ts.z, 0, b.counts, b.starts, cells,
                           b.status, b.ticket, true);
@@ -255,17 +263,19 @@ static void grid_build(hipStream_t st, const Sc& sc, uint32_t n, uint32_t cells,
     const Span tg = span(timer, KERNEL_GRID_SCATTER);
     hipExtLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, tg.a, nullptr, 0, n, b.cellid,
                           b.rank, b.starts, b.perm_tmp);
-    hipExtLaunchKernelGGL(k_grid_gather<true>, dim3(blocks_for(n)), dim3(kBlock), 0, st, nullptr, tg.z, 0, n, b.perm_tmp,
-                          b.cellid, b.starts, rest, work, b.perm);
+    const uint32_t rb = rj.regions ? blocks_for(rj.tiles) : 0u;
+    hipExtLaunchKernelGGL(k_grid_gather<true, Sc>, dim3(rb + blocks_for(n)), dim3(kBlock), 0, st, nullptr, tg.z, 0, n,
+                          b.perm_tmp, b.cellid, b.starts, rest, work, b.perm, sc, rj, rb);
 }
 
 void launch_grid_build(hipStream_t st, const OneScene& sc, uint32_t slots, uint32_t cells, const Records& rest,
-                       const Records& work, const GridBufs& b, KernelTimer* timer) {
-    grid_build(st, sc, slots, cells, rest, work, b, timer);
+                       const Records& work, const GridBufs& b, KernelTimer* timer, bool counted,
+                       const RegionJob& regions) {
+    grid_build(st, sc, slots, cells, rest, work, b, timer, counted, regions);
 }
 void launch_grid_build(hipStream_t st, const SceneTable& sc, uint32_t slots, uint32_t cells, const Records& rest,
                        const Records& work, const GridBufs& b, KernelTimer* timer) {
-    grid_build(st, sc, slots, cells, rest, work, b, timer);
+    grid_build(st, sc, slots, cells, rest, work, b, timer, false, RegionJob());
 }
 
 // The grid build's counting sort of `n` positions into an index list (the
@@ -284,8 +294,8 @@ void launch_index_sort(hipStream_t st, const OneScene& sc, uint32_t n, uint32_t 
     hipLaunchKernelGGL(k_grid_place, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.cellid, b.rank, b.starts,
                        b.perm_tmp);
     const Records none{};
-    hipLaunchKernelGGL(k_grid_gather<false>, dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp, b.cellid,
-                       b.starts, none, none, b.perm);
+    hipLaunchKernelGGL((k_grid_gather<false, OneScene>), dim3(blocks_for(n)), dim3(kBlock), 0, st, n, b.perm_tmp,
+                       b.cellid, b.starts, none, none, b.perm, sc, RegionJob(), 0u);
 }
 
 // A stand-alone exclusive scan of `cells` counts (the grid scan, exposed for

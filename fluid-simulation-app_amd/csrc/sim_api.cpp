@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fluid_common.hpp"
@@ -104,6 +105,15 @@ struct FluidSim {
     StatsBufs stats{};
     TracerBufs tracers{};
     bool track_ids = false;
+    // The last substep's integrate took the next grid build's count from the
+    // resting positions it wrote (NextCount); anything else that writes the
+    // resting set, or a grid build with no integrate after it, clears this.
+    bool counted = false;
+    // Where integrate writes that count's cells and ranks: the grid's own
+    // arrays keep the last build's (fluid_sim_download_grid), and a build
+    // that takes a count swaps the two.
+    uint32_t* next_cellid = nullptr;
+    uint32_t* next_rank = nullptr;
     size_t lds_bytes = 0;   // the dynamic LDS budget of the sweeps
     size_t lds_limit = 0;   // the device's per-workgroup LDS (FLUID_LDS_BYTES caps it)
     // The solve of the tile size in use (128 slots unless set).
@@ -195,6 +205,8 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     }
     A(s->grid.cellid, n);
     A(s->grid.rank, n);
+    A(s->next_cellid, n);
+    A(s->next_rank, n);
     A(s->grid.counts, c);
     A(s->grid.starts, c + 1);
     A(s->grid.perm_tmp, n);
@@ -263,15 +275,44 @@ int run_substeps(FluidSim* s, const uint8_t* step144, uint32_t nsub, uint32_t re
     std::memcpy(&sb, step144, sizeof sb);
     // dt = 0 encodes no solve: kappa divides by dt^2 (render.rs:3165-3167).
     if (!(sb.dt > 0.0f) || nsub == 0) return FLUID_OK;
-    const StepK k = make_step(sb, s->phys);
+    StepK k = make_step(sb, s->phys);
+    k.refine = refine;  // as the solve sets it (the region records carry it)
     const OneScene sc{s->scene, k};
+    // integrate takes the next build's count when its launch covers the
+    // scan's look-back words, which it clears.
+    const uint32_t scan_t = scan_tiles(s->cells);
+    const bool fuse = s->n > 0 && (uint64_t)s->V->tiles(s->n) * s->V->tile > scan_t;
+    SolveBufs b = s->solve;
+    // The gather writes the solve's tile region records.
+    RegionJob rj;
+    if (s->n > 0) {
+        rj.regions = s->solve.regions;
+        rj.tiles = s->V->tiles(s->n);
+        rj.tile = s->V->tile;
+        rj.stage_cap = s->solve.stage_cap;
+        b.regions_ready = true;
+    }
     for (uint32_t i = 0; i < nsub; ++i) {
-        launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
+        if (s->counted) {
+            std::swap(s->grid.cellid, s->next_cellid);
+            std::swap(s->grid.rank, s->next_rank);
+        }
+        if (fuse) {
+            b.next.cellid = s->next_cellid;
+            b.next.rank = s->next_rank;
+            b.next.counts = s->grid.counts;
+            b.next.status = s->grid.status;
+            b.next.ticket = s->grid.ticket;
+            b.next.nonfinite = s->grid.nonfinite;
+            b.next.scan_tiles = scan_t;
+        }
+        launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t(), s->counted, rj);
         if (s->tol.on())
-            s->V->tolerance_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, s->tol, refine,
+            s->V->tolerance_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, b, s->tol, refine,
                                 s->limits, s->t());
         else
-            s->V->solve_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, s->solve, refine, s->t());
+            s->V->solve_one(s->stream, sc, s->n, s->grid.starts, s->work, s->rest, b, refine, s->t());
+        s->counted = fuse;
     }
     return check_launch("substep");
 }
@@ -316,6 +357,12 @@ int fluid_sim_upload_state(FluidSim* s, const float* pos4, const float* vel4, co
     if (!valid(s)) return FLUID_ERR_ARG;
     const size_t n = s->n;
     HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->counted) {
+        // The count integrate took is of positions this upload replaces: the
+        // next build counts again, into a cleared histogram.
+        HIP_TRY(hipMemset(s->grid.counts, 0, (size_t)s->cells * 4));
+        s->counted = false;
+    }
     if (pos4) HIP_TRY(hipMemcpy(s->rest.pos, pos4, n * 16, hipMemcpyHostToDevice));
     if (vel4) HIP_TRY(hipMemcpy(s->rest.vel, vel4, n * 16, hipMemcpyHostToDevice));
     if (prev_vel4) HIP_TRY(hipMemcpy(s->rest.prev_vel, prev_vel4, n * 16, hipMemcpyHostToDevice));
@@ -518,7 +565,13 @@ int fluid_sim_density_only(FluidSim* s, const uint8_t* step144) {
     StepBlock sb;
     std::memcpy(&sb, step144, sizeof sb);
     const OneScene sc{s->scene, make_step(sb, s->phys)};
-    launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t());
+    if (s->counted) {
+        std::swap(s->grid.cellid, s->next_cellid);
+        std::swap(s->grid.rank, s->next_rank);
+    }
+    launch_grid_build(s->stream, sc, s->n, s->cells, s->rest, s->work, s->grid, s->t(), s->counted);
+    // The build consumed the count and no integrate follows to take the next.
+    s->counted = false;
     s->V->density_only(s->stream, sc, s->n, s->grid.starts, s->work, s->solve, s->t());
     return check_launch("density_only");
 }

@@ -63,13 +63,6 @@ __device__ __forceinline__ uint32_t tile_index(const Sc& sc) {
     return sc.tile0 + (sc.flip ? gridDim.x - 1u - t : t);
 }
 
-__device__ __forceinline__ uint32_t cell_of_pos(const SceneK& k, float4 p) {
-    const uint32_t cx = cell_coord(p.x, k.box_min[0], k.cell, k.dims[0]);
-    const uint32_t cy = cell_coord(p.y, k.box_min[1], k.cell, k.dims[1]);
-    const uint32_t cz = cell_coord(p.z, k.box_min[2], k.cell, k.dims[2]);
-    return (cz * k.dims[1] + cy) * k.dims[0] + cx;
-}
-
 // One thread a tile: the tile's nine row ranges (cells of the tile's own
 // scene; starts indexed at the scene's cell offset).
 template <class Sc>
@@ -81,38 +74,7 @@ __global__ __launch_bounds__(kBlock) void k_tile_regions(Sc sc, const uint32_t* 
     const SceneK& k = sc.scene(t);
     const uint32_t s0 = t * kTile;
     const uint32_t s1 = s0 + kTile < k.slot_end ? s0 + kTile : k.slot_end;
-    const int64_t c_first = cell_of_pos(k, pos[s0]);
-    const int64_t c_last = cell_of_pos(k, pos[s1 - 1]);
-    const int64_t cells = k.cells;
-    const uint32_t* __restrict__ sst = starts + k.cell_offset;
-    const int64_t plane = (int64_t)k.dims[0] * k.dims[1];
-    TileRegion R;
-    uint32_t total = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < kRows; ++r) {
-        const int64_t dz = (int64_t)(r / 3) - 1, dy = (int64_t)(r % 3) - 1;
-        const int64_t off = dz * plane + dy * (int64_t)k.dims[0];
-        int64_t lo = c_first + off - 1, hi = c_last + off + 1;
-        if (lo < 0) lo = 0;
-        if (hi > cells - 1) hi = cells - 1;
-        uint32_t a = 0, e = 0;
-        if (lo <= hi) {
-            a = sst[lo];
-            e = sst[hi + 1];
-        }
-        R.delta[r] = (int32_t)total - (int32_t)a;
-        R.base[r] = total;
-        total += e - a;
-    }
-    R.total = total;
-    R.staged = total <= stage_cap ? 1u : 0u;
-    const StepK& st = sc.step(t);
-    R.scene = sc.index(t);
-    R.slot_end = k.slot_end;
-    R.active = st.active;
-    R.refine = st.refine;
-    R.walls = 0;
-    out[t] = R;
+    write_region(sc, t, cell_of_pos(k, pos[s0]), cell_of_pos(k, pos[s1 - 1]), starts, out, stage_cap);
 }
 
 // A tile's region as uniform scalars (SGPRs).
@@ -1103,6 +1065,10 @@ __global__ __launch_bounds__(kSweepThreads) void k_forces(Sc sc, const float4* _
     const SceneK& k = sc.scene_at(R.scene);
     const StepK& s = sc.step_at(R.scene);
     const LaneId id = lane_id(tile, sc.slot_end_at(R.slot_end));
+    // The scene's first tile clears the grid build's non-finite flag: this
+    // substep's walk has read it, and the resting set integrate writes is the
+    // next build's to judge.
+    if (threadIdx.x == 0 && tile * kTile == k.slot_begin) b.counters[2 * sc.index_at(R.scene)] = 0u;
     const uint32_t slots = stage_slots(b);
     float2* const t2 = reinterpret_cast<float2*>(smem + 2 * slots);
     const ForcesLds L{smem, smem + slots, t2, reinterpret_cast<float*>(t2 + slots)};
@@ -1237,13 +1203,17 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);  // one tile a workgroup
     const SceneK& k = sc.scene(tile);
     const StepK& s = sc.step(tile);
-    if (!sc.runs(tile, kEveryPass)) return;
     const uint32_t i = tile * kTile + threadIdx.x;
-    // The scene's first tile clears the grid build's non-finite flag: the
-    // walk of this substep has read it, and the resting set written here is
-    // the next build's to judge.
-    if (threadIdx.x == 0 && i == k.slot_begin) b.counters[2 * sc.index(tile)] = 0u;
+    const NextCount& nx = b.next;
+    // The next build's look-back words and ticket, cleared as grid_count
+    // would (the ones of this substep's scan are read by now).
+    if (nx.cellid) {
+        if (i < nx.scan_tiles) nx.status[i] = 0;
+        if (i == 0) *nx.ticket = 0;
+    }
+    if (!sc.runs(tile, kEveryPass)) return;
     bool clamped_speed = false;
+    uint32_t cell = 0xffffffffu;
     if (i < k.slot_end) {
         // Every load first: a load issued after a store would wait for it.
         const float4 was = work.vel[i], pv = work.prev_vel[i], p4 = work.pos[i];
@@ -1269,12 +1239,32 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
             if (c != in[a]) vv[a] = 0.0f;
         }
         rest.pos[i] = make_float4(out[0], out[1], out[2], 0.0f);
+        if (nx.cellid) {
+            // The next grid build's cell of this resting position
+            // (grid_count's cell and non-finite test).
+            if (!(__builtin_isfinite(out[0]) && __builtin_isfinite(out[1]) && __builtin_isfinite(out[2])))
+                atomicOr(nx.nonfinite, 1u);
+            const uint32_t cx = cell_coord(out[0], k.box_min[0], k.cell, k.dims[0]);
+            const uint32_t cy = cell_coord(out[1], k.box_min[1], k.cell, k.dims[1]);
+            const uint32_t cz = cell_coord(out[2], k.box_min[2], k.cell, k.dims[2]);
+            cell = k.cell_offset + (cz * k.dims[1] + cy) * k.dims[0] + cx;
+        }
         rest.vel[i] = make_float4(vv[0], vv[1], vv[2], wmix(was.w, acc, s.chase));
         rest.prev_vel[i] = make_float4(vv[0], vv[1], vv[2], wmix(pv.w, pr, s.chase));
         const float dp = pr - pp;
         rest.prev_pressure[i] = pr;
         rest.temperature[i] = t + t * EXPANSION / (rho * HEAT_CAPACITY) * dp;
         if (rest.ids) rest.ids[i] = id;
+    }
+    if (nx.cellid) {
+        // The resting index is the slot (the resting set is written in this
+        // substep's slot order); live lanes are a prefix of the wave.
+        const bool live = i < k.slot_end;
+        const uint32_t r = count_runs(cell, live, nx.counts);
+        if (live) {
+            nx.cellid[i] = cell;
+            nx.rank[i] = r;
+        }
     }
     // One atomic per wave for the clamp count, into one of the scene's
     // spread words (by tile).
@@ -1510,16 +1500,26 @@ size_t configure_solve(SolveBufs& b, size_t lds_bytes) {
     return avail;
 }
 
+// The substep's region records, unless the grid build's gather wrote them;
+// returns whether it launched (and so recorded `start`).
+template <class Sc>
+static bool tile_regions(hipStream_t st, const Sc& sc, uint32_t tiles, const uint32_t* starts, const Records& work,
+                         const SolveBufs& b, hipEvent_t start) {
+    if (b.regions_ready) return false;
+    hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, start, nullptr, 0, sc,
+                          starts, work.pos, b.regions, tiles, b.stage_cap);
+    return true;
+}
+
 template <class Sc>
 static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* starts, const Records& work,
                          const SolveBufs& b, KernelTimer* timer) {
     if (n == 0) return;
     const uint32_t tiles = solve_tiles(n);
     const Span t = span(timer, KERNEL_DENSITY_DIV);
-    hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(tiles)), dim3(kBlock), 0, st, t.a, nullptr, 0, sc, starts,
-                          work.pos, b.regions, tiles, b.stage_cap);
-    hipExtLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), density_lds(b), st, nullptr,
-                          t.z, 0, sc, starts, work.pos, work.vel, b);
+    const bool own = tile_regions(st, sc, tiles, starts, work, b, t.a);
+    hipExtLaunchKernelGGL(k_density_div<Sc>, dim3(tiles), dim3(kSweepThreads), density_lds(b), st,
+                          own ? nullptr : t.a, t.z, 0, sc, starts, work.pos, work.vel, b);
 }
 
 struct NoCheck {
@@ -1560,8 +1560,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
     // kernel's time is its own.
     const uint32_t extra = ng > 1 && G.fork && !timer ? G.streams : 0u;
     const Span tdd0 = span(timer, KERNEL_DENSITY_DIV);
-    hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tdd0.a, nullptr, 0, sc,
-                          starts, work.pos, b.regions, ntiles, b.stage_cap);
+    const bool own_regions = tile_regions(st, sc, ntiles, starts, work, b, tdd0.a);
     if (extra) {
         (void)hipEventRecord(G.fork, st);
         for (uint32_t i = 0; i < extra; ++i) (void)hipStreamWaitEvent(G.stream[i], G.fork, 0);
@@ -1579,9 +1578,11 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         const Span tdd = g == 0 ? tdd0 : span(timer, KERNEL_DENSITY_DIV);
         if (b.div_residual)
             hipExtLaunchKernelGGL(k_density_div<Sc, true>, tiles, sweep, density_lds(b), s,
-                                  g == 0 ? nullptr : tdd.a, tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
+                                  g == 0 && own_regions ? nullptr : tdd.a, tdd.z, 0, fwd, starts, work.pos, V(0),
+                                  KB(0));
         else
-            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s, g == 0 ? nullptr : tdd.a,
+            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), s,
+                                  g == 0 && own_regions ? nullptr : tdd.a,
                                   tdd.z, 0, fwd, starts, work.pos, V(0), KB(0));
         if (div_passes > 1) check(s, 0, true);
         const Span td = span(timer, KERNEL_DIV_APPLY);
@@ -1728,14 +1729,13 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
                                divergence ? b.div_iterations : b.iterations, next, lists[next % 2u]);
         };
         const Span tr = span(timer, KERNEL_DENSITY_DIV);
-        hipExtLaunchKernelGGL(k_tile_regions<Sc>, dim3(blocks_for(ntiles)), dim3(kBlock), 0, st, tr.a,
-                              nullptr, 0, sc, starts, work.pos, b.regions, ntiles, b.stage_cap);
+        const bool own = tile_regions(st, sc, ntiles, starts, work, b, tr.a);
         if (div)
-            hipExtLaunchKernelGGL(k_density_div<Sc, true>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0,
-                                  sc, starts, work.pos, work.vel, t);
+            hipExtLaunchKernelGGL(k_density_div<Sc, true>, tiles, sweep, density_lds(b), st, own ? nullptr : tr.a,
+                                  tr.z, 0, sc, starts, work.pos, work.vel, t);
         else
-            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), st, nullptr, tr.z, 0, sc,
-                                  starts, work.pos, work.vel, t);
+            hipExtLaunchKernelGGL(k_density_div<Sc>, tiles, sweep, density_lds(b), st, own ? nullptr : tr.a,
+                                  tr.z, 0, sc, starts, work.pos, work.vel, t);
         if (div) converge(0, true, true, 1);
         const Span td = span(timer, KERNEL_DIV_APPLY);
         hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,

@@ -297,3 +297,42 @@ def test_dense_lists_are_bit_exact_with_the_oracle(h_factor, cap, lds, monkeypat
     for k in STATE:
         assert np.array_equal(_bits(st[k]), _bits(ost[k])), k
     assert counters == o.counters()
+
+
+def test_grid_count_taken_by_integrate_follows_uploads_and_density_sweeps():
+    """integrate takes the next grid build's count from the positions it
+    writes; an upload between substeps, or a density-only sweep, makes the
+    next build count again. Each path is held bit for bit to the oracle,
+    including the grid of the last build."""
+    sc = _scene_small(seed=21)
+    step, refine = _step(sc)
+    moved = sc.pos4.copy()
+    moved[:, 1] += np.float32(0.003)  # every particle shifted, many across a cell face
+    with fa.Sim(sc.params) as s:
+        s.upload_state(sc.pos4)
+        s.substeps(step, 2, refine)        # the second build takes the first integrate's count
+        g2 = s.state()
+        s.upload_state(moved)              # the count is stale: the next build counts again
+        s.substeps(step, 1, refine)
+        g3 = s.state()
+        grid3 = s.grid()
+        s.density_only(step)               # takes integrate's count, leaves none
+        s.substeps(step, 1, refine)
+        g4 = s.state()
+        grid4 = s.grid()
+    o = OracleSim(sc.params)
+    o.set_state(sc.pos4)
+    o.substep(step, refine)
+    o.substep(step, refine)
+    o2 = o.state()
+    o.set_state(moved)
+    o.substep(step, refine)
+    o3, og3 = o.state(), o.grid()
+    o.substep(step, refine)  # density_only then a substep: the substep rebuilds from the same positions
+    o4, og4 = o.state(), o.grid()
+    for got, want, what in ((g2, o2, "two substeps"), (g3, o3, "after the upload"), (g4, o4, "after density_only")):
+        for k in STATE:
+            assert np.array_equal(_bits(got[k]), _bits(want[k])), (what, k)
+    for got, want in ((grid3, og3), (grid4, og4)):
+        for k in ("cellid", "counts", "starts", "perm"):
+            assert np.array_equal(got[k], want[k]), k

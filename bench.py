@@ -32,15 +32,23 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Canonical algorithmic bytes per particle of each kernel class (per launch).
+# The plain step's integrate also takes the next grid build's count (the cell
+# and provisional rank it writes, 8 bytes a particle, from the positions it
+# already holds), so that step runs no grid_count; a batch counts in
+# grid_count (24 bytes: the position read, cell and rank written) and its
+# integrate moves 120.
 BYTES_PER_PARTICLE = {
     "grid_count": 24, "grid_scatter": 124, "density_div": 64, "div_apply": 68, "forces": 124,
-    "den_kappa": 68, "den_apply": 68, "integrate": 120,
+    "den_kappa": 68, "den_apply": 68, "integrate": 128,
 }
 BYTES_PER_CELL = {"grid_scan": 16}
+INTEGRATE_UNFUSED = 120
 
 
-def canonical_bytes(n, cells, refine):
-    return n * (524 + 136 * refine) + 16 * cells
+def canonical_bytes(n, cells, refine, count_fused=True):
+    """A substep's canonical bytes: N (508 + 136 R) + 16 C with the count in
+    integrate (the plain step), N (524 + 136 R) + 16 C with grid_count."""
+    return n * ((508 if count_fused else 524) + 136 * refine) + 16 * cells
 
 
 def kernel_bytes(name, n, cells):
@@ -485,7 +493,7 @@ def batched_ensemble(fa, scenes=256, steps=20, warmup=3, plain_rounds=10, seed=0
     tol_batched = total * steps / t_tol
     tol_plain = total * tol_plain_rounds / t_tol_plain
 
-    canon = sum(canonical_bytes(sc.n, sc.grid.cell_count(), sd[2]) for sc, sd in zip(ens, sched))
+    canon = sum(canonical_bytes(sc.n, sc.grid.cell_count(), sd[2], count_fused=False) for sc, sd in zip(ens, sched))
     return {
         "metric": "DFSPH particle-steps/s, batched ensemble",
         "value": batched, "unit": "particle-substeps/s",

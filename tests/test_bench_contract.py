@@ -1,6 +1,8 @@
 """bench.py's accounting, without a GPU: the per-kernel canonical bytes add
-up to the substep's B(N, C, R) = N (524 + 136 R) + 16 C; the committed
-counter summary names every kernel class; more than one GPU is refused."""
+up to the plain substep's B(N, C, R) = N (508 + 136 R) + 16 C (integrate takes
+the grid count), and to N (524 + 136 R) + 16 C with grid_count and integrate's
+own 120 bytes (a batch); the committed counter summary names every kernel
+class; more than one GPU is refused."""
 import os
 import subprocess
 import sys
@@ -11,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402
 
-SUBSTEP_ONCE = ("grid_count", "grid_scatter", "density_div", "div_apply", "forces", "integrate")
+SUBSTEP_ONCE = ("grid_scatter", "density_div", "div_apply", "forces", "integrate")
 PER_PASS = ("den_kappa", "den_apply")
 
 
@@ -22,6 +24,9 @@ def test_kernel_bytes_add_up_to_the_substep(refine):
     total += refine * sum(bench.kernel_bytes(k, n, cells) for k in PER_PASS)
     total += bench.kernel_bytes("grid_scan", n, cells)
     assert total == bench.canonical_bytes(n, cells, refine)
+    unfused = total - bench.kernel_bytes("integrate", n, cells) + bench.INTEGRATE_UNFUSED * n
+    unfused += bench.kernel_bytes("grid_count", n, cells)
+    assert unfused == bench.canonical_bytes(n, cells, refine, count_fused=False)
 
 
 def test_the_counter_summary_covers_every_kernel_class():

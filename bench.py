@@ -173,7 +173,9 @@ def pmc_summary(base, workload="block_1m", steps=0, warmup=0):
         tot[cls] = tot.get(cls, 0.0) + (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3
         occ[cls] = occ.get(cls, 0) + (1 if primary else 0)
     print("\nPer-kernel roofline, from the kernel trace of the same command: average launch time (a class's"
-          " launches summed, place + gather as grid_scatter, tile_regions with density_div), algorithmic"
+          " launches summed, place + gather as grid_scatter, tile_regions with density_div where it runs: the"
+          " plain step's gather writes the tile regions, and its integrate takes the next grid count, so"
+          " grid_count runs only in the first substep), algorithmic"
           f" bytes over that time, and its fraction of MI355X's {HBM_PEAK_GBS / 1000:.0f} TB/s.\n")
     print("| kernel | launches | avg µs | algorithmic GB/s | frac of HBM peak | measured traffic GB/s |")
     print("|---|---|---|---|---|---|")

@@ -30,7 +30,7 @@ namespace fluid {
 
 namespace {
 
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 8;  // 4, 16 and 32 measured slower at 358K cells
 constexpr uint32_t kScanTile = kBlock * kScanItems;
 constexpr uint64_t kFlagAggregate = 1ull << 32;
 constexpr uint64_t kFlagInclusive = 2ull << 32;

@@ -222,19 +222,7 @@ struct NoEarly {
 
 
 __device__ __forceinline__ f3 listed_grad(f3 pi, f3 pj, float g) { return scl(g, sub(pi, pj)); }
-#ifndef FLUID_RECOMPUTE_G
-#define FLUID_RECOMPUTE_G 0
-#endif
-// dW/r of a pair at x = x_i - x_j, as density_div computes and lists it.
-__device__ __forceinline__ float pair_g(const SceneK& k, f3 x) {
-    const float r = sqrt_rn(dot3(x, x));
-    const float q = r * k.inv_h;
-    return r < 1e-9f ? 0.0f : div_rn(grad_dw(k, q), r);
-}
-__device__ __forceinline__ f3 sweep_grad(const SceneK& k, f3 pi, f3 pj, float g) {
-    const f3 x = sub(pi, pj);
-    return scl(FLUID_RECOMPUTE_G ? pair_g(k, x) : g, x);
-}
+
 
 struct LaneId {
     uint32_t tile, p, lane, gid;
@@ -824,7 +812,7 @@ __device__ __forceinline__ f3 kappa_partial(const SceneK& k, const SolveBufs& b,
             const float4 pj = pos[j];
             q = make_float4(pj.x, pj.y, pj.z, b.kd[j]);
         }
-        dv = add(dv, scl(kdi + q.w, sweep_grad(k, pi, xyz(q), g)));
+        dv = add(dv, scl(kdi + q.w, listed_grad(pi, xyz(q), g)));
     });
     return dv;
 }
@@ -1147,7 +1135,7 @@ __device__ __forceinline__ float den_kappa_sweep(const SceneK& k, const LaneId& 
         for_list<STAGED, true>(b, id, h, [&](uint32_t j, float g) {
             f3 pj, vj;
             load_pv<STAGED>(L, pos, vel, j, pj, vj);
-            drho += dot3(sub(vii, vj), sweep_grad(k, pi, pj, g));
+            drho += dot3(sub(vii, vj), listed_grad(pi, pj, g));
         });
     }
     return fold4(drho);

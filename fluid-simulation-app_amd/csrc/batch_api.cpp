@@ -85,6 +85,8 @@ struct FluidBatch {
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
     Tolerance tol{0.0f, 1, 0, 0.0f, 1, true};
+    // Tolerance mode's early stop on work lists (pinned words, events).
+    PassStop stop{};
     Physics phys{};
     EventTimer timer;
     DeviceAllocs mem;
@@ -113,6 +115,9 @@ struct FluidBatch {
             if (r.done) (void)hipEventDestroy(r.done);
         }
         for (uint32_t i = 0; i < created; ++i) (void)hipStreamSynchronize(extra[i]);
+        for (hipEvent_t e : stop.ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stop.host) (void)hipHostFree(stop.host);
         mem.release();
         if (fork) (void)hipEventDestroy(fork);
         for (uint32_t i = 0; i < created; ++i) {
@@ -240,6 +245,8 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     if (const char* env = getenv("FLUID_TOL_PERSISTENT")) B->tol.persistent = strtol(env, nullptr, 10) != 0;
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    HIP_TRY(hipHostMalloc((void**)&B->stop.host, kStopRing * sizeof(uint32_t), hipHostMallocCoherent));
+    for (hipEvent_t& e : B->stop.ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     int rc = apply_layout(B, solve_variant(kSolveTile), B->group_slots, B->nstreams);
     if (rc) return rc;
     const size_t n = B->slots, c = B->cells, S = scenes;
@@ -280,8 +287,8 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     B->grid.nonfinite = b.counters;
     A(b.clamps, (size_t)kClampWords * S);
     A(b.zeros, kZeroRows);
-    A(b.residual, S);
-    A(b.div_residual, S);
+    A(b.residual, (size_t)S * kResidWords);
+    A(b.div_residual, (size_t)S * kResidWords);
     A(b.iterations, S);
     A(b.div_iterations, S);
     A(b.pass_hist, 2 * kHistBins);
@@ -316,6 +323,11 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     return FLUID_OK;
 }
 
+// Whether a substep issues its launches directly, not from a graph: tolerance
+// mode on work lists, which stops issuing passes once every scene has
+// converged (PassStop) and so waits on the device between passes.
+bool stops_early(const FluidBatch* B) { return B->tol.on() && B->tol.persistent; }
+
 // The launches of one substep (the copy of the Step table included), on the
 // batch stream; captured into a graph or issued directly.
 void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
@@ -323,8 +335,10 @@ void issue_substep(FluidBatch* B, const Ring& r, uint32_t passes) {
     const SceneTable sc = B->table(r);
     launch_grid_build(B->stream, sc, B->slots, B->cells, B->rest, B->work, B->grid, B->t());
     if (B->tol.on()) {
+        Tolerance tol = B->tol;
+        tol.stop = stops_early(B) ? &B->stop : nullptr;
         B->V->tolerance_table(B->stream, sc, B->scenes, B->slots, B->grid.starts, B->work, B->rest, B->solve,
-                              B->tol, passes, B->d_pass_limit, B->t());
+                              tol, passes, B->d_pass_limit, B->t());
     } else {
         TileGroups g;
         if (!B->groups.empty()) {
@@ -361,7 +375,7 @@ int run_substep(FluidBatch* B, const uint8_t* steps144, const uint32_t* refine) 
     }
     if (active == 0) return FLUID_OK;
     if (B->tol.tol > 0.0f) passes = B->tol.max_passes;
-    if (B->use_graph && !B->t()) {
+    if (B->use_graph && !B->t() && !stops_early(B)) {
         const uint64_t key = ((uint64_t)passes << 2) | (B->tol.tol > 0.0f ? 2u : 0u) | (B->tol.div_tol > 0.0f ? 1u : 0u);
         auto it = r.graphs.find(key);
         if (it == r.graphs.end()) {

@@ -311,7 +311,7 @@ struct SolveBufs {
     // Tolerance mode: per scene, the fixed-point (2^-32) sums of the current
     // pass's density errors max(rho* - rho0, 0) / rho0 and divergence errors
     // max(Drho/Dt, 0) dt / rho0, and the passes of each solve this substep.
-    // Null in the reference schedule.
+    // Null in the reference schedule. kResidWords each a scene.
     unsigned long long* residual;
     unsigned long long* div_residual;
     uint32_t* iterations;
@@ -337,6 +337,33 @@ constexpr uint32_t kHistBins = kMaxPasses + 1;
 constexpr uint32_t kClampSlots = 16;
 constexpr uint32_t kClampStride = 32;
 constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
+
+// Tolerance mode's per-scene error sums (SolveBufs::residual,
+// div_residual), each as kResidSlots 64-bit words kResidStride words (one
+// cache line) apart, a workgroup adding into the word of its tile: one
+// word took every workgroup's atomic in turn, 37 us a den_kappa launch at
+// 1M particles. Integer sums, so the total is the same in any order.
+constexpr uint32_t kResidSlots = 32;
+constexpr uint32_t kResidStride = 16;
+constexpr uint32_t kResidWords = kResidSlots * kResidStride;  // per scene
+__device__ __forceinline__ unsigned long long* resid_word(unsigned long long* base, uint32_t scene,
+                                                          uint32_t tile) {
+    return base + (size_t)scene * kResidWords + (tile % kResidSlots) * kResidStride;
+}
+// The scene's sum, its words cleared for the next pass.
+__device__ __forceinline__ unsigned long long take_resid(unsigned long long* base, uint32_t scene) {
+    unsigned long long* w = base + (size_t)scene * kResidWords;
+    unsigned long long t = 0;
+    for (uint32_t i = 0; i < kResidSlots; ++i) {
+        t += w[i * kResidStride];
+        w[i * kResidStride] = 0;
+    }
+    return t;
+}
+__device__ __forceinline__ void clear_resid(unsigned long long* base, uint32_t scene) {
+    unsigned long long* w = base + (size_t)scene * kResidWords;
+    for (uint32_t i = 0; i < kResidSlots; ++i) w[i * kResidStride] = 0;
+}
 
 // List lanes a particle in the sweeps (entry m belongs to lane m % 4).
 constexpr uint32_t kListLanes = 4;
@@ -493,12 +520,29 @@ struct StageBufs {
 // (`persistent`): a fixed grid of workgroups loops over the tiles of the
 // scenes still iterating, so a pass no scene needs costs a few microseconds
 // instead of a dispatch of every tile.
+//
+// A solve issued directly (not captured into a graph) can also stop issuing
+// passes once it has converged (`stop`): each pass's convergence check
+// writes to pinned host memory, with an event behind it, the scene's pass
+// limit (one scene) or the number of scenes listed for the next pass (work
+// lists), and before issuing pass r the host waits for the check of pass
+// r - kStopLead and issues nothing more when that limit is at most r (the
+// count is 0). The passes it leaves out are those the limits would turn into
+// no-ops, so the results are the same bits; kStopLead passes stay queued
+// ahead of the host's wait, so the device never waits for the host. The
+// full-grid form stops a single scene only.
+constexpr uint32_t kStopLead = 4, kStopRing = 8;
+struct PassStop {
+    uint32_t* host;  // pinned (coherent), kStopRing words the checks write
+    hipEvent_t ev[kStopRing];
+};
 struct Tolerance {
     float tol;
     uint32_t min_passes, max_passes;
     float div_tol;
     uint32_t div_max;
     bool persistent;
+    const PassStop* stop = nullptr;
     bool on() const { return tol > 0.0f || div_tol > 0.0f; }
 };
 

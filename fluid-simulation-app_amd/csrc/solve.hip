@@ -775,7 +775,7 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
         density_tile<true, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
     else
         density_tile<false, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
-    if (TOL) block_residual(err, b.div_residual + sc.index_at(R.scene));
+    if (TOL) block_residual(err, resid_word(b.div_residual, sc.index_at(R.scene), tile));
 }
 
 // ---- list sweeps -------------------------------------------------------------
@@ -1176,7 +1176,7 @@ __device__ __forceinline__ void den_kappa_on(const Sc& sc, uint32_t tile, uint32
             if (sum) err = fixed_err(excess / k.rho0);
         }
     }
-    if (sum) block_residual(err, sum + sc.index_at(R.scene));
+    if (sum) block_residual(err, resid_word(sum, sc.index_at(R.scene), tile));
 }
 template <class Sc, bool DIV = false>
 __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
@@ -1266,19 +1266,23 @@ __global__ __launch_bounds__(kTile) void k_integrate(Sc sc, Records work, Record
 // solve (`residual`, `limit`, `iterations` that solve's words): the mean
 // error of the pass; a scene at or below tol (after at least min_passes)
 // runs this pass's apply and no further pass.
+// `host` (a single scene's early stop, else null): the scene's limit after
+// the check, written to pinned host memory.
 template <class Sc>
 __global__ __launch_bounds__(kBlock) void k_converge(Sc sc, uint32_t scenes, uint32_t pass, float tol,
                                                      uint32_t min_passes, unsigned long long* __restrict__ residual,
                                                      uint32_t* __restrict__ limit,
-                                                     uint32_t* __restrict__ iterations) {
+                                                     uint32_t* __restrict__ iterations, uint32_t* host) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= scenes || sc.step_at(i).active == 0u || pass >= limit[i]) return;
-    const SceneK& k = sc.scene_at(i);
-    const uint32_t n = k.slot_end - k.slot_begin;
-    const double mean = (double)residual[i] * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
-    residual[i] = 0;
-    iterations[i] = pass + 1;
-    if (mean <= (double)tol && pass + 1 >= min_passes) limit[i] = pass + 1;
+    if (i >= scenes) return;
+    if (sc.step_at(i).active != 0u && pass < limit[i]) {
+        const SceneK& k = sc.scene_at(i);
+        const uint32_t n = k.slot_end - k.slot_begin;
+        const double mean = (double)take_resid(residual, i) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+        iterations[i] = pass + 1;
+        if (mean <= (double)tol && pass + 1 >= min_passes) limit[i] = pass + 1;
+    }
+    if (host) *host = limit[i];
 }
 
 // Tolerance mode, before a substep's solve: clears the residuals and sets
@@ -1289,8 +1293,8 @@ __global__ __launch_bounds__(kBlock) void k_reset_passes(Sc sc, uint32_t scenes,
                                                          SolveBufs b, uint32_t* __restrict__ limits) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= scenes) return;
-    b.residual[i] = 0;
-    b.div_residual[i] = 0;
+    clear_resid(b.residual, i);
+    clear_resid(b.div_residual, i);
     limits[i] = den_max ? den_max : sc.step_at(i).refine;
     limits[scenes + i] = div_max;
     b.iterations[i] = den_max ? 0u : sc.step_at(i).refine;
@@ -1324,7 +1328,7 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
                                                         float tol, uint32_t min_passes,
                                                         unsigned long long* __restrict__ residual,
                                                         uint32_t* __restrict__ limit, uint32_t* __restrict__ iterations,
-                                                        uint32_t next, uint32_t* __restrict__ list) {
+                                                        uint32_t next, uint32_t* __restrict__ list, uint32_t* host) {
     __shared__ uint32_t s_tiles[16], s_scenes[16];
     uint32_t* const first = list + 2;
     uint32_t* const tile0 = list + 3 + scenes;
@@ -1338,14 +1342,14 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
             uint32_t lim = limit[i];
             if (check && pass < lim) {
                 const uint32_t n = k.slot_end - k.slot_begin;
-                const double mean = (double)residual[i] * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
-                residual[i] = 0;
+                const double mean = (double)take_resid(residual, i) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
                 iterations[i] = pass + 1;
                 if (mean <= (double)tol && pass + 1 >= min_passes) {
                     lim = pass + 1;
                     limit[i] = lim;
                 }
             }
+
             if (next < lim && k.slot_end > k.slot_begin) {
                 t0 = k.slot_begin / kTile;
                 cnt = (k.slot_end + kTile - 1u) / kTile - t0;
@@ -1379,6 +1383,7 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
         list[0] = tiles_carry;
         list[1] = scenes_carry;
         first[scenes_carry] = tiles_carry;
+        if (host) *host = scenes_carry;  // the early stop: 0 ends the solve
     }
 }
 
@@ -1512,6 +1517,30 @@ static void density_only(hipStream_t st, const Sc& sc, uint32_t n, const uint32_
 
 struct NoCheck {
     void operator()(hipStream_t, uint32_t, bool) const {}
+    bool stopped(uint32_t, bool) const { return false; }
+};
+
+// The early stop (PassStop) on stream `st`: the convergence check of pass
+// `pass` writes to slot(pass) either the scene's limit (one scene, k_converge)
+// or, with `counts`, the number of scenes listed for the next pass
+// (k_converge_list), and note() puts an event behind it; stopped() before
+// issuing pass `pass` waits for the check kStopLead passes back and says
+// whether the solve ended there.
+struct Stopper {
+    const PassStop* stop;
+    hipStream_t st;
+    bool counts;
+    // Where the check of pass `pass` writes the limit (null: no early stop).
+    uint32_t* slot(uint32_t pass) const { return stop ? stop->host + pass % kStopRing : nullptr; }
+    void note(uint32_t pass) const {
+        if (stop) (void)hipEventRecord(stop->ev[pass % kStopRing], st);
+    }
+    bool stopped(uint32_t pass) const {
+        if (!stop || pass < kStopLead) return false;
+        const uint32_t k = (pass - kStopLead) % kStopRing;
+        if (hipEventSynchronize(stop->ev[k]) != hipSuccess) return false;
+        return counts ? stop->host[k] == 0u : stop->host[k] <= pass;
+    }
 };
 // The substep's launches. With stage buffers (the adjoint's taped forward)
 // each stage's velocities and kd go to arrays of their own instead of being
@@ -1579,6 +1608,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
         // Tolerance mode's further divergence passes, in place on the
         // velocities div_apply left (timed with div_apply).
         for (uint32_t r = 1; r < div_passes; ++r) {
+            if (check.stopped(r, true)) break;
             const Span tk = span(timer, KERNEL_DIV_APPLY);
             hipExtLaunchKernelGGL(k_den_kappa<Sc, true>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0,
                                   fwd, kDivPass | r, work.pos, V(1), KB(0));
@@ -1592,6 +1622,7 @@ static void solve(hipStream_t st, const Sc& sc, uint32_t n, const uint32_t* star
                               work.pos, V(1), work.temperature, work.prev_pressure, V(2), rest.temperature, KB(1));
         // the forces sweep ran forward, so the passes start in reverse
         for (uint32_t r = 0; r < passes; ++r) {
+            if (check.stopped(r, false)) break;
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
             hipExtLaunchKernelGGL(k_den_kappa<Sc>, tiles, sweep, stage_slots(b) * kPvEntry, s, tk.a, tk.z, 0, rev, r,
                                   work.pos, V(2 + r), KB(2 + r));
@@ -1667,13 +1698,27 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
     const dim3 grid(blocks_for(scenes));
     hipLaunchKernelGGL(k_reset_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, den ? tol.max_passes : 0u,
                        div ? tol.div_max : 1u, b, limits);
-    auto check = [&](hipStream_t s2, uint32_t pass, bool divergence) {
-        if (divergence ? !div : !den) return;
-        hipLaunchKernelGGL(k_converge<Sc>, grid, dim3(kBlock), 0, s2, sc, scenes, pass,
-                           divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
-                           divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
-                           divergence ? b.div_iterations : b.iterations);
+    struct Check {
+        const Sc& sc;
+        uint32_t scenes;
+        const Tolerance& tol;
+        bool den, div;
+        const SolveBufs& b;
+        uint32_t* limits;
+        Stopper stop;
+        void operator()(hipStream_t s2, uint32_t pass, bool divergence) const {
+            if (divergence ? !div : !den) return;
+            hipLaunchKernelGGL(k_converge<Sc>, dim3(blocks_for(scenes)), dim3(kBlock), 0, s2, sc, scenes, pass,
+                               divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
+                               divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
+                               divergence ? b.div_iterations : b.iterations, stop.slot(pass));
+            stop.note(pass);
+        }
+        bool stopped(uint32_t pass, bool divergence) const {
+            return (divergence ? div : den) && stop.stopped(pass);
+        }
     };
+    const Check check{sc, scenes, tol, den, div, b, limits, Stopper{scenes == 1u ? tol.stop : nullptr, st, false}};
     solve(st, sc, slots, starts, work, rest, t, den ? tol.max_passes : passes, timer, check, nullptr, TileGroups(),
           div ? tol.div_max : 1u);
     hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
@@ -1709,12 +1754,17 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
         uint32_t* const lists[2] = {b.tile_lists, b.tile_lists + words};
         Sc rev = sc;
         rev.flip = 1;
+        // The early stop (PassStop): once no scene is listed.
+        const Stopper stop{tol.stop, st, true};
+        auto stopped = [&](uint32_t pass) { return stop.stopped(pass); };
         // converge pass `pass` of one solve (check) and list the scenes of pass `next`
         auto converge = [&](uint32_t pass, bool check, bool divergence, uint32_t next) {
             hipLaunchKernelGGL(k_converge_list<Sc>, dim3(1), dim3(1024), 0, st, sc, scenes, pass,
                                check ? 1u : 0u, divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
-                               divergence ? b.div_iterations : b.iterations, next, lists[next % 2u]);
+                               divergence ? b.div_iterations : b.iterations, next, lists[next % 2u],
+                               check ? stop.slot(pass) : nullptr);
+            if (check) stop.note(pass);
         };
         const Span tr = span(timer, KERNEL_DENSITY_DIV);
         const bool own = tile_regions(st, sc, ntiles, starts, work, b, tr.a);
@@ -1729,6 +1779,7 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
         hipExtLaunchKernelGGL(k_kappa_apply<Sc>, tiles, sweep, lds_kappa, st, td.a, td.z, 0, rev, kEveryPass,
                               work.pos, work.vel, work.vel, t);
         for (uint32_t r = 1; div && r < tol.div_max; ++r) {
+            if (stopped(r)) break;
             const Span tk = span(timer, KERNEL_DIV_APPLY);
             hipExtLaunchKernelGGL(k_den_kappa_list<Sc, true>, dim3(lg), sweep, lds_pv, st, tk.a, nullptr, 0, sc,
                                   kDivPass | r, work.pos, work.vel, t, lists[r % 2u], scenes);
@@ -1744,6 +1795,7 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
         if (den) converge(0, false, false, 0);
         const uint32_t count = den ? tol.max_passes : passes;
         for (uint32_t r = 0; r < count; ++r) {
+            if (den && stopped(r)) break;
             const Span tk = span(timer, KERNEL_DEN_KAPPA);
             const Span tp = span(timer, KERNEL_DEN_APPLY);
             if (den) {

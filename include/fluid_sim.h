@@ -127,7 +127,11 @@ int fluid_sim_substeps(FluidSim *sim, const uint8_t *step144, uint32_t n, uint32
  * iterates to `tol` (min_passes .. max_passes <= 256 passes; the substeps'
  * `refine` is then ignored) and the divergence solve to `div_tol` (at most
  * div_max_passes <= 256 passes). A tolerance <= 0 keeps that solve's
- * reference schedule; both <= 0 is the reference step.
+ * reference schedule; both <= 0 is the reference step. In this mode
+ * fluid_sim_substeps waits on the device between passes: it stops issuing a
+ * solve's passes once that solve has converged (a few passes stay queued
+ * ahead of the wait, and those past convergence do nothing), so it returns
+ * while the last passes and integrate are still running.
  */
 int fluid_sim_set_tolerance(FluidSim *sim, float tol, uint32_t min_passes, uint32_t max_passes, float div_tol,
                             uint32_t div_max_passes);
@@ -402,7 +406,10 @@ int fluid_batch_set_div_tolerance(FluidBatch *batch, float div_tol, uint32_t max
  * work lists (the default: a fixed grid of workgroups loops over the tiles
  * of the scenes still iterating, so a pass no scene needs costs a few
  * microseconds) or dispatches every tile each pass. The results are the
- * same bits. */
+ * same bits. On work lists a substep is issued directly, never from a
+ * graph: fluid_batch_substep waits on the device between passes and stops
+ * issuing a solve's passes once every scene has converged (a few passes
+ * stay queued ahead of the wait, and those past convergence do nothing). */
 int fluid_batch_persistent_passes(FluidBatch *batch, bool enable);
 /* Density passes each scene ran in the last substep under tolerance mode. */
 int fluid_batch_iterations(FluidBatch *batch, uint32_t *out);
@@ -413,7 +420,8 @@ int fluid_batch_div_iterations(FluidBatch *batch, uint32_t *out);
  * p density passes, out[257 + p] those that ran p divergence passes. `out`
  * may be NULL; `reset` clears them afterwards. */
 int fluid_batch_pass_histogram(FluidBatch *batch, uint32_t *out, bool reset);
-/* Capture each substep shape once in a hipGraph and replay it. */
+/* Capture each substep shape once in a hipGraph and replay it (except
+ * tolerance mode on work lists, issued directly to stop early). */
 int fluid_batch_use_graph(FluidBatch *batch, bool enable);
 /* Per scene, as fluid_sim_reduce_stats / fluid_sim_read_stats: out holds
  * scenes x 14 floats. */

@@ -517,9 +517,8 @@ struct StageBufs {
 // and div_limit are set from it, and b's residuals, iterations and pass
 // histograms are used.
 // A batch may run the passes of each solve under tolerance on work lists
-// (`persistent`): a fixed grid of workgroups loops over the tiles of the
-// scenes still iterating, so a pass no scene needs costs a few microseconds
-// instead of a dispatch of every tile.
+// (`persistent`): each pass's sweeps take only the tiles of the scenes still
+// iterating, the workgroups past their count returning at once.
 //
 // A solve issued directly (not captured into a graph) can also stop issuing
 // passes once it has converged (`stop`): each pass's convergence check

@@ -1313,10 +1313,10 @@ __global__ __launch_bounds__(kBlock) void k_tally_passes(Sc sc, uint32_t scenes,
     atomicAdd(&b.pass_hist[kHistBins + div], 1u);
 }
 
-// Tolerance mode of a batch, work lists: each pass's sweeps run on a fixed
-// grid of workgroups that loop over the tiles of the scenes still iterating,
-// listed by k_converge_list, so that a pass costs the tiles it has and a
-// pass no scene needs costs a few microseconds. A list (list_words(scenes)
+// Tolerance mode of a batch, work lists: each pass's sweeps take the tiles
+// of the scenes still iterating, listed by k_converge_list, one workgroup a
+// list entry (the workgroups past the list's length return at once), so that
+// a pass costs the tiles it has. A list (list_words(scenes)
 // words) holds the tile count, the listed scenes, the prefix of their tile
 // counts (one more entry) and each one's first tile.
 __host__ __device__ __forceinline__ uint32_t list_words(uint32_t scenes) { return 2u + (scenes + 1u) + scenes; }
@@ -1387,15 +1387,16 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
     }
 }
 
-// Entry k of workgroup b's share of a list of `total` tiles on a grid of g
-// workgroups (a multiple of the XCD count): each XCD takes a contiguous run
-// of the list, as xcd_tile does for a launch of one workgroup a tile, and
-// its workgroups interleave over the run; `flip` walks the list backwards.
-__device__ __forceinline__ bool list_entry(uint32_t b, uint32_t g, uint32_t k, uint32_t total, bool flip,
-                                           uint32_t& e) {
-    const uint32_t x = b % kXcds, w = b / kXcds, g8 = g / kXcds;
+// The list entry of workgroup b of a launch of g workgroups (one a tile, a
+// multiple of the XCD count) over a list of `total` tiles: each XCD takes a
+// contiguous run of the list, as xcd_tile does for a launch of every tile,
+// and the workgroups past the list's length return at once; `flip` walks the
+// list backwards.
+__device__ __forceinline__ bool list_entry(uint32_t b, uint32_t g, uint32_t total, bool flip, uint32_t& e) {
+    (void)g;
+    const uint32_t x = b % kXcds, w = b / kXcds;
     const uint32_t hi = (uint32_t)((uint64_t)total * (x + 1) / kXcds);
-    const uint32_t idx = (uint32_t)((uint64_t)total * x / kXcds) + w + k * g8;
+    const uint32_t idx = (uint32_t)((uint64_t)total * x / kXcds) + w;
     if (idx >= hi) return false;
     e = flip ? total - 1u - idx : idx;
     return true;
@@ -1416,31 +1417,23 @@ __device__ __forceinline__ uint32_t list_tile(const uint32_t* __restrict__ list,
 }
 
 template <class Sc, bool DIV>
-__global__ __launch_bounds__(kSweepThreads) void k_den_kappa_list(Sc sc, uint32_t pass,
-                                                                  const float4* __restrict__ pos,
-                                                                  const float4* __restrict__ vel, SolveBufs b,
-                                                                  const uint32_t* __restrict__ list, uint32_t scenes) {
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_den_kappa_list(
+    Sc sc, uint32_t pass, const float4* __restrict__ pos, const float4* __restrict__ vel, SolveBufs b,
+    const uint32_t* __restrict__ list, uint32_t scenes) {
     extern __shared__ float4 smem[];
-    const uint32_t total = list[0];
     uint32_t e = 0;
-    for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, false, e); ++k) {
-        den_kappa_on<Sc, DIV>(sc, list_tile(list, scenes, e), pass, pos, vel, b, smem);
-        __syncthreads();
-    }
+    if (!list_entry(blockIdx.x, gridDim.x, list[0], false, e)) return;
+    den_kappa_on<Sc, DIV>(sc, list_tile(list, scenes, e), pass, pos, vel, b, smem);
 }
 
 template <class Sc>
-__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_kappa_apply_list(Sc sc, uint32_t pass,
-                                                                    const float4* __restrict__ pos, float4* vel,
-                                                                    SolveBufs b, const uint32_t* __restrict__ list,
-                                                                    uint32_t scenes) {
+__global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_kappa_apply_list(
+    Sc sc, uint32_t pass, const float4* __restrict__ pos, float4* vel, SolveBufs b,
+    const uint32_t* __restrict__ list, uint32_t scenes) {
     extern __shared__ float4 smem[];
-    const uint32_t total = list[0];
     uint32_t e = 0;
-    for (uint32_t k = 0; list_entry(blockIdx.x, gridDim.x, k, total, true, e); ++k) {
-        kappa_apply_on<Sc>(sc, list_tile(list, scenes, e), pass, pos, vel, vel, b, smem);
-        __syncthreads();
-    }
+    if (!list_entry(blockIdx.x, gridDim.x, list[0], true, e)) return;
+    kappa_apply_on<Sc>(sc, list_tile(list, scenes, e), pass, pos, vel, vel, b, smem);
 }
 
 constexpr size_t kKappaEntry = sizeof(float4);                         // 16
@@ -1724,10 +1717,6 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
     hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
 }
 
-// Workgroups of the work-list sweeps: enough to fill the device a few times
-// over, a multiple of the XCD count.
-constexpr uint32_t kListGrid = 2048;
-
 // A substep solve in tolerance mode with each solve under tolerance run on
 // work lists (k_converge_list, k_den_kappa_list, k_kappa_apply_list): a pass
 // costs the tiles of the scenes still iterating. The launches around them
@@ -1748,7 +1737,8 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
     if (n > 0) {
         const uint32_t ntiles = solve_tiles(n);
         const dim3 tiles(ntiles), sweep(kSweepThreads);
-        const uint32_t lg = ntiles < kListGrid ? (ntiles + kXcds - 1) / kXcds * kXcds : kListGrid;
+        // one workgroup a tile, a multiple of the XCD count
+        const uint32_t lg = (ntiles + kXcds - 1) / kXcds * kXcds;
         const size_t lds_kappa = stage_slots(b) * kKappaEntry, lds_pv = stage_slots(b) * kPvEntry;
         const uint32_t words = list_words(scenes);
         uint32_t* const lists[2] = {b.tile_lists, b.tile_lists + words};

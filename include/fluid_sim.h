@@ -403,9 +403,8 @@ int fluid_batch_set_tolerance(FluidBatch *batch, float tol, uint32_t min_passes,
  * div_tol <= 0 restores the one pass. */
 int fluid_batch_set_div_tolerance(FluidBatch *batch, float div_tol, uint32_t max_passes);
 /* Whether tolerance mode runs the passes of each solve under tolerance on
- * work lists (the default: a fixed grid of workgroups loops over the tiles
- * of the scenes still iterating, so a pass no scene needs costs a few
- * microseconds) or dispatches every tile each pass. The results are the
+ * work lists (the default: each pass's sweeps take only the tiles of the
+ * scenes still iterating) or dispatches every tile each pass. The results are the
  * same bits. On work lists a substep is issued directly, never from a
  * graph: fluid_batch_substep waits on the device between passes and stops
  * issuing a solve's passes once every scene has converged (a few passes

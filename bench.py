@@ -238,6 +238,54 @@ def trace_summary(base, workload="block_1m", steps=0, warmup=0):
               f"{float(r['TotalDurationNs']) / 1e6:.2f} | {float(r['Percentage']):.1f} |")
 
 
+def tolerance_summary(base):
+    """Prints the kernel table of the tolerance leg from a rocprofv3
+    --kernel-trace run of the default bench under base/trace
+    (trace_kernel_trace.csv): the substeps that ran convergence checks, less
+    the leg's untimed first one, each from the end of the previous
+    substep's integrate to the end of its own."""
+    import csv
+    import re
+    f = os.path.join(base, "trace", "trace_kernel_trace.csv")
+    ev = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_name(r["Kernel_Name"]))
+                for r in csv.DictReader(open(f)))
+    subs, cur, t_prev = [], [], None
+    for e in ev:
+        cur.append(e)
+        if e[2].startswith("k_integrate"):
+            subs.append((t_prev, cur))
+            t_prev, cur = e[1], []
+    tol = [(t0, s) for t0, s in subs if t0 is not None and any(re.match(r"k_converge", e[2]) for e in s)][1:]
+    if not tol:
+        print("no tolerance-mode substeps in the trace")
+        return
+    n = len(tol)
+    wall = sum(s[-1][1] - t0 for t0, s in tol) / n / 1e3
+    acc = {}
+    for _, s in tol:
+        for a, z, k in s:
+            c = acc.setdefault(k, [0, 0.0])
+            c[0] += 1
+            c[1] += (z - a) / 1e3
+    busy = sum(v[1] for v in acc.values()) / n
+    cmd = "python3 -u bench.py --no-cpu --no-batched --no-adjoint --no-cfl"
+    print("# DFSPH substep at the configs' tolerances, block_1m, kernel trace\n")
+    print("Commands, from the repository root on the MI355X box:\n")
+    print("```")
+    print(f"rocprofv3 --kernel-trace --stats --output-format csv -d {base}/trace -o trace -- {cmd}")
+    print(f"python3 bench.py --tolerance-summary {base} > profiles/kernels_tolerance_1m.md")
+    print("```\n")
+    print(f"The bench's tolerance leg (divergence {TOL_DIVERGENCE:g}, density {TOL_DENSITY:g}, caps of {MAX_PASSES} "
+          f"passes) continues the developed 1,048,576-particle scene; {n} substeps after its untimed first one. "
+          f"A substep spans {wall:.0f} µs from the previous substep's integrate to its own, {busy:.0f} µs of it "
+          "in kernels. The host issues no pass once a check says the solve has converged, so the launches "
+          "are the passes the scene ran plus at most four no-op passes a solve.\n")
+    print("| kernel | launches a substep | avg µs | µs a substep | % of kernel time |")
+    print("|---|---|---|---|---|")
+    for k, (c, t) in sorted(acc.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {k} | {c / n:.1f} | {t / c:.1f} | {t / n:.0f} | {100 * t / n / busy:.1f} |")
+
+
 def sq_summary(base, workload="block_1m"):
     """Prints per-wave shader counters of two rocprofv3 --pmc runs under
     base/sq_A and base/sq_B, averaged over every launch of each kernel."""
@@ -646,6 +694,9 @@ def main():
                     help="print only the oracle's CPU timings of configs 1-3 (the CPU baseline plan)")
     ap.add_argument("--trace-summary", metavar="DIR", default="",
                     help="print the kernel table of a rocprofv3 --kernel-trace --stats run under DIR/trace and exit")
+    ap.add_argument("--tolerance-summary", metavar="DIR", default="",
+                    help="print the tolerance leg's kernel table of a rocprofv3 --kernel-trace run under DIR/trace "
+                         "and exit")
     ap.add_argument("--sq-summary", metavar="DIR", default="",
                     help="print the per-wave shader counters of the runs under DIR/sq_A, DIR/sq_B and exit")
     ap.add_argument("--pmc-summary", metavar="DIR", default="",
@@ -657,6 +708,9 @@ def main():
         return
     if args.sq_summary:
         sq_summary(args.sq_summary, args.workload)
+        return
+    if args.tolerance_summary:
+        tolerance_summary(args.tolerance_summary)
         return
     if args.pmc_summary:
         pmc_summary(args.pmc_summary, args.workload, args.steps if args.steps != 300 else 0,

@@ -525,12 +525,14 @@ struct StageBufs {
 // writes to pinned host memory, with an event behind it, the scene's pass
 // limit (one scene) or the number of scenes listed for the next pass (work
 // lists), and before issuing pass r the host waits for the check of pass
-// r - kStopLead and issues nothing more when that limit is at most r (the
+// r - lead and issues nothing more when that limit is at most r (the
 // count is 0). The passes it leaves out are those the limits would turn into
-// no-ops, so the results are the same bits; kStopLead passes stay queued
-// ahead of the host's wait, so the device never waits for the host. The
-// full-grid form stops a single scene only.
-constexpr uint32_t kStopLead = 4, kStopRing = 8;
+// no-ops, so the results are the same bits; `lead` passes stay queued ahead
+// of the host's wait, so the device never waits for the host: kStopLead, or
+// kStopLeadLarge for launches of at least kStopLargeTiles tiles, whose passes
+// outlast the host's wait and issue. The full-grid form stops a single
+// scene only.
+constexpr uint32_t kStopLead = 4, kStopLeadLarge = 2, kStopLargeTiles = 2048, kStopRing = 8;
 struct PassStop {
     uint32_t* host;  // pinned (coherent), kStopRing words the checks write
     hipEvent_t ev[kStopRing];

@@ -1517,20 +1517,22 @@ struct NoCheck {
 // `pass` writes to slot(pass) either the scene's limit (one scene, k_converge)
 // or, with `counts`, the number of scenes listed for the next pass
 // (k_converge_list), and note() puts an event behind it; stopped() before
-// issuing pass `pass` waits for the check kStopLead passes back and says
+// issuing pass `pass` waits for the check `lead` passes back and says
 // whether the solve ended there.
+inline uint32_t stop_lead(uint32_t tiles) { return tiles >= kStopLargeTiles ? kStopLeadLarge : kStopLead; }
 struct Stopper {
     const PassStop* stop;
     hipStream_t st;
     bool counts;
+    uint32_t lead;
     // Where the check of pass `pass` writes the limit (null: no early stop).
     uint32_t* slot(uint32_t pass) const { return stop ? stop->host + pass % kStopRing : nullptr; }
     void note(uint32_t pass) const {
         if (stop) (void)hipEventRecord(stop->ev[pass % kStopRing], st);
     }
     bool stopped(uint32_t pass) const {
-        if (!stop || pass < kStopLead) return false;
-        const uint32_t k = (pass - kStopLead) % kStopRing;
+        if (!stop || pass < lead) return false;
+        const uint32_t k = (pass - lead) % kStopRing;
         if (hipEventSynchronize(stop->ev[k]) != hipSuccess) return false;
         return counts ? stop->host[k] == 0u : stop->host[k] <= pass;
     }
@@ -1711,7 +1713,7 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
             return (divergence ? div : den) && stop.stopped(pass);
         }
     };
-    const Check check{sc, scenes, tol, den, div, b, limits, Stopper{scenes == 1u ? tol.stop : nullptr, st, false}};
+    const Check check{sc, scenes, tol, den, div, b, limits, Stopper{scenes == 1u ? tol.stop : nullptr, st, false, stop_lead(solve_tiles(slots))}};
     solve(st, sc, slots, starts, work, rest, t, den ? tol.max_passes : passes, timer, check, nullptr, TileGroups(),
           div ? tol.div_max : 1u);
     hipLaunchKernelGGL(k_tally_passes<Sc>, grid, dim3(kBlock), 0, st, sc, scenes, b);
@@ -1745,7 +1747,7 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
         Sc rev = sc;
         rev.flip = 1;
         // The early stop (PassStop): once no scene is listed.
-        const Stopper stop{tol.stop, st, true};
+        const Stopper stop{tol.stop, st, true, stop_lead(ntiles)};
         auto stopped = [&](uint32_t pass) { return stop.stopped(pass); };
         // converge pass `pass` of one solve (check) and list the scenes of pass `next`
         auto converge = [&](uint32_t pass, bool check, bool divergence, uint32_t next) {

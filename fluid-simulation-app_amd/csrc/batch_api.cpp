@@ -288,6 +288,7 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     A(b.clamps, (size_t)kClampWords * S);
     A(b.zeros, kZeroRows);
     A(b.residual, (size_t)S * kResidWords);
+    b.resid_slots = resid_slots(S);
     A(b.div_residual, (size_t)S * kResidWords);
     A(b.iterations, S);
     A(b.div_iterations, S);

@@ -311,7 +311,8 @@ struct SolveBufs {
     // Tolerance mode: per scene, the fixed-point (2^-32) sums of the current
     // pass's density errors max(rho* - rho0, 0) / rho0 and divergence errors
     // max(Drho/Dt, 0) dt / rho0, and the passes of each solve this substep.
-    // Null in the reference schedule. kResidWords each a scene.
+    // Null in the reference schedule. kResidWords each a scene, of which
+    // resid_slots (below) are used.
     unsigned long long* residual;
     unsigned long long* div_residual;
     uint32_t* iterations;
@@ -327,6 +328,8 @@ struct SolveBufs {
     // The grid build's gather wrote this substep's region records (the
     // solve skips k_tile_regions).
     bool regions_ready = false;
+    // The error words a scene uses, a power of two (resid_slots()).
+    uint32_t resid_slots = 1;
 };
 
 constexpr uint32_t kZeroRows = 4096;
@@ -339,30 +342,44 @@ constexpr uint32_t kClampStride = 32;
 constexpr uint32_t kClampWords = kClampSlots * kClampStride;  // per scene
 
 // Tolerance mode's per-scene error sums (SolveBufs::residual,
-// div_residual), each as kResidSlots 64-bit words kResidStride words (one
-// cache line) apart, a workgroup adding into the word of its tile: one
+// div_residual), each as up to kResidSlots 64-bit words kResidStride words
+// (one cache line) apart, a workgroup adding into the word of its tile: one
 // word took every workgroup's atomic in turn, 37 us a den_kappa launch at
-// 1M particles. Integer sums, so the total is the same in any order.
+// 1M particles. A context uses `slots` of them a scene (resid_slots): all
+// for one scene, fewer for many, whose tiles spread over the scenes' words
+// already and whose checks read every scene's words. Integer sums, so the
+// total is the same in any order.
 constexpr uint32_t kResidSlots = 32;
 constexpr uint32_t kResidStride = 16;
 constexpr uint32_t kResidWords = kResidSlots * kResidStride;  // per scene
-__device__ __forceinline__ unsigned long long* resid_word(unsigned long long* base, uint32_t scene,
-                                                          uint32_t tile) {
-    return base + (size_t)scene * kResidWords + (tile % kResidSlots) * kResidStride;
+inline uint32_t resid_slots(uint32_t scenes) {
+    uint32_t s = kResidSlots;
+    while (s > 1u && s * scenes > kResidSlots) s /= 2u;
+    return s;
 }
-// The scene's sum, its words cleared for the next pass.
-__device__ __forceinline__ unsigned long long take_resid(unsigned long long* base, uint32_t scene) {
+__device__ __forceinline__ unsigned long long* resid_word(unsigned long long* base, uint32_t scene, uint32_t tile,
+                                                          uint32_t slots) {
+    return base + (size_t)scene * kResidWords + (tile & (slots - 1u)) * kResidStride;
+}
+// The scene's sum, its words cleared for the next pass. Every load is
+// issued before any store (a load issued after a store waits for it).
+__device__ __forceinline__ unsigned long long take_resid(unsigned long long* base, uint32_t scene,
+                                                         uint32_t slots) {
     unsigned long long* w = base + (size_t)scene * kResidWords;
+    unsigned long long v[kResidSlots];
+#pragma unroll
+    for (uint32_t i = 0; i < kResidSlots; ++i) v[i] = i < slots ? w[i * kResidStride] : 0ull;
     unsigned long long t = 0;
+#pragma unroll
     for (uint32_t i = 0; i < kResidSlots; ++i) {
-        t += w[i * kResidStride];
-        w[i * kResidStride] = 0;
+        t += v[i];
+        if (i < slots) w[i * kResidStride] = 0;
     }
     return t;
 }
-__device__ __forceinline__ void clear_resid(unsigned long long* base, uint32_t scene) {
+__device__ __forceinline__ void clear_resid(unsigned long long* base, uint32_t scene, uint32_t slots) {
     unsigned long long* w = base + (size_t)scene * kResidWords;
-    for (uint32_t i = 0; i < kResidSlots; ++i) w[i * kResidStride] = 0;
+    for (uint32_t i = 0; i < slots; ++i) w[i * kResidStride] = 0;
 }
 
 // List lanes a particle in the sweeps (entry m belongs to lane m % 4).

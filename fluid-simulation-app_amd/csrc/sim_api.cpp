@@ -240,6 +240,7 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.clamps, kClampWords);
     A(b.zeros, kZeroRows);
     A(b.residual, kResidWords);
+    b.resid_slots = resid_slots(1);
     A(b.div_residual, kResidWords);
     A(b.iterations, 1);
     A(b.div_iterations, 1);

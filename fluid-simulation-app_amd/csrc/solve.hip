@@ -775,7 +775,7 @@ __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(6
         density_tile<true, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
     else
         density_tile<false, TOL>(k, s, counters, id, R, starts, pos, vel, b, L, err);
-    if (TOL) block_residual(err, resid_word(b.div_residual, sc.index_at(R.scene), tile));
+    if (TOL) block_residual(err, resid_word(b.div_residual, sc.index_at(R.scene), tile, b.resid_slots));
 }
 
 // ---- list sweeps -------------------------------------------------------------
@@ -1176,7 +1176,7 @@ __device__ __forceinline__ void den_kappa_on(const Sc& sc, uint32_t tile, uint32
             if (sum) err = fixed_err(excess / k.rho0);
         }
     }
-    if (sum) block_residual(err, resid_word(sum, sc.index_at(R.scene), tile));
+    if (sum) block_residual(err, resid_word(sum, sc.index_at(R.scene), tile, b.resid_slots));
 }
 template <class Sc, bool DIV = false>
 __global__ __launch_bounds__(kSweepThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_den_kappa(Sc sc, uint32_t pass, const float4* __restrict__ pos,
@@ -1272,13 +1272,14 @@ template <class Sc>
 __global__ __launch_bounds__(kBlock) void k_converge(Sc sc, uint32_t scenes, uint32_t pass, float tol,
                                                      uint32_t min_passes, unsigned long long* __restrict__ residual,
                                                      uint32_t* __restrict__ limit,
-                                                     uint32_t* __restrict__ iterations, uint32_t* host) {
+                                                     uint32_t* __restrict__ iterations, uint32_t* host,
+                                                     uint32_t slots) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= scenes) return;
     if (sc.step_at(i).active != 0u && pass < limit[i]) {
         const SceneK& k = sc.scene_at(i);
         const uint32_t n = k.slot_end - k.slot_begin;
-        const double mean = (double)take_resid(residual, i) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+        const double mean = (double)take_resid(residual, i, slots) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
         iterations[i] = pass + 1;
         if (mean <= (double)tol && pass + 1 >= min_passes) limit[i] = pass + 1;
     }
@@ -1293,8 +1294,8 @@ __global__ __launch_bounds__(kBlock) void k_reset_passes(Sc sc, uint32_t scenes,
                                                          SolveBufs b, uint32_t* __restrict__ limits) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= scenes) return;
-    clear_resid(b.residual, i);
-    clear_resid(b.div_residual, i);
+    clear_resid(b.residual, i, b.resid_slots);
+    clear_resid(b.div_residual, i, b.resid_slots);
     limits[i] = den_max ? den_max : sc.step_at(i).refine;
     limits[scenes + i] = div_max;
     b.iterations[i] = den_max ? 0u : sc.step_at(i).refine;
@@ -1328,7 +1329,8 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
                                                         float tol, uint32_t min_passes,
                                                         unsigned long long* __restrict__ residual,
                                                         uint32_t* __restrict__ limit, uint32_t* __restrict__ iterations,
-                                                        uint32_t next, uint32_t* __restrict__ list, uint32_t* host) {
+                                                        uint32_t next, uint32_t* __restrict__ list, uint32_t* host,
+                                                        uint32_t slots) {
     __shared__ uint32_t s_tiles[16], s_scenes[16];
     uint32_t* const first = list + 2;
     uint32_t* const tile0 = list + 3 + scenes;
@@ -1342,7 +1344,7 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
             uint32_t lim = limit[i];
             if (check && pass < lim) {
                 const uint32_t n = k.slot_end - k.slot_begin;
-                const double mean = (double)take_resid(residual, i) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
+                const double mean = (double)take_resid(residual, i, slots) * (1.0 / 4294967296.0) / (double)(n ? n : 1u);
                 iterations[i] = pass + 1;
                 if (mean <= (double)tol && pass + 1 >= min_passes) {
                     lim = pass + 1;
@@ -1706,7 +1708,7 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
             hipLaunchKernelGGL(k_converge<Sc>, dim3(blocks_for(scenes)), dim3(kBlock), 0, s2, sc, scenes, pass,
                                divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
-                               divergence ? b.div_iterations : b.iterations, stop.slot(pass));
+                               divergence ? b.div_iterations : b.iterations, stop.slot(pass), b.resid_slots);
             stop.note(pass);
         }
         bool stopped(uint32_t pass, bool divergence) const {
@@ -1755,7 +1757,7 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
                                check ? 1u : 0u, divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
                                divergence ? b.div_iterations : b.iterations, next, lists[next % 2u],
-                               check ? stop.slot(pass) : nullptr);
+                               check ? stop.slot(pass) : nullptr, b.resid_slots);
             if (check) stop.note(pass);
         };
         const Span tr = span(timer, KERNEL_DENSITY_DIV);

@@ -85,7 +85,7 @@ struct FluidBatch {
     uint32_t stage_cap_max = 0;
     bool use_graph = false;
     Tolerance tol{0.0f, 1, 0, 0.0f, 1, true};
-    // Tolerance mode's early stop on work lists (pinned words, events).
+    // Tolerance mode's early stop on work lists (pinned slots).
     PassStop stop{};
     Physics phys{};
     EventTimer timer;
@@ -115,8 +115,6 @@ struct FluidBatch {
             if (r.done) (void)hipEventDestroy(r.done);
         }
         for (uint32_t i = 0; i < created; ++i) (void)hipStreamSynchronize(extra[i]);
-        for (hipEvent_t e : stop.ev)
-            if (e) (void)hipEventDestroy(e);
         if (stop.host) (void)hipHostFree(stop.host);
         mem.release();
         if (fork) (void)hipEventDestroy(fork);
@@ -245,8 +243,8 @@ int init_batch(FluidBatch* B, uint32_t scenes, const uint8_t* params, uint32_t n
     if (const char* env = getenv("FLUID_TOL_PERSISTENT")) B->tol.persistent = strtol(env, nullptr, 10) != 0;
     HIP_TRY(hipGetDevice(&B->device));
     HIP_TRY(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    HIP_TRY(hipHostMalloc((void**)&B->stop.host, kStopRing * sizeof(uint32_t), hipHostMallocCoherent));
-    for (hipEvent_t& e : B->stop.ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc((void**)&B->stop.host, kStopRing * sizeof(unsigned long long), hipHostMallocCoherent));
+    std::memset(B->stop.host, 0, kStopRing * sizeof(unsigned long long));
     int rc = apply_layout(B, solve_variant(kSolveTile), B->group_slots, B->nstreams);
     if (rc) return rc;
     const size_t n = B->slots, c = B->cells, S = scenes;

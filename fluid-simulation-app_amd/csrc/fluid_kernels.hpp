@@ -539,11 +539,13 @@ struct StageBufs {
 //
 // A solve issued directly (not captured into a graph) can also stop issuing
 // passes once it has converged (`stop`): each pass's convergence check
-// writes to pinned host memory, with an event behind it, the scene's pass
-// limit (one scene) or the number of scenes listed for the next pass (work
-// lists), and before issuing pass r the host waits for the check of pass
-// r - lead and issues nothing more when that limit is at most r (the
-// count is 0). The passes it leaves out are those the limits would turn into
+// writes to pinned host memory, tagged with a number the host gave that
+// check, the scene's pass limit (one scene) or the number of scenes listed
+// for the next pass (work lists), and before issuing pass r the host polls
+// until the check of pass r - lead has written its slot and issues nothing
+// more when that limit is at most r (the count is 0). No event or stream
+// query goes into the queue while it polls: each one held the queue about
+// 5.6 us. The passes it leaves out are those the limits would turn into
 // no-ops, so the results are the same bits; `lead` passes stay queued ahead
 // of the host's wait, so the device never waits for the host: kStopLead, or
 // kStopLeadLarge for launches of at least kStopLargeTiles tiles, whose passes
@@ -551,8 +553,10 @@ struct StageBufs {
 // scene only.
 constexpr uint32_t kStopLead = 4, kStopLeadLarge = 2, kStopLargeTiles = 2048, kStopRing = 8;
 struct PassStop {
-    uint32_t* host;  // pinned (coherent), kStopRing words the checks write
-    hipEvent_t ev[kStopRing];
+    // pinned (coherent), kStopRing slots the checks write: tag << 32 | value
+    unsigned long long* host;
+    uint32_t next = 0;              // the last tag handed out (0: none)
+    uint32_t tag[kStopRing] = {};   // the tag of the check each slot awaits
 };
 struct Tolerance {
     float tol;
@@ -560,7 +564,7 @@ struct Tolerance {
     float div_tol;
     uint32_t div_max;
     bool persistent;
-    const PassStop* stop = nullptr;
+    PassStop* stop = nullptr;
     bool on() const { return tol > 0.0f || div_tol > 0.0f; }
 };
 

@@ -124,7 +124,7 @@ struct FluidSim {
     // 50 + 39 us against 71 + 51 us on work lists.
     Tolerance tol{0.0f, 1, 0, 0.0f, 1, false};
     uint32_t* limits = nullptr;
-    // The solve's early stop in tolerance mode (pinned limit words, events).
+    // The solve's early stop in tolerance mode (pinned slots).
     PassStop stop{};
     EventTimer timer;
     DeviceAllocs mem;
@@ -138,8 +138,6 @@ struct FluidSim {
 
     ~FluidSim() {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (hipEvent_t e : stop.ev)
-            if (e) (void)hipEventDestroy(e);
         if (stop.host) (void)hipHostFree(stop.host);
         mem.release();
         if (stream) (void)hipStreamDestroy(stream);
@@ -247,8 +245,8 @@ int init_sim(FluidSim* s, const uint8_t* params64, uint32_t ntr, uint32_t nbr_ca
     A(b.pass_hist, 2 * kHistBins);
     A(b.tile_lists, 2 * (2 + 2 + 1));
     A(s->limits, 2);
-    HIP_TRY(hipHostMalloc((void**)&s->stop.host, kStopRing * sizeof(uint32_t), hipHostMallocCoherent));
-    for (hipEvent_t& e : s->stop.ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipHostMalloc((void**)&s->stop.host, kStopRing * sizeof(unsigned long long), hipHostMallocCoherent));
+    std::memset(s->stop.host, 0, kStopRing * sizeof(unsigned long long));
     s->tol.stop = &s->stop;
     // FLUID_TOL_PERSISTENT=1: tolerance mode's passes on work lists.
     if (const char* env = getenv("FLUID_TOL_PERSISTENT")) s->tol.persistent = strtol(env, nullptr, 10) != 0;

@@ -33,6 +33,8 @@
 // pair, the neighbour and the scalar dW/r; the later sweeps rebuild each
 // gradient as dW/r times x_i - x_j, the product grad_q itself forms, so it is
 // bit-identical.
+#include <chrono>
+
 #include "fluid_common.hpp"
 #include "fluid_kernels.hpp"
 
@@ -1272,8 +1274,8 @@ template <class Sc>
 __global__ __launch_bounds__(kBlock) void k_converge(Sc sc, uint32_t scenes, uint32_t pass, float tol,
                                                      uint32_t min_passes, unsigned long long* __restrict__ residual,
                                                      uint32_t* __restrict__ limit,
-                                                     uint32_t* __restrict__ iterations, uint32_t* host,
-                                                     uint32_t slots) {
+                                                     uint32_t* __restrict__ iterations, unsigned long long* host,
+                                                     uint32_t tag, uint32_t slots) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= scenes) return;
     if (sc.step_at(i).active != 0u && pass < limit[i]) {
@@ -1283,7 +1285,7 @@ __global__ __launch_bounds__(kBlock) void k_converge(Sc sc, uint32_t scenes, uin
         iterations[i] = pass + 1;
         if (mean <= (double)tol && pass + 1 >= min_passes) limit[i] = pass + 1;
     }
-    if (host) *host = limit[i];
+    if (host) *host = ((unsigned long long)tag << 32) | limit[i];
 }
 
 // Tolerance mode, before a substep's solve: clears the residuals and sets
@@ -1329,8 +1331,8 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
                                                         float tol, uint32_t min_passes,
                                                         unsigned long long* __restrict__ residual,
                                                         uint32_t* __restrict__ limit, uint32_t* __restrict__ iterations,
-                                                        uint32_t next, uint32_t* __restrict__ list, uint32_t* host,
-                                                        uint32_t slots) {
+                                                        uint32_t next, uint32_t* __restrict__ list,
+                                                        unsigned long long* host, uint32_t tag, uint32_t slots) {
     __shared__ uint32_t s_tiles[16], s_scenes[16];
     uint32_t* const first = list + 2;
     uint32_t* const tile0 = list + 3 + scenes;
@@ -1385,7 +1387,7 @@ __global__ __launch_bounds__(1024) void k_converge_list(Sc sc, uint32_t scenes, 
         list[0] = tiles_carry;
         list[1] = scenes_carry;
         first[scenes_carry] = tiles_carry;
-        if (host) *host = scenes_carry;  // the early stop: 0 ends the solve
+        if (host) *host = ((unsigned long long)tag << 32) | scenes_carry;  // the early stop: 0 ends the solve
     }
 }
 
@@ -1515,28 +1517,55 @@ struct NoCheck {
     bool stopped(uint32_t, bool) const { return false; }
 };
 
-// The early stop (PassStop) on stream `st`: the convergence check of pass
-// `pass` writes to slot(pass) either the scene's limit (one scene, k_converge)
-// or, with `counts`, the number of scenes listed for the next pass
-// (k_converge_list), and note() puts an event behind it; stopped() before
-// issuing pass `pass` waits for the check `lead` passes back and says
-// whether the solve ended there.
+// The early stop (PassStop) on stream `st`: issue() gives the convergence
+// check of pass `pass` its tag and slot(pass) its slot, where it writes
+// either the scene's limit (one scene, k_converge) or, with `counts`, the
+// number of scenes listed for the next pass (k_converge_list); stopped()
+// before issuing pass `pass` waits for the check `lead` passes back and
+// says whether the solve ended there.
 inline uint32_t stop_lead(uint32_t tiles) { return tiles >= kStopLargeTiles ? kStopLeadLarge : kStopLead; }
+// Polls a slot until its check has written it. After two seconds without
+// it, the stream's state decides: still running, the wait goes on; done or
+// failed with the slot unwritten, the wait ends and the caller issues the
+// pass, which the limits make a no-op if it is one.
+inline bool wait_slot(const PassStop& p, uint32_t k, hipStream_t st, uint32_t& value) {
+    auto since = std::chrono::steady_clock::now();
+    for (uint32_t i = 1;; ++i) {
+        const unsigned long long v = __atomic_load_n(&p.host[k], __ATOMIC_ACQUIRE);
+        if ((uint32_t)(v >> 32) == p.tag[k]) {
+            value = (uint32_t)v;
+            return true;
+        }
+        if (i % 65536u == 0u && std::chrono::steady_clock::now() - since > std::chrono::seconds(2)) {
+            if (hipStreamQuery(st) == hipErrorNotReady) {
+                since = std::chrono::steady_clock::now();
+                continue;
+            }
+            const unsigned long long w = __atomic_load_n(&p.host[k], __ATOMIC_ACQUIRE);
+            value = (uint32_t)w;
+            return (uint32_t)(w >> 32) == p.tag[k];
+        }
+    }
+}
 struct Stopper {
-    const PassStop* stop;
+    PassStop* stop;
     hipStream_t st;
     bool counts;
     uint32_t lead;
-    // Where the check of pass `pass` writes the limit (null: no early stop).
-    uint32_t* slot(uint32_t pass) const { return stop ? stop->host + pass % kStopRing : nullptr; }
-    void note(uint32_t pass) const {
-        if (stop) (void)hipEventRecord(stop->ev[pass % kStopRing], st);
+    // Where the check of pass `pass` writes (null: no early stop).
+    unsigned long long* slot(uint32_t pass) const { return stop ? stop->host + pass % kStopRing : nullptr; }
+    // The tag of the check of pass `pass`, issued next.
+    uint32_t issue(uint32_t pass) const {
+        if (!stop) return 0u;
+        if (++stop->next == 0u) ++stop->next;
+        stop->tag[pass % kStopRing] = stop->next;
+        return stop->next;
     }
     bool stopped(uint32_t pass) const {
         if (!stop || pass < lead) return false;
-        const uint32_t k = (pass - lead) % kStopRing;
-        if (hipEventSynchronize(stop->ev[k]) != hipSuccess) return false;
-        return counts ? stop->host[k] == 0u : stop->host[k] <= pass;
+        uint32_t v = 0;
+        if (!wait_slot(*stop, (pass - lead) % kStopRing, st, v)) return false;
+        return counts ? v == 0u : v <= pass;
     }
 };
 // The substep's launches. With stage buffers (the adjoint's taped forward)
@@ -1705,11 +1734,11 @@ static void solve_tolerance(hipStream_t st, Sc sc, uint32_t scenes, uint32_t slo
         Stopper stop;
         void operator()(hipStream_t s2, uint32_t pass, bool divergence) const {
             if (divergence ? !div : !den) return;
+            const uint32_t tag = stop.issue(pass);
             hipLaunchKernelGGL(k_converge<Sc>, dim3(blocks_for(scenes)), dim3(kBlock), 0, s2, sc, scenes, pass,
                                divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
-                               divergence ? b.div_iterations : b.iterations, stop.slot(pass), b.resid_slots);
-            stop.note(pass);
+                               divergence ? b.div_iterations : b.iterations, stop.slot(pass), tag, b.resid_slots);
         }
         bool stopped(uint32_t pass, bool divergence) const {
             return (divergence ? div : den) && stop.stopped(pass);
@@ -1753,12 +1782,12 @@ static void solve_listed(hipStream_t st, Sc sc, uint32_t scenes, uint32_t n, con
         auto stopped = [&](uint32_t pass) { return stop.stopped(pass); };
         // converge pass `pass` of one solve (check) and list the scenes of pass `next`
         auto converge = [&](uint32_t pass, bool check, bool divergence, uint32_t next) {
+            const uint32_t tag = check ? stop.issue(pass) : 0u;
             hipLaunchKernelGGL(k_converge_list<Sc>, dim3(1), dim3(1024), 0, st, sc, scenes, pass,
                                check ? 1u : 0u, divergence ? tol.div_tol : tol.tol, divergence ? 1u : tol.min_passes,
                                divergence ? b.div_residual : b.residual, divergence ? limits + scenes : limits,
                                divergence ? b.div_iterations : b.iterations, next, lists[next % 2u],
-                               check ? stop.slot(pass) : nullptr, b.resid_slots);
-            if (check) stop.note(pass);
+                               check ? stop.slot(pass) : nullptr, tag, b.resid_slots);
         };
         const Span tr = span(timer, KERNEL_DENSITY_DIV);
         const bool own = tile_regions(st, sc, ntiles, starts, work, b, tr.a);

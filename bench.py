@@ -372,6 +372,9 @@ def cpu_baseline(scene, step, refine, budget_s=10.0, max_substeps=64):
 
 # The configs' solver tolerances: mean divergence error 0.1%, mean density
 # error 0.01%; each solve stops at MAX_PASSES passes.
+# The timed region's dominant-kernel events time one launch in every 7 (7
+# and the 5 density passes are coprime, so every pass is sampled).
+DOMINANT_STRIDE = 7
 TOL_DIVERGENCE = 1e-3
 TOL_DENSITY = 1e-4
 MAX_PASSES = 256
@@ -765,9 +768,11 @@ def main():
     dominant = max((k for k in pre if kernel_bytes(k, sc.n, sim.cells)), key=lambda k: pre[k][0])
 
     # the timed region: exactly `steps` substeps, the dominant kernel timed by
-    # start/stop events carried on its own dispatches (no extra packets)
+    # start/stop events carried on its own dispatches, one launch in every
+    # DOMINANT_STRIDE (the events slow the stream: about 2% of a 1M substep
+    # with every den_kappa launch timed)
     clamps0, overflow0 = sim.counters()
-    sim.profile([dominant])
+    sim.profile([dominant], stride=DOMINANT_STRIDE)
     sim.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):

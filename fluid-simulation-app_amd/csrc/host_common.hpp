@@ -89,6 +89,10 @@ struct DeviceAllocs {
 class EventTimer final : public KernelTimer {
   public:
     uint32_t mask = 0;
+    // Times one launch in every `stride` of each class: a timed launch's
+    // events go into the queue beside it and slow the stream a little.
+    uint32_t stride = 1;
+    uint64_t seen[KERNEL_COUNT_] = {};
     struct Pair {
         hipEvent_t a, b;
         int kernel;
@@ -117,8 +121,10 @@ class EventTimer final : public KernelTimer {
         return e;
     }
     bool wants(int kernel) const { return (mask >> kernel) & 1u; }
+    // Whether this launch of the class is one to time.
+    bool sample(int kernel) { return seen[kernel]++ % (stride ? stride : 1u) == 0u; }
     void begin(hipStream_t st, int kernel) override {
-        if (!wants(kernel)) return;
+        if (!wants(kernel) || !sample(kernel)) return;
         open[kernel] = take();
         (void)hipEventRecord(open[kernel], st);
     }
@@ -130,7 +136,7 @@ class EventTimer final : public KernelTimer {
         open[kernel] = nullptr;
     }
     hipEvent_t start(int kernel) override {
-        if (!wants(kernel)) return nullptr;
+        if (!wants(kernel) || !sample(kernel)) return nullptr;
         open[kernel] = take();
         return open[kernel];
     }
@@ -157,6 +163,7 @@ class EventTimer final : public KernelTimer {
         collect();
         std::fill(ms, ms + KERNEL_COUNT_, 0.0);
         std::fill(launches, launches + KERNEL_COUNT_, 0);
+        std::fill(seen, seen + KERNEL_COUNT_, 0);
     }
 };
 

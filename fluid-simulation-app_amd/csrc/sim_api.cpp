@@ -695,11 +695,14 @@ int fluid_sim_synchronize(FluidSim* s) {
     return FLUID_OK;
 }
 
-int fluid_sim_profile(FluidSim* s, uint32_t kernel_mask) {
+int fluid_sim_profile(FluidSim* s, uint32_t kernel_mask) { return fluid_sim_profile_sampled(s, kernel_mask, 1); }
+
+int fluid_sim_profile_sampled(FluidSim* s, uint32_t kernel_mask, uint32_t stride) {
     if (!valid(s)) return FLUID_ERR_ARG;
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->timer.reset();
     s->timer.mask = kernel_mask;
+    s->timer.stride = stride ? stride : 1u;
     return FLUID_OK;
 }
 

@@ -59,6 +59,7 @@ def _declare(L):
         "fluid_sim_counters": ([vp, P(u64), P(u64)], i32),
         "fluid_sim_synchronize": ([vp], i32),
         "fluid_sim_profile": ([vp, u32], i32),
+        "fluid_sim_profile_sampled": ([vp, u32, u32], i32),
         "fluid_sim_profile_read": ([vp, P(f64), P(u64)], i32),
         "fluid_sim_kernel_name": ([u32], C.c_char_p),
         "fluid_sim_describe": ([vp, C.c_char_p, C.c_size_t], i32),

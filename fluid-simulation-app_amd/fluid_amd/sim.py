@@ -324,11 +324,13 @@ class Sim:
         check(self.L.fluid_sim_describe(self.h, buf, 512), "describe")
         return json.loads(buf.value.decode())
 
-    def profile(self, kernels=None):
+    def profile(self, kernels=None, stride: int = 1):
+        """Times the launches of the named kernel classes (one in every
+        `stride` of each class)."""
         mask = 0
         for k in (kernels or []):
             mask |= 1 << (KERNELS.index(k) if isinstance(k, str) else int(k))
-        check(self.L.fluid_sim_profile(self.h, mask), "profile")
+        check(self.L.fluid_sim_profile_sampled(self.h, mask, int(stride)), "profile")
 
     def profile_read(self) -> dict:
         ms = np.zeros(len(KERNELS), np.float64)

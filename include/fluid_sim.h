@@ -225,6 +225,12 @@ int fluid_sim_synchronize(FluidSim *sim);
 #define FLUID_KERNEL_TRACERS 11u
 #define FLUID_KERNEL_CLASSES 12u
 int fluid_sim_profile(FluidSim *sim, uint32_t kernel_mask);
+/* The same, timing one launch in every `stride` of each class in the mask
+ * (stride 0 or 1: every launch). A timed launch's start and stop events go
+ * into the stream beside it and slow it a little (about 2% of a 1M substep
+ * with every den_kappa launch timed); the read-out's averages cover the
+ * timed launches. No counterpart in the reference. */
+int fluid_sim_profile_sampled(FluidSim *sim, uint32_t kernel_mask, uint32_t stride);
 /* Total milliseconds and launch count per kernel class since profiling began (synchronises). */
 int fluid_sim_profile_read(FluidSim *sim, double *ms, uint64_t *launches);
 const char *fluid_sim_kernel_name(uint32_t kernel);
